@@ -1,0 +1,176 @@
+/*
+ * kestrel_step.h -- C-ABI of the MI355X (gfx950) finite-volume morphodynamic
+ * time step.
+ *
+ * The library replaces the body of Kestrel's IntegrateTo
+ * (src/TimeStepper.f90:116-277).  A Fortran host keeps calling Run
+ * (src/TimeStepper.f90:73-113) once per output interval; Run calls
+ * kst_integrate_to instead of IntegrateTo and afterwards downloads the state
+ * with kst_get_state into grid%tileContainer, so OutputSolutionData,
+ * CalculateVolume and OutputAggregateData (src/Output.f90) read the same data
+ * as before.
+ *
+ * Conventions follow the reference's own C bridges (src/GeoTiffRead.f90:43-100,
+ * src/utm.f90:55-100): plain pointers and sizes, the caller owns every host
+ * array, an int status (0 = success, negative = fatal) that the Fortran side
+ * turns into FatalErrorMessage -> exit(1) (src/Messages.f90:73-89).
+ * One host thread per handle; every call blocks until host-visible results
+ * are ready.
+ *
+ * Grid layout at the boundary (all arrays row-major, x fastest):
+ *   cells     NX*NY            with NX = nx_tiles*nx_per_tile, NY likewise;
+ *                              cell (I,J) of the global grid is cell
+ *                              (ii,jj) = (I mod nx_per_tile, J mod ny_per_tile)
+ *                              of tile TileID(I/nx_per_tile+1, J/ny_per_tile+1)
+ *                              (src/Grid.f90:249-256), 0-based here.
+ *   vertices  (NX+1)*(NY+1)    vertex (I,J) is the south-west corner of cell (I,J).
+ *   tiles     nx_tiles*ny_tiles tile t = ti + tj*nx_tiles (TileID - 1).
+ */
+#ifndef KESTREL_STEP_H
+#define KESTREL_STEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enumerations: the runtime-selected closures (src/Closures.f90:67-151,
+ *      src/Limiters.f90:68-75, bound from the input deck in
+ *      src/Parameters.f90:175-420 and src/SolverSettings.f90:102-126) ---- */
+enum { KST_BC_HALT = 0 };                                   /* "halt" (src/UpdateTiles.f90:61-65) */
+enum { KST_LIMITER_MINMOD1 = 0, KST_LIMITER_MINMOD2 = 1, KST_LIMITER_VANALBADA = 2,
+       KST_LIMITER_WENO = 3, KST_LIMITER_NONE = 4 };
+enum { KST_DRAG_CHEZY = 0, KST_DRAG_COULOMB = 1, KST_DRAG_VOELLMY = 2, KST_DRAG_POULIQUEN = 3,
+       KST_DRAG_EDWARDS2019 = 4, KST_DRAG_MANNING = 5, KST_DRAG_VARIABLE = 6 };
+enum { KST_EROSION_NONE = 0, KST_EROSION_SIMPLE = 1, KST_EROSION_FLUID = 2,
+       KST_EROSION_GRANULAR = 3, KST_EROSION_MIXED = 4 };
+enum { KST_DEPOSITION_NONE = 0, KST_DEPOSITION_SIMPLE = 1, KST_DEPOSITION_SPEARMAN_MANNING = 2 };
+enum { KST_TRANSITION_NONE = 0, KST_TRANSITION_SMOOTH = 1, KST_TRANSITION_STEP = 2 };
+enum { KST_DAMPING_NONE = 0, KST_DAMPING_TANH = 1, KST_DAMPING_RAT3 = 2 };
+enum { KST_SWITCH_TANH = 0, KST_SWITCH_RAT3 = 1, KST_SWITCH_COS = 2, KST_SWITCH_EQUAL = 3,
+       KST_SWITCH_ZERO = 4, KST_SWITCH_ONE = 5, KST_SWITCH_LINEAR = 6, KST_SWITCH_STEP = 7 };
+/* Storage/arithmetic precision of the device state. */
+enum { KST_PREC_FP64 = 0, KST_PREC_MIXED = 1 };
+/* Per-tile state (TileType%TileOn / %isGhostTile, src/Grid.f90:59-60). */
+enum { KST_TILE_OFF = 0, KST_TILE_ACTIVE = 1, KST_TILE_GHOST = 2 };
+
+/* Status codes. */
+enum { KST_OK = 0, KST_ERR_CONFIG = -1, KST_ERR_HIP = -2, KST_ERR_HALT = -3,
+       KST_ERR_NONFINITE = -4, KST_ERR_STATE = -5, KST_ERR_TOO_MANY_STEPS = -6 };
+
+/* One flux source (RunSet%FluxSources, src/RunSettings.f90; evaluated in
+ * ExplicitSourceTerms, src/Equations.f90:456-620).  time/flux/psi hold
+ * n_series knots; num_cells is NumCellsInSrc (src/SetSources.f90:357-375). */
+typedef struct kst_flux_source {
+    double x, y, radius;
+    int32_t n_series;
+    int32_t num_cells;
+    const double* time;
+    const double* flux;
+    const double* psi;
+} kst_flux_source;
+
+/* Every RunSet scalar the step reads (src/RunSettings.f90:161-340), plus the
+ * derived parameters of src/Parameters.f90:629-648 (filled in by
+ * kst_config_defaults / kst_config_derive). */
+typedef struct kst_config {
+    int32_t nx_tiles, ny_tiles;          /* nXtiles, nYtiles */
+    int32_t nx_per_tile, ny_per_tile;    /* nXpertile, nYpertile */
+    double dx, dy;                       /* deltaX, deltaY */
+    double x_size, y_size;               /* xSize, ySize: cell centres at -size/2 + dx*(I+1/2) */
+    int32_t bc;
+    int32_t limiter;
+    int32_t geometric_factors;           /* 1: gamma = sqrt(1+bx^2+by^2); 0: gamma = 1 */
+    int32_t drag;
+    int32_t erosion;
+    int32_t deposition;
+    int32_t ero_transition;
+    int32_t morpho_damping;
+    int32_t fswitch;
+    int32_t morphodynamics;              /* MorphodynamicsOn */
+    int32_t tile_buffer;                 /* TileBuffer */
+    int32_t track_maxima;                /* UpdateMaximum* each accepted step */
+    int32_t precision;                   /* KST_PREC_FP64 / KST_PREC_MIXED */
+    int32_t n_sources;
+    double g, rhow, rhos;
+    double chezy_co, coulomb_co, manning_co;
+    double pouliquen_min, pouliquen_max, pouliquen_intermediate, pouliquen_beta;
+    double edwards_betastar, edwards_kappa, edwards_gamma;
+    double ero_rate, ero_rate_granular, ero_depth, ero_critical_height;
+    double voellmy_switch_rate, voellmy_switch_value;
+    double max_pack, bed_porosity, solid_diameter, eddy_viscosity;
+    double cfl, max_dt, height_threshold;
+    /* derived (src/Parameters.f90:629-648) */
+    double gred, rep, ws0, nsettling, critical_shields, diffusive_time_scale;
+    const kst_flux_source* sources;
+} kst_config;
+
+/* Statistics of one kst_integrate_to call. */
+typedef struct kst_stats {
+    int64_t steps;            /* accepted outer steps */
+    int64_t rejected;         /* rolled-back attempts (refineTimeStep) */
+    int64_t rhs_evals;        /* hydraulic RHS evaluations */
+    int64_t cell_updates;     /* sum over accepted steps of active cells */
+    int32_t active_tiles;     /* after the call */
+    int32_t ghost_tiles;
+    double t;                 /* simulation time after the call */
+    double last_dt;
+    double min_dt, max_dt;
+} kst_stats;
+
+typedef struct kst_handle kst_handle;
+
+/* Fill the reference defaults (src/Parameters.f90:41-77,
+ * src/SolverSettings.f90:42-51) for everything but the grid. */
+void kst_config_defaults(kst_config* cfg);
+/* Compute the derived parameters (src/Parameters.f90:629-648). */
+void kst_config_derive(kst_config* cfg);
+
+/* Create a handle on the current HIP device. Replaces InitialiseGrid's
+ * allocation of tileContainer/intermed0..3 (src/Grid.f90:166-220). */
+int kst_create(const kst_config* cfg, kst_handle** out);
+
+/* Global vertex elevations b0, (NX+1)*(NY+1) doubles, after the tile-edge
+ * equalisation of EqualiseTopographicBoundaryData
+ * (src/MorphodynamicRHS.f90:588-687). Replaces GetHeights (src/dem.f90:360-393). */
+int kst_set_topography(kst_handle* h, const double* b0_vertex);
+
+/* Conserved fields w, rhoHnu, rhoHnv, Hnpsi (NX*NY each), vertex bt
+ * ((NX+1)*(NY+1), may be NULL = 0), tile state (KST_TILE_*), time t and
+ * optional initial maxima (Hnmax, may be NULL).  Ghost-tile w reconstructions
+ * are rebuilt as at the end of LoadSourceConditions
+ * (src/SetSources.f90:388-392). */
+int kst_set_state(kst_handle* h, const double* w, const double* rhoHnu,
+                  const double* rhoHnv, const double* Hnpsi, const double* bt_vertex,
+                  const uint8_t* tile_state, double t, const double* hnmax0);
+
+/* Integrate from the current time to tend with the adaptive SSP-RK3 /
+ * Strang-split step of IntegrateTo (src/TimeStepper.f90:116-277). */
+int kst_integrate_to(kst_handle* h, double tend, kst_stats* stats_out);
+
+/* Take exactly n_steps accepted steps (a step that would pass tend is not
+ * clamped); used for fixed-step parity runs and throughput measurement. */
+int kst_step_n(kst_handle* h, int64_t n_steps, kst_stats* stats_out);
+
+/* Download the state: w, rhoHnu, rhoHnv, Hnpsi, Hn (NX*NY each, any may be
+ * NULL), vertex bt ((NX+1)*(NY+1), may be NULL), tile states, time. */
+int kst_get_state(kst_handle* h, double* w, double* rhoHnu, double* rhoHnv,
+                  double* Hnpsi, double* Hn, double* bt_vertex, uint8_t* tile_state,
+                  double* t);
+
+/* Running maxima (src/TimeStepper.f90:1155-1303): 11 arrays of NX*NY doubles
+ * in the order Hnmax, Hnmax_t, umax, umax_t, emax, emax_t, dmax, dmax_t,
+ * psimax, psimax_t, tfirst. */
+int kst_get_maxima(kst_handle* h, double* const maxima[11]);
+
+/* Copy the last error message (NUL-terminated) into buf. */
+int kst_last_error(const kst_handle* h, char* buf, int buflen);
+
+void kst_destroy(kst_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KESTREL_STEP_H */

@@ -1,0 +1,75 @@
+/*
+ * kestrel_oracle.h -- CPU restatement of Kestrel's finite-volume step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP path is checked
+ * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker or as the timed CPU baseline; the
+ * product (kestrel_amd/) never calls it.
+ *
+ * It keeps the reference's data model -- tiles with AoS-by-field arrays, a
+ * tileContainer plus intermed0..3 deep copies, sorted active-tile list and
+ * ghost tiles with frozen data -- so that every sweep can be compared with
+ * the reference routine it restates (file:line citations in the .c file).
+ */
+#ifndef KESTREL_ORACLE_H
+#define KESTREL_ORACLE_H
+
+#include <stdint.h>
+#include "../include/kestrel_step.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ko_sim ko_sim;
+
+ko_sim* ko_create(const kst_config* cfg);
+void ko_destroy(ko_sim* s);
+const char* ko_error(const ko_sim* s);
+
+/* Global vertex b0, (NX+1)*(NY+1), row-major (same layout as kst_set_topography). */
+int ko_set_topography(ko_sim* s, const double* b0_vertex);
+
+/* Initial conditions (src/SetSources.f90:47-392): caps (shape 0 flat,
+ * 1 para, 2 level), flux-source cell counting, then the closing
+ * ReconstructwAtEdges sweep over ghost tiles. */
+int ko_add_caps(ko_sim* s, int ncaps, const double* x, const double* y, const double* radius,
+                const double* height, const double* psi, const double* u, const double* v,
+                const int* shape);
+/* Counts NumCellsInSrc for cfg->sources (writes into num_cells_out[n_sources])
+ * and activates the tiles holding source cells. */
+int ko_add_flux_sources(ko_sim* s, int32_t* num_cells_out);
+int ko_finish_sources(ko_sim* s);
+
+/* Direct state load (the C-ABI's kst_set_state semantics). */
+int ko_set_state(ko_sim* s, const double* w, const double* rhoHnu, const double* rhoHnv,
+                 const double* Hnpsi, const double* bt_vertex, const uint8_t* tile_state,
+                 double t, const double* hnmax0);
+
+int ko_integrate_to(ko_sim* s, double tend, kst_stats* st);
+int ko_step_n(ko_sim* s, int64_t n, kst_stats* st);
+
+int ko_get_state(ko_sim* s, double* w, double* rhoHnu, double* rhoHnv, double* Hnpsi,
+                 double* Hn, double* bt_vertex, uint8_t* tile_state, double* t);
+int ko_get_maxima(ko_sim* s, double* const maxima[11]);
+
+/* Full tile-level dump for cross-checks: all 13 fields of every allocated
+ * tile's u (tileContainer), cell-major global layout, NX*NY per field
+ * (unallocated tiles give 0). */
+int ko_get_fields13(ko_sim* s, double* out13);
+/* Ghost-tile w slopes (uLimX/uLimY of field w), global NX*NY each. */
+int ko_get_ghost_slopes(ko_sim* s, double* sx, double* sy);
+
+/* One CalculateHydraulicRHS on tileContainer (substep 1 semantics without
+ * touching grid%dt): writes ddtExplicit (4*NX*NY) and ddtImplicit (4*NX*NY)
+ * in global layout and returns the advised time step. */
+int ko_rhs(ko_sim* s, int substep, double t, double* ddte, double* ddti, double* advised);
+
+/* Number of active / ghost tiles and the sorted active list (0-based ids). */
+int ko_tile_lists(ko_sim* s, int32_t* active, int32_t* nactive, int32_t* ghost, int32_t* nghost);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

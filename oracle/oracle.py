@@ -1,0 +1,225 @@
+"""ctypes wrapper of the oracle library (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the kestrel_amd package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from kestrel_amd.config import KstConfig
+from kestrel_amd import scenario
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+class KstStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("rejected", ctypes.c_int64),
+                ("rhs_evals", ctypes.c_int64), ("cell_updates", ctypes.c_int64),
+                ("active_tiles", ctypes.c_int32), ("ghost_tiles", ctypes.c_int32),
+                ("t", ctypes.c_double), ("last_dt", ctypes.c_double),
+                ("min_dt", ctypes.c_double), ("max_dt", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.ko_create.restype = ctypes.c_void_p
+        L.ko_create.argtypes = [ctypes.POINTER(KstConfig)]
+        L.ko_destroy.argtypes = [ctypes.c_void_p]
+        L.ko_error.restype = ctypes.c_char_p
+        L.ko_error.argtypes = [ctypes.c_void_p]
+        for name in ["ko_set_topography", "ko_add_caps", "ko_add_flux_sources", "ko_finish_sources",
+                     "ko_set_state", "ko_integrate_to", "ko_step_n", "ko_get_state",
+                     "ko_get_maxima", "ko_get_fields13", "ko_get_ghost_slopes", "ko_rhs",
+                     "ko_tile_lists"]:
+            getattr(L, name).restype = ctypes.c_int
+        L.ko_set_topography.argtypes = [ctypes.c_void_p, _dp]
+        L.ko_add_caps.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
+        L.ko_add_flux_sources.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        L.ko_finish_sources.argtypes = [ctypes.c_void_p]
+        L.ko_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(ctypes.c_uint8), ctypes.c_double, _dp]
+        L.ko_integrate_to.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(KstStats)]
+        L.ko_step_n.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(KstStats)]
+        L.ko_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(ctypes.c_uint8), _dp]
+        L.ko_get_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
+        L.ko_get_fields13.argtypes = [ctypes.c_void_p, _dp]
+        L.ko_get_ghost_slopes.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ko_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.ko_tile_lists.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
+        _LIB = L
+    return _LIB
+
+
+SHAPES = {"flat": 0, "para": 1, "level": 2}
+
+
+class Oracle:
+    """One simulation in the oracle."""
+
+    def __init__(self, cfg: KstConfig):
+        self.cfg = cfg
+        self.L = lib()
+        self.h = self.L.ko_create(ctypes.byref(cfg))
+        self._check(0)
+
+    def close(self):
+        if self.h:
+            self.L.ko_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        msg = self.L.ko_error(self.h).decode()
+        if rc != 0 or msg:
+            raise RuntimeError(f"oracle error {rc}: {msg}")
+
+    def set_topography(self, b0v):
+        self.b0v = np.ascontiguousarray(b0v, dtype=np.float64)
+        self._check(self.L.ko_set_topography(self.h, _ptr(self.b0v)))
+
+    def add_caps(self, caps):
+        n = len(caps)
+        arrs = [np.array([getattr(c, f) for c in caps], dtype=np.float64)
+                for f in ("x", "y", "radius", "height", "psi", "u", "v")]
+        shp = (ctypes.c_int * max(n, 1))(*[SHAPES[c.shape] for c in caps])
+        self._arrs = arrs
+        self._check(self.L.ko_add_caps(self.h, n, *[_ptr(a) for a in arrs], shp))
+
+    def add_flux_sources(self):
+        n = max(1, self.cfg.n_sources)
+        out = (ctypes.c_int32 * n)()
+        self._check(self.L.ko_add_flux_sources(self.h, out))
+        return list(out)[: self.cfg.n_sources]
+
+    def finish_sources(self):
+        self._check(self.L.ko_finish_sources(self.h))
+
+    def set_state(self, st: "scenario.InitialState", t=0.0):
+        ts = np.ascontiguousarray(st.tile_state, dtype=np.uint8)
+        self._check(self.L.ko_set_state(
+            self.h, _ptr(np.ascontiguousarray(st.w)), _ptr(np.ascontiguousarray(st.rhoHnu)),
+            _ptr(np.ascontiguousarray(st.rhoHnv)), _ptr(np.ascontiguousarray(st.Hnpsi)), None,
+            ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), float(t),
+            _ptr(np.ascontiguousarray(st.hnmax0))))
+
+    def integrate_to(self, tend):
+        st = KstStats()
+        self._check(self.L.ko_integrate_to(self.h, float(tend), ctypes.byref(st)))
+        return st.as_dict()
+
+    def step_n(self, n):
+        st = KstStats()
+        self._check(self.L.ko_step_n(self.h, int(n), ctypes.byref(st)))
+        return st.as_dict()
+
+    def state(self):
+        c = self.cfg
+        shp = (c.NY, c.NX)
+        out = {k: np.zeros(shp) for k in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "Hn")}
+        ts = np.zeros(c.ntiles, dtype=np.uint8)
+        t = ctypes.c_double()
+        self._check(self.L.ko_get_state(self.h, _ptr(out["w"]), _ptr(out["rhoHnu"]), _ptr(out["rhoHnv"]),
+                                        _ptr(out["Hnpsi"]), _ptr(out["Hn"]), None,
+                                        ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(t)))
+        out["tile_state"] = ts
+        out["t"] = t.value
+        return out
+
+    def maxima(self):
+        c = self.cfg
+        arrs = [np.zeros((c.NY, c.NX)) for _ in range(11)]
+        ptrs = (_dp * 11)(*[_ptr(a) for a in arrs])
+        self._check(self.L.ko_get_maxima(self.h, ptrs))
+        return arrs
+
+    def fields13(self):
+        c = self.cfg
+        out = np.zeros((13, c.NY, c.NX))
+        self._check(self.L.ko_get_fields13(self.h, _ptr(out)))
+        return out
+
+    def ghost_slopes(self):
+        c = self.cfg
+        sx = np.zeros((c.NY, c.NX)); sy = np.zeros((c.NY, c.NX))
+        self._check(self.L.ko_get_ghost_slopes(self.h, _ptr(sx), _ptr(sy)))
+        return sx, sy
+
+    def rhs(self, substep=1, t=0.0):
+        c = self.cfg
+        e = np.zeros((4, c.NY, c.NX)); i = np.zeros((4, c.NY, c.NX))
+        adv = ctypes.c_double()
+        self._check(self.L.ko_rhs(self.h, substep, float(t), _ptr(e), _ptr(i), ctypes.byref(adv)))
+        return e, i, adv.value
+
+    def tile_lists(self):
+        n = self.cfg.ntiles
+        a = (ctypes.c_int32 * n)(); g = (ctypes.c_int32 * n)()
+        na = ctypes.c_int32(); ng = ctypes.c_int32()
+        self._check(self.L.ko_tile_lists(self.h, a, ctypes.byref(na), g, ctypes.byref(ng)))
+        return list(a)[: na.value], list(g)[: ng.value]
+
+
+# ---------------------------------------------------------------------------
+# oracle/_ref: the reference's own hydraulic operator (flang build)
+# ---------------------------------------------------------------------------
+_REF = None
+
+
+def ref_lib():
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libkestrel_ref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/_ref/libkestrel_ref.so is not built (make -C oracle ref)")
+        L = ctypes.CDLL(path)
+        u8 = ctypes.POINTER(ctypes.c_int8)
+        L.kref_rhs.restype = None
+        L.kref_rhs.argtypes = [ctypes.POINTER(KstConfig), u8, u8, _dp, _dp, _dp, _dp,
+                               ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                               _dp, _dp, _dp]
+        _REF = L
+    return _REF
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libkestrel_ref.so"))
+
+
+def ref_rhs(cfg: KstConfig, tile_state, has_src, b0v, f13, sx, sy, *, recompute_ghost=False,
+            substep=1, t=0.0, grid_t=0.0):
+    """CalculateHydraulicRHS of the reference on the given global state."""
+    L = ref_lib()
+    NX, NY = cfg.NX, cfg.NY
+    ts = np.ascontiguousarray(tile_state, dtype=np.int8)
+    hs = np.ascontiguousarray(has_src, dtype=np.int8)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (b0v, f13, sx, sy)]
+    e = np.zeros((4, NY, NX)); i = np.zeros((4, NY, NX)); adv = ctypes.c_double()
+    u8 = ctypes.POINTER(ctypes.c_int8)
+    L.kref_rhs(ctypes.byref(cfg), ts.ctypes.data_as(u8), hs.ctypes.data_as(u8),
+               *[_ptr(a) for a in arrs], int(recompute_ghost), int(substep), float(t), float(grid_t),
+               _ptr(e), _ptr(i), ctypes.byref(adv))
+    return e, i, adv.value
