@@ -169,6 +169,14 @@ int kst_last_error(const kst_handle* h, char* buf, int buflen);
 
 void kst_destroy(kst_handle* h);
 
+/* Diagnostics (no counterpart in the reference, which has no timers;
+ * src/Messages.f90:275-283 prints only t and dt).  While profiling is on,
+ * every RK-stage kernel launch is bracketed by HIP events on the handle's
+ * stream; kst_profile_read returns the mean duration in ms of the launches
+ * that did work, per stage kernel (P1..P4), and their counts, then resets. */
+int kst_profile(kst_handle* h, int enable);
+int kst_profile_read(kst_handle* h, double stage_ms[4], int64_t launches[4]);
+
 #ifdef __cplusplus
 }
 #endif

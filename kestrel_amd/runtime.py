@@ -1,0 +1,158 @@
+"""ctypes binding of libkestrel_amd.so (the C-ABI of include/kestrel_step.h).
+
+``Stepper`` is the host-side mirror of the reference's step driver: one
+handle per simulation, created after the initial conditions are painted,
+then ``integrate_to(tend)`` per output interval as Run does with IntegrateTo
+(src/TimeStepper.f90:73-113).  The HIP library is required: there is no CPU
+fallback, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import KstConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkestrel_amd.so")
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_topography",
+           "kst_set_state", "kst_integrate_to", "kst_step_n", "kst_get_state", "kst_get_maxima",
+           "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read"]
+
+
+class KstStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("rejected", ctypes.c_int64),
+                ("rhs_evals", ctypes.c_int64), ("cell_updates", ctypes.c_int64),
+                ("active_tiles", ctypes.c_int32), ("ghost_tiles", ctypes.c_int32),
+                ("t", ctypes.c_double), ("last_dt", ctypes.c_double),
+                ("min_dt", ctypes.c_double), ("max_dt", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class KestrelError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"kestrel_amd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the HIP library; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m kestrel_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.kst_config_defaults.argtypes = [ctypes.POINTER(KstConfig)]
+        L.kst_config_defaults.restype = None
+        L.kst_config_derive.argtypes = [ctypes.POINTER(KstConfig)]
+        L.kst_config_derive.restype = None
+        L.kst_create.argtypes = [ctypes.POINTER(KstConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.kst_set_topography.argtypes = [ctypes.c_void_p, _dp]
+        L.kst_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _u8p, ctypes.c_double, _dp]
+        L.kst_integrate_to.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(KstStats)]
+        L.kst_step_n.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(KstStats)]
+        L.kst_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, _dp]
+        L.kst_get_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
+        L.kst_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        L.kst_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.kst_profile.restype = ctypes.c_int
+        L.kst_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]
+        L.kst_profile_read.restype = ctypes.c_int
+        L.kst_destroy.argtypes = [ctypes.c_void_p]
+        L.kst_destroy.restype = None
+        for n in ["kst_create", "kst_set_topography", "kst_set_state", "kst_integrate_to",
+                  "kst_step_n", "kst_get_state", "kst_get_maxima", "kst_last_error"]:
+            getattr(L, n).restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Stepper:
+    """One simulation resident on the current HIP device."""
+
+    def __init__(self, cfg: KstConfig):
+        self.cfg = cfg
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.kst_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise KestrelError(rc, "kst_create failed (configuration or device)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.kst_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            buf = ctypes.create_string_buffer(512)
+            self.L.kst_last_error(self.h, buf, 512)
+            raise KestrelError(rc, buf.value.decode())
+
+    def set_topography(self, b0v: np.ndarray):
+        self._b0v = np.ascontiguousarray(b0v, dtype=np.float64)
+        self._check(self.L.kst_set_topography(self.h, _ptr(self._b0v)))
+
+    def set_state(self, init, t: float = 0.0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (init.w, init.rhoHnu, init.rhoHnv, init.Hnpsi)]
+        ts = np.ascontiguousarray(init.tile_state, dtype=np.uint8)
+        hm = None if init.hnmax0 is None else np.ascontiguousarray(init.hnmax0, dtype=np.float64)
+        self._check(self.L.kst_set_state(self.h, *[_ptr(a) for a in arrs], None,
+                                         ts.ctypes.data_as(_u8p), float(t), _ptr(hm)))
+
+    def integrate_to(self, tend: float) -> dict:
+        st = KstStats()
+        self._check(self.L.kst_integrate_to(self.h, float(tend), ctypes.byref(st)))
+        return st.as_dict()
+
+    def step_n(self, n: int) -> dict:
+        st = KstStats()
+        self._check(self.L.kst_step_n(self.h, int(n), ctypes.byref(st)))
+        return st.as_dict()
+
+    def state(self) -> dict:
+        c = self.cfg
+        out = {k: np.zeros((c.NY, c.NX)) for k in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "Hn")}
+        ts = np.zeros(c.ntiles, dtype=np.uint8)
+        t = ctypes.c_double()
+        self._check(self.L.kst_get_state(self.h, _ptr(out["w"]), _ptr(out["rhoHnu"]), _ptr(out["rhoHnv"]),
+                                         _ptr(out["Hnpsi"]), _ptr(out["Hn"]), None,
+                                         ts.ctypes.data_as(_u8p), ctypes.byref(t)))
+        out["tile_state"] = ts
+        out["t"] = t.value
+        return out
+
+    def profile(self, enable: bool = True):
+        self._check(self.L.kst_profile(self.h, int(enable)))
+
+    def profile_read(self) -> tuple[list, list]:
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_int64 * 4)()
+        self._check(self.L.kst_profile_read(self.h, ms, n))
+        return list(ms), list(n)
+
+    def maxima(self) -> list:
+        c = self.cfg
+        arrs = [np.zeros((c.NY, c.NX)) for _ in range(11)]
+        ptrs = (_dp * 11)(*[_ptr(a) for a in arrs])
+        self._check(self.L.kst_get_maxima(self.h, ptrs))
+        return arrs

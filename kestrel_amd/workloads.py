@@ -52,3 +52,15 @@ def plain_cap(n: int = 256, steps: int = 1000, tile: int = 32, dx: float = 1.0,
     caps = [S.Cap(x0, 0.0, r, 1.0, shape="para")]
     init = S.initial_state(cfg, b0v, caps)
     return Workload(f"plain_cap_{n}", cfg, b0v, init, steps, caps, [])
+
+
+def plain_cap_2048(steps: int = 200) -> Workload:
+    """The plain hydraulic physics (10 degree plane, Chezy 0.01, CFL 0.25,
+    max dt 0.05 s) at 2048x2048 cells, dx = 1 m, with a paraboloid cap of
+    height 1 m and radius 700 m centred at x = +200 m: about 1.6 M cells in
+    about 1550 active tiles, so the step is measured on a full chip."""
+    return plain_cap(2048, steps, tile=32, dx=1.0, cap_x=200.0, cap_radius=700.0)
+
+
+def plain_cap_256_sample() -> Workload:
+    return plain_cap(256, 1000)

@@ -1,0 +1,85 @@
+"""Parity of the HIP FP64 path with the oracle, through the C-ABI.
+
+The bar (FP64 plain path): every conserved field within 1e-12 relative
+(max-norm over active cells, relative to the oracle's max-norm); in practice
+the two agree bit for bit because both follow the reference's operation order
+with no FMA contraction.
+"""
+import numpy as np
+import pytest
+
+from kestrel_amd import workloads
+from tests.helpers import FIELDS, active_mask, oracle_for, rel_inf, stepper_for
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(wl, g, o, label):
+    gs, os_ = g.state(), o.state()
+    assert np.array_equal(gs["tile_state"], os_["tile_state"]), f"{label}: tile states differ"
+    assert gs["t"] == os_["t"], f"{label}: t {gs['t']} != {os_['t']}"
+    m = active_mask(wl.cfg, os_["tile_state"])
+    worst = 0.0
+    for f in FIELDS:
+        r = rel_inf(gs[f], os_[f], m)
+        worst = max(worst, r)
+        assert r <= TOL, f"{label}: field {f} rel err {r:.3e}"
+    return worst, all(np.array_equal(gs[f][m], os_[f][m]) for f in FIELDS)
+
+
+@pytest.mark.parametrize("n,tile,steps", [(128, 32, 1), (128, 32, 25), (120, 20, 25), (128, 16, 25),
+                                          (256, 32, 100)])
+def test_plain_cap_parity(gpu, n, tile, steps):
+    if n == 256:
+        wl = workloads.plain_cap(n, steps)
+    else:
+        wl = workloads.plain_cap(n, steps, tile=tile, cap_x=0.0, cap_radius=10.0)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    so = o.step_n(steps)
+    sg = g.step_n(steps)
+    assert sg["steps"] == so["steps"] == steps
+    assert sg["rejected"] == so["rejected"]
+    worst, bitwise = _compare(wl, g, o, f"{n}^2/{tile} x {steps}")
+    print(f"plain cap {n}^2 tile {tile}, {steps} steps: worst rel {worst:.3e}, bitwise={bitwise}")
+
+
+def test_activation_parity(gpu):
+    """Flow crosses tile edges: tiles are activated and ghosts created on the device."""
+    wl = workloads.plain_cap(128, 150, tile=16, cap_x=0.0, cap_radius=10.0)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    for chunk in range(3):
+        so = o.step_n(50)
+        sg = g.step_n(50)
+        assert sg["active_tiles"] == so["active_tiles"]
+        assert sg["ghost_tiles"] == so["ghost_tiles"]
+        _compare(wl, g, o, f"activation chunk {chunk}")
+    assert so["active_tiles"] > 4, "the test must exercise activation"
+
+
+def test_plain_cap_maxima(gpu):
+    wl = workloads.plain_cap(128, 30, cap_x=0.0, cap_radius=10.0)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    o.step_n(30)
+    g.step_n(30)
+    mo, mg = o.maxima(), g.maxima()
+    st = o.state()
+    m = active_mask(wl.cfg, st["tile_state"])
+    for k in (0, 1, 2, 3, 10):
+        assert rel_inf(mg[k], mo[k], m) <= TOL, f"maxima array {k}"
+
+
+def test_integrate_to_matches(gpu):
+    wl = workloads.plain_cap(128, 0, cap_x=0.0, cap_radius=10.0)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    for tend in (0.1, 0.25, 0.4):
+        so = o.integrate_to(tend)
+        sg = g.integrate_to(tend)
+        assert sg["steps"] == so["steps"], (sg, so)
+        assert sg["t"] == so["t"]
+    _compare(wl, g, o, "integrate_to")
