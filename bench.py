@@ -96,6 +96,8 @@ def main():
     if args.warmup:
         g.step_n(args.warmup)
     torch.cuda.synchronize()
+    # timed region: K steps; every stage kernel is bracketed by HIP events on
+    # the library's stream (direct launches)
     g.profile(True)
     t0 = time.perf_counter()
     st = g.step_n(args.steps)
@@ -103,6 +105,11 @@ def main():
     el = time.perf_counter() - t0
     stage_ms, launches = g.profile_read()
     g.profile(False)
+    # the same K steps again without events, as the library runs them (hipGraph)
+    t1 = time.perf_counter()
+    st2 = g.step_n(args.steps)
+    torch.cuda.synchronize()
+    el2 = time.perf_counter() - t1
 
     cells = st["cell_updates"]
     value = cells / el
@@ -132,6 +139,8 @@ def main():
                      "kernel": "stage_kernel<2> (RK stage 2)",
                      "algorithmic_bytes_per_cell": P2_WORDS * 8},
         "step_roofline_frac": value * STEP_WORDS * 8 / (HBM_PEAK_GBS * 1e9),
+        "graph_run": {"value": st2["cell_updates"] / el2, "ms_per_step": el2 * 1e3 / args.steps,
+                      "note": "next K steps, hipGraph launches, no events"},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline("plain_cap_256_sample", args.cpu_budget)

@@ -717,9 +717,31 @@ static constexpr int MAXNEW = 1024;
 
 __global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) {
     __shared__ int newt[MAXNEW];
-    __shared__ int nnew, bad;
+    __shared__ int nnew, bad, need;
     Ctrl* c = d.ctrl;
     if (c->status != ST_NEW) return;
+    // fast path: does any flagged edge face a tile that is not active?
+    if (threadIdx.x == 0) need = 0;
+    __syncthreads();
+    {
+        const int dirs[4] = {p.ntx, -p.ntx, 1, -1};
+        int mine = 0;
+        for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) {
+            const int k = d.active[q];
+            const unsigned f = d.eflags[k];
+            if (!f) continue;
+            for (int r = 0; r < 4; ++r) {
+                const int n = k + dirs[r];
+                if ((f & (1u << r)) && n > -1 && !(n < p.ntiles && d.tstate[n] == KST_TILE_ACTIVE)) mine = 1;
+            }
+        }
+        if (mine) atomicOr(&need, 1);
+    }
+    __syncthreads();
+    if (!need) {
+        for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) d.eflags[d.active[q]] = 0;
+        return;
+    }
     if (threadIdx.x == 0) {
         nnew = 0;
         bad = 0;
@@ -985,6 +1007,7 @@ struct kst_handle {
     double t = 0.0;
     // profiling: per-iteration event pairs around the four stage kernels
     bool profiling = false;
+    hipGraphExec_t gexec = nullptr;      // graph of kGraphIters whole iterations
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[4] = {0, 0, 0, 0};
     long long prof_n[4] = {0, 0, 0, 0};
@@ -1071,7 +1094,7 @@ void from_tiles(const kst_handle* h, const double* t, double* g) {
 int launch_iteration(kst_handle* h, int slot) {
     const Params& p = h->p;
     const Dev& d = h->d;
-    hipEvent_t* E = h->profiling ? &h->ev[8 * slot] : nullptr;
+    hipEvent_t* E = (h->profiling && slot >= 0) ? &h->ev[8 * slot] : nullptr;
 #define EVREC(k) if (E) HIPCHK(h, hipEventRecord(E[k], h->stream))
     const int spt = ((p.nx + B - 1) / B) * ((p.ny + B - 1) / B);
     // 8 * ceil(work/8) blocks so the XCD-chunked mapping is a bijection
@@ -1105,11 +1128,34 @@ int launch_iteration(kst_handle* h, int slot) {
     return 0;
 }
 
+constexpr int kGraphIters = 8;
+
+// One hipGraph holding kGraphIters whole iterations (activation, four stage
+// kernels, finalize); kernel arguments never change for a handle.
+int ensure_graph(kst_handle* h) {
+    if (h->gexec) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < kGraphIters; ++q) {
+        int rc = launch_iteration(h, -1);
+        if (rc) {
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(h->stream, &dummy);
+            if (dummy) (void)hipGraphDestroy(dummy);
+            return rc;
+        }
+    }
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    HIPCHK(h, hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    HIPCHK(h, hipGraphDestroy(g));
+    return 0;
+}
+
 int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
     Ctrl c;
     int rc = read_ctrl(h, &c);
     if (rc) return rc;
-    int batch = 4;
+    int batch = h->profiling ? 4 : kGraphIters;
     long long launched = 0;
     for (;;) {
         if (launched > max_iters)
@@ -1122,9 +1168,15 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
             h->ev.resize(8 * batch);
             for (size_t q = old; q < h->ev.size(); ++q) HIPCHK(h, hipEventCreate(&h->ev[q]));
         }
-        for (int q = 0; q < batch; ++q) {
-            rc = launch_iteration(h, q);
+        if (h->profiling) {
+            for (int q = 0; q < batch; ++q) {
+                rc = launch_iteration(h, q);
+                if (rc) return rc;
+            }
+        } else {
+            rc = ensure_graph(h);
             if (rc) return rc;
+            for (int q = 0; q < batch; q += kGraphIters) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
         }
         rc = read_ctrl(h, &c);
         if (rc) return rc;
@@ -1503,6 +1555,7 @@ int kst_profile_read(kst_handle* h, double stage_ms[4], int64_t launches[4]) {
 
 void kst_destroy(kst_handle* h) {
     if (!h) return;
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     if (h->stream) hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) hipFree(p);

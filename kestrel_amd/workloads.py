@@ -64,3 +64,43 @@ def plain_cap_2048(steps: int = 200) -> Workload:
 
 def plain_cap_256_sample() -> Workload:
     return plain_cap(256, 1000)
+
+
+def fixture_case(name: str) -> Workload:
+    """Small 40x40 decks (5x5 tiles of 8 cells, dx = 1 m) exercising each
+    runtime-selected closure, for the reference-vector fixtures."""
+    cfg = C.default_config(5, 5, 8, 8, dx=1.0)
+    C.hydraulic_only(cfg)
+    cfg.cfl = 0.25
+    cfg.max_dt = 0.05
+    slope = math.tan(math.radians(10.0))
+    psi = 0.0
+    if name == "minmod1":
+        cfg.limiter = C.LIMITERS["minmod1"]
+    elif name == "vanalbada":
+        cfg.limiter = C.LIMITERS["vanalbada"]
+    elif name == "weno":
+        cfg.limiter = C.LIMITERS["weno"]
+    elif name == "nolimiter":
+        cfg.limiter = C.LIMITERS["none"]
+    elif name in ("coulomb", "voellmy", "pouliquen", "edwards2019", "manning", "variable"):
+        cfg.drag = C.DRAGS[name]
+        psi = 0.3 if name == "variable" else 0.0
+    elif name == "flat_nogeo":
+        cfg.geometric_factors = 0
+        slope = 0.0
+    elif name == "eddy":
+        cfg.eddy_viscosity = 0.05
+    elif name == "psi":
+        psi = 0.4
+    elif name != "chezy":
+        raise ValueError(name)
+    C.derive(cfg)
+    b0v = S.topo_xslope(cfg, slope) if slope else S.topo_flat(cfg)
+    caps = [S.Cap(0.5, -0.5, 6.0, 1.0, psi=psi, shape="para")]
+    init = S.initial_state(cfg, b0v, caps)
+    return Workload(f"fixture_{name}", cfg, b0v, init, 0, caps, [])
+
+
+FIXTURE_CASES = ["chezy", "minmod1", "vanalbada", "weno", "nolimiter", "coulomb", "voellmy",
+                 "pouliquen", "edwards2019", "manning", "variable", "flat_nogeo", "eddy", "psi"]
