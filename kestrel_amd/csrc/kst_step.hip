@@ -53,6 +53,8 @@ struct Dev {
     double *b0c, *btc, *cbx, *cby, *cgam;      // cell topography [tile][NC]
     double *xb0, *xbt, *xby, *xgam;            // x-face topography [tile][NFX]
     double *yb0, *ybt, *ybx, *ygam;            // y-face topography [tile][NFY]
+    double *xnp, *xnm, *ynp, *ynm;             // CFL numerators gr^2*dx of the face's two cells
+    double *pxu, *pxv, *pyv;                   // surface-gradient prefactors [tile][NC]
     double *vb0, *vbt;                         // tile vertex elevations [tile][NV]
     double *gsx, *gsy;                         // ghost w slopes [tile][NC]
     double* maxima;                            // [11][tile][NC]
@@ -152,15 +154,33 @@ __device__ void flux_sources(const Dev& d, const Params& p, double t, double gri
 // the fused stage kernel
 // ---------------------------------------------------------------------------
 
-struct Smem {
+// Limiters at compile time (src/Limiters.f90:83-184).  MinMod1/2 use the
+// identity max(-x,-y) = -min(x,y): negation and |.| are exact, so the value
+// equals the reference's branchy form.
+template <int LIM>
+__device__ __forceinline__ double lim(double a, double b) {
+    if (LIM == KST_LIMITER_MINMOD2) {
+        const double th = 1.3;
+        const double m = fmin(th * fabs(a), fmin(th * fabs(b), 0.5 * fabs(a + b)));
+        return (a * b <= 0.0) ? 0.0 : copysign(m, a);
+    } else if (LIM == KST_LIMITER_MINMOD1) {
+        const double m = fmin(fabs(a), fabs(b));
+        return (a * b <= 0.0) ? 0.0 : copysign(m, a);
+    } else {
+        return limiter(LIM, a, b);
+    }
+}
+
+template <int NQ>
+struct SmemT {
     double w[NREG], h[NREG], u[NREG], v[NREG], rho[NREG], gc[NREG];
+    double xbed[B][B + 3];   // bed of x-faces -1..B+1 of each row (relative)
+    double ybed[B][B + 3];   // bed of y-faces -1..B+1 of each column
     int gidx[NREG];
     uint8_t act[NREG];
-    // face fluxes: hw, hru, hrv, hh, g, pu, pv
-    double fx[7][NFACE];
-    double fy[7][NFACE];
+    double fx[NQ][NFACE];    // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
+    double fy[NQ][NFACE];
     double red[NTHREADS / 64];
-    unsigned flg;
     int tile, bx0, by0, bxa, bya, skip;
     double dt, tstage;
 };
@@ -180,77 +200,53 @@ __device__ __forceinline__ void map_cell(const Params& p, int t, int i, int j, i
 
 // Fused work of one x- or y-face (CalculateLimitedDerivs + Reconstruct +
 // CorrectSlopes + CalculateFluxes, src/HydraulicRHS.f90:181-1182).
-// Along the face normal the four cells are c[-2], L=c[-1], R=c[0], c[1].
-template <bool YDIR, bool VISC>
+// Along the face normal the four cells are c[-2], L=c[-1], R=c[0], c[1];
+// fi is the face position along the normal, fj the row (x) / column (y).
+template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem>
 __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s, int fi, int fj,
                                           double& cfl_min) {
-    // region indices of LL, L, R, RR along the normal
     int iLL, iL, iR, iRR;
     if (!YDIR) { iLL = rg(fi - 2, fj); iL = rg(fi - 1, fj); iR = rg(fi, fj); iRR = rg(fi + 1, fj); }
     else       { iLL = rg(fj, fi - 2); iL = rg(fj, fi - 1); iR = rg(fj, fi); iRR = rg(fj, fi + 1); }
     const double dxr = YDIR ? p.dyr : p.dxr;
     const double dx = YDIR ? p.dy : p.dx;
     const bool actL = s.act[iL], actR = s.act[iR];
-
-    // tile-relative position of the face (along the normal) and its row
     const int nn = YDIR ? p.ny : p.nx;
-    const int fpos = (YDIR ? s.by0 : s.bx0) + fi;     // face index within tile, 0..nn
-    const int frow = (YDIR ? s.bx0 : s.by0) + fj;     // row (x-face) / column (y-face)
-    const int t = s.tile;
+    const int fpos = (YDIR ? s.by0 : s.bx0) + fi;     // face index within the tile, 0..nn
 
-    // --- slopes (CalculateLimitedDerivs(iFlux/iDesing), src/HydraulicRHS.f90:181-386);
+    // --- slopes (CalculateLimitedDerivs, src/HydraulicRHS.f90:181-386);
     // non-active tiles keep their stored w slope and zero slopes otherwise.
     const double wLL = s.w[iLL], wL = s.w[iL], wR = s.w[iR], wRR = s.w[iRR];
-    double sLw, sRw;
     const double* gs = YDIR ? d.gsy : d.gsx;
-    sLw = actL ? dxr * limiter(p.limiter, wR - wL, wL - wLL) : gs[s.gidx[iL]];
-    sRw = actR ? dxr * limiter(p.limiter, wRR - wR, wR - wL) : gs[s.gidx[iR]];
+    const double sLw = actL ? dxr * lim<LIM>(wR - wL, wL - wLL) : gs[s.gidx[iL]];
+    const double sRw = actR ? dxr * lim<LIM>(wRR - wR, wR - wL) : gs[s.gidx[iR]];
     const double hLL = s.h[iLL], hL = s.h[iL], hR = s.h[iR], hRR = s.h[iRR];
-    const double sLh = actL ? dxr * limiter(p.limiter, hR - hL, hL - hLL) : 0.0;
-    const double sRh = actR ? dxr * limiter(p.limiter, hRR - hR, hR - hL) : 0.0;
+    const double sLh = actL ? dxr * lim<LIM>(hR - hL, hL - hLL) : 0.0;
+    const double sRh = actR ? dxr * lim<LIM>(hRR - hR, hR - hL) : 0.0;
     const double uLL = s.u[iLL], uL = s.u[iL], uR = s.u[iR], uRR = s.u[iRR];
-    const double sLu = actL ? dxr * limiter(p.limiter, uR - uL, uL - uLL) : 0.0;
-    const double sRu = actR ? dxr * limiter(p.limiter, uRR - uR, uR - uL) : 0.0;
+    const double sLu = actL ? dxr * lim<LIM>(uR - uL, uL - uLL) : 0.0;
+    const double sRu = actR ? dxr * lim<LIM>(uRR - uR, uR - uL) : 0.0;
     const double vLL = s.v[iLL], vL = s.v[iL], vR = s.v[iR], vRR = s.v[iRR];
-    const double sLv = actL ? dxr * limiter(p.limiter, vR - vL, vL - vLL) : 0.0;
-    const double sRv = actR ? dxr * limiter(p.limiter, vRR - vR, vR - vL) : 0.0;
+    const double sLv = actL ? dxr * lim<LIM>(vR - vL, vL - vLL) : 0.0;
+    const double sRv = actR ? dxr * lim<LIM>(vRR - vR, vR - vL) : 0.0;
     const double rLL = s.rho[iLL], rL = s.rho[iL], rR = s.rho[iR], rRR = s.rho[iRR];
-    const double sLr = actL ? dxr * limiter(p.limiter, rR - rL, rL - rLL) : 0.0;
-    const double sRr = actR ? dxr * limiter(p.limiter, rRR - rR, rR - rL) : 0.0;
+    const double sLr = actL ? dxr * lim<LIM>(rR - rL, rL - rLL) : 0.0;
+    const double sRr = actR ? dxr * lim<LIM>(rRR - rR, rR - rL) : 0.0;
 
     // --- MUSCL face values (Reconstruct, src/HydraulicRHS.f90:397-490)
     const double wEL = wL + sLw * 0.5 * dx, wWL = wL - sLw * 0.5 * dx;   // L's east / west
     const double wWR = wR - sRw * 0.5 * dx, wER = wR + sRw * 0.5 * dx;   // R's west / east
 
-    // face beds of L and R, each from its own tile (InterpolateB == face b0
-    // when bt = 0; src/HydraulicRHS.f90:741-753)
-    int tL, ciL, cjL, tR, ciR, cjR;
-    if (!YDIR) {
-        map_cell(p, t, fpos - 1, frow, tL, ciL, cjL);
-        map_cell(p, t, fpos, frow, tR, ciR, cjR);
-    } else {
-        map_cell(p, t, frow, fpos - 1, tL, ciL, cjL);
-        map_cell(p, t, frow, fpos, tR, ciR, cjR);
-    }
-    double bWL, bEL, bWR, bER;
-    if (!YDIR) {
-        const int rowL = cjL * (p.nx + 1), rowR = cjR * (p.nx + 1);
-        bWL = d.xb0[(size_t)tL * p.NFX + rowL + ciL];
-        bEL = d.xb0[(size_t)tL * p.NFX + rowL + ciL + 1];
-        bWR = d.xb0[(size_t)tR * p.NFX + rowR + ciR];
-        bER = d.xb0[(size_t)tR * p.NFX + rowR + ciR + 1];
-    } else {
-        bWL = d.yb0[(size_t)tL * p.NFY + cjL * p.nx + ciL];
-        bEL = d.yb0[(size_t)tL * p.NFY + (cjL + 1) * p.nx + ciL];
-        bWR = d.yb0[(size_t)tR * p.NFY + cjR * p.nx + ciR];
-        bER = d.yb0[(size_t)tR * p.NFY + (cjR + 1) * p.nx + ciR];
-    }
+    // beds of the faces of L and R (InterpolateB == face b0 when bt = 0,
+    // src/HydraulicRHS.f90:741-753); the face itself is entry fi + 1
+    const double* bed = YDIR ? &s.ybed[fj][0] : &s.xbed[fj][0];
+    const double bWL = bed[fi], fb0 = bed[fi + 1], bER = bed[fi + 2];
 
     // --- CorrectSlopes for w (src/HydraulicRHS.f90:560-736): each cell's
     // pair of face values is replaced by the bed slope if either is below bed.
     double wFL = wEL, wFR = wWR;
-    if ((wEL < bEL) || (wWL < bWL)) wFL = wL + 0.5 * (bEL - bWL);
-    if ((wER < bER) || (wWR < bWR)) wFR = wR + 0.5 * (bWR - bER);
+    if ((wEL < fb0) || (wWL < bWL)) wFL = wL + 0.5 * (fb0 - bWL);
+    if ((wER < bER) || (wWR < fb0)) wFR = wR + 0.5 * (fb0 - bER);
     // Hnpsi: inside the tile both faces of a cell are reset; across the
     // tile edge only the copy this tile holds is checked.
     double hFL = hL + sLh * 0.5 * dx, hFR = hR - sRh * 0.5 * dx;
@@ -266,25 +262,25 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
         const double hER = hR + sRh * 0.5 * dx;
         if ((hFR < 0.0) || (hER < 0.0)) hFR = hR;
     }
-    // derived face values
     const double uFL = uL + sLu * 0.5 * dx, uFR = uR - sRu * 0.5 * dx;
     const double vFL = vL + sLv * 0.5 * dx, vFR = vR - sRv * 0.5 * dx;
     const double rFL = rL + sLr * 0.5 * dx, rFR = rR - sRr * 0.5 * dx;
 
     // face topography of this tile
-    double fb0, fbp, fg;
+    const int t = s.tile;
+    int fidx;
+    double fbp, fg;
     if (!YDIR) {
-        size_t fidx = (size_t)t * p.NFX + frow * (p.nx + 1) + fpos;
-        fb0 = d.xb0[fidx]; fbp = d.xby[fidx]; fg = d.xgam[fidx];
+        fidx = t * p.NFX + (s.by0 + fj) * (p.nx + 1) + fpos;
+        fbp = d.xby[fidx]; fg = d.xgam[fidx];
     } else {
-        size_t fidx = (size_t)t * p.NFY + fpos * p.nx + frow;
-        fb0 = d.yb0[fidx]; fbp = d.ybx[fidx]; fg = d.ygam[fidx];
+        fidx = t * p.NFY + fpos * p.nx + (s.bx0 + fj);
+        fbp = d.ybx[fidx]; fg = d.ygam[fidx];
     }
     const double fbt = 0.0;
     // face Hn from the corrected w (src/HydraulicRHS.f90:492-517)
     const double HnL = compute_hn(wFL, fb0, fbt, fg);
     const double HnR = compute_hn(wFR, fb0, fbt, fg);
-    // normal / tangential velocities
     const double unL = YDIR ? vFL : uFL, unR = YDIR ? vFR : uFR;
     const double ruL = rFL * HnL * uFL, ruR = rFR * HnR * uFR;
     const double rvL = rFL * HnL * vFL, rvR = rFR * HnR * vFR;
@@ -298,15 +294,10 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     wsP = unR - rP; wsM = unL - rM;
     double aNeg = (wsP < wsM) ? wsP : wsM;
     if (aNeg > 0.0) aNeg = 0.0;
-    if (cfl_min >= 0.0) {
-        if (aPos > kEps) {
-            double gr = dmin(s.gc[iL] / fg, 1.0);
-            cfl_min = dmin(gr * gr * dx / aPos, cfl_min);
-        }
-        if (fabs(aNeg) > kEps) {
-            double gr = dmin(s.gc[iR] / fg, 1.0);
-            cfl_min = dmin(gr * gr * dx / fabs(aNeg), cfl_min);
-        }
+    if (CFL) {
+        // numerators (min(gam_cell/gam_face,1))^2 * dx are static topography
+        if (aPos > kEps) cfl_min = fmin((YDIR ? d.ynp : d.xnp)[fidx] / aPos, cfl_min);
+        if (fabs(aNeg) > kEps) cfl_min = fmin((YDIR ? d.ynm : d.xnm)[fidx] / fabs(aNeg), cfl_min);
     }
 
     // --- fluxes (src/HydraulicRHS.f90:1027-1067, src/Equations.f90:53-245)
@@ -357,15 +348,32 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
 }
 
 // Block-wide minimum of a positive double into a control slot.
+template <typename Smem>
 __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
-    for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) s.red[wv] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = s.red[0];
-        for (int i = 1; i < NTHREADS / 64; ++i) m = dmin(m, s.red[i]);
+        for (int i = 1; i < NTHREADS / 64; ++i) m = fmin(m, s.red[i]);
         if (m < kHuge) atomicMin(slot, dbits(m));
+    }
+}
+
+// Bed of a face given by its tile-relative index K in [-1, nn+1] along the
+// normal and its row/column r (single-valued b0: a tile's edge faces equal
+// its neighbours').
+template <bool YDIR>
+__device__ __forceinline__ double face_bed(const Dev& d, const Params& p, int t, int K, int r) {
+    if (!YDIR) {
+        if (K < 0) return d.xb0[(t - 1) * p.NFX + r * (p.nx + 1) + (p.nx - 1)];
+        if (K > p.nx) return d.xb0[(t + 1) * p.NFX + r * (p.nx + 1) + 1];
+        return d.xb0[t * p.NFX + r * (p.nx + 1) + K];
+    } else {
+        if (K < 0) return d.yb0[(t - p.ntx) * p.NFY + (p.ny - 1) * p.nx + r];
+        if (K > p.ny) return d.yb0[(t + p.ntx) * p.NFY + 1 * p.nx + r];
+        return d.yb0[t * p.NFY + K * p.nx + r];
     }
 }
 
@@ -374,9 +382,9 @@ __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
 // STAGE 3: RHS(u2); u3 -> S3.                   (src/TimeStepper.f90:447-498)
 // STAGE 4: RHS(u3); implicit drag -> S0; maxima; activation flags.
 //                                               (src/TimeStepper.f90:501-524)
-template <int STAGE, bool VISC>
+template <int STAGE, bool VISC, int LIM>
 __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
-    __shared__ Smem s;
+    __shared__ SmemT<VISC ? 7 : 5> s;
     const int tid = threadIdx.x;
     if (tid == 0) {
         const Ctrl* c = d.ctrl;
@@ -408,22 +416,21 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             s.bya = min(B, p.ny - s.by0);
         }
         s.skip = skip;
-        s.flg = 0;
     }
     __syncthreads();
     if (s.skip) return;
     const int t = s.tile;
     const double dt = s.dt;
-    const long long FS = d.FS;
+    const int FS = (int)d.FS;
+    const int bxa = s.bxa, bya = s.bya;
 
     // ---- phase A: stage state + derived variables on the region ----------
     const int tx = tid % B, ty = tid / B;
-    const bool own = (tx < s.bxa) && (ty < s.bya);
-    // registers of the owned cell
+    const bool own = (tx < bxa) && (ty < bya);
     double ow = 0, oru = 0, orv = 0, oh = 0, ohn = 0, ou = 0, ov = 0, opsi = 0, orho = 0;
-    double ob0 = 0, obx = 0, oby = 0, og = 1;
-    double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stages 2, 3)
-    size_t oidx = 0;
+    double ob0 = 0, og = 1;
+    double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stage 2)
+    int oidx = 0;
     for (int round = 0; round < 2; ++round) {
         int rx, ry;
         bool valid;
@@ -432,15 +439,15 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         } else {
             if (tid >= 128) break;
             const int strip = tid >> 5, k = tid & 31, dd = k >> 4, pos = k & 15;
-            if (strip == 0) { rx = -2 + dd; ry = pos; valid = pos < s.bya; }
-            else if (strip == 1) { rx = s.bxa + dd; ry = pos; valid = pos < s.bya; }
-            else if (strip == 2) { rx = pos; ry = -2 + dd; valid = pos < s.bxa; }
-            else { rx = pos; ry = s.bya + dd; valid = pos < s.bxa; }
+            if (strip == 0) { rx = -2 + dd; ry = pos; valid = pos < bya; }
+            else if (strip == 1) { rx = bxa + dd; ry = pos; valid = pos < bya; }
+            else if (strip == 2) { rx = pos; ry = -2 + dd; valid = pos < bxa; }
+            else { rx = pos; ry = bya + dd; valid = pos < bxa; }
         }
         if (!valid) continue;
         int tc, ci, cj;
         map_cell(p, t, s.bx0 + rx, s.by0 + ry, tc, ci, cj);
-        const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
+        const int gi = tc * p.NC + cj * p.nx + ci;
         double w, ru, rv, h;
         if (STAGE == 1 || STAGE == 2) {
             w = d.S0[gi]; ru = d.S0[FS + gi]; rv = d.S0[2 * FS + gi]; h = d.S0[3 * FS + gi];
@@ -471,25 +478,34 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         const double vv = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
         const int r = rg(rx, ry);
         s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho; s.gc[r] = gam;
-        s.gidx[r] = (int)gi;
+        s.gidx[r] = gi;
         s.act[r] = (d.tstate[tc] == KST_TILE_ACTIVE) ? 1 : 0;
         if (round == 0) {
             ow = w; oru = ru; orv = rv; oh = h; ohn = hn; ou = uu; ov = vv; opsi = psi; orho = rho;
-            ob0 = b0c; og = gam; obx = d.cbx[gi]; oby = d.cby[gi]; oidx = gi;
+            ob0 = b0c; og = gam; oidx = gi;
         }
+    }
+    // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
+    for (int q = tid; q < B * (B + 3); q += NTHREADS) {
+        const int r = q / (B + 3), k = q % (B + 3) - 1;
+        if (r < bya && k <= bxa + 1) s.xbed[r][k + 1] = face_bed<false>(d, p, t, s.bx0 + k, s.by0 + r);
+        if (r < bxa && k <= bya + 1) s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
     }
     __syncthreads();
 
     // ---- phase B: face fluxes --------------------------------------------
-    double cflm = (STAGE == 4) ? -1.0 : kHuge;
-    // x-faces (fi, fj): fi in [0, bxa], fj in [0, bya); y-faces likewise.
-    for (int f = tid; f < (s.bxa + 1) * s.bya; f += NTHREADS) {
-        const int fi = f % (s.bxa + 1), fj = f / (s.bxa + 1);
-        face_flux<false, VISC>(d, p, s, fi, fj, cflm);
-    }
-    for (int f = tid; f < (s.bya + 1) * s.bxa; f += NTHREADS) {
-        const int fi = f % (s.bya + 1), fj = f / (s.bya + 1);
-        face_flux<true, VISC>(d, p, s, fi, fj, cflm);
+    double cflm = kHuge;
+    // x-faces: consecutive lanes walk along a row; y-faces: along a row too
+    if (bxa == B && bya == B) {
+        for (int f = tid; f < (B + 1) * B; f += NTHREADS)
+            face_flux<false, VISC, LIM, STAGE != 4>(d, p, s, f % (B + 1), f / (B + 1), cflm);
+        for (int f = tid; f < (B + 1) * B; f += NTHREADS)
+            face_flux<true, VISC, LIM, STAGE != 4>(d, p, s, f / B, f % B, cflm);
+    } else {
+        for (int f = tid; f < (bxa + 1) * bya; f += NTHREADS)
+            face_flux<false, VISC, LIM, STAGE != 4>(d, p, s, f % (bxa + 1), f / (bxa + 1), cflm);
+        for (int f = tid; f < (bya + 1) * bxa; f += NTHREADS)
+            face_flux<true, VISC, LIM, STAGE != 4>(d, p, s, f / bxa, f % bxa, cflm);
     }
     if (STAGE != 4) {
         reduce_min_to(s, cflm, &d.ctrl->slot[STAGE - 1]);
@@ -504,19 +520,13 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     const double* FYp = &s.fy[0][0];
 #define FX(q, o) FXp[(q) * NFACE + xs + (o)]
 #define FY(q, o) FYp[(q) * NFACE + ys + (o)]
-    const double gam = og, bx = obx, by = oby;
-    double gxu, gxv, gyu, gyv;
-    if (p.geometric) {
-        gxu = (1.0 + by * by) / gam;
-        gxv = -(bx * by / gam);
-        gyu = -(bx * by / gam);
-        gyv = (1.0 + bx * bx) / gam;
-    } else {
-        gxu = 1.0; gxv = 0.0; gyu = 0.0; gyv = 1.0;
-    }
+    const double gam = og;
+    const double bx = d.cbx[oidx], by = d.cby[oidx];
+    // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are static topography
+    const double gxu = d.pxu[oidx], gxv = d.pxv[oidx], gyu = gxv, gyv = d.pyv[oidx];
     // ConstructHydraulicRHS (src/HydraulicRHS.f90:1228-1271)
-    double stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
-    double sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
+    const double stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
+    const double sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
     double stu, stv;
     {
         Kahan k1;
@@ -542,32 +552,35 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     }
 #undef FX
 #undef FY
-    // ExplicitSourceTerms (src/Equations.f90:456-620)
-    double Qt = 0.0, psiQt = 0.0;
+    // ExplicitSourceTerms (src/Equations.f90:456-620); with no source the
+    // flux terms are 0/(gam*gam) = 0 exactly
+    double Ew, Eh;
     if (p.n_sources > 0 && d.has_src[t]) {
+        double Qt = 0.0, psiQt = 0.0;
         const int ti = t % p.ntx, tj = t / p.ntx;
         const int i = s.bx0 + tx, j = s.by0 + ty;
         const double x = -0.5 * p.x_size + p.dx * (((ti + 1) - 1.0) * p.nx + ((i + 1) - 0.5));
         const double y = -0.5 * p.y_size + p.dy * (((tj + 1) - 1.0) * p.ny + ((j + 1) - 0.5));
         flux_sources(d, p, s.tstage, d.ctrl->t, x, y, Qt, psiQt);
+        Ew = stw + (0.0 + Qt / (gam * gam));
+        Eh = sth + (0.0 + psiQt / gam);
+    } else {
+        Ew = stw + 0.0;
+        Eh = sth + 0.0;
     }
     double hpg = -0.0;
     hpg = hpg + (ow - ob0);
     hpg = hpg / gam;
-    const double Ew = stw + (0.0 + Qt / (gam * gam));
-    const double Eh = sth + (0.0 + psiQt / gam);
     const double Eu = stu + (0.0 - p.g * orho * hpg * bx);
     const double Ev = stv + (0.0 - p.g * orho * hpg * by);
     // DragClosure + ImplicitSourceTerms (src/Equations.f90:627-658)
     double Im = 0.0;
-    {
+    if (ohn > p.hthr) {
         const double fric = drag(p, ohn, ou, ov, opsi, bx, by, gam);
-        if (ohn > p.hthr) {
-            const double modu = sqrt(speed2(p.geometric, ou, ov, bx, by));
-            if (modu > 1.0e-8) {
-                const double hr = 1.0 / ohn;
-                Im = -(fric * hr / modu);
-            }
+        const double modu = sqrt(speed2(p.geometric, ou, ov, bx, by));
+        if (modu > 1.0e-8) {
+            const double hr = 1.0 / ohn;
+            Im = -(fric * hr / modu);
         }
     }
 
@@ -617,23 +630,23 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             const double h0 = d.S0[3 * FS + oidx];
             double hn = compute_hn(w0, ob0, 0.0, gam);
             if (hn < 0.0) hn = 0.0;
-            double hp0 = h0;
-            if (hp0 < 0.0) hp0 = 0.0;
-            const double eps = p.hthr;
-            const double den = hn * hn + dmax(hn * hn, eps * eps);
-            const double psi = dmin(2.0 * hn * hp0 / den, p.max_pack);
-            const double rho = density(p, psi);
-            const double uu = 2.0 * hn * u0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-            const double vv = 2.0 * hn * v0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-            const double tn = d.ctrl->t + dt;
-            double* M = d.maxima;
             if (hn > p.hthr) {
+                double hp0 = h0;
+                if (hp0 < 0.0) hp0 = 0.0;
+                const double eps = p.hthr;
+                const double den = hn * hn + dmax(hn * hn, eps * eps);
+                const double psi = dmin(2.0 * hn * hp0 / den, p.max_pack);
+                const double rho = density(p, psi);
+                const double uu = 2.0 * hn * u0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+                const double vv = 2.0 * hn * v0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+                const double tn = d.ctrl->t + dt;
+                double* M = d.maxima;
                 if (M[10 * FS + oidx] == -1) M[10 * FS + oidx] = tn;
                 if (hn > M[oidx]) { M[oidx] = hn; M[FS + oidx] = tn; }
+                const double spd = sqrt(speed2(p.geometric, uu, vv, bx, by));
+                if (spd > M[2 * FS + oidx]) { M[2 * FS + oidx] = spd; M[3 * FS + oidx] = tn; }
+                if (psi > M[8 * FS + oidx]) { M[8 * FS + oidx] = psi; M[9 * FS + oidx] = tn; }
             }
-            const double spd = sqrt(speed2(p.geometric, uu, vv, bx, by));
-            if (spd > M[2 * FS + oidx] && hn > p.hthr) { M[2 * FS + oidx] = spd; M[3 * FS + oidx] = tn; }
-            if (hn > p.hthr && psi > M[8 * FS + oidx]) { M[8 * FS + oidx] = psi; M[9 * FS + oidx] = tn; }
         }
         d.S0[oidx] = ow; d.S0[FS + oidx] = un; d.S0[2 * FS + oidx] = vn; d.S0[3 * FS + oidx] = oh;
         // activation flags from this state's Hn (CheckIfNearBoundaries,
@@ -987,6 +1000,48 @@ __global__ __launch_bounds__(256) void topo_kernel(Dev d, Params p) {
 #undef V
 }
 
+// Static per-face CFL numerators and per-cell surface-gradient prefactors.
+// gam_ratio = min(gam(cell)/gam(face), 1) of the cell on each side of a face
+// (src/HydraulicRHS.f90:983-1006) enters the CFL as (gr*gr)*dx; the
+// prefactors of ConstructHydraulicRHS (src/HydraulicRHS.f90:1233-1243).
+// Needs cgam of the neighbour tiles, so it runs after topo_kernel.
+__global__ __launch_bounds__(256) void topo2_kernel(Dev d, Params p) {
+    const int t = blockIdx.x;
+    const int nx = p.nx, ny = p.ny;
+    const int ti = t % p.ntx, tj = t / p.ntx;
+    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
+    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
+    for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
+        const int gi = t * p.NC + c;
+        const double bx = d.cbx[gi], by = d.cby[gi], gam = d.cgam[gi];
+        if (p.geometric) {
+            d.pxu[gi] = (1.0 + by * by) / gam;
+            d.pxv[gi] = -(bx * by / gam);
+            d.pyv[gi] = (1.0 + bx * bx) / gam;
+        } else {
+            d.pxu[gi] = 1.0; d.pxv[gi] = 0.0; d.pyv[gi] = 1.0;
+        }
+    }
+    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
+        const int i = f % (nx + 1), j = f / (nx + 1);
+        const double fg = d.xgam[t * p.NFX + f];
+        const double gl = (i == 0) ? d.cgam[tW * p.NC + j * nx + nx - 1] : d.cgam[t * p.NC + j * nx + i - 1];
+        const double gr_ = (i == nx) ? d.cgam[tE * p.NC + j * nx] : d.cgam[t * p.NC + j * nx + i];
+        const double a = fmin(gl / fg, 1.0), b = fmin(gr_ / fg, 1.0);
+        d.xnp[t * p.NFX + f] = a * a * p.dx;
+        d.xnm[t * p.NFX + f] = b * b * p.dx;
+    }
+    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
+        const int i = f % nx, j = f / nx;
+        const double fg = d.ygam[t * p.NFY + f];
+        const double gl = (j == 0) ? d.cgam[tS * p.NC + (ny - 1) * nx + i] : d.cgam[t * p.NC + (j - 1) * nx + i];
+        const double gr_ = (j == ny) ? d.cgam[tN * p.NC + i] : d.cgam[t * p.NC + j * nx + i];
+        const double a = fmin(gl / fg, 1.0), b = fmin(gr_ / fg, 1.0);
+        d.ynp[t * p.NFY + f] = a * a * p.dy;
+        d.ynm[t * p.NFY + f] = b * b * p.dy;
+    }
+}
+
 }  // namespace kst
 
 // ===========================================================================
@@ -1091,6 +1146,22 @@ void from_tiles(const kst_handle* h, const double* t, double* g) {
         }
 }
 
+typedef void (*StageFn)(Dev, Params);
+struct StageSet { StageFn fn[4]; };
+
+template <bool VISC, int LIM>
+StageSet make_set() {
+    return StageSet{{stage_kernel<1, VISC, LIM>, stage_kernel<2, VISC, LIM>,
+                     stage_kernel<3, VISC, LIM>, stage_kernel<4, VISC, LIM>}};
+}
+
+const StageSet& stage_set(bool visc, int limiter) {
+    static const StageSet sets[2][5] = {
+        {make_set<false, 0>(), make_set<false, 1>(), make_set<false, 2>(), make_set<false, 3>(), make_set<false, 4>()},
+        {make_set<true, 0>(), make_set<true, 1>(), make_set<true, 2>(), make_set<true, 3>(), make_set<true, 4>()}};
+    return sets[visc ? 1 : 0][(limiter >= 0 && limiter < 5) ? limiter : 1];
+}
+
 int launch_iteration(kst_handle* h, int slot) {
     const Params& p = h->p;
     const Dev& d = h->d;
@@ -1101,26 +1172,11 @@ int launch_iteration(kst_handle* h, int slot) {
     const int nblk = 8 * ((p.ntiles * spt + 7) / 8);
     const bool visc = p.nu != 0.0;
     hipLaunchKernelGGL(activate_kernel, dim3(1), dim3(256), 0, h->stream, d, p);
-    if (visc) {
-        EVREC(0);
-        hipLaunchKernelGGL((stage_kernel<1, true>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(1); EVREC(2);
-        hipLaunchKernelGGL((stage_kernel<2, true>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(3); EVREC(4);
-        hipLaunchKernelGGL((stage_kernel<3, true>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(5); EVREC(6);
-        hipLaunchKernelGGL((stage_kernel<4, true>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(7);
-    } else {
-        EVREC(0);
-        hipLaunchKernelGGL((stage_kernel<1, false>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(1); EVREC(2);
-        hipLaunchKernelGGL((stage_kernel<2, false>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(3); EVREC(4);
-        hipLaunchKernelGGL((stage_kernel<3, false>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(5); EVREC(6);
-        hipLaunchKernelGGL((stage_kernel<4, false>), dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(7);
+    const StageSet& k = stage_set(visc, p.limiter);
+    for (int q = 0; q < 4; ++q) {
+        EVREC(2 * q);
+        hipLaunchKernelGGL(k.fn[q], dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
+        EVREC(2 * q + 1);
     }
 #undef EVREC
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1), 0, h->stream, d, p);
@@ -1290,6 +1346,9 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     rc |= dalloc(h, &d.yb0, (size_t)h->ntiles * h->p.NFY); rc |= dalloc(h, &d.ybt, (size_t)h->ntiles * h->p.NFY);
     rc |= dalloc(h, &d.ybx, (size_t)h->ntiles * h->p.NFY); rc |= dalloc(h, &d.ygam, (size_t)h->ntiles * h->p.NFY);
     rc |= dalloc(h, &d.vb0, (size_t)h->ntiles * NV); rc |= dalloc(h, &d.vbt, (size_t)h->ntiles * NV);
+    rc |= dalloc(h, &d.xnp, (size_t)h->ntiles * h->p.NFX); rc |= dalloc(h, &d.xnm, (size_t)h->ntiles * h->p.NFX);
+    rc |= dalloc(h, &d.ynp, (size_t)h->ntiles * h->p.NFY); rc |= dalloc(h, &d.ynm, (size_t)h->ntiles * h->p.NFY);
+    rc |= dalloc(h, &d.pxu, FS); rc |= dalloc(h, &d.pxv, FS); rc |= dalloc(h, &d.pyv, FS);
     rc |= dalloc(h, &d.gsx, FS); rc |= dalloc(h, &d.gsy, FS);
     rc |= dalloc(h, &d.maxima, 11 * FS);
     rc |= dalloc(h, &d.tstate, (size_t)h->ntiles); rc |= dalloc(h, &d.has_src, (size_t)h->ntiles);
@@ -1339,6 +1398,7 @@ int kst_set_topography(kst_handle* h, const double* b0v) {
         }
     HIPCHK(h, hipMemcpy(h->d.vb0, tv.data(), tv.size() * 8, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(topo_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p);
+    hipLaunchKernelGGL(topo2_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->have_topo = true;

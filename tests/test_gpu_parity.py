@@ -83,3 +83,22 @@ def test_integrate_to_matches(gpu):
         assert sg["steps"] == so["steps"], (sg, so)
         assert sg["t"] == so["t"]
     _compare(wl, g, o, "integrate_to")
+
+
+@pytest.mark.parametrize("case", workloads.FIXTURE_CASES)
+def test_closure_decks_parity(gpu, case):
+    """Every limiter, drag law, gamma on/off, eddy viscosity and psi > 0 on the
+    device equals the oracle (bitwise where no transcendental enters)."""
+    wl = workloads.fixture_case(case)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    so = o.step_n(15)
+    sg = g.step_n(15)
+    assert sg["steps"] == so["steps"]
+    gs, os_ = g.state(), o.state()
+    m = active_mask(wl.cfg, os_["tile_state"])
+    # tanh/pow/cos of the device library may differ from glibc by ulps
+    tol = 1e-12 if case not in ("edwards2019", "manning", "variable") else 1e-9
+    for f in FIELDS:
+        r = rel_inf(gs[f], os_[f], m)
+        assert r <= tol, f"{case}: {f} rel err {r:.3e}"
