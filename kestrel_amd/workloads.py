@@ -102,5 +102,96 @@ def fixture_case(name: str) -> Workload:
     return Workload(f"fixture_{name}", cfg, b0v, init, 0, caps, [])
 
 
+def morpho_hills(n: int = 1024, steps: int = 2000, tile: int = 32, dx: float = 5.0,
+                 seed: int = 20240611, q: float = 20.0, psi: float = 0.3,
+                 src_x: float | None = None, src_y: float = 0.0, n_hills: int = 32,
+                 drag: str = "pouliquen") -> Workload:
+    """Full morphodynamics: an n x n grid of dx cells over a seeded sum of
+    n_hills Gaussian hills (amplitude U[5, 50] m, width U[50, 300] m) on a
+    5 degree x-ramp, fed by a one-cell flux source (radius 0.4 dx at a cell
+    centre) of Q = q*dx m^3/s with solids fraction psi, Pouliquen drag, Mixed
+    erosion, Spearman-Manning deposition, smooth erosion transition, tanh
+    damping, eddy viscosity 1e-2 m^2/s.  The flow runs toward -x."""
+    nt = n // tile
+    cfg = C.default_config(nt, nt, tile, tile, dx=dx)
+    cfg.drag = C.DRAGS[drag]
+    cfg.erosion = C.EROSIONS["mixed"]
+    cfg.deposition = C.DEPOSITIONS["spearman manning"]
+    cfg.morphodynamics = 1
+    cfg.eddy_viscosity = 1e-2
+    C.derive(cfg)
+    hills = S.Hills(seed, n=n_hills)
+    b0v = S.topo_gaussian_hills(cfg, hills)
+    L = n * dx
+    x0 = 0.25 * L if src_x is None else src_x
+    # snap to a cell centre
+    xc, yc = S.cell_centres(cfg)
+    x0 = float(xc[np.argmin(np.abs(xc - x0))])
+    y0 = float(yc[np.argmin(np.abs(yc - src_y))])
+    src = C.FluxSource(x0, y0, 0.4 * dx, [0.0, 1.0e9], [q * dx, q * dx], [psi, psi])
+    init = S.initial_state(cfg, b0v, [], [src])
+    C.set_sources(cfg, [src])
+    return Workload(f"morpho_hills_{n}", cfg, b0v, init, steps, [], [src])
+
+
+def morpho_cap(n: int = 128, steps: int = 100, tile: int = 16, dx: float = 1.0,
+               psi: float = 0.3, radius: float = 12.0, height: float = 1.0,
+               slope_deg: float = 10.0, u: float = 0.0, drag: str = "chezy",
+               erosion: str = "mixed", deposition: str = "spearman manning",
+               transition: str = "smooth", damping: str = "tanh", ero_rate: float | None = None,
+               settling_speed: float | None = None, ero_critical_height: float | None = None) -> Workload:
+    """Small morphodynamic decks for parity tests: a paraboloid cap with solids
+    fraction psi released on an x-plane (centred at x = 0), with selectable
+    closures."""
+    nt = n // tile
+    cfg = C.default_config(nt, nt, tile, tile, dx=dx)
+    cfg.drag = C.DRAGS[drag]
+    cfg.erosion = C.EROSIONS[erosion]
+    cfg.deposition = C.DEPOSITIONS[deposition]
+    cfg.ero_transition = C.TRANSITIONS[transition]
+    cfg.morpho_damping = C.DAMPINGS[damping]
+    cfg.morphodynamics = 1
+    cfg.cfl = 0.25
+    cfg.max_dt = 0.05
+    if ero_rate is not None:
+        cfg.ero_rate = ero_rate
+    if ero_critical_height is not None:
+        cfg.ero_critical_height = ero_critical_height
+    C.derive(cfg, settling_speed=settling_speed)
+    b0v = S.topo_xslope(cfg, math.tan(math.radians(slope_deg))) if slope_deg else S.topo_flat(cfg)
+    caps = [S.Cap(0.0, 0.0, radius, height, psi=psi, u=u, shape="para")]
+    init = S.initial_state(cfg, b0v, caps)
+    return Workload(f"morpho_cap_{n}", cfg, b0v, init, steps, caps, [])
+
+
 FIXTURE_CASES = ["chezy", "minmod1", "vanalbada", "weno", "nolimiter", "coulomb", "voellmy",
                  "pouliquen", "edwards2019", "manning", "variable", "flat_nogeo", "eddy", "psi"]
+
+
+def morpho_fixture_case(name: str) -> Workload:
+    """Small 40x40 morphodynamic decks (5x5 tiles of 8 cells, dx = 1 m): a
+    paraboloid cap (radius 6 m, height 1 m, psi 0.3) released on a 10 degree
+    x-plane, one deck per erosion/deposition/transition/damping choice."""
+    kw = dict(n=40, tile=8, radius=6.0, psi=0.3)
+    if name == "mixed":
+        pass
+    elif name == "simple":
+        kw.update(erosion="simple", deposition="simple", transition="step", damping="rat3")
+    elif name == "fluid":
+        kw.update(erosion="fluid", deposition="none", transition="none", damping="none")
+    elif name == "granular":
+        kw.update(erosion="granular", transition="step", damping="tanh")
+    elif name == "fastsettling":
+        kw.update(settling_speed=0.2, ero_critical_height=0.05)
+    elif name == "nogeo":
+        kw.update(slope_deg=0.0, u=1.0)
+    else:
+        raise ValueError(name)
+    wl = morpho_cap(**kw)
+    if name == "nogeo":
+        wl.cfg.geometric_factors = 0
+    wl.name = f"morpho_fixture_{name}"
+    return wl
+
+
+MORPHO_FIXTURE_CASES = ["mixed", "simple", "fluid", "granular", "fastsettling", "nogeo"]

@@ -76,6 +76,12 @@ struct ko_sim {
     double* gb0;              /* global vertex b0 */
     char err[256];
     int status;
+    /* probe of the first redistribution of a run (for pinning RedistributeGrid) */
+    int probe_on, probe_done;
+    tdata* probe[3];          /* I0, I3 before and I3 after redistribution */
+    void* probe_list; int probe_n;
+    uint8_t* probe_ts;        /* tile states at the probe */
+    double probe_dt, probe_dt_pending;
 };
 
 #define NCw(s) ((s)->NC)
@@ -266,6 +272,113 @@ static double drag(const kst_config* c, const cstate* s) {
         return ch * (1.0 - fc) + po * fc;
     }
     }
+}
+
+/* Shields number and particle speed (src/Closures.f90:208-241). */
+static double shields_number(const kst_config* c, const cstate* s) {
+    double gred = c->gred / gamma_of(c, s->bx, s->by);
+    double m2 = speed2(c, s->u, s->v, s->bx, s->by);
+    double chezy = c->chezy_co * m2;
+    return chezy / (gred * c->solid_diameter);
+}
+static double particle_speed(const kst_config* c, const cstate* s) {
+    double gred = c->gred / gamma_of(c, s->bx, s->by);
+    return sqrt(gred * c->solid_diameter);
+}
+
+static double erosion_fluid(const kst_config* c, const cstate* s) {
+    double sh = shields_number(c, s);
+    if (sh > c->critical_shields) {
+        double e = c->ero_rate * (sh - c->critical_shields);
+        return e * particle_speed(c, s);
+    }
+    return 0.0;
+}
+static double erosion_granular(const kst_config* c, const cstate* s) {
+    const double pi = 4.0 * atan(1.0);
+    double hn = s->hn;
+    double m2 = speed2(c, s->u, s->v, s->bx, s->by);
+    double mu1 = c->pouliquen_min;
+    double t1 = tan(pi / 180.0);
+    double stat = (mu1 + t1) / (1.0 - mu1 * t1);
+    double gcos = c->g / gamma_of(c, s->bx, s->by);
+    double mu = pouliquen_mu(c, gcos, hn, sqrt(m2));
+    double mun = mu1 + (stat - mu1) / (1.0 + pow(hn / 25.0 / c->solid_diameter, 2.0));
+    if (mu > mun) {
+        double e = c->ero_rate_granular * (mu - mun);
+        return e * particle_speed(c, s);
+    }
+    return 0.0;
+}
+
+/* ErosionClosure family (src/Closures.f90:566-675). */
+static double erosion_closure(const kst_config* c, const cstate* s) {
+    switch (c->erosion) {
+    case KST_EROSION_SIMPLE: {
+        double e = c->ero_rate * shields_number(c, s);
+        return e * particle_speed(c, s);
+    }
+    case KST_EROSION_FLUID: return erosion_fluid(c, s);
+    case KST_EROSION_GRANULAR: return erosion_granular(c, s);
+    case KST_EROSION_MIXED: {
+        double fc = fswitch(c, s->psi);
+        double fl = erosion_fluid(c, s);
+        double gr = erosion_granular(c, s);
+        return (1.0 - fc) * fl + fc * gr;
+    }
+    default: return 0.0;
+    }
+}
+
+/* ErosionTransition family (src/Closures.f90:684-732). */
+static double erosion_transition(const kst_config* c, const cstate* s) {
+    switch (c->ero_transition) {
+    case KST_TRANSITION_SMOOTH: return 0.5 * (1.0 + tanh(1e5 * (s->bt + c->ero_depth)));
+    case KST_TRANSITION_STEP: return (s->bt < -c->ero_depth) ? 0.0 : 1.0;
+    default: return 1.0;
+    }
+}
+
+/* DepositionClosure family (src/Closures.f90:320-356). */
+static double deposition_closure(const kst_config* c, double psi) {
+    switch (c->deposition) {
+    case KST_DEPOSITION_SIMPLE: return psi * (1.0 - psi / c->max_pack);
+    case KST_DEPOSITION_SPEARMAN_MANNING: {
+        double a = 2.7 - 0.15 * c->nsettling;
+        double b = 0.62 * c->nsettling - 1.46;
+        return psi * pow(1.0 - psi, a) * pow(1.0 - psi / c->max_pack, b);
+    }
+    default: return 0.0;
+    }
+}
+
+/* MorphoDamping family (src/Closures.f90:744-791). */
+static double morpho_damping(const kst_config* c, double hn) {
+    switch (c->morpho_damping) {
+    case KST_DAMPING_TANH: return 0.5 * (1.0 + tanh(10.0 * log(hn / c->ero_critical_height)));
+    case KST_DAMPING_RAT3: {
+        double hc = c->ero_critical_height;
+        if (hn < hc) return 0.0;
+        if (hn > 2.0 * hc) return 1.0;
+        double t = hn / hc - 1.0;
+        return (t * t * t) / ((1.0 - t) * (1.0 - t) * (1.0 - t) + t * t * t);
+    }
+    default: return 1.0;
+    }
+}
+
+/* ErosionDepositionTerms, Erosion, Deposition (src/Equations.f90:385-448). */
+static void erosion_deposition(const kst_config* c, const cstate* s, double* E, double* D) {
+    double e = erosion_closure(c, s);
+    e = e * erosion_transition(c, s);
+    double alpha;
+    if (s->psi >= c->max_pack) alpha = 0.0;
+    else if (s->psi > 0.0) alpha = deposition_closure(c, s->psi);
+    else alpha = 0.0;
+    double dep = c->ws0 * alpha;
+    double damp = morpho_damping(c, s->hn);
+    *E = e * damp;
+    *D = dep * damp;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1519,10 +1632,407 @@ static double next_flux_series_time(ko_sim* s, double t) {
     return nextT;
 }
 
+/* ------------------------------------------------------------------ */
+/* Morphodynamics (src/MorphodynamicRHS.f90, src/TimeStepper.f90:532-781,
+ * src/Redistribute.f90)                                              */
+/* ------------------------------------------------------------------ */
+
+static double kahan4(double a, double b, double c, double d) {
+    double x[4] = {a, b, c, d};
+    return kahan_sum(x, 4);
+}
+static double kahan3(double a, double b, double c) {
+    double x[3] = {a, b, c};
+    return kahan_sum(x, 3);
+}
+
+#define EMD(s, T, i, j) (T)->emd[CI(s, i, j)]
+#define DTBT(s, T, i, j) (T)->dtbt[VI(s, i, j)]
+
+/* BtSourceTerm (src/MorphodynamicRHS.f90:154-305), 2D, including the
+ * three-term KahanSums of the west edge (:210) and the NE-corner dbdx (:266). */
+static void bt_source(ko_sim* s, tdata* C) {
+    const kst_config* c = &s->cfg;
+    int nx = s->nx, ny = s->ny;
+    double psib = 1.0 - c->bed_porosity;
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        tdata* T = &C[k];
+        tmeta* m = &s->meta[k];
+        for (int i = 1; i < nx; ++i)
+            for (int j = 1; j < ny; ++j) {
+                double bx = 0.25 * kahan4(UF(s, T, DBDX, i - 1, j - 1), UF(s, T, DBDX, i - 1, j), UF(s, T, DBDX, i, j - 1), UF(s, T, DBDX, i, j));
+                double by = 0.25 * kahan4(UF(s, T, DBDY, i - 1, j - 1), UF(s, T, DBDY, i - 1, j), UF(s, T, DBDY, i, j - 1), UF(s, T, DBDY, i, j));
+                double gam = gamma_of(c, bx, by);
+                DTBT(s, T, i, j) = -0.25 * gam / psib *
+                    kahan4(EMD(s, T, i - 1, j - 1), EMD(s, T, i - 1, j), EMD(s, T, i, j - 1), EMD(s, T, i, j));
+            }
+        tdata* W = &C[m->nb[0]];
+        tdata* E = &C[m->nb[1]];
+        tdata* N = &C[m->nb[2]];
+        tdata* S = &C[m->nb[3]];
+        tdata* SW = &C[s->meta[m->nb[0]].nb[3]];
+        tdata* SE = &C[s->meta[m->nb[1]].nb[3]];
+        tdata* NW = &C[s->meta[m->nb[0]].nb[2]];
+        tdata* NE = &C[s->meta[m->nb[1]].nb[2]];
+        for (int j = 1; j < ny; ++j) {
+            double bx = 0.25 * kahan4(UF(s, W, DBDX, nx - 1, j - 1), UF(s, W, DBDX, nx - 1, j), UF(s, T, DBDX, 0, j - 1), UF(s, T, DBDX, 0, j));
+            double by = 0.25 * kahan4(UF(s, W, DBDY, nx - 1, j - 1), UF(s, W, DBDY, nx - 1, j), UF(s, T, DBDY, 0, j - 1), UF(s, T, DBDY, 0, j));
+            double gam = gamma_of(c, bx, by);
+            DTBT(s, T, 0, j) = -0.25 * gam / psib *
+                kahan3(EMD(s, W, nx - 1, j - 1), EMD(s, W, nx - 1, j), EMD(s, T, 0, j - 1) + EMD(s, T, 0, j));
+            bx = 0.25 * kahan4(UF(s, T, DBDX, nx - 1, j - 1), UF(s, T, DBDX, nx - 1, j), UF(s, E, DBDX, 0, j - 1), UF(s, E, DBDX, 0, j));
+            by = 0.25 * kahan4(UF(s, T, DBDY, nx - 1, j - 1), UF(s, T, DBDY, nx - 1, j), UF(s, E, DBDY, 0, j - 1), UF(s, E, DBDY, 0, j));
+            gam = gamma_of(c, bx, by);
+            DTBT(s, T, nx, j) = -0.25 * gam / psib *
+                kahan4(EMD(s, T, nx - 1, j - 1), EMD(s, T, nx - 1, j), EMD(s, E, 0, j - 1), EMD(s, E, 0, j));
+        }
+        for (int i = 1; i < nx; ++i) {
+            double bx = 0.25 * kahan4(UF(s, T, DBDX, i - 1, ny - 1), UF(s, N, DBDX, i - 1, 0), UF(s, T, DBDX, i, ny - 1), UF(s, N, DBDX, i, 0));
+            double by = 0.25 * kahan4(UF(s, T, DBDY, i - 1, ny - 1), UF(s, N, DBDY, i - 1, 0), UF(s, T, DBDY, i, ny - 1), UF(s, N, DBDY, i, 0));
+            double gam = gamma_of(c, bx, by);
+            DTBT(s, T, i, ny) = -0.25 * gam / psib *
+                kahan4(EMD(s, T, i - 1, ny - 1), EMD(s, N, i - 1, 0), EMD(s, T, i, ny - 1), EMD(s, N, i, 0));
+            bx = 0.25 * kahan4(UF(s, S, DBDX, i - 1, ny - 1), UF(s, T, DBDX, i - 1, 0), UF(s, S, DBDX, i, ny - 1), UF(s, T, DBDX, i, 0));
+            by = 0.25 * kahan4(UF(s, S, DBDY, i - 1, ny - 1), UF(s, T, DBDY, i - 1, 0), UF(s, S, DBDY, i, ny - 1), UF(s, T, DBDY, i, 0));
+            gam = gamma_of(c, bx, by);
+            DTBT(s, T, i, 0) = -0.25 * gam / psib *
+                kahan4(EMD(s, S, i - 1, ny - 1), EMD(s, T, i - 1, 0), EMD(s, S, i, ny - 1), EMD(s, T, i, 0));
+        }
+        /* corners */
+        double bx = 0.25 * kahan4(UF(s, SW, DBDX, nx - 1, ny - 1), UF(s, W, DBDX, nx - 1, 0), UF(s, S, DBDX, 0, ny - 1), UF(s, T, DBDX, 0, 0));
+        double by = 0.25 * kahan4(UF(s, SW, DBDY, nx - 1, ny - 1), UF(s, W, DBDY, nx - 1, 0), UF(s, S, DBDY, 0, ny - 1), UF(s, T, DBDY, 0, 0));
+        double gam = gamma_of(c, bx, by);
+        DTBT(s, T, 0, 0) = -0.25 * gam / psib *
+            kahan4(EMD(s, SW, nx - 1, ny - 1), EMD(s, W, nx - 1, 0), EMD(s, S, 0, ny - 1), EMD(s, T, 0, 0));
+        bx = 0.25 * kahan4(UF(s, S, DBDX, nx - 1, ny - 1), UF(s, T, DBDX, nx - 1, 0), UF(s, SE, DBDX, 0, ny - 1), UF(s, E, DBDX, 0, 0));
+        by = 0.25 * kahan4(UF(s, S, DBDY, nx - 1, ny - 1), UF(s, T, DBDY, nx - 1, 0), UF(s, SE, DBDY, 0, ny - 1), UF(s, E, DBDY, 0, 0));
+        gam = gamma_of(c, bx, by);
+        DTBT(s, T, nx, 0) = -0.25 * gam / psib *
+            kahan4(EMD(s, S, nx - 1, ny - 1), EMD(s, T, nx - 1, 0), EMD(s, SE, 0, ny - 1), EMD(s, E, 0, 0));
+        bx = 0.25 * kahan4(UF(s, W, DBDX, nx - 1, ny - 1), UF(s, NW, DBDX, nx - 1, 0), UF(s, T, DBDX, 0, ny - 1), UF(s, N, DBDX, 0, 0));
+        by = 0.25 * kahan4(UF(s, W, DBDY, nx - 1, ny - 1), UF(s, NW, DBDY, nx - 1, 0), UF(s, T, DBDY, 0, ny - 1), UF(s, N, DBDY, 0, 0));
+        gam = gamma_of(c, bx, by);
+        DTBT(s, T, 0, ny) = -0.25 * gam / psib *
+            kahan4(EMD(s, W, nx - 1, ny - 1), EMD(s, NW, nx - 1, 0), EMD(s, T, 0, ny - 1), EMD(s, N, 0, 0));
+        bx = 0.25 * kahan3(UF(s, T, DBDX, nx - 1, ny - 1), UF(s, N, DBDX, nx - 1, 0), UF(s, E, DBDX, 0, ny - 1) + UF(s, NE, DBDX, 0, 0));
+        by = 0.25 * kahan4(UF(s, T, DBDY, nx - 1, ny - 1), UF(s, N, DBDY, nx - 1, 0), UF(s, E, DBDY, 0, ny - 1), UF(s, NE, DBDY, 0, 0));
+        gam = gamma_of(c, bx, by);
+        DTBT(s, T, nx, ny) = -0.25 * gam / psib *
+            kahan4(EMD(s, T, nx - 1, ny - 1), EMD(s, N, nx - 1, 0), EMD(s, E, 0, ny - 1), EMD(s, NE, 0, 0));
+    }
+}
+
+/* CalculateMorphodynamicRHS (src/MorphodynamicRHS.f90:72-149), 2D. */
+static void morpho_rhs(ko_sim* s, tdata* C) {
+    const kst_config* c = &s->cfg;
+    int nx = s->nx, ny = s->ny;
+    double eps = c->height_threshold;
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        tdata* T = &C[k];
+        tmeta* m = &s->meta[k];
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                double hW = (i == 0) ? UF(s, &C[m->nb[0]], HN, nx - 1, j) : UF(s, T, HN, i - 1, j);
+                double hE = (i == nx - 1) ? UF(s, &C[m->nb[1]], HN, 0, j) : UF(s, T, HN, i + 1, j);
+                double hS = (j == 0) ? UF(s, &C[m->nb[3]], HN, i, ny - 1) : UF(s, T, HN, i, j - 1);
+                double hN = (j == ny - 1) ? UF(s, &C[m->nb[2]], HN, i, 0) : UF(s, T, HN, i, j + 1);
+                if (UF(s, T, HN, i, j) < eps || hW < eps || hE < eps || hS < eps || hN < eps) {
+                    for (int d = 0; d < 4; ++d) T->dte[d * s->NC + CI(s, i, j)] = 0.0;
+                    EMD(s, T, i, j) = 0.0;
+                } else {
+                    cstate cs = {UF(s, T, HN, i, j), UF(s, T, UU, i, j), UF(s, T, VV, i, j), UF(s, T, PSI, i, j),
+                                 UF(s, T, BT, i, j), UF(s, T, DBDX, i, j), UF(s, T, DBDY, i, j)};
+                    double E, D;
+                    erosion_deposition(c, &cs, &E, &D);
+                    EMD(s, T, i, j) = E - D;
+                }
+            }
+    }
+    bt_source(s, C);
+}
+
+/* ComputeCellCentredTopographicData_ij (src/MorphodynamicRHS.f90:363-416), 2D. */
+static void cell_topo_ij(ko_sim* s, tdata* T, int i, int j) {
+    double bt = 0.25 * kahan4(BTV(s, T, i, j), BTV(s, T, i + 1, j), BTV(s, T, i, j + 1), BTV(s, T, i + 1, j + 1));
+    UF(s, T, BT, i, j) = bt;
+    double a[8];
+    a[0] = B0V(s, T, i + 1, j); a[1] = BTV(s, T, i + 1, j);
+    a[2] = -B0V(s, T, i, j); a[3] = -BTV(s, T, i, j);
+    a[4] = B0V(s, T, i + 1, j + 1); a[5] = BTV(s, T, i + 1, j + 1);
+    a[6] = -B0V(s, T, i, j + 1); a[7] = -BTV(s, T, i, j + 1);
+    UF(s, T, DBDX, i, j) = 0.5 * s->dxr * kahan_sum(a, 8);
+    a[0] = B0V(s, T, i, j + 1); a[1] = BTV(s, T, i, j + 1);
+    a[2] = -B0V(s, T, i, j); a[3] = -BTV(s, T, i, j);
+    a[4] = B0V(s, T, i + 1, j + 1); a[5] = BTV(s, T, i + 1, j + 1);
+    a[6] = -B0V(s, T, i + 1, j); a[7] = -BTV(s, T, i + 1, j);
+    UF(s, T, DBDY, i, j) = 0.5 * s->dyr * kahan_sum(a, 8);
+}
+
+/* ExcessDeposition (src/Redistribute.f90:158-198). */
+static void excess_deposition(ko_sim* s, int k, int i, int j, double* excess, double* dbt, double* psiold) {
+    const kst_config* c = &s->cfg;
+    tdata* B = &s->cont[C_I0][k];
+    tdata* A = &s->cont[C_I3][k];
+    double gamold = gam_cell(s, B, i, j);
+    double gamnew = gam_cell(s, A, i, j);
+    double hnold = compute_hn(UF(s, B, W, i, j), UF(s, B, B0, i, j), UF(s, B, BT, i, j), gamold);
+    (void)gamnew;
+    *dbt = UF(s, A, BT, i, j) - UF(s, B, BT, i, j);
+    *psiold = UF(s, B, PSI, i, j);
+    double e = -(UF(s, B, HNPSI, i, j) * gamold / (1.0 - c->bed_porosity) - *dbt);
+    *excess = fmax_(e, -(hnold * gamold - *dbt));
+}
+
+typedef struct { int k, i, j; double excess; } rnode;
+
+/* AddToRedistList (src/Redistribute.f90:69-101): ascending, stable. */
+static void redist_add(rnode** list, int* n, int* cap, int k, int i, int j, double excess) {
+    if (*n == *cap) {
+        *cap = *cap ? 2 * *cap : 64;
+        *list = realloc(*list, *cap * sizeof(rnode));
+    }
+    int p = *n;
+    while (p > 0 && (*list)[p - 1].excess > excess) { (*list)[p] = (*list)[p - 1]; --p; }
+    (*list)[p].k = k; (*list)[p].i = i; (*list)[p].j = j; (*list)[p].excess = excess;
+    (*n)++;
+}
+
+/* RedistributeCell (src/Redistribute.f90:249-475), 2D. */
+static int redistribute_cell(ko_sim* s, int k, int i, int j, double corr) {
+    const kst_config* c = &s->cfg;
+    tdata* I0 = s->cont[C_I0];
+    tdata* I3 = s->cont[C_I3];
+    int nx = s->nx, ny = s->ny;
+    double bd[4] = {0, 0, 0, 0}, sum = 0.0;
+    int di[4], dj[4], N = 0;
+    int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
+    for (int q = 0; q < 4; ++q) {
+        bd[N] = BTV(s, &I3[k], vi[q], vj[q]) - BTV(s, &I0[k], vi[q], vj[q]);
+        if (bd[N] > 0.0) {
+            di[N] = vi[q]; dj[N] = vj[q];
+            sum = sum + bd[N];
+            N++;
+        }
+    }
+    if (N == 0) return 0;
+    double delta = 4.0 * corr / sum;
+    double hnold = UF(s, &I0[k], HN, i, j);
+    double gamold = gam_cell(s, &I0[k], i, j);
+    double db = UF(s, &I3[k], BT, i, j) - UF(s, &I0[k], BT, i, j);
+    double tol = EPS_WP * UF(s, &I3[k], W, i, j) * 10.0;
+    double adj = 0.0;
+    double disc = kahan3(hnold * gamold, -db, corr);
+    if (fabs(disc) < tol) {
+        adj = kahan3(tol, -hnold * gamold, db);
+        adj = adj * (4.0 / sum);
+        adj = adj - delta;
+        adj = fmax_(adj, 0.0);
+    }
+    double hgpo = UF(s, &I0[k], HNPSI, i, j) * gamold / (1.0 - c->bed_porosity);
+    tol = EPS_WP * 10.0;
+    disc = kahan3(hgpo, -db, corr);
+    if (fabs(disc) < tol) {
+        double ap = kahan3(tol, -hgpo, db);
+        ap = ap * (4.0 / sum);
+        ap = ap - delta;
+        adj = fmax_(adj, ap);
+    }
+    delta = delta + adj;
+    for (int q = 0; q < N; ++q) {
+        double nv = BTV(s, &I3[k], di[q], dj[q]) - delta * bd[q];
+        BTV(s, &I3[k], di[q], dj[q]) = nv;
+        tmeta* m = &s->meta[k];
+        if (di[q] == 0) BTV(s, &I3[m->nb[0]], nx, dj[q]) = nv;
+        else if (di[q] == nx) BTV(s, &I3[m->nb[1]], 0, dj[q]) = nv;
+        if (dj[q] == 0) {
+            BTV(s, &I3[m->nb[3]], di[q], ny) = nv;
+            if (di[q] == 0) BTV(s, &I3[s->meta[m->nb[3]].nb[0]], nx, ny) = nv;
+            if (di[q] == nx) BTV(s, &I3[s->meta[m->nb[3]].nb[1]], 0, ny) = nv;
+        }
+        if (dj[q] == ny) {
+            BTV(s, &I3[m->nb[2]], di[q], 0) = nv;
+            if (di[q] == 0) BTV(s, &I3[s->meta[m->nb[2]].nb[0]], nx, 0) = nv;
+            if (di[q] == nx) BTV(s, &I3[s->meta[m->nb[2]].nb[1]], 0, 0) = nv;
+        }
+    }
+    /* the 3x3 cells around (i, j), walking tiles as the reference does */
+    int ct, ci, cj;
+    if (i == 0) { ct = s->meta[k].nb[0]; ci = nx - 1; }
+    else { ct = k; ci = i - 1; }
+    for (int kk = -1; kk <= 1; ++kk) {
+        if (j == 0) { ct = s->meta[ct].nb[3]; cj = ny - 1; }
+        else cj = j - 1;
+        for (int l = -1; l <= 1; ++l) {
+            tdata* A = &I3[ct];
+            tdata* Bo = &I0[ct];
+            cell_topo_ij(s, A, ci, cj);
+            double dbb = UF(s, A, BT, ci, cj) - UF(s, Bo, BT, ci, cj);
+            double go = gam_cell(s, Bo, ci, cj);
+            double gn = gam_cell(s, A, ci, cj);
+            double ho = UF(s, Bo, HN, ci, cj);
+            double w = UF(s, A, BT, ci, cj);
+            w = w + (ho * go / gn - dbb / gn) / gn;
+            w = w + UF(s, A, B0, ci, cj);
+            UF(s, A, W, ci, cj) = w;
+            double hpo = UF(s, Bo, HNPSI, ci, cj);
+            UF(s, A, HNPSI, ci, cj) = hpo * go / gn - (1.0 - c->bed_porosity) * dbb / gn;
+            cj = cj + 1;
+            if (cj == ny && l < 1) { ct = s->meta[ct].nb[2]; cj = 0; }
+        }
+        if (j + 1 >= ny) ct = s->meta[ct].nb[3];
+        ci = ci + 1;
+        if (ci == nx) { ct = s->meta[ct].nb[1]; ci = 0; }
+    }
+    return 1;
+}
+
+static void probe_copy(ko_sim* s, tdata** dst, tdata* src) {
+    if (!*dst) *dst = xcalloc(s->ntiles, sizeof(tdata));
+    for (int k = 0; k < s->ntiles; ++k) {
+        if (!src[k].alloc) continue;
+        if (!(*dst)[k].alloc) tdata_alloc(s, &(*dst)[k]);
+        tdata_copy(s, &(*dst)[k], &src[k]);
+    }
+}
+
+static void probe_capture(ko_sim* s, int after, const rnode* list, int nl) {
+    if (!after) {
+        s->probe_dt = s->probe_dt_pending;
+    probe_copy(s, &s->probe[0], s->cont[C_I0]);
+        probe_copy(s, &s->probe[1], s->cont[C_I3]);
+        free(s->probe_list);
+        s->probe_list = xcalloc(nl, sizeof(rnode));
+        memcpy(s->probe_list, list, nl * sizeof(rnode));
+        s->probe_n = nl;
+        if (!s->probe_ts) s->probe_ts = xcalloc(s->ntiles, 1);
+        for (int k = 0; k < s->ntiles; ++k)
+            s->probe_ts[k] = s->meta[k].on ? KST_TILE_ACTIVE : (s->meta[k].ghost ? KST_TILE_GHOST : KST_TILE_OFF);
+    } else {
+        probe_copy(s, &s->probe[2], s->cont[C_I3]);
+        s->probe_done = 1;
+    }
+}
+
+/* RedistributeGrid (src/Redistribute.f90:203-244): the sorted list in order,
+ * re-evaluating each cell's excess; 0 if a cell could not be corrected. */
+static int redistribute_grid(ko_sim* s, const rnode* list, int nl) {
+    for (int q = 0; q < nl; ++q) {
+        double ex, dbt, psio;
+        excess_deposition(s, list[q].k, list[q].i, list[q].j, &ex, &dbt, &psio);
+        if (ex > EPS_WP && !redistribute_cell(s, list[q].k, list[q].i, list[q].j, ex)) return 0;
+    }
+    return 1;
+}
+
+/* One linear update of w and Hnpsi from the change of bt
+ * (src/TimeStepper.f90:587-610). */
+static void morpho_mass_update(ko_sim* s, tdata* I0, tdata* Ik, int k) {
+    const kst_config* c = &s->cfg;
+    cell_topo(s, &Ik[k]);
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j) {
+            double go = gam_cell(s, &I0[k], i, j);
+            double gn = gam_cell(s, &Ik[k], i, j);
+            double ho = compute_hn(UF(s, &I0[k], W, i, j), UF(s, &I0[k], B0, i, j), UF(s, &I0[k], BT, i, j), go);
+            double db = UF(s, &Ik[k], BT, i, j) - UF(s, &I0[k], BT, i, j);
+            double w = -db / gn / gn;
+            w = w + UF(s, &Ik[k], BT, i, j);
+            w = w + ho * go / gn / gn;
+            w = w + UF(s, &Ik[k], B0, i, j);
+            UF(s, &Ik[k], W, i, j) = w;
+            double hpo = UF(s, &I0[k], HNPSI, i, j);
+            double hp = -(1.0 - c->bed_porosity) * db / gn;
+            hp = hp + hpo * go / gn;
+            UF(s, &Ik[k], HNPSI, i, j) = hp;
+        }
+    desingularise(s, Ik, k, 0);
+}
+
+/* MorphodynamicTimeStepper (src/TimeStepper.f90:532-781), no sponge layer. */
+static void morpho_stepper(ko_sim* s, double* thisdt, int* refine) {
+    const kst_config* c = &s->cfg;
+    tdata *I0 = s->cont[C_I0], *I1 = s->cont[C_I1], *I2 = s->cont[C_I2], *I3 = s->cont[C_I3];
+    int NV = s->NV;
+    double dt = *thisdt;
+    *refine = 0;
+    double relmax = 0.0;
+    s->probe_dt_pending = dt;
+    morpho_rhs(s, I0);
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        for (int v = 0; v < NV; ++v) I1[k].bt[v] = fmax_(-c->ero_depth, I0[k].bt[v] + dt * I0[k].dtbt[v]);
+    }
+    for (int q = 0; q < s->nactive; ++q) morpho_mass_update(s, I0, I1, s->active[q]);
+    morpho_rhs(s, I1);
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        for (int v = 0; v < NV; ++v)
+            I2[k].bt[v] = fmax_(-c->ero_depth, 0.75 * I0[k].bt[v] + 0.25 * (I1[k].bt[v] + dt * I1[k].dtbt[v]));
+    }
+    for (int q = 0; q < s->nactive; ++q) morpho_mass_update(s, I0, I2, s->active[q]);
+    morpho_rhs(s, I2);
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        for (int v = 0; v < NV; ++v)
+            I3[k].bt[v] = fmax_(-c->ero_depth, (1.0 / 3.0) * I0[k].bt[v] + (2.0 / 3.0) * (I2[k].bt[v] + dt * I2[k].dtbt[v]));
+    }
+    for (int q = 0; q < s->nactive; ++q) morpho_mass_update(s, I0, I3, s->active[q]);
+
+    rnode* list = NULL;
+    int nl = 0, cap = 0;
+    rnode* scan = NULL;       /* the same candidates in scan order (probe only) */
+    int ns = 0, scap = 0;
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        for (int j = 0; j < s->ny; ++j)
+            for (int i = 0; i < s->nx; ++i) {
+                double go = gam_cell(s, &I0[k], i, j);
+                double gn = gam_cell(s, &I3[k], i, j);
+                double ho = compute_hn(UF(s, &I0[k], W, i, j), UF(s, &I0[k], B0, i, j), UF(s, &I0[k], BT, i, j), go);
+                double hn = compute_hn(UF(s, &I3[k], W, i, j), UF(s, &I3[k], B0, i, j), UF(s, &I3[k], BT, i, j), gn);
+                double ex, dbt, psio;
+                excess_deposition(s, k, i, j, &ex, &dbt, &psio);
+                if (!*refine && ex > EPS_WP && dbt > EPS_WP && psio > -EPS_WP) {
+                    if (ho < c->ero_critical_height || fabs(hn - ho) < c->ero_critical_height) {
+                        redist_add(&list, &nl, &cap, k, i, j, ex);
+                        if (s->probe_on && !s->probe_done) {
+                            if (ns == scap) { scap = scap ? 2 * scap : 64; scan = realloc(scan, scap * sizeof(rnode)); }
+                            scan[ns].k = k; scan[ns].i = i; scan[ns].j = j; scan[ns].excess = ex; ns++;
+                        }
+                    } else {
+                        *refine = 1;
+                        break;
+                    }
+                    continue;
+                }
+                if (ho < c->ero_critical_height) continue;
+                double rel = fabs(hn - ho) / fabs(ho);
+                if (rel > 0.1 && rel > relmax) {
+                    relmax = rel;
+                    *refine = 1;
+                    break;
+                }
+            }
+    }
+    int probing = ((nl > 0 || s->probe_on == 2) && !*refine && s->probe_on && !s->probe_done);
+    if (probing) probe_capture(s, 0, scan, ns);
+    if (nl > 0 && !*refine && !redistribute_grid(s, list, nl)) *refine = 1;
+    if (probing) probe_capture(s, 1, NULL, 0);
+    free(list);
+    free(scan);
+    if (*refine) {
+        *thisdt = 0.5 * dt;
+        return;
+    }
+    for (int q = 0; q < s->nactive; ++q) face_topo(s, I3, s->active[q]);
+}
+
 static int64_t count_cells(ko_sim* s) { return (int64_t)s->nactive * s->NC; }
 
-/* One accepted outer step of IntegrateTo (src/TimeStepper.f90:151-275),
- * hydraulic only. */
+/* One accepted outer step of IntegrateTo (src/TimeStepper.f90:151-275):
+ * H(dt) alone, or the Strang step H(dt) M(2dt) H(dt) with morphodynamics. */
 static int one_step(ko_sim* s, double tend, kst_stats* st) {
     int rc = check_near_boundaries(s);
     if (rc) return rc;
@@ -1535,19 +2045,50 @@ static int one_step(ko_sim* s, double tend, kst_stats* st) {
     s->dt = adv;
     double dth = adv;
     s->t0 = s->t;
+    tdata *TC = s->cont[C_TC], *I0 = s->cont[C_I0], *I1 = s->cont[C_I1];
+    tdata *I2 = s->cont[C_I2], *I3 = s->cont[C_I3];
     for (;;) {
-        copy_solution(s, s->cont[C_TC], s->cont[C_I0]);
-        copy_mutable_topo(s, s->cont[C_TC], s->cont[C_I1]);
-        copy_mutable_topo(s, s->cont[C_TC], s->cont[C_I2]);
-        copy_solution(s, s->cont[C_TC], s->cont[C_I3]);
+        copy_solution(s, TC, I0);
+        copy_mutable_topo(s, TC, I1);
+        copy_mutable_topo(s, TC, I2);
+        copy_solution(s, TC, I3);
         int refine = 0;
         hydraulic_stepper(s, &dth, &refine, &st->rhs_evals);
         if (refine) { st->rejected++; continue; }
         if (!s->cfg.morphodynamics) break;
-        return fail(s, KST_ERR_CONFIG, "morphodynamics not available in this oracle build");
+        copy_solution(s, I3, I0);
+        copy_solution(s, I3, I1);
+        copy_solution(s, I3, I2);
+        double dtm = 2.0 * dth;
+        morpho_stepper(s, &dtm, &refine);
+        if (refine) {
+            dth = 0.5 * dtm;
+            s->dt = dth;
+            st->rejected++;
+            continue;
+        }
+        copy_solution(s, I3, I0);
+        copy_mutable_topo(s, I3, I1);
+        copy_mutable_topo(s, I3, I2);
+        s->t = s->t + dth;
+        double adv2 = hydraulic_rhs(s, I0, s->t, 1);
+        st->rhs_evals++;
+        if (adv2 < dth) {
+            if ((dth - adv2) / dth < 0.1) dth = 0.9 * dth;
+            else dth = adv2;
+            s->dt = dth;
+            s->t = s->t0;
+            st->rejected++;
+            continue;
+        }
+        hydraulic_stepper(s, &dth, &refine, &st->rhs_evals);
+        if (!refine) break;
+        s->t = s->t0;
+        st->rejected++;
     }
-    copy_solution(s, s->cont[C_I3], s->cont[C_TC]);
-    s->t = s->t0 + dth;
+    copy_solution(s, I3, TC);
+    if (!s->cfg.morphodynamics) s->t = s->t0 + dth;
+    else s->t = s->t0 + 2.0 * dth;
     st->steps++;
     st->cell_updates += count_cells(s);
     st->last_dt = dth;
@@ -1627,7 +2168,6 @@ ko_sim* ko_create(const kst_config* cfg) {
         s->src[i].psi = s->src_arrays[3 * i + 2];
     }
     s->cfg.sources = s->src;
-    if (cfg->morphodynamics) fail(s, KST_ERR_CONFIG, "morphodynamics not supported by this oracle build");
     if (cfg->bc != KST_BC_HALT) fail(s, KST_ERR_CONFIG, "only the halt boundary condition is supported");
     return s;
 }
@@ -1643,6 +2183,13 @@ void ko_destroy(ko_sim* s) {
         free(m->x); free(m->y); free(m->xv); free(m->yv);
         free(m->hnmax); free(m->umax); free(m->emax); free(m->dmax); free(m->psimax); free(m->tfirst);
     }
+    for (int p = 0; p < 3; ++p) {
+        if (!s->probe[p]) continue;
+        for (int k = 0; k < s->ntiles; ++k) tdata_free(&s->probe[p][k]);
+        free(s->probe[p]);
+    }
+    free(s->probe_list);
+    free(s->probe_ts);
     for (int i = 0; i < 3 * s->cfg.n_sources; ++i) free(s->src_arrays[i]);
     free(s->src_arrays); free(s->src);
     free(s->meta); free(s->active); free(s->ghosts); free(s->gb0);
@@ -1761,13 +2308,27 @@ int ko_finish_sources(ko_sim* s) {
 int ko_set_state(ko_sim* s, const double* w, const double* rhu, const double* rhv,
                  const double* hnpsi, const double* btv, const uint8_t* ts, double t,
                  const double* hnmax0) {
-    (void)btv;
     for (int k = 0; k < s->ntiles; ++k)
         if (ts[k] == KST_TILE_ACTIVE && !s->meta[k].on) {
             int rc = add_tile(s, k);
             if (rc) return rc;
         }
     int nx = s->nx, ny = s->ny;
+    if (btv) {
+        /* every allocated tile takes its vertex bt from the global array; the
+         * cell (and, for active tiles, face) topography follows */
+        for (int k = 0; k < s->ntiles; ++k) {
+            tdata* T = &s->cont[C_TC][k];
+            if (!T->alloc) continue;
+            int gi, gj;
+            tile_coords(s, k, &gi, &gj);
+            for (int j = 0; j <= ny; ++j)
+                for (int i = 0; i <= nx; ++i)
+                    BTV(s, T, i, j) = btv[(size_t)(gj * ny + j) * (s->NX + 1) + gi * nx + i];
+            cell_topo(s, T);
+        }
+        for (int q = 0; q < s->nactive; ++q) face_topo(s, s->cont[C_TC], s->active[q]);
+    }
     for (int q = 0; q < s->nactive; ++q) {
         int k = s->active[q], gi, gj;
         tile_coords(s, k, &gi, &gj);
@@ -1807,7 +2368,18 @@ int ko_get_state(ko_sim* s, double* w, double* rhu, double* rhv, double* hnpsi, 
         for (int a = 0; a < 5; ++a)
             if (outs[a]) scatter_field(s, k, &T->u[fld[a] * s->NC], outs[a]);
     }
-    if (btv) memset(btv, 0, 8 * (size_t)(s->NX + 1) * (s->NY + 1));
+    if (btv) {
+        /* vertex (I, J) from the tile holding cell (min(I, NX-1), min(J, NY-1)) */
+        memset(btv, 0, 8 * (size_t)(s->NX + 1) * (s->NY + 1));
+        for (int J = 0; J <= s->NY; ++J)
+            for (int I = 0; I <= s->NX; ++I) {
+                int ci = I < s->NX ? I : s->NX - 1, cj = J < s->NY ? J : s->NY - 1;
+                int k = (cj / s->ny) * s->ntx + ci / s->nx;
+                tdata* T = &s->cont[C_TC][k];
+                if (!T->alloc) continue;
+                btv[(size_t)J * (s->NX + 1) + I] = BTV(s, T, I - (ci / s->nx) * s->nx, J - (cj / s->ny) * s->ny);
+            }
+    }
     if (ts)
         for (int k = 0; k < s->ntiles; ++k)
             ts[k] = s->meta[k].on ? KST_TILE_ACTIVE : (s->meta[k].ghost ? KST_TILE_GHOST : KST_TILE_OFF);
@@ -1879,5 +2451,113 @@ int ko_tile_lists(ko_sim* s, int32_t* active, int32_t* nactive, int32_t* ghost, 
     if (ghost) for (int q = 0; q < s->nghost; ++q) ghost[q] = s->ghosts[q];
     if (nactive) *nactive = s->nactive;
     if (nghost) *nghost = s->nghost;
+    return 0;
+}
+
+/* ---- probes for the tests (no counterpart in the reference) ---- */
+
+static void gather13(ko_sim* s, tdata* C, double* f13, double* btt) {
+    size_t N = (size_t)s->NX * s->NY;
+    if (f13) memset(f13, 0, 8 * N * NF);
+    if (btt) memset(btt, 0, 8 * (size_t)s->ntiles * s->NV);
+    for (int k = 0; k < s->ntiles; ++k) {
+        tdata* T = &C[k];
+        if (!T->alloc) continue;
+        if (f13)
+            for (int d = 0; d < NF; ++d) scatter_field(s, k, &T->u[d * s->NC], f13 + d * N);
+        if (btt) memcpy(btt + (size_t)k * s->NV, T->bt, 8 * s->NV);
+    }
+}
+
+/* Per-tile vertex bt of the tile container, ntiles x (ny+1) x (nx+1). */
+int ko_get_tile_bt(ko_sim* s, double* btt) {
+    gather13(s, s->cont[C_TC], NULL, btt);
+    return 0;
+}
+
+/* CalculateMorphodynamicRHS on a copy of the current state: E - D at cells
+ * (global layout) and the vertex bt rate per tile. */
+int ko_morpho_rhs(ko_sim* s, double* emd, double* dtbt) {
+    init_stepping_arrays(s);
+    tdata* I0 = s->cont[C_I0];
+    morpho_rhs(s, I0);
+    size_t N = (size_t)s->NX * s->NY;
+    memset(emd, 0, 8 * N);
+    memset(dtbt, 0, 8 * (size_t)s->ntiles * s->NV);
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        scatter_field(s, k, I0[k].emd, emd);
+        memcpy(dtbt + (size_t)k * s->NV, I0[k].dtbt, 8 * s->NV);
+    }
+    return 0;
+}
+
+void ko_probe_enable(ko_sim* s, int on) {
+    s->probe_on = on;
+    s->probe_done = 0;
+}
+
+int ko_probe_count(ko_sim* s) { return s->probe_done ? s->probe_n : -1; }
+
+/* The morphodynamic dt of the probed step. */
+double ko_probe_dt(ko_sim* s) { return s->probe_dt; }
+
+/* which: 0 = intermed0, 1 = intermed3 before and 2 = after redistribution. */
+int ko_probe_get(ko_sim* s, int which, double* f13, double* btt) {
+    if (!s->probe_done || which < 0 || which > 2) return KST_ERR_STATE;
+    gather13(s, s->probe[which], f13, btt);
+    return 0;
+}
+
+/* Candidates in the order the check loop marked them (1-based tile ids and
+ * cell indices, as AddToRedistList receives them). */
+int ko_probe_candidates(ko_sim* s, int32_t* k, int32_t* i, int32_t* j, double* ex, uint8_t* ts) {
+    if (!s->probe_done) return KST_ERR_STATE;
+    memcpy(ts, s->probe_ts, s->ntiles);
+    const rnode* L = (const rnode*)s->probe_list;
+    for (int q = 0; q < s->probe_n; ++q) {
+        k[q] = L[q].k + 1; i[q] = L[q].i + 1; j[q] = L[q].j + 1; ex[q] = L[q].excess;
+    }
+    return 0;
+}
+
+static void load13(ko_sim* s, tdata* C, const double* f13, const double* btt) {
+    size_t N = (size_t)s->NX * s->NY;
+    int nx = s->nx, ny = s->ny;
+    for (int k = 0; k < s->ntiles; ++k) {
+        tdata* T = &C[k];
+        if (!T->alloc) continue;
+        int gi, gj;
+        tile_coords(s, k, &gi, &gj);
+        for (int d = 0; d < NF; ++d)
+            for (int jj = 0; jj < ny; ++jj)
+                for (int ii = 0; ii < nx; ++ii)
+                    UF(s, T, d, ii, jj) = f13[d * N + (size_t)(gj * ny + jj) * s->NX + gi * nx + ii];
+        if (btt) memcpy(T->bt, btt + (size_t)k * s->NV, 8 * s->NV);
+    }
+}
+
+/* Overwrite tileContainer's 13 fields (global layout) and, if given, the
+ * per-tile vertex bt of every allocated tile, as given. */
+int ko_set_fields13(ko_sim* s, const double* f13, const double* btt) {
+    load13(s, s->cont[C_TC], f13, btt);
+    return 0;
+}
+
+/* AddToRedistList over the candidates (1-based tile, i, j, as AddToRedistList
+ * receives them) and RedistributeGrid, with intermed0 and intermed3 loaded
+ * from (f0, bt0) and (f3, bt3); returns intermed3 and whether it failed. */
+int ko_redistribute(ko_sim* s, const double* f0, const double* bt0, const double* f3,
+                    const double* bt3, int n, const int32_t* k, const int32_t* i,
+                    const int32_t* j, const double* ex, double* fout, double* btout, int* failed) {
+    init_stepping_arrays(s);
+    load13(s, s->cont[C_I0], f0, bt0);
+    load13(s, s->cont[C_I3], f3, bt3);
+    rnode* list = NULL;
+    int nl = 0, cap = 0;
+    for (int q = 0; q < n; ++q) redist_add(&list, &nl, &cap, k[q] - 1, i[q] - 1, j[q] - 1, ex[q]);
+    *failed = (nl > 0) ? !redistribute_grid(s, list, nl) : 0;
+    free(list);
+    gather13(s, s->cont[C_I3], fout, btout);
     return 0;
 }

@@ -68,6 +68,37 @@ int ko_rhs(ko_sim* s, int substep, double t, double* ddte, double* ddti, double*
 /* Number of active / ghost tiles and the sorted active list (0-based ids). */
 int ko_tile_lists(ko_sim* s, int32_t* active, int32_t* nactive, int32_t* ghost, int32_t* nghost);
 
+/* Per-tile vertex bt of tileContainer: ntiles x (ny+1) x (nx+1), the
+ * reference's tileContainer(k)%bt (shared vertices may differ by tile). */
+int ko_get_tile_bt(ko_sim* s, double* bt_tiles);
+
+/* CalculateMorphodynamicRHS on a copy of tileContainer: EminusD (NX*NY,
+ * global) and ddtExplicitBt per tile (ntiles x (ny+1) x (nx+1)). */
+int ko_morpho_rhs(ko_sim* s, double* eminusd, double* dtbt_tiles);
+
+/* Probe of the first redistribution of a run: after ko_probe_enable, the
+ * first morphodynamic step that redistributes (with on = 2: the first
+ * morphodynamic step that is not refined) records intermed0, intermed3
+ * before and after RedistributeGrid (13 fields, global, and per-tile vertex
+ * bt) and the candidate cells in the order the check loop marked them. */
+void ko_probe_enable(ko_sim* s, int on);   /* on = 2: first morphodynamic step */
+double ko_probe_dt(ko_sim* s);
+int ko_probe_count(ko_sim* s);
+int ko_probe_get(ko_sim* s, int which, double* f13, double* bt_tiles);
+int ko_probe_candidates(ko_sim* s, int32_t* tile, int32_t* i, int32_t* j, double* excess,
+                        uint8_t* tile_state);
+
+/* Overwrite tileContainer's fields 1..13 (global layout) and per-tile vertex
+ * bt (may be NULL) of every allocated tile. */
+int ko_set_fields13(ko_sim* s, const double* f13, const double* bt_tiles);
+
+/* AddToRedistList over the given candidates (1-based) and RedistributeGrid
+ * with intermed0 = (f0, bt0), intermed3 = (f3, bt3): intermed3 afterwards. */
+int ko_redistribute(ko_sim* s, const double* f0, const double* bt0, const double* f3,
+                    const double* bt3, int n, const int32_t* tile, const int32_t* i,
+                    const int32_t* j, const double* excess, double* fout, double* btout,
+                    int* failed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,27 @@ def lib():
         L.ko_get_ghost_slopes.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.ko_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.ko_tile_lists.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
+        for name in ["ko_get_tile_bt", "ko_morpho_rhs", "ko_probe_count", "ko_probe_get",
+                     "ko_probe_candidates"]:
+            getattr(L, name).restype = ctypes.c_int
+        L.ko_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
+        L.ko_morpho_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ko_set_fields13.restype = ctypes.c_int
+        L.ko_set_fields13.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ko_redistribute.restype = ctypes.c_int
+        L.ko_redistribute.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32), _dp, _dp, _dp,
+                                      ctypes.POINTER(ctypes.c_int)]
+        L.ko_probe_enable.restype = None
+        L.ko_probe_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ko_probe_count.argtypes = [ctypes.c_void_p]
+        L.ko_probe_dt.restype = ctypes.c_double
+        L.ko_probe_dt.argtypes = [ctypes.c_void_p]
+        L.ko_probe_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.ko_probe_candidates.argtypes = [ctypes.c_void_p, i32p, i32p, i32p, _dp,
+                                          ctypes.POINTER(ctypes.c_uint8)]
         _LIB = L
     return _LIB
 
@@ -140,10 +161,11 @@ class Oracle:
         c = self.cfg
         shp = (c.NY, c.NX)
         out = {k: np.zeros(shp) for k in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "Hn")}
+        out["btv"] = np.zeros((c.NY + 1, c.NX + 1))
         ts = np.zeros(c.ntiles, dtype=np.uint8)
         t = ctypes.c_double()
         self._check(self.L.ko_get_state(self.h, _ptr(out["w"]), _ptr(out["rhoHnu"]), _ptr(out["rhoHnv"]),
-                                        _ptr(out["Hnpsi"]), _ptr(out["Hn"]), None,
+                                        _ptr(out["Hnpsi"]), _ptr(out["Hn"]), _ptr(out["btv"]),
                                         ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(t)))
         out["tile_state"] = ts
         out["t"] = t.value
@@ -175,6 +197,70 @@ class Oracle:
         self._check(self.L.ko_rhs(self.h, substep, float(t), _ptr(e), _ptr(i), ctypes.byref(adv)))
         return e, i, adv.value
 
+    def _nv(self):
+        c = self.cfg
+        return (c.ny_per_tile + 1, c.nx_per_tile + 1)
+
+    def tile_bt(self):
+        out = np.zeros((self.cfg.ntiles,) + self._nv())
+        self._check(self.L.ko_get_tile_bt(self.h, _ptr(out)))
+        return out
+
+    def morpho_rhs(self):
+        c = self.cfg
+        emd = np.zeros((c.NY, c.NX))
+        dtbt = np.zeros((c.ntiles,) + self._nv())
+        self._check(self.L.ko_morpho_rhs(self.h, _ptr(emd), _ptr(dtbt)))
+        return emd, dtbt
+
+    def set_fields13(self, f13, bt_tiles=None):
+        f13 = np.ascontiguousarray(f13, dtype=np.float64)
+        btt = None if bt_tiles is None else np.ascontiguousarray(bt_tiles, dtype=np.float64)
+        self._check(self.L.ko_set_fields13(self.h, _ptr(f13), _ptr(btt)))
+
+    def redistribute(self, i0, i3, cand):
+        """AddToRedistList + RedistributeGrid on the given containers."""
+        f0, bt0 = [np.ascontiguousarray(a, dtype=np.float64) for a in i0]
+        f3, bt3 = [np.ascontiguousarray(a, dtype=np.float64) for a in i3]
+        k, i, j = [np.ascontiguousarray(a, dtype=np.int32) for a in cand[:3]]
+        ex = np.ascontiguousarray(cand[3], dtype=np.float64)
+        fout = np.zeros_like(f3); btout = np.zeros_like(bt3)
+        failed = ctypes.c_int()
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        self._check(self.L.ko_redistribute(self.h, _ptr(f0), _ptr(bt0), _ptr(f3), _ptr(bt3), len(k),
+                                           k.ctypes.data_as(i32p), i.ctypes.data_as(i32p),
+                                           j.ctypes.data_as(i32p), _ptr(ex), _ptr(fout), _ptr(btout),
+                                           ctypes.byref(failed)))
+        return fout, btout, failed.value
+
+    def probe_enable(self, on=True):
+        """on=True: first redistribution; on=2: first morphodynamic step."""
+        self.L.ko_probe_enable(self.h, int(on))
+
+    def probe(self):
+        """The first redistribution recorded since probe_enable, or None."""
+        n = self.L.ko_probe_count(self.h)
+        if n < 0:
+            return None
+        c = self.cfg
+        res = {}
+        for which, name in enumerate(("i0", "i3_before", "i3_after")):
+            f13 = np.zeros((13, c.NY, c.NX))
+            btt = np.zeros((c.ntiles,) + self._nv())
+            self._check(self.L.ko_probe_get(self.h, which, _ptr(f13), _ptr(btt)))
+            res[name] = (f13, btt)
+        k = np.zeros(n, np.int32); i = np.zeros(n, np.int32); j = np.zeros(n, np.int32)
+        ex = np.zeros(n)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        ts = np.zeros(c.ntiles, np.uint8)
+        self._check(self.L.ko_probe_candidates(self.h, k.ctypes.data_as(i32p), i.ctypes.data_as(i32p),
+                                               j.ctypes.data_as(i32p), _ptr(ex),
+                                               ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        res["cand"] = (k, i, j, ex)
+        res["tile_state"] = ts
+        res["dt"] = self.L.ko_probe_dt(self.h)
+        return res
+
     def tile_lists(self):
         n = self.cfg.ntiles
         a = (ctypes.c_int32 * n)(); g = (ctypes.c_int32 * n)()
@@ -201,6 +287,12 @@ def ref_lib():
         L.kref_rhs.argtypes = [ctypes.POINTER(KstConfig), u8, u8, _dp, _dp, _dp, _dp,
                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                _dp, _dp, _dp]
+        L.kref_morpho_rhs.restype = None
+        L.kref_morpho_rhs.argtypes = [ctypes.POINTER(KstConfig), u8, _dp, _dp, _dp, _dp, _dp]
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.kref_redistribute.restype = None
+        L.kref_redistribute.argtypes = [ctypes.POINTER(KstConfig), u8, _dp, _dp, _dp, _dp, _dp,
+                                        ctypes.c_int32, i32p, i32p, i32p, _dp, _dp, _dp, i32p]
         _REF = L
     return _REF
 
@@ -223,3 +315,34 @@ def ref_rhs(cfg: KstConfig, tile_state, has_src, b0v, f13, sx, sy, *, recompute_
                *[_ptr(a) for a in arrs], int(recompute_ghost), int(substep), float(t), float(grid_t),
                _ptr(e), _ptr(i), ctypes.byref(adv))
     return e, i, adv.value
+
+
+def ref_morpho_rhs(cfg: KstConfig, tile_state, b0v, f13, bt_tiles):
+    """CalculateMorphodynamicRHS of the reference: (EminusD, ddtExplicitBt per tile)."""
+    L = ref_lib()
+    ts = np.ascontiguousarray(tile_state, dtype=np.int8)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (b0v, f13, bt_tiles)]
+    emd = np.zeros((cfg.NY, cfg.NX))
+    dtbt = np.zeros_like(arrs[2])
+    L.kref_morpho_rhs(ctypes.byref(cfg), ts.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                      *[_ptr(a) for a in arrs], _ptr(emd), _ptr(dtbt))
+    return emd, dtbt
+
+
+def ref_redistribute(cfg: KstConfig, tile_state, b0v, i0, i3, cand):
+    """AddToRedistList over the candidates, then RedistributeGrid, in the
+    reference: returns (fields13, bt per tile) of intermed3 and the refine flag."""
+    L = ref_lib()
+    ts = np.ascontiguousarray(tile_state, dtype=np.int8)
+    f0, bt0 = [np.ascontiguousarray(a, dtype=np.float64) for a in i0]
+    f3, bt3 = [np.ascontiguousarray(a, dtype=np.float64) for a in i3]
+    k, i, j, ex = [np.ascontiguousarray(a) for a in cand]
+    fout = np.zeros_like(f3); btout = np.zeros_like(bt3)
+    ref = ctypes.c_int32()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.kref_redistribute(ctypes.byref(cfg), ts.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                        _ptr(np.ascontiguousarray(b0v, dtype=np.float64)), _ptr(f0), _ptr(bt0),
+                        _ptr(f3), _ptr(bt3), len(k), k.astype(np.int32).ctypes.data_as(i32p),
+                        i.astype(np.int32).ctypes.data_as(i32p), j.astype(np.int32).ctypes.data_as(i32p),
+                        _ptr(ex.astype(np.float64)), _ptr(fout), _ptr(btout), ctypes.byref(ref))
+    return fout, btout, ref.value

@@ -5,22 +5,25 @@
 ! builds a GridType from a global state, the way InitialiseGrid / AllocateTile /
 ! AddGhostTiles / GetHeights would have left it (src/Grid.f90:166-220,
 ! src/UpdateTiles.f90:120-481, src/dem.f90:360-393), and calls the reference's
-! CalculateHydraulicRHS (src/HydraulicRHS.f90:64-136) on it.  The result pins
-! the C restatement's RHS bit for bit.
+! CalculateHydraulicRHS (src/HydraulicRHS.f90:64-136), CalculateMorphodynamicRHS
+! (src/MorphodynamicRHS.f90:72-149) or AddToRedistList + RedistributeGrid
+! (src/Redistribute.f90:69-244) on it.  The results pin the C restatement bit
+! for bit.
 module kestrel_ref_driver
    use, intrinsic :: iso_c_binding
    use set_precision_module, only: wp
    use runsettings_module, only: RunSet, SetImmutableRunSettings
-   use grid_module, only: GridType, InitialiseGrid, GridToPhysical
+   use grid_module, only: GridType, TileType, InitialiseGrid, GridToPhysical
    use hydraulic_rhs_module, only: CalculateHydraulicRHS, CalculateLimitedDerivsBoundary
    use morphodynamic_rhs_module, only: ComputeCellCentredTopographicData, &
-      ComputeInterfacialTopographicData
+      ComputeInterfacialTopographicData, CalculateMorphodynamicRHS
+   use redistribute_module, only: RedistList, AddToRedistList, RedistributeGrid
    use limiters_module
    use closures_module
    use kestrel_step_binding, only: kst_config, kst_flux_source
    implicit none
    private
-   public :: kref_rhs
+   public :: kref_rhs, kref_morpho_rhs, kref_redistribute
 
    ! Module variables: RunSet holds a proj_transformer component, and a local
    ! RunSet would be finalized through utm.f90's bindings (PROJ).  Module
@@ -147,24 +150,21 @@ contains
       end if
    end subroutine fill_runset
 
-   ! One hydraulic RHS of the reference on a state given in global layout:
-   !   tstate(ntiles)   0 off / 1 active / 2 ghost
-   !   has_src(ntiles)  containsSource
-   !   b0v((NX+1)*(NY+1)), f13(13*NX*NY) fields 1..13, sx/sy(NX*NY) ghost w slopes
-   !   recompute_ghost: 1 -> rebuild the ghost w slopes with the reference's
-   !   CalculateLimitedDerivsBoundary as ReconstructwAtEdges does; 0 -> use sx/sy.
-   subroutine kref_rhs(c, tstate, has_src, b0v, f13, sx, sy, recompute_ghost, substep, t, &
-                       gridt, ddte, ddti, advised) bind(C, name="kref_rhs")
+   ! Build grid%tileContainer from a global state the way InitialiseGrid /
+   ! AllocateTile / AddGhostTiles / GetHeights leave it: tstate(ntiles) 0 off /
+   ! 1 active / 2 ghost, b0v((NX+1)*(NY+1)), f13(13*NX*NY) fields 1..13 as
+   ! given, sx/sy(NX*NY) ghost w slopes, btt(ntiles*(nx+1)*(ny+1)) per-tile
+   ! vertex bt when use_btt /= 0 (else 0).
+   subroutine setup_grid(c, tstate, has_src, b0v, f13, sx, sy, btt, use_btt, gridt)
       type(kst_config), intent(in) :: c
       integer(c_int8_t), intent(in) :: tstate(*), has_src(*)
-      real(c_double), intent(in) :: b0v(*), f13(*), sx(*), sy(*)
-      integer(c_int32_t), value :: recompute_ghost, substep
-      real(c_double), value :: t, gridt
-      real(c_double), intent(out) :: ddte(*), ddti(*), advised
+      real(c_double), intent(in) :: b0v(*), f13(*), sx(*), sy(*), btt(*)
+      integer, intent(in) :: use_btt
+      real(c_double), intent(in) :: gridt
 
-      integer :: nx, ny, ntx, nty, NXg, NYg, k, gi, gj, i, j, d, nact, ngh, nb
+      integer :: nx, ny, ntx, nty, NXg, NYg, k, gi, gj, i, j, d, nact, ngh
       integer(c_int64_t) :: cidx, vidx, ncell
-      real(wp) :: x, y, adv
+      real(wp) :: x, y
 
       call bind_closures(c)
       call fill_runset(c, rp)
@@ -200,6 +200,8 @@ contains
             allocate(tl%hYFlux(13, nx, ny + 1), tl%gYFlux(13, nx, ny + 1), tl%pYFlux(13, nx, ny + 1))
             allocate(tl%ddtExplicit(4, nx, ny), tl%ddtImplicit(4, nx, ny))
             allocate(tl%b0(nx + 1, ny + 1), tl%bt(nx + 1, ny + 1), tl%EminusD(nx, ny))
+            allocate(tl%ddtExplicitBt(nx + 1, ny + 1))
+            tl%ddtExplicitBt = 0
             allocate(tl%x(nx), tl%y(ny))
             tl%u = 0; tl%uLimX = 0; tl%uLimY = 0
             tl%uPlusX = 0; tl%uMinusX = 0; tl%uPlusY = 0; tl%uMinusY = 0
@@ -218,6 +220,8 @@ contains
                do i = 1, nx + 1
                   vidx = int((gj - 1) * ny + j - 1, c_int64_t) * (NXg + 1) + (gi - 1) * nx + i
                   tl%b0(i, j) = b0v(vidx)
+                  if (use_btt /= 0) tl%bt(i, j) = btt(int(k - 1, c_int64_t) * (nx + 1) * (ny + 1) &
+                                                      + (j - 1) * (nx + 1) + i)
                end do
             end do
             ! face b0 as GetHeights leaves it (src/dem.f90:378-386)
@@ -250,6 +254,33 @@ contains
             grid%ghostTiles%List(grid%ghostTiles%size) = k
          end if
       end do
+
+   end subroutine setup_grid
+
+   ! One hydraulic RHS of the reference on a state given in global layout:
+   !   tstate(ntiles)   0 off / 1 active / 2 ghost
+   !   has_src(ntiles)  containsSource
+   !   b0v((NX+1)*(NY+1)), f13(13*NX*NY) fields 1..13, sx/sy(NX*NY) ghost w slopes
+   !   recompute_ghost: 1 -> rebuild the ghost w slopes with the reference's
+   !   CalculateLimitedDerivsBoundary as ReconstructwAtEdges does; 0 -> use sx/sy.
+   subroutine kref_rhs(c, tstate, has_src, b0v, f13, sx, sy, recompute_ghost, substep, t, &
+                       gridt, ddte, ddti, advised) bind(C, name="kref_rhs")
+      type(kst_config), intent(in) :: c
+      integer(c_int8_t), intent(in) :: tstate(*), has_src(*)
+      real(c_double), intent(in) :: b0v(*), f13(*), sx(*), sy(*)
+      integer(c_int32_t), value :: recompute_ghost, substep
+      real(c_double), value :: t, gridt
+      real(c_double), intent(out) :: ddte(*), ddti(*), advised
+
+      integer :: nx, ny, ntx, nty, NXg, NYg, k, gi, gj, i, j, d, nb
+      integer(c_int64_t) :: cidx, ncell
+      real(wp) :: adv
+      real(c_double) :: nobt(1)
+
+      call setup_grid(c, tstate, has_src, b0v, f13, sx, sy, nobt, 0, gridt)
+      nx = c%nx_per_tile; ny = c%ny_per_tile; ntx = c%nx_tiles; nty = c%ny_tiles
+      NXg = ntx * nx; NYg = nty * ny
+      ncell = int(NXg, c_int64_t) * NYg
 
       do k = 1, rp%nTiles
          if (tstate(k) == 0) cycle
@@ -336,5 +367,150 @@ contains
          end do
       end do
    end subroutine kref_rhs
+
+
+   ! Scatter a container's fields 1..13 into global layout and its vertex bt
+   ! per tile.
+   subroutine gather_container(c, tstate, tiles, f13, btt)
+      type(kst_config), intent(in) :: c
+      integer(c_int8_t), intent(in) :: tstate(*)
+      type(TileType), intent(in) :: tiles(:)
+      real(c_double), intent(out) :: f13(*), btt(*)
+      integer :: nx, ny, ntx, NXg, NYg, k, gi, gj, i, j, d
+      integer(c_int64_t) :: cidx, ncell, nv
+      nx = c%nx_per_tile; ny = c%ny_per_tile; ntx = c%nx_tiles
+      NXg = ntx * nx; NYg = c%ny_tiles * ny
+      ncell = int(NXg, c_int64_t) * NYg
+      nv = int(nx + 1, c_int64_t) * (ny + 1)
+      f13(1:13 * ncell) = 0.0_wp
+      btt(1:nv * rp%nTiles) = 0.0_wp
+      do k = 1, rp%nTiles
+         if (tstate(k) == 0) cycle
+         gi = mod(k - 1, ntx) + 1; gj = (k - 1) / ntx + 1
+         do j = 1, ny
+            do i = 1, nx
+               cidx = int((gj - 1) * ny + j - 1, c_int64_t) * NXg + (gi - 1) * nx + i
+               do d = 1, 13
+                  f13((d - 1) * ncell + cidx) = tiles(k)%u(d, i, j)
+               end do
+            end do
+         end do
+         do j = 1, ny + 1
+            do i = 1, nx + 1
+               btt((k - 1) * nv + (j - 1) * (nx + 1) + i) = tiles(k)%bt(i, j)
+            end do
+         end do
+      end do
+   end subroutine gather_container
+
+   ! Load fields 1..13 (global layout) and per-tile vertex bt into a container.
+   subroutine scatter_container(c, tstate, tiles, f13, btt)
+      type(kst_config), intent(in) :: c
+      integer(c_int8_t), intent(in) :: tstate(*)
+      type(TileType), intent(inout) :: tiles(:)
+      real(c_double), intent(in) :: f13(*), btt(*)
+      integer :: nx, ny, ntx, NXg, NYg, k, gi, gj, i, j, d
+      integer(c_int64_t) :: cidx, ncell, nv
+      nx = c%nx_per_tile; ny = c%ny_per_tile; ntx = c%nx_tiles
+      NXg = ntx * nx; NYg = c%ny_tiles * ny
+      ncell = int(NXg, c_int64_t) * NYg
+      nv = int(nx + 1, c_int64_t) * (ny + 1)
+      do k = 1, rp%nTiles
+         if (tstate(k) == 0) cycle
+         gi = mod(k - 1, ntx) + 1; gj = (k - 1) / ntx + 1
+         do j = 1, ny
+            do i = 1, nx
+               cidx = int((gj - 1) * ny + j - 1, c_int64_t) * NXg + (gi - 1) * nx + i
+               do d = 1, 13
+                  tiles(k)%u(d, i, j) = f13((d - 1) * ncell + cidx)
+               end do
+            end do
+         end do
+         do j = 1, ny + 1
+            do i = 1, nx + 1
+               tiles(k)%bt(i, j) = btt((k - 1) * nv + (j - 1) * (nx + 1) + i)
+            end do
+         end do
+      end do
+   end subroutine scatter_container
+
+   ! CalculateMorphodynamicRHS of the reference on tileContainer built from
+   ! fields 1..13 and per-tile vertex bt as given: returns EminusD (NX*NY,
+   ! active tiles) and ddtExplicitBt per tile (ntiles*(nx+1)*(ny+1)).
+   subroutine kref_morpho_rhs(c, tstate, b0v, f13, btt, emd, dtbt) bind(C, name="kref_morpho_rhs")
+      type(kst_config), intent(in) :: c
+      integer(c_int8_t), intent(in) :: tstate(*)
+      real(c_double), intent(in) :: b0v(*), f13(*), btt(*)
+      real(c_double), intent(out) :: emd(*), dtbt(*)
+      integer(c_int8_t), allocatable :: nosrc(:)
+      real(c_double), allocatable :: zero(:)
+      integer :: nx, ny, ntx, NXg, k, gi, gj, i, j
+      integer(c_int64_t) :: cidx, ncell, nv
+
+      nx = c%nx_per_tile; ny = c%ny_per_tile; ntx = c%nx_tiles
+      NXg = ntx * nx
+      ncell = int(NXg, c_int64_t) * (c%ny_tiles * ny)
+      nv = int(nx + 1, c_int64_t) * (ny + 1)
+      allocate(nosrc(c%nx_tiles * c%ny_tiles), zero(ncell))
+      nosrc = 0; zero = 0.0_wp
+      call setup_grid(c, tstate, nosrc, b0v, f13, zero, zero, btt, 1, 0.0_wp)
+      call CalculateMorphodynamicRHS(rp, grid, grid%tileContainer)
+      emd(1:ncell) = 0.0_wp
+      dtbt(1:nv * rp%nTiles) = 0.0_wp
+      do k = 1, rp%nTiles
+         if (tstate(k) /= 1) cycle
+         gi = mod(k - 1, ntx) + 1; gj = (k - 1) / ntx + 1
+         do j = 1, ny
+            do i = 1, nx
+               cidx = int((gj - 1) * ny + j - 1, c_int64_t) * NXg + (gi - 1) * nx + i
+               emd(cidx) = grid%tileContainer(k)%EminusD(i, j)
+            end do
+         end do
+         do j = 1, ny + 1
+            do i = 1, nx + 1
+               dtbt((k - 1) * nv + (j - 1) * (nx + 1) + i) = grid%tileContainer(k)%ddtExplicitBt(i, j)
+            end do
+         end do
+      end do
+   end subroutine kref_morpho_rhs
+
+   ! AddToRedistList for the candidates in the given order, then
+   ! RedistributeGrid, with intermed0 = (f0, bt0) and intermed3 = (f3, bt3)
+   ! (fields 1..13 global, vertex bt per tile).  Returns intermed3 afterwards
+   ! and the refine flag.
+   subroutine kref_redistribute(c, tstate, b0v, f0, bt0, f3, bt3, ncand, ck, ci, cj, cex, &
+                                fout, btout, refine) bind(C, name="kref_redistribute")
+      type(kst_config), intent(in) :: c
+      integer(c_int8_t), intent(in) :: tstate(*)
+      real(c_double), intent(in) :: b0v(*), f0(*), bt0(*), f3(*), bt3(*), cex(*)
+      integer(c_int32_t), value :: ncand
+      integer(c_int32_t), intent(in) :: ck(*), ci(*), cj(*)
+      real(c_double), intent(out) :: fout(*), btout(*)
+      integer(c_int32_t), intent(out) :: refine
+      integer(c_int8_t), allocatable :: nosrc(:)
+      real(c_double), allocatable :: zero(:)
+      type(RedistList) :: list
+      logical :: ref
+      integer :: k, q
+      integer(c_int64_t) :: ncell
+
+      ncell = int(c%nx_tiles * c%nx_per_tile, c_int64_t) * (c%ny_tiles * c%ny_per_tile)
+      allocate(nosrc(c%nx_tiles * c%ny_tiles), zero(ncell))
+      nosrc = 0; zero = 0.0_wp
+      call setup_grid(c, tstate, nosrc, b0v, f0, zero, zero, bt0, 1, 0.0_wp)
+      do k = 1, rp%nTiles
+         if (tstate(k) == 0) cycle
+         grid%intermed0(k) = grid%tileContainer(k)
+         grid%intermed3(k) = grid%tileContainer(k)
+      end do
+      call scatter_container(c, tstate, grid%intermed3, f3, bt3)
+      do q = 1, ncand
+         call AddToRedistList(list, int(ck(q)), int(ci(q)), int(cj(q)), cex(q))
+      end do
+      ref = .false.
+      if (ncand > 0) call RedistributeGrid(rp, grid, list, ref)
+      refine = merge(1, 0, ref)
+      call gather_container(c, tstate, grid%intermed3, fout, btout)
+   end subroutine kref_redistribute
 
 end module kestrel_ref_driver

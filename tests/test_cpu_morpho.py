@@ -1,0 +1,228 @@
+"""CPU tests of the oracle's morphodynamics: pinned to the reference's own
+CalculateMorphodynamicRHS and RedistributeGrid, and to what the mathematics
+fixes for the stepper (src/TimeStepper.f90:532-781, unbuildable here).
+
+No test here touches a GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from kestrel_amd import config as C
+from kestrel_amd import scenario as S, workloads
+from tests.helpers import oracle_for
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+# 0-based field indices kept in the fixtures (tests/golden/make_morpho_vectors.py)
+FIELDS_RHS = [4, 5, 6, 7, 10, 11, 12]
+FIELDS_REDIST = [0, 3, 4, 9, 10, 11, 12]
+REDIST_CASES = [c for c in workloads.MORPHO_FIXTURE_CASES if c != "fluid"]
+
+
+def _fixture(case):
+    return np.load(os.path.join(GOLDEN, f"morpho_{case}.npz"), allow_pickle=False)
+
+
+def _oracle_on_tiles(wl, tile_state):
+    """An oracle whose allocated tiles are those of tile_state."""
+    from oracle.oracle import Oracle
+    o = Oracle(wl.cfg)
+    o.set_topography(wl.b0v)
+    z = np.zeros((wl.cfg.NY, wl.cfg.NX))
+    o.set_state(S.InitialState(z, z, z, z, np.asarray(tile_state, np.uint8), z))
+    return o
+
+
+def _full13(cfg, idx, part):
+    f = np.zeros((13, cfg.NY, cfg.NX))
+    f[idx] = part
+    return f
+
+
+# ---------------------------------------------------------------------------
+# the reference's own arithmetic
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", workloads.MORPHO_FIXTURE_CASES)
+def test_oracle_morpho_rhs_matches_reference_vectors(case):
+    """CalculateMorphodynamicRHS of the reference (E - D at cells and the
+    vertex bt rate per tile, including the west-edge and NE-corner
+    three-term KahanSums) == the oracle's, bit for bit."""
+    z = _fixture(case)
+    wl = workloads.morpho_fixture_case(case)
+    o = _oracle_on_tiles(wl, z["tile_state"])
+    o.set_fields13(_full13(wl.cfg, FIELDS_RHS, z["f_rhs"]), z["btt"])
+    emd, dtbt = o.morpho_rhs()
+    assert np.array_equal(emd, z["emd"]), np.abs(emd - z["emd"]).max()
+    assert np.array_equal(dtbt, z["dtbt"]), np.abs(dtbt - z["dtbt"]).max()
+    assert np.count_nonzero(z["emd"]) > 100
+
+
+@pytest.mark.parametrize("case", REDIST_CASES)
+def test_oracle_redistribution_matches_reference_vectors(case):
+    """AddToRedistList + RedistributeGrid of the reference on a recorded
+    intermed0 / intermed3 and candidate sequence == the oracle's."""
+    z = _fixture(case)
+    wl = workloads.morpho_fixture_case(case)
+    o = _oracle_on_tiles(wl, z["r_tile_state"])
+    f0 = _full13(wl.cfg, FIELDS_REDIST, z["r_f0"])
+    f3 = _full13(wl.cfg, FIELDS_REDIST, z["r_f3"])
+    cand = (z["r_k"], z["r_i"], z["r_j"], z["r_ex"])
+    fout, btout, failed = o.redistribute((f0, z["r_bt0"]), (f3, z["r_bt3"]), cand)
+    assert failed == int(z["r_refine"])
+    assert np.array_equal(fout[FIELDS_REDIST], z["r_fout"]), np.abs(fout[FIELDS_REDIST] - z["r_fout"]).max()
+    assert np.array_equal(btout, z["r_btout"])
+    assert np.count_nonzero(z["r_btout"] != z["r_bt3"]) > 0
+
+
+@pytest.mark.parametrize("mutation", ["porosity", "settling", "critical_height", "erosion_law"])
+def test_planted_morpho_mistakes_are_caught(mutation):
+    """A plausible slip (wrong porosity, settling speed, damping height or
+    erosion law) must fail the reference vectors."""
+    z = _fixture("mixed")
+    wl = workloads.morpho_fixture_case("mixed")
+    cfg = wl.cfg
+    if mutation == "porosity":
+        cfg.bed_porosity = 0.4
+    elif mutation == "settling":
+        cfg.ws0 *= 1.01
+    elif mutation == "critical_height":
+        cfg.ero_critical_height = 0.02
+    else:
+        cfg.erosion = C.EROSIONS["fluid"]
+    o = _oracle_on_tiles(wl, z["tile_state"])
+    o.set_fields13(_full13(cfg, FIELDS_RHS, z["f_rhs"]), z["btt"])
+    emd, dtbt = o.morpho_rhs()
+    assert not (np.array_equal(emd, z["emd"]) and np.array_equal(dtbt, z["dtbt"]))
+
+
+def test_planted_redistribution_mistake_is_caught():
+    z = _fixture("mixed")
+    wl = workloads.morpho_fixture_case("mixed")
+    wl.cfg.bed_porosity = 0.3
+    o = _oracle_on_tiles(wl, z["r_tile_state"])
+    f0 = _full13(wl.cfg, FIELDS_REDIST, z["r_f0"])
+    f3 = _full13(wl.cfg, FIELDS_REDIST, z["r_f3"])
+    fout, btout, failed = o.redistribute((f0, z["r_bt0"]), (f3, z["r_bt3"]),
+                                         (z["r_k"], z["r_i"], z["r_j"], z["r_ex"]))
+    assert not np.array_equal(fout[FIELDS_REDIST], z["r_fout"])
+
+
+def test_oracle_morphodynamics_matches_live_reference():
+    """With oracle/_ref built: along a run with tile activation, the oracle's
+    morphodynamic RHS equals the reference's at several points, and the first
+    redistribution replays identically in the reference."""
+    from oracle import oracle as OR
+    if not OR.ref_available():
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    wl = workloads.morpho_cap(128, tile=16)
+    o = oracle_for(wl)
+    o.probe_enable(True)
+    for _ in range(3):
+        o.step_n(25)
+        st = o.state()
+        emd, dtbt = o.morpho_rhs()
+        remd, rdtbt = OR.ref_morpho_rhs(wl.cfg, st["tile_state"], wl.b0v, o.fields13(), o.tile_bt())
+        assert np.array_equal(emd, remd) and np.array_equal(dtbt, rdtbt)
+    assert st["tile_state"].tolist().count(1) > 4
+    p = o.probe()
+    assert p is not None and len(p["cand"][0]) > 10
+    f, b, ref = OR.ref_redistribute(wl.cfg, p["tile_state"], wl.b0v, p["i0"], p["i3_before"], p["cand"])
+    assert ref == 0
+    assert np.array_equal(f, p["i3_after"][0]) and np.array_equal(b, p["i3_after"][1])
+
+
+# ---------------------------------------------------------------------------
+# the stepper: what the mathematics fixes
+# ---------------------------------------------------------------------------
+
+def _volumes(cfg, o):
+    """Volume.txt's flow volume, bed volume and solids (src/Output.f90:617-735)
+    from the tile container's own cell topography."""
+    f = o.fields13()
+    st = o.state()
+    ts = st["tile_state"].reshape(cfg.ny_tiles, cfg.nx_tiles)
+    m = np.repeat(np.repeat(ts == 1, cfg.ny_per_tile, 0), cfg.nx_per_tile, 1)
+    gam = np.sqrt(1.0 + f[11] ** 2 + f[12] ** 2)
+    vol = ((f[0] - f[9] - f[10]) * gam * gam)[m].sum()
+    bed = f[10][m].sum()
+    solids = (f[3] * gam)[m].sum()
+    return vol, bed, solids
+
+
+def test_morphodynamic_conservation():
+    """Flow plus bed volume, and suspended plus bed solids, are conserved with
+    erosion, deposition and redistribution in a closed domain (the
+    reference's check_conservativity, tests/testlib.jl:104-158, rtol 1e-10)."""
+    wl = workloads.morpho_cap(128, tile=16)
+    o = oracle_for(wl)
+    o.probe_enable(True)
+    phi = wl.cfg.bed_porosity
+    v0, b0, s0 = _volumes(wl.cfg, o)
+    for _ in range(4):
+        o.step_n(25)
+        v, b, s = _volumes(wl.cfg, o)
+        assert abs((v + b) - (v0 + b0)) / v0 < 1e-10
+        assert abs((s + (1 - phi) * b) - (s0 + (1 - phi) * b0)) / s0 < 1e-10
+    assert abs(b) > 1.0           # the bed did change
+    assert o.probe() is not None  # and redistribution happened
+
+
+def test_morphodynamic_positivity_and_erosion_depth():
+    """Hn >= -1e-14 and bt >= -EroDepth (tests/testlib.jl:286-328)."""
+    wl = workloads.morpho_cap(128, tile=16)
+    wl.cfg.ero_depth = 0.01
+    o = oracle_for(wl)
+    o.step_n(80)
+    st = o.state()
+    assert st["Hn"].min() >= -1e-14
+    assert o.tile_bt().min() >= -0.01
+    assert o.tile_bt().min() == -0.01     # the clamp was reached
+
+
+def test_ssp_stages_exact_for_a_uniform_rate():
+    """For a rate that does not change over the step (fluid erosion under a
+    uniform sheet on a flat bed, no deposition, damping or transition), the
+    three SSP-RK stages return bt0 + dt*rate exactly at every vertex where the
+    rate is uniform; the rate is the (reference-pinned) morphodynamic RHS of
+    intermed0."""
+    from oracle.oracle import Oracle
+    wl = workloads.morpho_cap(128, tile=16, slope_deg=0.0, psi=0.0, erosion="fluid",
+                              deposition="none", transition="none", damping="none")
+    cfg = wl.cfg
+    init = S.initial_state(cfg, wl.b0v, [S.Cap(0.0, 0.0, 40.0, 1.0, u=2.0, shape="flat")])
+    o = Oracle(cfg)
+    o.set_topography(wl.b0v)
+    o.set_state(init)
+    o.probe_enable(2)
+    st = o.step_n(1)
+    p = o.probe()
+    assert p["dt"] == 2.0 * st["last_dt"]
+    (f0, bt0), (f3, bt3) = p["i0"], p["i3_before"]
+    o2 = Oracle(cfg)
+    o2.set_topography(wl.b0v)
+    o2.set_state(init)
+    o2.set_fields13(f0, bt0)
+    _, rate = o2.morpho_rhs()
+    vals, counts = np.unique(rate[rate != 0.0], return_counts=True)
+    r0 = vals[np.argmax(counts)]
+    m = rate == r0
+    assert m.sum() > 1000 and r0 < 0.0
+    assert np.array_equal(bt3[m], bt0[m] + p["dt"] * rate[m])
+
+
+def test_morphodynamic_symmetry_about_the_x_axis():
+    """Without deposition (so no order-dependent redistribution), a cap on an
+    x-only bed stays mirror-symmetric in y through erosion."""
+    wl = workloads.morpho_cap(128, tile=16, erosion="fluid", deposition="none",
+                              transition="none", damping="none")
+    o = oracle_for(wl)
+    o.step_n(60)
+    st = o.state()
+    assert st["btv"].min() < -1e-4
+    assert np.abs(st["Hn"] - st["Hn"][::-1]).max() < 1e-12
+    assert np.abs(st["rhoHnv"] + st["rhoHnv"][::-1]).max() < 1e-9
+    assert np.abs(st["btv"] - st["btv"][::-1]).max() < 1e-15
