@@ -2314,6 +2314,21 @@ int ko_set_state(ko_sim* s, const double* w, const double* rhu, const double* rh
             if (rc) return rc;
         }
     int nx = s->nx, ny = s->ny;
+    /* containsSource: tiles holding a cell within a flux source's radius
+     * (src/SetSources.f90:357-375); NumCellsInSrc comes with the config */
+    const kst_config* c = &s->cfg;
+    for (int k = 0; k < s->ntiles; ++k) {
+        int gi, gj;
+        tile_coords(s, k, &gi, &gj);
+        for (int q = 0; q < c->n_sources; ++q)
+            for (int jj = 0; jj < ny; ++jj)
+                for (int ii = 0; ii < nx; ++ii) {
+                    double x = -0.5 * c->x_size + c->dx * (((gi + 1) - 1.0) * nx + ((ii + 1) - 0.5));
+                    double y = -0.5 * c->y_size + c->dy * (((gj + 1) - 1.0) * ny + ((jj + 1) - 0.5));
+                    double R2 = (x - s->src[q].x) * (x - s->src[q].x) + (y - s->src[q].y) * (y - s->src[q].y);
+                    if (R2 <= s->src[q].radius * s->src[q].radius) s->meta[k].src = 1;
+                }
+    }
     if (btv) {
         /* every allocated tile takes its vertex bt from the global array; the
          * cell (and, for active tiles, face) topography follows */
@@ -2559,5 +2574,19 @@ int ko_redistribute(ko_sim* s, const double* f0, const double* bt0, const double
     *failed = (nl > 0) ? !redistribute_grid(s, list, nl) : 0;
     free(list);
     gather13(s, s->cont[C_I3], fout, btout);
+    return 0;
+}
+
+/* Per-tile vertex bt views of every allocated tile (ntiles x (ny+1) x
+ * (nx+1)), then the cell topography of those tiles and the face topography
+ * of the active ones. */
+int ko_set_tile_bt(ko_sim* s, const double* btt) {
+    for (int k = 0; k < s->ntiles; ++k) {
+        tdata* T = &s->cont[C_TC][k];
+        if (!T->alloc) continue;
+        memcpy(T->bt, btt + (size_t)k * s->NV, 8 * s->NV);
+        cell_topo(s, T);
+    }
+    for (int q = 0; q < s->nactive; ++q) face_topo(s, s->cont[C_TC], s->active[q]);
     return 0;
 }

@@ -88,6 +88,10 @@ int ko_probe_get(ko_sim* s, int which, double* f13, double* bt_tiles);
 int ko_probe_candidates(ko_sim* s, int32_t* tile, int32_t* i, int32_t* j, double* excess,
                         uint8_t* tile_state);
 
+/* Set the per-tile vertex bt of every allocated tile, then recompute the
+ * cell (all allocated tiles) and face (active tiles) topography. */
+int ko_set_tile_bt(ko_sim* s, const double* bt_tiles);
+
 /* Overwrite tileContainer's fields 1..13 (global layout) and per-tile vertex
  * bt (may be NULL) of every allocated tile. */
 int ko_set_fields13(ko_sim* s, const double* f13, const double* bt_tiles);

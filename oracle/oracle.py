@@ -73,6 +73,8 @@ def lib():
             getattr(L, name).restype = ctypes.c_int
         L.ko_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
         L.ko_morpho_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ko_set_tile_bt.restype = ctypes.c_int
+        L.ko_set_tile_bt.argtypes = [ctypes.c_void_p, _dp]
         L.ko_set_fields13.restype = ctypes.c_int
         L.ko_set_fields13.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.ko_redistribute.restype = ctypes.c_int
@@ -213,6 +215,10 @@ class Oracle:
         self._check(self.L.ko_morpho_rhs(self.h, _ptr(emd), _ptr(dtbt)))
         return emd, dtbt
 
+    def set_tile_bt(self, bt_tiles):
+        btt = np.ascontiguousarray(bt_tiles, dtype=np.float64)
+        self._check(self.L.ko_set_tile_bt(self.h, _ptr(btt)))
+
     def set_fields13(self, f13, bt_tiles=None):
         f13 = np.ascontiguousarray(f13, dtype=np.float64)
         btt = None if bt_tiles is None else np.ascontiguousarray(bt_tiles, dtype=np.float64)
@@ -284,9 +290,9 @@ def ref_lib():
         L = ctypes.CDLL(path)
         u8 = ctypes.POINTER(ctypes.c_int8)
         L.kref_rhs.restype = None
-        L.kref_rhs.argtypes = [ctypes.POINTER(KstConfig), u8, u8, _dp, _dp, _dp, _dp,
-                               ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
-                               _dp, _dp, _dp]
+        L.kref_rhs.argtypes = [ctypes.POINTER(KstConfig), u8, u8, _dp, _dp, _dp, _dp, _dp,
+                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                               ctypes.c_double, _dp, _dp, _dp]
         L.kref_morpho_rhs.restype = None
         L.kref_morpho_rhs.argtypes = [ctypes.POINTER(KstConfig), u8, _dp, _dp, _dp, _dp, _dp]
         i32p = ctypes.POINTER(ctypes.c_int32)
@@ -302,18 +308,20 @@ def ref_available() -> bool:
 
 
 def ref_rhs(cfg: KstConfig, tile_state, has_src, b0v, f13, sx, sy, *, recompute_ghost=False,
-            substep=1, t=0.0, grid_t=0.0):
-    """CalculateHydraulicRHS of the reference on the given global state."""
+            substep=1, t=0.0, grid_t=0.0, bt_tiles=None):
+    """CalculateHydraulicRHS of the reference on the given global state
+    (per-tile vertex bt views if given, else bt = 0)."""
     L = ref_lib()
     NX, NY = cfg.NX, cfg.NY
     ts = np.ascontiguousarray(tile_state, dtype=np.int8)
     hs = np.ascontiguousarray(has_src, dtype=np.int8)
-    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (b0v, f13, sx, sy)]
+    btt = np.zeros(1) if bt_tiles is None else bt_tiles
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (b0v, f13, sx, sy, btt)]
     e = np.zeros((4, NY, NX)); i = np.zeros((4, NY, NX)); adv = ctypes.c_double()
     u8 = ctypes.POINTER(ctypes.c_int8)
     L.kref_rhs(ctypes.byref(cfg), ts.ctypes.data_as(u8), hs.ctypes.data_as(u8),
-               *[_ptr(a) for a in arrs], int(recompute_ghost), int(substep), float(t), float(grid_t),
-               _ptr(e), _ptr(i), ctypes.byref(adv))
+               *[_ptr(a) for a in arrs], int(bt_tiles is not None), int(recompute_ghost),
+               int(substep), float(t), float(grid_t), _ptr(e), _ptr(i), ctypes.byref(adv))
     return e, i, adv.value
 
 

@@ -257,27 +257,27 @@ contains
 
    end subroutine setup_grid
 
-   ! One hydraulic RHS of the reference on a state given in global layout:
+   ! One hydraulic RHS of the reference on a state given in global layout
+   ! (btt: per-tile vertex bt when use_btt /= 0, else bt = 0):
    !   tstate(ntiles)   0 off / 1 active / 2 ghost
    !   has_src(ntiles)  containsSource
    !   b0v((NX+1)*(NY+1)), f13(13*NX*NY) fields 1..13, sx/sy(NX*NY) ghost w slopes
    !   recompute_ghost: 1 -> rebuild the ghost w slopes with the reference's
    !   CalculateLimitedDerivsBoundary as ReconstructwAtEdges does; 0 -> use sx/sy.
-   subroutine kref_rhs(c, tstate, has_src, b0v, f13, sx, sy, recompute_ghost, substep, t, &
+   subroutine kref_rhs(c, tstate, has_src, b0v, f13, sx, sy, btt, use_btt, recompute_ghost, substep, t, &
                        gridt, ddte, ddti, advised) bind(C, name="kref_rhs")
       type(kst_config), intent(in) :: c
       integer(c_int8_t), intent(in) :: tstate(*), has_src(*)
-      real(c_double), intent(in) :: b0v(*), f13(*), sx(*), sy(*)
-      integer(c_int32_t), value :: recompute_ghost, substep
+      real(c_double), intent(in) :: b0v(*), f13(*), sx(*), sy(*), btt(*)
+      integer(c_int32_t), value :: use_btt, recompute_ghost, substep
       real(c_double), value :: t, gridt
       real(c_double), intent(out) :: ddte(*), ddti(*), advised
 
       integer :: nx, ny, ntx, nty, NXg, NYg, k, gi, gj, i, j, d, nb
       integer(c_int64_t) :: cidx, ncell
       real(wp) :: adv
-      real(c_double) :: nobt(1)
 
-      call setup_grid(c, tstate, has_src, b0v, f13, sx, sy, nobt, 0, gridt)
+      call setup_grid(c, tstate, has_src, b0v, f13, sx, sy, btt, int(use_btt), gridt)
       nx = c%nx_per_tile; ny = c%ny_per_tile; ntx = c%nx_tiles; nty = c%ny_tiles
       NXg = ntx * nx; NYg = nty * ny
       ncell = int(NXg, c_int64_t) * NYg

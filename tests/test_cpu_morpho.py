@@ -61,6 +61,27 @@ def test_oracle_morpho_rhs_matches_reference_vectors(case):
     assert np.count_nonzero(z["emd"]) > 100
 
 
+@pytest.mark.parametrize("case", workloads.MORPHO_FIXTURE_CASES)
+def test_oracle_hydraulic_rhs_over_changed_bed_matches_reference_vectors(case):
+    """CalculateHydraulicRHS of the reference over an eroded/deposited bed
+    whose shared tile-edge vertices hold different bt in neighbouring tiles
+    (face bt, InterpolateB beds in CorrectSlopes, hp = -bt + (w - b0)) ==
+    the oracle's, bit for bit."""
+    from oracle.oracle import Oracle
+    z = _fixture(case)
+    wl = workloads.morpho_fixture_case(case)
+    o = Oracle(wl.cfg)
+    o.set_topography(wl.b0v)
+    o.set_state(S.InitialState(z["h_w"], z["h_rhu"], z["h_rhv"], z["h_hnpsi"], z["tile_state"],
+                               np.zeros_like(z["h_w"])), t=float(z["h_t"]))
+    o.set_tile_bt(z["btt"])
+    e, i, adv = o.rhs(1, float(z["h_t"]))
+    assert np.array_equal(e, z["h_ddte"]), np.abs(e - z["h_ddte"]).max()
+    assert np.array_equal(i, z["h_ddti"])
+    assert adv == float(z["h_adv"])
+    assert np.abs(z["btt"]).max() > 1e-4
+
+
 @pytest.mark.parametrize("case", REDIST_CASES)
 def test_oracle_redistribution_matches_reference_vectors(case):
     """AddToRedistList + RedistributeGrid of the reference on a recorded
@@ -113,8 +134,9 @@ def test_planted_redistribution_mistake_is_caught():
 
 def test_oracle_morphodynamics_matches_live_reference():
     """With oracle/_ref built: along a run with tile activation, the oracle's
-    morphodynamic RHS equals the reference's at several points, and the first
-    redistribution replays identically in the reference."""
+    morphodynamic RHS equals the reference's at several points, so does the
+    hydraulic RHS over the changed bed, and the first redistribution replays
+    identically in the reference."""
     from oracle import oracle as OR
     if not OR.ref_available():
         pytest.skip("oracle/_ref not built (needs the reference sources)")
@@ -128,6 +150,12 @@ def test_oracle_morphodynamics_matches_live_reference():
         remd, rdtbt = OR.ref_morpho_rhs(wl.cfg, st["tile_state"], wl.b0v, o.fields13(), o.tile_bt())
         assert np.array_equal(emd, remd) and np.array_equal(dtbt, rdtbt)
     assert st["tile_state"].tolist().count(1) > 4
+    hs = np.zeros(wl.cfg.ntiles, np.int8)
+    sx, sy = o.ghost_slopes()
+    e, i, adv = o.rhs(1, st["t"])
+    re_, ri, radv = OR.ref_rhs(wl.cfg, st["tile_state"], hs, wl.b0v, o.fields13(), sx, sy,
+                               t=st["t"], grid_t=st["t"], bt_tiles=o.tile_bt())
+    assert np.array_equal(e, re_) and np.array_equal(i, ri) and adv == radv
     p = o.probe()
     assert p is not None and len(p["cand"][0]) > 10
     f, b, ref = OR.ref_redistribute(wl.cfg, p["tile_state"], wl.b0v, p["i0"], p["i3_before"], p["cand"])
