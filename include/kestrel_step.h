@@ -115,8 +115,10 @@ typedef struct kst_stats {
     int32_t active_tiles;     /* after the call */
     int32_t ghost_tiles;
     double t;                 /* simulation time after the call */
-    double last_dt;
+    double last_dt;           /* hydraulic dt of the last step (a Strang step spans 2 dt) */
     double min_dt, max_dt;
+    int64_t redist_cells;     /* cells corrected by RedistributeCell (src/Redistribute.f90:249) */
+    int64_t redist_ghost;     /* of their writes, those into ghost tiles' part of intermed3 */
 } kst_stats;
 
 typedef struct kst_handle kst_handle;
@@ -137,7 +139,8 @@ int kst_create(const kst_config* cfg, kst_handle** out);
 int kst_set_topography(kst_handle* h, const double* b0_vertex);
 
 /* Conserved fields w, rhoHnu, rhoHnv, Hnpsi (NX*NY each), vertex bt
- * ((NX+1)*(NY+1), may be NULL = 0), tile state (KST_TILE_*), time t and
+ * ((NX+1)*(NY+1), may be NULL = 0; taken by the active tiles, ghost tiles
+ * keep bt = 0), tile state (KST_TILE_*), time t and
  * optional initial maxima (Hnmax, may be NULL).  Ghost-tile w reconstructions
  * are rebuilt as at the end of LoadSourceConditions
  * (src/SetSources.f90:388-392). */
@@ -154,10 +157,21 @@ int kst_integrate_to(kst_handle* h, double tend, kst_stats* stats_out);
 int kst_step_n(kst_handle* h, int64_t n_steps, kst_stats* stats_out);
 
 /* Download the state: w, rhoHnu, rhoHnv, Hnpsi, Hn (NX*NY each, any may be
- * NULL), vertex bt ((NX+1)*(NY+1), may be NULL), tile states, time. */
+ * NULL), vertex bt ((NX+1)*(NY+1), may be NULL; vertex (I,J) as the tile
+ * holding cell (min(I,NX-1), min(J,NY-1)) sees it), tile states, time. */
 int kst_get_state(kst_handle* h, double* w, double* rhoHnu, double* rhoHnv,
                   double* Hnpsi, double* Hn, double* bt_vertex, uint8_t* tile_state,
                   double* t);
+
+/* Per-tile vertex bt, ntiles x (ny_per_tile+1) x (nx_per_tile+1): each
+ * tile's own copy, as tileContainer(k)%bt holds it (src/Grid.f90:57-121);
+ * copies of a shared vertex may differ in the last bits. */
+int kst_get_tile_bt(kst_handle* h, double* bt_tiles);
+
+/* Cell-centred b0, bt, db/dx, db/dy (NX*NY each, any may be NULL): fields
+ * 10-13 of tileContainer(k)%u (ComputeCellCentredTopographicData,
+ * src/MorphodynamicRHS.f90:308-416). */
+int kst_get_cell_topography(kst_handle* h, double* b0c, double* btc, double* dbdx, double* dbdy);
 
 /* Running maxima (src/TimeStepper.f90:1155-1303): 11 arrays of NX*NY doubles
  * in the order Hnmax, Hnmax_t, umax, umax_t, emax, emax_t, dmax, dmax_t,

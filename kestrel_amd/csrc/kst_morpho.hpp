@@ -1,0 +1,545 @@
+// kst_morpho.hpp -- gfx950 kernels of the morphodynamic operator M(2dt) of
+// the Strang step H(dt) M(2dt) H(dt) (src/TimeStepper.f90:532-781) and of
+// the bed bookkeeping around it.  Included by kst_step.hip after the stage
+// kernels (it uses Dev, Ctrl, Topo and the work mapping defined there).
+//
+//   morpho_kernel<1..3>   one SSP stage on the vertex bt of every active tile:
+//                         E - D with the dryness check, vertex rates, the
+//                         clamped stage update, cell topography and the
+//                         conservative w / Hnpsi update; stage 3 also runs
+//                         the excess-deposition and depth-change checks.
+//   redistribute_kernel   the sorted redistribution list (one workgroup).
+//   face_topo_kernel      face topography and CFL numerators of the new bed.
+//   commit_kernel         the new bed becomes current once the step is accepted.
+#pragma once
+
+namespace kst {
+
+constexpr int MR1 = B + 2;   // sub-block plus a 1-cell halo
+constexpr int MV = B + 1;    // vertices along a sub-block edge
+
+// Thread 0: whether the morphodynamic operator runs in this attempt (the
+// first hydraulic operator passed its re-checks), its dt, and the block's
+// work item (sub-block of an active tile, XCD-chunked as the stage kernel).
+struct MWork {
+    int tile, q, bx0, by0, bxa, bya, skip;
+    double dt;
+};
+
+__device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_no_refine) {
+    const Ctrl* c = d.ctrl;
+    int skip = (c->status != ST_NEW && c->status != ST_RETRY);
+    double dt = 0.0;
+    if (!skip) {
+        dt = step_dt(d, p);
+        if (!first_operator_ok(c, p, dt)) skip = 1;
+        if (need_no_refine && c->mref) skip = 1;
+    }
+    const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
+    const int nwork = c->n_active * spt;
+    const int per = (nwork + 7) / 8;
+    const int b = blockIdx.x;
+    const int w = (b % 8) * per + b / 8;
+    if (b / 8 >= per || w >= nwork) skip = 1;
+    if (!skip) {
+        m.q = w / spt;
+        m.tile = d.active[m.q];
+        const int sub = w % spt;
+        m.bx0 = (sub % sbx) * B;
+        m.by0 = (sub / sbx) * B;
+        m.bxa = min(B, p.nx - m.bx0);
+        m.bya = min(B, p.ny - m.by0);
+    }
+    m.skip = skip;
+    m.dt = dt;
+}
+
+// ExcessDeposition (src/Redistribute.f90:158-198) of cell gi, between
+// intermed0 = (SH, TA) and intermed3 = (SM[2], TB); also returns Hn_old,
+// deltaBt and psiold.
+__device__ __forceinline__ double excess_deposition(const Dev& d, const Params& p, size_t gi,
+                                                    double btc3, double& ho, double& dbt,
+                                                    double& psio) {
+    const long long FS = d.FS;
+    const double go = d.TA.cgam[gi], btc0 = d.TA.btc[gi];
+    const double w0 = d.SH[gi], h0 = d.SH[3 * FS + gi];
+    ho = compute_hn(w0, d.b0c[gi], btc0, go);
+    dbt = btc3 - btc0;
+    // intermed0's psi as ComputeDesingularisedVariables left it
+    const double hoc = ho < 0.0 ? 0.0 : ho;
+    const double hpc = h0 < 0.0 ? 0.0 : h0;
+    const double eps = p.hthr;
+    const double den = hoc * hoc + dmax(hoc * hoc, eps * eps);
+    psio = dmin(2.0 * hoc * hpc / den, p.max_pack);
+    const double e = -(h0 * go / (1.0 - p.porosity) - dbt);
+    const double lim = -(ho * go - dbt);
+    return (e > lim) ? e : lim;
+}
+
+// One SSP stage S of MorphodynamicTimeStepper (src/TimeStepper.f90:562-710)
+// on one 16x16 sub-block:
+//   E - D of intermed_{S-1} (CalculateMorphodynamicRHS, src/MorphodynamicRHS.f90:72-149)
+//   vertex rate (BtSourceTerm, src/MorphodynamicRHS.f90:154-305)
+//   bt_S = max(-EroDepth, SSP combination) at this tile's vertices
+//   cell topography (ComputeCellCentredTopographicData, :308-360)
+//   w, Hnpsi from the change of bt (src/TimeStepper.f90:587-610)
+// Stage 3 then runs the checks of src/TimeStepper.f90:713-752.
+template <int S>
+__global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
+    __shared__ double hn[NREG];            // Hn of intermed_{S-1}, 2-cell halo
+    __shared__ double emd[MR1 * MR1];      // E - D, 1-cell halo
+    __shared__ double sbx[MR1 * MR1], sby[MR1 * MR1];
+    __shared__ double vb[MV * MV];         // new vertex bt of this tile's copy
+    __shared__ MWork m;
+    const int tid = threadIdx.x;
+    if (tid == 0) morpho_work(d, p, m, false);
+    __syncthreads();
+    if (m.skip) return;
+    const long long FS = d.FS;
+    const int t = m.tile, bx0 = m.bx0, by0 = m.by0, bxa = m.bxa, bya = m.bya;
+    const double dtm = 2.0 * m.dt;
+    const double* Win = (S == 1) ? d.SH : d.SM[S - 2];
+    const double* Hin = (S == 1) ? d.SH + 3 * FS : d.SM[S - 2] + FS;
+    const Topo& Tin = (S == 1) ? d.TA : d.TM[S - 2];
+    const Topo& Tout = (S == 3) ? d.TB : d.TM[S - 1];
+    double* Wout = d.SM[S - 1];
+    double* Hout = d.SM[S - 1] + FS;
+    const double eps = p.hthr;
+
+    // ---- Hn of the stage state (ComputeDesingularisedVariables) -----------
+    for (int q = tid; q < NREG; q += NTHREADS) {
+        const int rx = q % RW - 2, ry = q / RW - 2;
+        if (rx > bxa + 1 || ry > bya + 1) continue;
+        int tc, ci, cj;
+        map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
+        const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
+        // ghost tiles keep the zero derived fields SetDefaultTileData gave them
+        double h = 0.0;
+        if (d.tstate[tc] == KST_TILE_ACTIVE) h = compute_hn(Win[gi], d.b0c[gi], Tin.btc[gi], Tin.cgam[gi]);
+        hn[q] = h < 0.0 ? 0.0 : h;
+    }
+    __syncthreads();
+
+    // ---- E - D with the dryness check (src/MorphodynamicRHS.f90:90-145) ---
+    for (int q = tid; q < MR1 * MR1; q += NTHREADS) {
+        const int rx = q % MR1 - 1, ry = q / MR1 - 1;
+        if (rx > bxa || ry > bya) continue;
+        int tc, ci, cj;
+        map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
+        const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
+        const double bxc = Tin.cbx[gi], byc = Tin.cby[gi];
+        sbx[q] = bxc;
+        sby[q] = byc;
+        double e = 0.0;
+        const bool on = d.tstate[tc] == KST_TILE_ACTIVE;
+        const bool own = rx >= 0 && ry >= 0 && rx < bxa && ry < bya;
+        if (on) {
+            const double h = hn[rg(rx, ry)];
+            double hp = Hin[gi];
+            if (hp < 0.0) hp = 0.0;
+            const double den = h * h + dmax(h * h, eps * eps);
+            const double psi = dmin(2.0 * h * hp / den, p.max_pack);
+            // the velocities intermed_k holds: desingularised from the first
+            // operator's stage-3 momentum, never recomputed in M
+            double u, v;
+            if (S == 1) {
+                const double rho = density(p, psi);
+                u = 2.0 * h * d.S3[FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
+                v = 2.0 * h * d.S3[2 * FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
+                if (own) { d.UV[gi] = u; d.UV[FS + gi] = v; }
+            } else {
+                u = d.UV[gi];
+                v = d.UV[FS + gi];
+            }
+            if (!(h < eps || hn[rg(rx - 1, ry)] < eps || hn[rg(rx + 1, ry)] < eps ||
+                  hn[rg(rx, ry - 1)] < eps || hn[rg(rx, ry + 1)] < eps)) {
+                CellState cs{h, u, v, psi, Tin.btc[gi], bxc, byc};
+                e = erosion_minus_deposition(p, cs);
+            }
+        }
+        emd[q] = e;
+    }
+    __syncthreads();
+
+    // ---- vertex rates and the stage update of this tile's copy ------------
+    const int NV1 = p.nx + 1;
+    const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
+    const double psib = 1.0 - p.porosity;
+    for (int q = tid; q < MV * MV; q += NTHREADS) {
+        const int a = q % MV, b = q / MV;
+        if (a > bxa || b > bya) continue;
+        const int vi = bx0 + a, vj = by0 + b;
+        // cells (vi-1, vj-1), (vi-1, vj), (vi, vj-1), (vi, vj)
+        const int i00 = b * MR1 + a, i01 = (b + 1) * MR1 + a;
+        const int i10 = b * MR1 + a + 1, i11 = (b + 1) * MR1 + a + 1;
+        Kahan kx, ky, ke;
+        kx.add(sbx[i00]); kx.add(sbx[i01]);
+        if (vi == p.nx && vj == p.ny) kx.add(sbx[i10] + sbx[i11]);   // src/MorphodynamicRHS.f90:266
+        else { kx.add(sbx[i10]); kx.add(sbx[i11]); }
+        ky.add(sby[i00]); ky.add(sby[i01]); ky.add(sby[i10]); ky.add(sby[i11]);
+        ke.add(emd[i00]); ke.add(emd[i01]);
+        if (vi == 0 && vj >= 1 && vj <= p.ny - 1) ke.add(emd[i10] + emd[i11]);   // :210
+        else { ke.add(emd[i10]); ke.add(emd[i11]); }
+        const double gam = gamma_of(p.geometric, 0.25 * kx.s, 0.25 * ky.s);
+        const double rate = -0.25 * gam / psib * ke.s;
+        const size_t vidx = (size_t)t * NV + vj * NV1 + vi;
+        const double bt0 = d.TA.vbt[vidx];
+        double bn;
+        if (S == 1) bn = bt0 + dtm * rate;
+        else if (S == 2) bn = 0.75 * bt0 + 0.25 * (Tin.vbt[vidx] + dtm * rate);
+        else bn = (1.0 / 3.0) * bt0 + (2.0 / 3.0) * (Tin.vbt[vidx] + dtm * rate);
+        bn = dmax(-p.ero_depth, bn);
+        vb[q] = bn;
+        if ((a < bxa || vi == p.nx) && (b < bya || vj == p.ny)) Tout.vbt[vidx] = bn;
+    }
+    __syncthreads();
+
+    // ---- cell topography, w and Hnpsi of the owned cells -------------------
+    const int tx = tid % B, ty = tid / B;
+    if (tx >= bxa || ty >= bya) return;
+    const int i = bx0 + tx, j = by0 + ty;
+    const double* b0 = d.vb0 + (size_t)t * NV;
+#define VB(a, b) vb[(b) * MV + (a)]
+#define V0(ii, jj) b0[(jj) * NV1 + (ii)]
+    const double t00 = VB(tx, ty), t10 = VB(tx + 1, ty), t01 = VB(tx, ty + 1), t11 = VB(tx + 1, ty + 1);
+    Kahan kt;
+    kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
+    const double btc = 0.25 * kt.s;
+    Kahan kx;
+    kx.add(V0(i + 1, j)); kx.add(t10); kx.add(-V0(i, j)); kx.add(-t00);
+    kx.add(V0(i + 1, j + 1)); kx.add(t11); kx.add(-V0(i, j + 1)); kx.add(-t01);
+    const double dbdx = 0.5 * p.dxr * kx.s;
+    Kahan ky;
+    ky.add(V0(i, j + 1)); ky.add(t01); ky.add(-V0(i, j)); ky.add(-t00);
+    ky.add(V0(i + 1, j + 1)); ky.add(t11); ky.add(-V0(i + 1, j)); ky.add(-t10);
+    const double dbdy = 0.5 * p.dyr * ky.s;
+#undef VB
+#undef V0
+    const double gn = gamma_of(p.geometric, dbdx, dbdy);
+    const size_t gi = (size_t)t * p.NC + j * p.nx + i;
+    const double b0c = d.b0c[gi];
+    const double w0 = d.SH[gi], h0 = d.SH[3 * FS + gi];
+    const double btc0 = d.TA.btc[gi], go = d.TA.cgam[gi];
+    const double ho = compute_hn(w0, b0c, btc0, go);
+    const double db = btc - btc0;
+    double w = -db / gn / gn;
+    w = w + btc;
+    w = w + ho * go / gn / gn;
+    w = w + b0c;
+    double hp = -(1.0 - p.porosity) * db / gn;
+    hp = hp + h0 * go / gn;
+    Wout[gi] = w;
+    Hout[gi] = hp;
+    Tout.btc[gi] = btc;
+    Tout.cbx[gi] = dbdx;
+    Tout.cby[gi] = dbdy;
+    Tout.cgam[gi] = gn;
+
+    if (S == 3) {
+        // excess deposition and depth-change checks (src/TimeStepper.f90:713-752)
+        const double hnew = compute_hn(w, b0c, btc, gn);
+        double ho2, dbt, psio;
+        const double ex = excess_deposition(d, p, gi, btc, ho2, dbt, psio);
+        Ctrl* c = d.ctrl;
+        if (ex > kEps && dbt > kEps && psio > -kEps) {
+            if (ho2 < p.ero_hcrit || fabs(hnew - ho2) < p.ero_hcrit) {
+                const int k = atomicAdd(&c->ncand, 1);
+                if (k < d.cand_cap) {
+                    d.cand_key[k] = (long long)m.q * p.NC + j * p.nx + i;
+                    d.cand_ex[k] = ex;
+                } else {
+                    atomicOr(&c->mref, 4);
+                }
+            } else {
+                atomicOr(&c->mref, 1);
+            }
+        } else if (!(ho2 < p.ero_hcrit)) {
+            const double rel = fabs(hnew - ho2) / fabs(ho2);
+            if (rel > 0.1) atomicOr(&c->mref, 1);
+        }
+    }
+}
+
+// ComputeCellCentredTopographicData_ij (src/MorphodynamicRHS.f90:363-416) on
+// the new bed TB: stored for active tiles, returned (bt, gamma) always.
+__device__ void cell_topo_ij(const Dev& d, const Params& p, int t, int i, int j, bool store,
+                             double& btc, double& gam) {
+    const int NV1 = p.nx + 1;
+    const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
+    const double* b = d.vb0 + (size_t)t * NV;
+    const double* bt = d.TB.vbt + (size_t)t * NV;
+#define V(a, ii, jj) a[(jj) * NV1 + (ii)]
+    Kahan kt;
+    kt.add(V(bt, i, j)); kt.add(V(bt, i + 1, j)); kt.add(V(bt, i, j + 1)); kt.add(V(bt, i + 1, j + 1));
+    Kahan kx;
+    kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
+    kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1)); kx.add(-V(b, i, j + 1)); kx.add(-V(bt, i, j + 1));
+    Kahan ky;
+    ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1)); ky.add(-V(b, i, j)); ky.add(-V(bt, i, j));
+    ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(-V(b, i + 1, j)); ky.add(-V(bt, i + 1, j));
+#undef V
+    const size_t gi = (size_t)t * p.NC + j * p.nx + i;
+    const double dbdx = 0.5 * p.dxr * kx.s, dbdy = 0.5 * p.dyr * ky.s;
+    btc = 0.25 * kt.s;
+    gam = gamma_of(p.geometric, dbdx, dbdy);
+    if (store) {
+        d.TB.btc[gi] = btc;
+        d.TB.cbx[gi] = dbdx;
+        d.TB.cby[gi] = dbdy;
+        d.TB.cgam[gi] = gam;
+    }
+}
+
+// A write into a ghost tile's part of intermed3, listed for the next step's refresh.
+__device__ __forceinline__ void note_dirty(const Dev& d, long long e, long long* ghost) {
+    Ctrl* c = d.ctrl;
+    const long long q = c->ndirty++;
+    if (q < d.dirty_cap) d.dirty[q] = e;
+    ++*ghost;
+}
+
+// RedistributeCell (src/Redistribute.f90:249-475), 2D, on intermed0 =
+// (SH, TA) and intermed3 = (SM[2], TB) for active tiles.  In ghost tiles the
+// reference writes intermed3 too: vertex copies go to TB.vbt, cell w and
+// Hnpsi to S3 (the stage state the last RHS of each hydraulic operator
+// reads); the cell topography of a ghost cell is recomputed where it is used
+// and not stored, since only intermed3 would hold it.
+__device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, double corr,
+                                  long long* ghost) {
+    const long long FS = d.FS;
+    const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
+    const size_t NV = (size_t)(nx + 1) * (ny + 1);
+    double* bt3 = d.TB.vbt;
+    const double* bt0 = d.TA.vbt;
+    double bd[4] = {0.0, 0.0, 0.0, 0.0};
+    int di[4] = {0, 0, 0, 0}, dj[4] = {0, 0, 0, 0};
+    const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
+    int N = 0;
+    double sum = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        const size_t v = (size_t)k * NV + vj[q] * NV1 + vi[q];
+        bd[N] = bt3[v] - bt0[v];
+        if (bd[N] > 0.0) {
+            di[N] = vi[q];
+            dj[N] = vj[q];
+            sum = sum + bd[N];
+            N++;
+        }
+    }
+    if (N == 0) return false;
+    double delta = 4.0 * corr / sum;
+    const size_t gc = (size_t)k * p.NC + j * nx + i;
+    const double gamold = d.TA.cgam[gc];
+    double hnold = compute_hn(d.SH[gc], d.b0c[gc], d.TA.btc[gc], gamold);
+    if (hnold < 0.0) hnold = 0.0;
+    const double db = d.TB.btc[gc] - d.TA.btc[gc];
+    double tol = kEps * d.SM[2][gc] * 10.0;
+    double adj = 0.0;
+    {
+        Kahan ks;
+        ks.add(hnold * gamold); ks.add(-db); ks.add(corr);
+        if (fabs(ks.s) < tol) {
+            Kahan ka;
+            ka.add(tol); ka.add(-hnold * gamold); ka.add(db);
+            adj = ka.s;
+            adj = adj * (4.0 / sum);
+            adj = adj - delta;
+            adj = dmax(adj, 0.0);
+        }
+    }
+    const double hgpo = d.SH[3 * FS + gc] * gamold / (1.0 - p.porosity);
+    tol = kEps * 10.0;
+    {
+        Kahan ks;
+        ks.add(hgpo); ks.add(-db); ks.add(corr);
+        if (fabs(ks.s) < tol) {
+            Kahan ka;
+            ka.add(tol); ka.add(-hgpo); ka.add(db);
+            double ap = ka.s;
+            ap = ap * (4.0 / sum);
+            ap = ap - delta;
+            adj = dmax(adj, ap);
+        }
+    }
+    delta = delta + adj;
+    auto put = [&](int tile, int ii, int jj, double val) {
+        const size_t v = (size_t)tile * NV + jj * NV1 + ii;
+        bt3[v] = val;
+        if (d.tstate[tile] != KST_TILE_ACTIVE) note_dirty(d, (long long)v | (1LL << 62), ghost);
+    };
+    for (int q = 0; q < N; ++q) {
+        const size_t v = (size_t)k * NV + dj[q] * NV1 + di[q];
+        const double nv = bt3[v] - delta * bd[q];
+        bt3[v] = nv;
+        if (di[q] == 0) put(k - 1, nx, dj[q], nv);
+        else if (di[q] == nx) put(k + 1, 0, dj[q], nv);
+        if (dj[q] == 0) {
+            put(k - ntx, di[q], ny, nv);
+            if (di[q] == 0) put(k - ntx - 1, nx, ny, nv);
+            if (di[q] == nx) put(k - ntx + 1, 0, ny, nv);
+        }
+        if (dj[q] == ny) {
+            put(k + ntx, di[q], 0, nv);
+            if (di[q] == 0) put(k + ntx - 1, nx, 0, nv);
+            if (di[q] == nx) put(k + ntx + 1, 0, 0, nv);
+        }
+    }
+    // the 3x3 cells around (i, j), walking tiles as the reference does
+    int ct, ci, cj;
+    if (i == 0) { ct = k - 1; ci = nx - 1; }
+    else { ct = k; ci = i - 1; }
+    for (int kk = -1; kk <= 1; ++kk) {
+        if (j == 0) { ct = ct - ntx; cj = ny - 1; }
+        else cj = j - 1;
+        for (int l = -1; l <= 1; ++l) {
+            {
+                const bool act = d.tstate[ct] == KST_TILE_ACTIVE;
+                const size_t g = (size_t)ct * p.NC + cj * nx + ci;
+                double btc, gn;
+                cell_topo_ij(d, p, ct, ci, cj, act, btc, gn);
+                const double dbb = btc - d.TA.btc[g];
+                const double go = d.TA.cgam[g];
+                // intermed0's Hn (ComputeDesingularisedVariables; zero in ghost tiles)
+                double ho = 0.0;
+                if (act) {
+                    ho = compute_hn(d.SH[g], d.b0c[g], d.TA.btc[g], go);
+                    if (ho < 0.0) ho = 0.0;
+                }
+                double w = btc;
+                w = w + (ho * go / gn - dbb / gn) / gn;
+                w = w + d.b0c[g];
+                const double hp = d.SH[3 * FS + g] * go / gn - (1.0 - p.porosity) * dbb / gn;
+                if (act) {
+                    d.SM[2][g] = w;
+                    d.SM[2][FS + g] = hp;
+                } else {
+                    d.S3[g] = w;
+                    d.S3[3 * FS + g] = hp;
+                    note_dirty(d, (long long)g, ghost);
+                }
+            }
+            cj = cj + 1;
+            if (cj == ny && l < 1) { ct = ct + ntx; cj = 0; }
+        }
+        if (j + 1 >= ny) ct = ct - ntx;
+        ci = ci + 1;
+        if (ci == nx) { ct = ct + 1; ci = 0; }
+    }
+    return true;
+}
+
+// AddToRedistList + RedistributeGrid (src/Redistribute.f90:69-244): the
+// candidates the checks marked, sorted by excess deposition (ties in the
+// check loop's order: active-list position, row, column), each re-evaluated
+// and corrected in turn.  One workgroup; the list is bitonic-sorted in place.
+__global__ __launch_bounds__(256) void redistribute_kernel(Dev d, Params p) {
+    __shared__ int sh_n, sh_skip;
+    Ctrl* c = d.ctrl;
+    if (threadIdx.x == 0) {
+        int skip = (c->status != ST_NEW && c->status != ST_RETRY);
+        if (!skip) {
+            const double dt = step_dt(d, p);
+            if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
+        }
+        if (!skip && c->ncand <= 0) skip = 1;
+        sh_skip = skip;
+        sh_n = c->ncand;
+    }
+    __syncthreads();
+    if (sh_skip) return;
+    const int n = min(sh_n, d.cand_cap);
+    int m2 = 1;
+    while (m2 < n) m2 <<= 1;
+    long long* K = d.cand_key;
+    double* X = d.cand_ex;
+    for (int q = n + threadIdx.x; q < m2; q += blockDim.x) {
+        K[q] = 0x7fffffffffffffffLL;
+        X[q] = kHuge;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= m2; kk <<= 1) {
+        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+            for (int q = threadIdx.x; q < m2; q += blockDim.x) {
+                const int l = q ^ jj;
+                if (l <= q) continue;
+                const double xq = X[q], xl = X[l];
+                const long long kq = K[q], kl = K[l];
+                const bool l_less = (xl < xq) || (xl == xq && kl < kq);
+                const bool up = (q & kk) == 0;
+                if (l_less == up) {
+                    X[q] = xl; X[l] = xq;
+                    K[q] = kl; K[l] = kq;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x != 0) return;
+    long long done = 0, ghost = 0;
+    bool fail = false;
+    for (int q = 0; q < n && !fail; ++q) {
+        const long long key = K[q];
+        const int tq = (int)(key / p.NC), cell = (int)(key % p.NC);
+        const int k = d.active[tq], i = cell % p.nx, j = cell / p.nx;
+        const size_t gi = (size_t)k * p.NC + cell;
+        double ho, dbt, psio;
+        const double ex = excess_deposition(d, p, gi, d.TB.btc[gi], ho, dbt, psio);
+        if (ex > kEps) {
+            if (!redistribute_cell(d, p, k, i, j, ex, &ghost)) fail = true;
+            else ++done;
+        }
+    }
+    c->redist_cells += done;
+    c->redist_ghost += ghost;
+    if (fail) c->mref |= 2;
+}
+
+// ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),
+// plus its CFL numerators and prefactors; one active tile per block.
+__global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
+    __shared__ int sh_skip;
+    const Ctrl* c = d.ctrl;
+    if (threadIdx.x == 0) {
+        int skip = (c->status != ST_NEW && c->status != ST_RETRY);
+        if (!skip) {
+            const double dt = step_dt(d, p);
+            if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
+        }
+        if ((int)blockIdx.x >= c->n_active) skip = 1;
+        sh_skip = skip;
+    }
+    __syncthreads();
+    if (sh_skip) return;
+    const int t = d.active[blockIdx.x];
+    tile_face_topo(d, p, d.TB, t);
+    __syncthreads();
+    tile_static_terms(d, p, d.TB, t);
+}
+
+// The accepted step's bed becomes the current one (CopySolutionData
+// intermed3 -> tileContainer, src/TimeStepper.f90:263); one active tile per block.
+__global__ __launch_bounds__(256) void commit_kernel(Dev d, Params p) {
+    const Ctrl* c = d.ctrl;
+    if (!c->commit || (int)blockIdx.x >= c->n_active) return;
+    const int t = d.active[blockIdx.x];
+    const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
+    for (size_t q = threadIdx.x; q < NV; q += blockDim.x) d.TA.vbt[t * NV + q] = d.TB.vbt[t * NV + q];
+    for (int q = threadIdx.x; q < p.NC; q += blockDim.x) {
+        const size_t g = (size_t)t * p.NC + q;
+        d.TA.btc[g] = d.TB.btc[g]; d.TA.cbx[g] = d.TB.cbx[g]; d.TA.cby[g] = d.TB.cby[g];
+        d.TA.cgam[g] = d.TB.cgam[g]; d.TA.pxu[g] = d.TB.pxu[g]; d.TA.pxv[g] = d.TB.pxv[g];
+        d.TA.pyv[g] = d.TB.pyv[g];
+    }
+    for (int q = threadIdx.x; q < p.NFX; q += blockDim.x) {
+        const size_t f = (size_t)t * p.NFX + q;
+        d.TA.xbt[f] = d.TB.xbt[f]; d.TA.xby[f] = d.TB.xby[f]; d.TA.xgam[f] = d.TB.xgam[f];
+        d.TA.xib[f] = d.TB.xib[f]; d.TA.xnp[f] = d.TB.xnp[f]; d.TA.xnm[f] = d.TB.xnm[f];
+    }
+    for (int q = threadIdx.x; q < p.NFY; q += blockDim.x) {
+        const size_t f = (size_t)t * p.NFY + q;
+        d.TA.ybt[f] = d.TB.ybt[f]; d.TA.ybx[f] = d.TB.ybx[f]; d.TA.ygam[f] = d.TB.ygam[f];
+        d.TA.yib[f] = d.TB.yib[f]; d.TA.ynp[f] = d.TB.ynp[f]; d.TA.ynm[f] = d.TB.ynm[f];
+    }
+}
+
+}  // namespace kst

@@ -197,6 +197,121 @@ __device__ inline double drag(const Params& p, double hn, double u, double v, do
     }
 }
 
+// ---- morphodynamic closures ------------------------------------------------
+
+// Cell state the erosion/deposition closures read (fields Hn, u, v, psi, bt,
+// dbdx, dbdy of the reference's u vector).
+struct CellState {
+    double hn, u, v, psi, bt, bx, by;
+};
+
+// FlowShieldsNumber / FlowParticleSpeed (src/Closures.f90:208-241).
+__device__ inline double shields_number(const Params& p, const CellState& c) {
+    const double gred = p.gred / gamma_of(p.geometric, c.bx, c.by);
+    const double m2 = speed2(p.geometric, c.u, c.v, c.bx, c.by);
+    const double chezy = p.chezy * m2;
+    return chezy / (gred * p.diam);
+}
+__device__ inline double particle_speed(const Params& p, const CellState& c) {
+    const double gred = p.gred / gamma_of(p.geometric, c.bx, c.by);
+    return sqrt(gred * p.diam);
+}
+
+// FluidErosion, GranularErosion (src/Closures.f90:588-655).
+__device__ inline double erosion_fluid(const Params& p, const CellState& c) {
+    const double sh = shields_number(p, c);
+    if (sh > p.crit_shields) {
+        double e = p.ero_rate * (sh - p.crit_shields);
+        return e * particle_speed(p, c);
+    }
+    return 0.0;
+}
+__device__ inline double erosion_granular(const Params& p, const CellState& c) {
+    const double pi = 4.0 * atan(1.0);
+    const double m2 = speed2(p.geometric, c.u, c.v, c.bx, c.by);
+    const double mu1 = p.pmin;
+    const double t1 = tan(pi / 180.0);
+    const double stat = (mu1 + t1) / (1.0 - mu1 * t1);
+    const double gcos = p.g / gamma_of(p.geometric, c.bx, c.by);
+    const double mu = pouliquen_mu(p, gcos, c.hn, sqrt(m2));
+    const double mun = mu1 + (stat - mu1) / (1.0 + pow(c.hn / 25.0 / p.diam, 2.0));
+    if (mu > mun) {
+        double e = p.ero_rate_granular * (mu - mun);
+        return e * particle_speed(p, c);
+    }
+    return 0.0;
+}
+
+// ErosionClosure family (src/Closures.f90:566-675).
+__device__ inline double erosion_closure(const Params& p, const CellState& c) {
+    switch (p.erosion) {
+    case KST_EROSION_SIMPLE: {
+        double e = p.ero_rate * shields_number(p, c);
+        return e * particle_speed(p, c);
+    }
+    case KST_EROSION_FLUID: return erosion_fluid(p, c);
+    case KST_EROSION_GRANULAR: return erosion_granular(p, c);
+    case KST_EROSION_MIXED: {
+        const double fc = fswitch(p, c.psi);
+        const double fl = erosion_fluid(p, c);
+        const double gr = erosion_granular(p, c);
+        return (1.0 - fc) * fl + fc * gr;
+    }
+    default: return 0.0;
+    }
+}
+
+// ErosionTransition family (src/Closures.f90:684-732).
+__device__ inline double erosion_transition(const Params& p, double bt) {
+    switch (p.ero_transition) {
+    case KST_TRANSITION_SMOOTH: return 0.5 * (1.0 + tanh(1e5 * (bt + p.ero_depth)));
+    case KST_TRANSITION_STEP: return (bt < -p.ero_depth) ? 0.0 : 1.0;
+    default: return 1.0;
+    }
+}
+
+// DepositionClosure family (src/Closures.f90:320-356).
+__device__ inline double deposition_closure(const Params& p, double psi) {
+    switch (p.deposition) {
+    case KST_DEPOSITION_SIMPLE: return psi * (1.0 - psi / p.max_pack);
+    case KST_DEPOSITION_SPEARMAN_MANNING: {
+        const double a = 2.7 - 0.15 * p.nsettling;
+        const double b = 0.62 * p.nsettling - 1.46;
+        return psi * pow(1.0 - psi, a) * pow(1.0 - psi / p.max_pack, b);
+    }
+    default: return 0.0;
+    }
+}
+
+// MorphoDamping family (src/Closures.f90:744-791).
+__device__ inline double morpho_damping(const Params& p, double hn) {
+    switch (p.morpho_damping) {
+    case KST_DAMPING_TANH: return 0.5 * (1.0 + tanh(10.0 * log(hn / p.ero_hcrit)));
+    case KST_DAMPING_RAT3: {
+        const double hc = p.ero_hcrit;
+        if (hn < hc) return 0.0;
+        if (hn > 2.0 * hc) return 1.0;
+        const double t = hn / hc - 1.0;
+        return (t * t * t) / ((1.0 - t) * (1.0 - t) * (1.0 - t) + t * t * t);
+    }
+    default: return 1.0;
+    }
+}
+
+// ErosionDepositionTerms, Erosion, Deposition (src/Equations.f90:385-448):
+// returns E - D.
+__device__ inline double erosion_minus_deposition(const Params& p, const CellState& c) {
+    double e = erosion_closure(p, c);
+    e = e * erosion_transition(p, c.bt);
+    double alpha;
+    if (c.psi >= p.max_pack) alpha = 0.0;
+    else if (c.psi > 0.0) alpha = deposition_closure(p, c.psi);
+    else alpha = 0.0;
+    const double dep = p.ws0 * alpha;
+    const double damp = morpho_damping(p, c.hn);
+    return e * damp - dep * damp;
+}
+
 // Square root of the gravity-wave speed term of the wave speeds
 // (src/Equations.f90:249-381).
 __device__ __forceinline__ double wave_root(const Params& p, double hn, double bperp, double gam) {

@@ -39,25 +39,48 @@ enum Status { ST_NEW = 0, ST_RETRY = 1, ST_DONE = 2, ST_ERROR = 3 };
 
 struct Ctrl {
     double t, tend, dt_retry, last_dt, min_dt, max_dt;
-    unsigned long long slot[3];
+    // CFL minima (bit patterns): first hydraulic operator stages 1..3, then
+    // the second operator of a Strang step stages 1..3
+    unsigned long long slot[6];
     long long steps, rejected, rhs_evals, cell_updates;
     long long steps_left;
-    int status, err, n_active, n_ghost, mode, pad;
+    long long redist_cells;    // cells corrected by RedistributeCell
+    long long redist_ghost;    // of their writes, those into ghost tiles
+    long long ndirty;          // ghost entries of intermed3 written in this step
+    int status, err, n_active, n_ghost, mode;
+    int mref;                  // morphodynamic step refused: 1 checks, 2 redistribution
+    int ncand;                 // redistribution candidates of this attempt
+    int commit;                // the step was accepted: the new bed becomes current
+};
+
+// Bed-dependent topography of one generation, per tile.  Each tile holds its
+// own copy of the vertices it shares with its neighbours (TileType%bt,
+// src/Grid.f90:57-121); the copies may differ in the last bits because the
+// reference computes their rates with different summation orders.
+struct Topo {
+    double* vbt;                               // vertex bt [tile][NV]
+    double *btc, *cbx, *cby, *cgam;            // cell bt, db/dx, db/dy, gamma [tile][NC]
+    double *pxu, *pxv, *pyv;                   // surface-gradient prefactors [tile][NC]
+    double *xbt, *xby, *xgam, *xib;            // x-face bt, db/dy, gamma, CorrectSlopes bed [tile][NFX]
+    double *ybt, *ybx, *ygam, *yib;            // y-face bt, db/dx, gamma, CorrectSlopes bed [tile][NFY]
+    double *xnp, *xnm, *ynp, *ynm;             // CFL numerators gr^2*dx of the face's two cells
 };
 
 struct Dev {
-    double* S0;  // u^n, 4 fields
-    double* S2;  // stage 2
-    double* S3;  // stage 3
-    double* R;   // RHS of u^n: E_w, E_rhu, E_rhv, E_hnpsi, I
-    double *b0c, *btc, *cbx, *cby, *cgam;      // cell topography [tile][NC]
-    double *xb0, *xbt, *xby, *xgam;            // x-face topography [tile][NFX]
-    double *yb0, *ybt, *ybx, *ygam;            // y-face topography [tile][NFY]
-    double *xnp, *xnm, *ynp, *ynm;             // CFL numerators gr^2*dx of the face's two cells
-    double *pxu, *pxv, *pyv;                   // surface-gradient prefactors [tile][NC]
-    double *vb0, *vbt;                         // tile vertex elevations [tile][NV]
-    double *gsx, *gsy;                         // ghost w slopes [tile][NC]
-    double* maxima;                            // [11][tile][NC]
+    // ---- the hydraulic operator this launch evaluates
+    const double *Uw, *Uu, *Uv, *Uh;  // its u^n: w, rhoHnu, rhoHnv, Hnpsi
+    double* R;                        // RHS of its u^n: E_w, E_rhu, E_rhv, E_hnpsi, I
+    double* S2;                       // stage 2
+    double* S3;                       // stage 3
+    double* Sout;                     // result of the final implicit pass
+    int second;                       // 1: the second operator of a Strang step
+    int last;                         // 1: ends the step (activation flags)
+    Topo T;                           // the topography it sees
+    // ---- step state and static data
+    double* S0;                       // u^n of the step (tileContainer), 4 fields
+    double *b0c, *xb0, *yb0, *vb0;    // b0 at cells, x-/y-faces, vertices [tile][...]
+    double *gsx, *gsy;                // ghost w slopes [tile][NC]
+    double* maxima;                   // [11][tile][NC]
     uint8_t* tstate;
     uint8_t* has_src;
     int* active;
@@ -68,6 +91,25 @@ struct Dev {
     double *src_x, *src_y, *src_r, *src_t, *src_q, *src_p;
     int *src_n, *src_off, *src_cells;
     long long FS;  // field stride = ntiles*NC
+    // ---- morphodynamics (src/TimeStepper.f90:532-781)
+    double* SH;                       // u after the first hydraulic operator, 4 fields
+    double* SM[3];                    // w, Hnpsi after morphodynamic stages 1..3
+    double* UV;                       // velocities the morphodynamic RHS reads, 2 fields
+    double* R2;                       // RHS of the second operator's u^n
+    Topo TA;                          // bed at the start of the step (tileContainer)
+    Topo TM[2];                       // bed after morphodynamic stages 1 and 2
+    Topo TB;                          // bed after the morphodynamic operator
+    long long* cand_key;              // redistribution candidates: scan-order key
+    double* cand_ex;                  // and excess deposition
+    int cand_cap;
+    int morpho;
+    // Redistribution may write ghost tiles' copies of shared vertices and
+    // ghost cells' w, Hnpsi into intermed3 (src/Redistribute.f90:362-470); the
+    // reference keeps them until the next step refreshes intermed3 from the
+    // tileContainer.  Here they live in TB.vbt and S3 and are listed for
+    // that refresh: bit 62 set = vertex tile*NV + v, else cell tile*NC + c.
+    long long* dirty;
+    int dirty_cap;
 };
 
 // ---------------------------------------------------------------------------
@@ -94,14 +136,23 @@ __device__ double advised_from_slot(const Params& p, unsigned long long slot, in
     return dmin(m, p.max_dt);
 }
 
-// The step's dt: substep-1 advice clamped to the next output/flux-series time
+// The step's (hydraulic) dt: substep-1 advice clamped to the next
+// output/flux-series time, or to half of it with morphodynamics
 // (src/TimeStepper.f90:155-168), or the refined dt of a retried step.
 __device__ double step_dt(const Dev& d, const Params& p) {
     const Ctrl* c = d.ctrl;
     if (c->status == ST_RETRY) return c->dt_retry;
     double adv = 0.9 * advised_from_slot(p, c->slot[0], c->n_active);
     double tmax = dmin(c->tend, next_flux_time(d, p, c->t));
-    return dmin(adv, tmax - c->t);
+    if (!d.morpho) return dmin(adv, tmax - c->t);
+    return dmin(adv, 0.5 * (tmax - c->t));
+}
+
+// The first hydraulic operator of the attempt passed both CFL re-checks
+// (src/TimeStepper.f90:393,452).
+__device__ __forceinline__ bool first_operator_ok(const Ctrl* c, const Params& p, double dt) {
+    return advised_from_slot(p, c->slot[1], c->n_active) >= dt &&
+           advised_from_slot(p, c->slot[2], c->n_active) >= dt;
 }
 
 // ---------------------------------------------------------------------------
@@ -173,16 +224,19 @@ __device__ __forceinline__ double lim(double a, double b) {
 
 template <int NQ>
 struct SmemT {
-    double w[NREG], h[NREG], u[NREG], v[NREG], rho[NREG], gc[NREG];
-    double xbed[B][B + 3];   // bed of x-faces -1..B+1 of each row (relative)
-    double ybed[B][B + 3];   // bed of y-faces -1..B+1 of each column
+    double w[NREG], h[NREG], u[NREG], v[NREG], rho[NREG];
+    double wst[NREG];        // w behind the ghost tiles' stored face values (their tileContainer w)
+    double xbed[B][B + 3];   // CorrectSlopes bed of x-faces -1..B+1 of each row (relative)
+    double ybed[B][B + 3];   // CorrectSlopes bed of y-faces -1..B+1 of each column
+    double xside[2][B];      // the W / E tile's own bed at this tile's edge x-faces
+    double yside[2][B];      // the S / N tile's own bed at this tile's edge y-faces
     int gidx[NREG];
     uint8_t act[NREG];
     double fx[NQ][NFACE];    // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
     double fy[NQ][NFACE];
     double red[NTHREADS / 64];
     int tile, bx0, by0, bxa, bya, skip;
-    double dt, tstage;
+    double dt, tstage, tgrid;
 };
 
 __device__ __forceinline__ int rg(int rx, int ry) { return (ry + 2) * RW + (rx + 2); }
@@ -234,19 +288,25 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     const double sRr = actR ? dxr * lim<LIM>(rRR - rR, rR - rL) : 0.0;
 
     // --- MUSCL face values (Reconstruct, src/HydraulicRHS.f90:397-490)
-    const double wEL = wL + sLw * 0.5 * dx, wWL = wL - sLw * 0.5 * dx;   // L's east / west
-    const double wWR = wR - sRw * 0.5 * dx, wER = wR + sRw * 0.5 * dx;   // R's west / east
+    const double wEL = wL + sLw * 0.5 * dx;   // L's east face
+    const double wWR = wR - sRw * 0.5 * dx;   // R's west face
 
-    // beds of the faces of L and R (InterpolateB == face b0 when bt = 0,
-    // src/HydraulicRHS.f90:741-753); the face itself is entry fi + 1
+    // beds of the faces of L and R (InterpolateB, src/HydraulicRHS.f90:741-753);
+    // the face itself is entry fi + 1, except at a tile edge, where the cell
+    // on the far side sees its own tile's copy of the shared vertices
     const double* bed = YDIR ? &s.ybed[fj][0] : &s.xbed[fj][0];
-    const double bWL = bed[fi], fb0 = bed[fi + 1], bER = bed[fi + 2];
+    const double bWL = bed[fi], bER = bed[fi + 2];
+    const double fbL = (fpos == 0) ? (YDIR ? s.yside[0][fj] : s.xside[0][fj]) : bed[fi + 1];
+    const double fbR = (fpos == nn) ? (YDIR ? s.yside[1][fj] : s.xside[1][fj]) : bed[fi + 1];
 
     // --- CorrectSlopes for w (src/HydraulicRHS.f90:560-736): each cell's
     // pair of face values is replaced by the bed slope if either is below bed.
+    // A ghost cell is tested with the face values ReconstructwAtEdges stored
+    // from its tileContainer w (src/UpdateTiles.f90:669-750).
+    const double wLc = s.wst[iL], wRc = s.wst[iR];
     double wFL = wEL, wFR = wWR;
-    if ((wEL < fb0) || (wWL < bWL)) wFL = wL + 0.5 * (fb0 - bWL);
-    if ((wER < bER) || (wWR < fb0)) wFR = wR + 0.5 * (fb0 - bER);
+    if ((wLc + sLw * 0.5 * dx < fbL) || (wLc - sLw * 0.5 * dx < bWL)) wFL = wL + 0.5 * (fbL - bWL);
+    if ((wRc + sRw * 0.5 * dx < bER) || (wRc - sRw * 0.5 * dx < fbR)) wFR = wR + 0.5 * (fbR - bER);
     // Hnpsi: inside the tile both faces of a cell are reset; across the
     // tile edge only the copy this tile holds is checked.
     double hFL = hL + sLh * 0.5 * dx, hFR = hR - sRh * 0.5 * dx;
@@ -269,15 +329,14 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     // face topography of this tile
     const int t = s.tile;
     int fidx;
-    double fbp, fg;
+    double fbp, fg, fb0, fbt;
     if (!YDIR) {
         fidx = t * p.NFX + (s.by0 + fj) * (p.nx + 1) + fpos;
-        fbp = d.xby[fidx]; fg = d.xgam[fidx];
+        fbp = d.T.xby[fidx]; fg = d.T.xgam[fidx]; fb0 = d.xb0[fidx]; fbt = d.T.xbt[fidx];
     } else {
         fidx = t * p.NFY + fpos * p.nx + (s.bx0 + fj);
-        fbp = d.ybx[fidx]; fg = d.ygam[fidx];
+        fbp = d.T.ybx[fidx]; fg = d.T.ygam[fidx]; fb0 = d.yb0[fidx]; fbt = d.T.ybt[fidx];
     }
-    const double fbt = 0.0;
     // face Hn from the corrected w (src/HydraulicRHS.f90:492-517)
     const double HnL = compute_hn(wFL, fb0, fbt, fg);
     const double HnR = compute_hn(wFR, fb0, fbt, fg);
@@ -296,8 +355,8 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     if (aNeg > 0.0) aNeg = 0.0;
     if (CFL) {
         // numerators (min(gam_cell/gam_face,1))^2 * dx are static topography
-        if (aPos > kEps) cfl_min = fmin((YDIR ? d.ynp : d.xnp)[fidx] / aPos, cfl_min);
-        if (fabs(aNeg) > kEps) cfl_min = fmin((YDIR ? d.ynm : d.xnm)[fidx] / fabs(aNeg), cfl_min);
+        if (aPos > kEps) cfl_min = fmin((YDIR ? d.T.ynp : d.T.xnp)[fidx] / aPos, cfl_min);
+        if (fabs(aNeg) > kEps) cfl_min = fmin((YDIR ? d.T.ynm : d.T.xnm)[fidx] / fabs(aNeg), cfl_min);
     }
 
     // --- fluxes (src/HydraulicRHS.f90:1027-1067, src/Equations.f90:53-245)
@@ -361,27 +420,45 @@ __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
     }
 }
 
-// Bed of a face given by its tile-relative index K in [-1, nn+1] along the
-// normal and its row/column r (single-valued b0: a tile's edge faces equal
-// its neighbours').
+// InterpolateB (src/HydraulicRHS.f90:741-753) of tile tn's copy of vertices
+// (i0, j0) and (i1, j1) in the bed the morphodynamic operator left (TB), which
+// is intermed3's: a ghost tile's copy there may carry redistribution writes.
+__device__ __forceinline__ double ghost_bed(const Dev& d, const Params& p, int tn, int i0, int j0,
+                                           int i1, int j1) {
+    const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
+    const size_t a = (size_t)tn * NV + j0 * (p.nx + 1) + i0, b = (size_t)tn * NV + j1 * (p.nx + 1) + i1;
+    double ib = d.TB.vbt[a] + d.TB.vbt[b];
+    ib = ib + d.vb0[a];
+    ib = ib + d.vb0[b];
+    return ib * 0.5;
+}
+
+// CorrectSlopes bed of a face given by its tile-relative index K in
+// [-1, nn+1] along the normal and its row/column r, as tile t sees it
+// (faces beyond the tile edge belong to the neighbour; they do not touch the
+// shared vertices, so every tile sees the same value there).
 template <bool YDIR>
 __device__ __forceinline__ double face_bed(const Dev& d, const Params& p, int t, int K, int r) {
     if (!YDIR) {
-        if (K < 0) return d.xb0[(t - 1) * p.NFX + r * (p.nx + 1) + (p.nx - 1)];
-        if (K > p.nx) return d.xb0[(t + 1) * p.NFX + r * (p.nx + 1) + 1];
-        return d.xb0[t * p.NFX + r * (p.nx + 1) + K];
+        if (K < 0) return d.T.xib[(t - 1) * p.NFX + r * (p.nx + 1) + (p.nx - 1)];
+        if (K > p.nx) return d.T.xib[(t + 1) * p.NFX + r * (p.nx + 1) + 1];
+        return d.T.xib[t * p.NFX + r * (p.nx + 1) + K];
     } else {
-        if (K < 0) return d.yb0[(t - p.ntx) * p.NFY + (p.ny - 1) * p.nx + r];
-        if (K > p.ny) return d.yb0[(t + p.ntx) * p.NFY + 1 * p.nx + r];
-        return d.yb0[t * p.NFY + K * p.nx + r];
+        if (K < 0) return d.T.yib[(t - p.ntx) * p.NFY + (p.ny - 1) * p.nx + r];
+        if (K > p.ny) return d.T.yib[(t + p.ntx) * p.NFY + 1 * p.nx + r];
+        return d.T.yib[t * p.NFY + K * p.nx + r];
     }
 }
 
-// STAGE 1: RHS(u^n) -> R.            (IntegrateTo, src/TimeStepper.f90:155)
+// STAGE 1: RHS(u^n) -> R.            (IntegrateTo, src/TimeStepper.f90:155,220)
 // STAGE 2: u1 = f(u^n, R); RHS(u1); u2 -> S2.   (src/TimeStepper.f90:371-444)
 // STAGE 3: RHS(u2); u3 -> S3.                   (src/TimeStepper.f90:447-498)
-// STAGE 4: RHS(u3); implicit drag -> S0; maxima; activation flags.
+// STAGE 4: RHS(u3); implicit drag -> Sout; maxima; activation flags.
 //                                               (src/TimeStepper.f90:501-524)
+// With morphodynamics the same kernels evaluate both hydraulic operators of
+// the Strang step H(dt) M(2dt) H(dt); the second starts at t + dt from the
+// morphodynamic result and first checks its substep-1 advice
+// (src/TimeStepper.f90:214-258).
 template <int STAGE, bool VISC, int LIM>
 __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     __shared__ SmemT<VISC ? 7 : 5> s;
@@ -390,15 +467,20 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         const Ctrl* c = d.ctrl;
         int skip = 0;
         if (c->status == ST_DONE || c->status == ST_ERROR) skip = 1;
-        if (STAGE == 1 && c->status != ST_NEW) skip = 1;
+        if (STAGE == 1 && !d.second && c->status != ST_NEW) skip = 1;
+        const int sb = d.second ? 3 : 0;
         double dt = 0.0;
-        if (!skip && STAGE >= 2) {
+        if (!skip && (STAGE >= 2 || d.second)) {
             dt = step_dt(d, p);
-            if (STAGE >= 3 && advised_from_slot(p, c->slot[1], c->n_active) < dt) skip = 1;
-            if (STAGE >= 4 && advised_from_slot(p, c->slot[2], c->n_active) < dt) skip = 1;
+            if (d.second && (!first_operator_ok(c, p, dt) || c->mref)) skip = 1;
+            if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3], c->n_active) < dt) skip = 1;
+            if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1], c->n_active) < dt) skip = 1;
+            if (STAGE >= 4 && advised_from_slot(p, c->slot[sb + 2], c->n_active) < dt) skip = 1;
         }
         s.dt = dt;
-        s.tstage = (STAGE == 1) ? c->t : (STAGE == 3 ? c->t + 0.5 * dt : c->t + dt);
+        const double tb = d.second ? c->t + dt : c->t;   // grid%t of this operator
+        s.tgrid = tb;
+        s.tstage = (STAGE == 1) ? tb : (STAGE == 3 ? tb + 0.5 * dt : tb + dt);
         // work item -> (tile, sub-block); contiguous chunks of the active
         // list go to the same XCD (blocks are dealt round-robin to 8 XCDs)
         const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
@@ -428,7 +510,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     const int tx = tid % B, ty = tid / B;
     const bool own = (tx < bxa) && (ty < bya);
     double ow = 0, oru = 0, orv = 0, oh = 0, ohn = 0, ou = 0, ov = 0, opsi = 0, orho = 0;
-    double ob0 = 0, og = 1;
+    double ob0 = 0, obt = 0, og = 1;
     double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stage 2)
     int oidx = 0;
     for (int round = 0; round < 2; ++round) {
@@ -450,7 +532,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         const int gi = tc * p.NC + cj * p.nx + ci;
         double w, ru, rv, h;
         if (STAGE == 1 || STAGE == 2) {
-            w = d.S0[gi]; ru = d.S0[FS + gi]; rv = d.S0[2 * FS + gi]; h = d.S0[3 * FS + gi];
+            w = d.Uw[gi]; ru = d.Uu[gi]; rv = d.Uv[gi]; h = d.Uh[gi];
             if (STAGE == 2) {
                 if (round == 0) { pw = w; pru = ru; prv = rv; ph = h; }
                 const double ew = d.R[gi], eu = d.R[FS + gi], ev = d.R[2 * FS + gi];
@@ -464,25 +546,31 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             const double* S = (STAGE == 3) ? d.S2 : d.S3;
             w = S[gi]; ru = S[FS + gi]; rv = S[2 * FS + gi]; h = S[3 * FS + gi];
         }
-        const double b0c = d.b0c[gi], gam = d.cgam[gi];
-        // ComputeDesingularisedVariables (src/HydraulicRHS.f90:762-878)
-        double hn = compute_hn(w, b0c, 0.0, gam);
-        if (hn < 0.0) hn = 0.0;
-        double hpc = h;
-        if (hpc < 0.0) hpc = 0.0;
-        const double eps = p.hthr;
-        const double den = hn * hn + dmax(hn * hn, eps * eps);
-        const double psi = dmin(2.0 * hn * hpc / den, p.max_pack);
-        const double rho = density(p, psi);
-        const double uu = 2.0 * hn * ru / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-        const double vv = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+        const double b0c = d.b0c[gi], btc = d.T.btc[gi], gam = d.T.cgam[gi];
+        const bool act = d.tstate[tc] == KST_TILE_ACTIVE;
+        // ComputeDesingularisedVariables (src/HydraulicRHS.f90:762-878) runs
+        // on active tiles only; ghost tiles keep SetDefaultTileData's zeros
+        double hn = 0.0, psi = 0.0, rho = p.rhow, uu = 0.0, vv = 0.0;
+        if (act) {
+            hn = compute_hn(w, b0c, btc, gam);
+            if (hn < 0.0) hn = 0.0;
+            double hpc = h;
+            if (hpc < 0.0) hpc = 0.0;
+            const double eps = p.hthr;
+            const double den = hn * hn + dmax(hn * hn, eps * eps);
+            psi = dmin(2.0 * hn * hpc / den, p.max_pack);
+            rho = density(p, psi);
+            uu = 2.0 * hn * ru / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+            vv = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+        }
         const int r = rg(rx, ry);
-        s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho; s.gc[r] = gam;
+        s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho;
+        s.wst[r] = act ? w : d.S0[gi];
         s.gidx[r] = gi;
-        s.act[r] = (d.tstate[tc] == KST_TILE_ACTIVE) ? 1 : 0;
+        s.act[r] = act ? 1 : 0;
         if (round == 0) {
             ow = w; oru = ru; orv = rv; oh = h; ohn = hn; ou = uu; ov = vv; opsi = psi; orho = rho;
-            ob0 = b0c; og = gam; oidx = gi;
+            ob0 = b0c; obt = btc; og = gam; oidx = gi;
         }
     }
     // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
@@ -490,6 +578,43 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         const int r = q / (B + 3), k = q % (B + 3) - 1;
         if (r < bya && k <= bxa + 1) s.xbed[r][k + 1] = face_bed<false>(d, p, t, s.bx0 + k, s.by0 + r);
         if (r < bxa && k <= bya + 1) s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
+    }
+    // the neighbour tiles' own beds at the shared edge faces; in the last
+    // stage a ghost neighbour's copy of the shared vertices is the one the
+    // redistribution may have written (intermed3, src/Redistribute.f90:362-410)
+    if (tid < B) {
+        const int r = tid;
+        const bool ovl = STAGE == 4 && d.morpho;
+        if (r < bya) {
+            const int row = (s.by0 + r) * (p.nx + 1);
+            if (s.bx0 == 0) {
+                const int tn = t - 1;
+                s.xside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
+                                    ? ghost_bed(d, p, tn, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
+                                    : d.T.xib[tn * p.NFX + row + p.nx];
+            }
+            if (s.bx0 + bxa == p.nx) {
+                const int tn = t + 1;
+                s.xside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
+                                    ? ghost_bed(d, p, tn, 0, s.by0 + r, 0, s.by0 + r + 1)
+                                    : d.T.xib[tn * p.NFX + row];
+            }
+        }
+        if (r < bxa) {
+            const int col = s.bx0 + r;
+            if (s.by0 == 0) {
+                const int tn = t - p.ntx;
+                s.yside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
+                                    ? ghost_bed(d, p, tn, col, p.ny, col + 1, p.ny)
+                                    : d.T.yib[tn * p.NFY + p.ny * p.nx + col];
+            }
+            if (s.by0 + bya == p.ny) {
+                const int tn = t + p.ntx;
+                s.yside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
+                                    ? ghost_bed(d, p, tn, col, 0, col + 1, 0)
+                                    : d.T.yib[tn * p.NFY + col];
+            }
+        }
     }
     __syncthreads();
 
@@ -508,7 +633,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             face_flux<true, VISC, LIM, STAGE != 4>(d, p, s, f / bxa, f % bxa, cflm);
     }
     if (STAGE != 4) {
-        reduce_min_to(s, cflm, &d.ctrl->slot[STAGE - 1]);
+        reduce_min_to(s, cflm, &d.ctrl->slot[(d.second ? 3 : 0) + STAGE - 1]);
     } else {
         __syncthreads();
     }
@@ -521,9 +646,9 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
 #define FX(q, o) FXp[(q) * NFACE + xs + (o)]
 #define FY(q, o) FYp[(q) * NFACE + ys + (o)]
     const double gam = og;
-    const double bx = d.cbx[oidx], by = d.cby[oidx];
-    // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are static topography
-    const double gxu = d.pxu[oidx], gxv = d.pxv[oidx], gyu = gxv, gyv = d.pyv[oidx];
+    const double bx = d.T.cbx[oidx], by = d.T.cby[oidx];
+    // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are per-bed topography
+    const double gxu = d.T.pxu[oidx], gxv = d.T.pxv[oidx], gyu = gxv, gyv = d.T.pyv[oidx];
     // ConstructHydraulicRHS (src/HydraulicRHS.f90:1228-1271)
     const double stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
     const double sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
@@ -561,14 +686,14 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         const int i = s.bx0 + tx, j = s.by0 + ty;
         const double x = -0.5 * p.x_size + p.dx * (((ti + 1) - 1.0) * p.nx + ((i + 1) - 0.5));
         const double y = -0.5 * p.y_size + p.dy * (((tj + 1) - 1.0) * p.ny + ((j + 1) - 0.5));
-        flux_sources(d, p, s.tstage, d.ctrl->t, x, y, Qt, psiQt);
+        flux_sources(d, p, s.tstage, s.tgrid, x, y, Qt, psiQt);
         Ew = stw + (0.0 + Qt / (gam * gam));
         Eh = sth + (0.0 + psiQt / gam);
     } else {
         Ew = stw + 0.0;
         Eh = sth + 0.0;
     }
-    double hpg = -0.0;
+    double hpg = -obt;
     hpg = hpg + (ow - ob0);
     hpg = hpg / gam;
     const double Eu = stu + (0.0 - p.g * orho * hpg * bx);
@@ -591,11 +716,11 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         // second SSP substep (src/TimeStepper.f90:407-444)
         const double u2 = 0.75 * pru + 0.25 * (oru + dt * Eu) / (1.0 - dt * Im);
         const double v2 = 0.75 * prv + 0.25 * (orv + dt * Ev) / (1.0 - dt * Im);
-        double hpo = -0.0;
+        double hpo = -obt;
         hpo = hpo + (pw - ob0);
-        double hpn = -0.0;
+        double hpn = -obt;
         hpn = hpn + (ow - ob0);
-        double wu = 0.0;
+        double wu = obt;
         wu = wu + 0.25 * hpn;
         wu = wu + 0.75 * hpo;
         wu = wu + 0.25 * dt * Ew;
@@ -604,16 +729,16 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         d.S2[oidx] = wu; d.S2[FS + oidx] = u2; d.S2[2 * FS + oidx] = v2; d.S2[3 * FS + oidx] = h2;
     } else if (STAGE == 3) {
         // third SSP substep (src/TimeStepper.f90:466-498)
-        const double w0 = d.S0[oidx], u0 = d.S0[FS + oidx], v0 = d.S0[2 * FS + oidx];
-        const double h0 = d.S0[3 * FS + oidx];
+        const double w0 = d.Uw[oidx], u0 = d.Uu[oidx], v0 = d.Uv[oidx];
+        const double h0 = d.Uh[oidx];
         const double third = 1.0 / 3.0, twothird = 2.0 / 3.0;
         const double u3 = third * u0 + twothird * (oru + dt * Eu) / (1.0 - dt * Im);
         const double v3 = third * v0 + twothird * (orv + dt * Ev) / (1.0 - dt * Im);
-        double hpo = -0.0;
+        double hpo = -obt;
         hpo = hpo + (w0 - ob0);
-        double hpn = -0.0;
+        double hpn = -obt;
         hpn = hpn + (ow - ob0);
-        double wu = 0.0;
+        double wu = obt;
         wu = wu + third * hpo;
         wu = wu + twothird * hpn;
         wu = wu + twothird * dt * Ew;
@@ -624,11 +749,17 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         // final implicit substep (src/TimeStepper.f90:504-518)
         const double un = (oru - dt * dt * Eu * Im) / (1.0 + dt * dt * Im * Im);
         const double vn = (orv - dt * dt * Ev * Im) / (1.0 + dt * dt * Im * Im);
-        if (p.track_maxima) {
-            // UpdateMaximum* on the pre-step state at nextT (src/TimeStepper.f90:1155-1303)
+        if (p.track_maxima && !d.second) {
+            // UpdateMaximum* on the pre-step state at nextT (src/TimeStepper.f90:1155-1303);
+            // the second operator of a Strang step repeats the first one's
+            // update with the same state, which changes nothing
             const double w0 = d.S0[oidx], u0 = d.S0[FS + oidx], v0 = d.S0[2 * FS + oidx];
             const double h0 = d.S0[3 * FS + oidx];
-            double hn = compute_hn(w0, ob0, 0.0, gam);
+            const double tn = d.ctrl->t + dt;
+            double* M = d.maxima;
+            if (obt < 0.0 && -obt > M[4 * FS + oidx]) { M[4 * FS + oidx] = -obt; M[5 * FS + oidx] = tn; }
+            if (obt > 0.0 && obt > M[6 * FS + oidx]) { M[6 * FS + oidx] = obt; M[7 * FS + oidx] = tn; }
+            double hn = compute_hn(w0, ob0, obt, gam);
             if (hn < 0.0) hn = 0.0;
             if (hn > p.hthr) {
                 double hp0 = h0;
@@ -639,8 +770,6 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
                 const double rho = density(p, psi);
                 const double uu = 2.0 * hn * u0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
                 const double vv = 2.0 * hn * v0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-                const double tn = d.ctrl->t + dt;
-                double* M = d.maxima;
                 if (M[10 * FS + oidx] == -1) M[10 * FS + oidx] = tn;
                 if (hn > M[oidx]) { M[oidx] = hn; M[FS + oidx] = tn; }
                 const double spd = sqrt(speed2(p.geometric, uu, vv, bx, by));
@@ -648,10 +777,10 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
                 if (psi > M[8 * FS + oidx]) { M[8 * FS + oidx] = psi; M[9 * FS + oidx] = tn; }
             }
         }
-        d.S0[oidx] = ow; d.S0[FS + oidx] = un; d.S0[2 * FS + oidx] = vn; d.S0[3 * FS + oidx] = oh;
+        d.Sout[oidx] = ow; d.Sout[FS + oidx] = un; d.Sout[2 * FS + oidx] = vn; d.Sout[3 * FS + oidx] = oh;
         // activation flags from this state's Hn (CheckIfNearBoundaries,
         // src/TimeStepper.f90:946-1150): bit 0 N, 1 S, 2 E, 3 W
-        if (ohn > p.hthr) {
+        if (d.last && ohn > p.hthr) {
             const int i = s.bx0 + tx, j = s.by0 + ty, tb = p.tile_buffer;
             unsigned f = 0;
             if (j + 1 > p.ny - tb) f |= 1u;
@@ -660,6 +789,159 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             if (i + 1 <= tb) f |= 8u;
             if (f) atomicOr(&d.eflags[t], f);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// topography
+// ---------------------------------------------------------------------------
+
+// Cell and face topography of tile t from its vertex elevations
+// (ComputeCellCentredTopographicData / ComputeInterfacialTopographicData,
+// src/MorphodynamicRHS.f90:308-578; face b0 of GetHeights, src/dem.f90:378-386;
+// CorrectSlopes beds of InterpolateB, src/HydraulicRHS.f90:741-753).  Block-wide.
+#define V(a, i, j) a[(j) * NV1 + (i)]
+__device__ void tile_cell_topo(const Dev& d, const Params& p, const Topo& T, int t, bool zero_btc) {
+    const int nx = p.nx, NV1 = nx + 1;
+    const size_t NV = (size_t)(nx + 1) * (p.ny + 1);
+    const double* b = d.vb0 + (size_t)t * NV;
+    const double* bt = T.vbt + (size_t)t * NV;
+    for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
+        const int i = c % nx, j = c / nx;
+        Kahan k;
+        k.add(V(b, i, j)); k.add(V(b, i + 1, j)); k.add(V(b, i, j + 1)); k.add(V(b, i + 1, j + 1));
+        const double b0c = 0.25 * k.s;
+        Kahan kt;
+        kt.add(V(bt, i, j)); kt.add(V(bt, i + 1, j)); kt.add(V(bt, i, j + 1)); kt.add(V(bt, i + 1, j + 1));
+        const double btc = zero_btc ? 0.0 : 0.25 * kt.s;
+        Kahan kx;
+        kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
+        kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1)); kx.add(-V(b, i, j + 1)); kx.add(-V(bt, i, j + 1));
+        const double dbdx = 0.5 * p.dxr * kx.s;
+        Kahan ky;
+        ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1)); ky.add(-V(b, i, j)); ky.add(-V(bt, i, j));
+        ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(-V(b, i + 1, j)); ky.add(-V(bt, i + 1, j));
+        const double dbdy = 0.5 * p.dyr * ky.s;
+        const size_t gi = (size_t)t * p.NC + c;
+        d.b0c[gi] = b0c; T.btc[gi] = btc; T.cbx[gi] = dbdx; T.cby[gi] = dbdy;
+        T.cgam[gi] = gamma_of(p.geometric, dbdx, dbdy);
+    }
+}
+
+__device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int t) {
+    const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
+    const int ti = t % p.ntx, tj = t / p.ntx;
+    // neighbour tiles (clamped at the domain edge: their values are never read)
+    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
+    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
+    const size_t NV = (size_t)(nx + 1) * (ny + 1);
+    const double* b = d.vb0 + (size_t)t * NV;
+    const double* bt = T.vbt + (size_t)t * NV;
+    const double* bW = d.vb0 + (size_t)tW * NV; const double* btW = T.vbt + (size_t)tW * NV;
+    const double* bE = d.vb0 + (size_t)tE * NV; const double* btE = T.vbt + (size_t)tE * NV;
+    const double* bS = d.vb0 + (size_t)tS * NV; const double* btS = T.vbt + (size_t)tS * NV;
+    const double* bN = d.vb0 + (size_t)tN * NV; const double* btN = T.vbt + (size_t)tN * NV;
+    // x-faces: i in [0, nx], j in [0, ny)
+    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
+        const int i = f % (nx + 1), j = f / (nx + 1);
+        Kahan kx;
+        if (i >= 1 && i <= nx - 1) {
+            kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1));
+            kx.add(-V(b, i - 1, j)); kx.add(-V(bt, i - 1, j)); kx.add(-V(b, i - 1, j + 1)); kx.add(-V(bt, i - 1, j + 1));
+        } else if (i == 0) {
+            kx.add(V(b, 1, j)); kx.add(V(bt, 1, j)); kx.add(V(b, 1, j + 1)); kx.add(V(bt, 1, j + 1));
+            kx.add(-V(bW, nx - 1, j)); kx.add(-V(btW, nx - 1, j)); kx.add(-V(bW, nx - 1, j + 1)); kx.add(-V(btW, nx - 1, j + 1));
+        } else {
+            kx.add(V(bE, 1, j)); kx.add(V(btE, 1, j)); kx.add(V(bE, 1, j + 1)); kx.add(V(btE, 1, j + 1));
+            kx.add(-V(b, nx - 1, j)); kx.add(-V(bt, nx - 1, j)); kx.add(-V(b, nx - 1, j + 1)); kx.add(-V(bt, nx - 1, j + 1));
+        }
+        const double dbdx = 0.25 * p.dxr * kx.s;
+        const double btf = 0.5 * (V(bt, i, j) + V(bt, i, j + 1));
+        Kahan ky;
+        ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1)); ky.add(-V(b, i, j)); ky.add(-V(bt, i, j));
+        const double dbdy = p.dyr * ky.s;
+        double ib = V(bt, i, j) + V(bt, i, j + 1);
+        ib = ib + V(b, i, j);
+        ib = ib + V(b, i, j + 1);
+        ib = ib * 0.5;
+        const size_t fi = (size_t)t * p.NFX + f;
+        d.xb0[fi] = 0.5 * (V(b, i, j) + V(b, i, j + 1));
+        T.xbt[fi] = btf;
+        T.xby[fi] = dbdy;
+        T.xgam[fi] = gamma_of(p.geometric, dbdx, dbdy);
+        T.xib[fi] = ib;
+    }
+    // y-faces: i in [0, nx), j in [0, ny]
+    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
+        const int i = f % nx, j = f / nx;
+        Kahan kx;
+        kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
+        const double dbdx = p.dxr * kx.s;
+        const double btf = 0.5 * (V(bt, i, j) + V(bt, i + 1, j));
+        Kahan ky;
+        if (j >= 1 && j <= ny - 1) {
+            ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1));
+            ky.add(-V(b, i + 1, j - 1)); ky.add(-V(bt, i + 1, j - 1)); ky.add(-V(b, i, j - 1)); ky.add(-V(bt, i, j - 1));
+        } else if (j == 0) {
+            ky.add(V(b, i + 1, 1)); ky.add(V(bt, i + 1, 1)); ky.add(V(b, i, 1)); ky.add(V(bt, i, 1));
+            ky.add(-V(bS, i + 1, ny - 1)); ky.add(-V(btS, i + 1, ny - 1)); ky.add(-V(bS, i, ny - 1)); ky.add(-V(btS, i, ny - 1));
+        } else {
+            ky.add(V(bN, i + 1, 1)); ky.add(V(btN, i + 1, 1)); ky.add(V(bN, i, 1)); ky.add(V(btN, i, 1));
+            ky.add(-V(b, i + 1, ny - 1)); ky.add(-V(bt, i + 1, ny - 1)); ky.add(-V(b, i, ny - 1)); ky.add(-V(bt, i, ny - 1));
+        }
+        const double dbdy = 0.25 * p.dyr * ky.s;
+        double ib = V(bt, i, j) + V(bt, i + 1, j);
+        ib = ib + V(b, i, j);
+        ib = ib + V(b, i + 1, j);
+        ib = ib * 0.5;
+        const size_t fi = (size_t)t * p.NFY + f;
+        d.yb0[fi] = 0.5 * (V(b, i, j) + V(b, i + 1, j));
+        T.ybt[fi] = btf;
+        T.ybx[fi] = dbdx;
+        T.ygam[fi] = gamma_of(p.geometric, dbdx, dbdy);
+        T.yib[fi] = ib;
+    }
+}
+#undef V
+
+// Per-face CFL numerators and per-cell surface-gradient prefactors of tile t.
+// gam_ratio = min(gam(cell)/gam(face), 1) of the cell on each side of a face
+// (src/HydraulicRHS.f90:983-1006) enters the CFL as (gr*gr)*dx; the
+// prefactors of ConstructHydraulicRHS (src/HydraulicRHS.f90:1233-1243).
+// Needs the cell gamma of the neighbour tiles.
+__device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, int t) {
+    const int nx = p.nx, ny = p.ny;
+    const int ti = t % p.ntx, tj = t / p.ntx;
+    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
+    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
+    for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
+        const int gi = t * p.NC + c;
+        const double bx = T.cbx[gi], by = T.cby[gi], gam = T.cgam[gi];
+        if (p.geometric) {
+            T.pxu[gi] = (1.0 + by * by) / gam;
+            T.pxv[gi] = -(bx * by / gam);
+            T.pyv[gi] = (1.0 + bx * bx) / gam;
+        } else {
+            T.pxu[gi] = 1.0; T.pxv[gi] = 0.0; T.pyv[gi] = 1.0;
+        }
+    }
+    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
+        const int i = f % (nx + 1), j = f / (nx + 1);
+        const double fg = T.xgam[t * p.NFX + f];
+        const double gl = (i == 0) ? T.cgam[tW * p.NC + j * nx + nx - 1] : T.cgam[t * p.NC + j * nx + i - 1];
+        const double gr_ = (i == nx) ? T.cgam[tE * p.NC + j * nx] : T.cgam[t * p.NC + j * nx + i];
+        const double a = fmin(gl / fg, 1.0), b = fmin(gr_ / fg, 1.0);
+        T.xnp[t * p.NFX + f] = a * a * p.dx;
+        T.xnm[t * p.NFX + f] = b * b * p.dx;
+    }
+    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
+        const int i = f % nx, j = f / nx;
+        const double fg = T.ygam[t * p.NFY + f];
+        const double gl = (j == 0) ? T.cgam[tS * p.NC + (ny - 1) * nx + i] : T.cgam[t * p.NC + (j - 1) * nx + i];
+        const double gr_ = (j == ny) ? T.cgam[tN * p.NC + i] : T.cgam[t * p.NC + j * nx + i];
+        const double a = fmin(gl / fg, 1.0), b = fmin(gr_ / fg, 1.0);
+        T.ynp[t * p.NFY + f] = a * a * p.dy;
+        T.ynm[t * p.NFY + f] = b * b * p.dy;
     }
 }
 
@@ -711,14 +993,19 @@ __device__ void dry_tile(const Dev& d, const Params& p, int k, bool active) {
     const long long FS = d.FS;
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
         const double b = d.b0c[base + c];
-        double* S[3] = {d.S0, d.S2, d.S3};
-        for (int q = 0; q < 3; ++q) {
+        double* S[4] = {d.S0, d.S2, d.S3, d.SH};
+        for (int q = 0; q < (d.morpho ? 4 : 3); ++q) {
             S[q][base + c] = b;
             S[q][FS + base + c] = 0.0;
             S[q][2 * FS + base + c] = 0.0;
             S[q][3 * FS + base + c] = 0.0;
         }
         for (int q = 0; q < 5; ++q) d.R[q * FS + base + c] = 0.0;
+        if (d.morpho) {
+            for (int q = 0; q < 5; ++q) d.R2[q * FS + base + c] = 0.0;
+            for (int q = 0; q < 3; ++q) { d.SM[q][base + c] = b; d.SM[q][FS + base + c] = 0.0; }
+            d.UV[base + c] = 0.0; d.UV[FS + base + c] = 0.0;
+        }
         if (active) {
             for (int q = 0; q < 10; ++q) d.maxima[q * FS + base + c] = 0.0;
             d.maxima[10 * FS + base + c] = -1.0;
@@ -728,11 +1015,49 @@ __device__ void dry_tile(const Dev& d, const Params& p, int k, bool active) {
 
 static constexpr int MAXNEW = 1024;
 
+// A new step refreshes intermed3's ghost tiles from the tileContainer
+// (InitialiseTimeSteppingArrays, src/TimeStepper.f90:785-808): undo the
+// redistribution writes of the previous step.
+__device__ void refresh_ghost_overlay(const Dev& d, const Params& p) {
+    Ctrl* c = d.ctrl;
+    const long long n = c->ndirty;
+    if (n == 0) return;
+    const long long FS = d.FS;
+    const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
+    const long long vbit = 1LL << 62;
+    if (n <= d.dirty_cap) {
+        for (long long q = threadIdx.x; q < n; q += blockDim.x) {
+            const long long e = d.dirty[q];
+            if (e & vbit) {
+                const size_t v = (size_t)(e & ~vbit);
+                d.TB.vbt[v] = d.TA.vbt[v];
+            } else {
+                d.S3[e] = d.S0[e];
+                d.S3[3 * FS + e] = d.S0[3 * FS + e];
+            }
+        }
+    } else {
+        for (int g = 0; g < c->n_ghost; ++g) {
+            const int k = d.ghosts[g];
+            for (size_t q = threadIdx.x; q < NV; q += blockDim.x) d.TB.vbt[k * NV + q] = d.TA.vbt[k * NV + q];
+            for (int q = threadIdx.x; q < p.NC; q += blockDim.x) {
+                const size_t gi = (size_t)k * p.NC + q;
+                d.S3[gi] = d.S0[gi];
+                d.S3[3 * FS + gi] = d.S0[3 * FS + gi];
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) c->ndirty = 0;
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) {
     __shared__ int newt[MAXNEW];
     __shared__ int nnew, bad, need;
     Ctrl* c = d.ctrl;
     if (c->status != ST_NEW) return;
+    if (d.morpho) refresh_ghost_overlay(d, p);
     // fast path: does any flagged edge face a tile that is not active?
     if (threadIdx.x == 0) need = 0;
     __syncthreads();
@@ -802,6 +1127,16 @@ __global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) {
     for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) d.eflags[d.active[q]] = 0;
     if (nnew == 0) return;
     __syncthreads();
+    // ActivateTile (src/UpdateTiles.f90:342-384): cell bt = 0, face
+    // topography from the tile's and its neighbours' current vertex bt
+    for (int q = 0; q < nnew; ++q) {
+        tile_cell_topo(d, p, d.T, newt[q], true);
+        __syncthreads();
+        tile_face_topo(d, p, d.T, newt[q]);
+        __syncthreads();
+        tile_static_terms(d, p, d.T, newt[q]);
+        __syncthreads();
+    }
     for (int q = 0; q < nnew; ++q) dry_tile(d, p, newt[q], true);
     // new ghosts around each new tile (AddGhostTiles, src/UpdateTiles.f90:389-481)
     __shared__ int ng_start;
@@ -852,7 +1187,7 @@ __global__ __launch_bounds__(256) void init_flags_kernel(Dev d, Params p) {
     const int tb = p.tile_buffer;
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
         const size_t gi = (size_t)t * p.NC + c;
-        double hn = compute_hn(d.S0[gi], d.b0c[gi], 0.0, d.cgam[gi]);
+        double hn = compute_hn(d.S0[gi], d.b0c[gi], d.T.btc[gi], d.T.cgam[gi]);
         if (hn > p.hthr) {
             const int i = c % p.nx, j = c / p.nx;
             if (j + 1 > p.ny - tb) f |= 1u;
@@ -864,44 +1199,83 @@ __global__ __launch_bounds__(256) void init_flags_kernel(Dev d, Params p) {
     if (f) atomicOr(&d.eflags[t], f);
 }
 
-// Accept or roll back the step (src/TimeStepper.f90:173-275).
+// Accept or roll back the step (src/TimeStepper.f90:173-275).  With
+// morphodynamics the attempt is H(dt) M(2dt) H(dt) and fails at the first
+// of: a CFL re-check of the first operator (dt <- 0.9 dt_k), the
+// morphodynamic checks or redistribution (dt <- dt/2), the second
+// operator's substep-1 advice (dt <- 0.9 dt or the advice), or its CFL
+// re-checks (dt <- 0.9 dt_k).
 __global__ void finalize_kernel(Dev d, Params p) {
     Ctrl* c = d.ctrl;
+    c->commit = 0;
     if (c->status == ST_DONE || c->status == ST_ERROR) return;
     const bool ran_p1 = (c->status == ST_NEW);
     const double dt = step_dt(d, p);
     c->rhs_evals += (ran_p1 ? 1 : 0) + 1;
-    const double dt1 = advised_from_slot(p, c->slot[1], c->n_active);
     bool reject = false;
+    double retry = 0.0;
+    const double dt1 = advised_from_slot(p, c->slot[1], c->n_active);
     if (dt1 < dt) {
         reject = true;
-        c->dt_retry = 0.9 * dt1;
+        retry = 0.9 * dt1;
     } else {
         c->rhs_evals += 1;
         const double dt2 = advised_from_slot(p, c->slot[2], c->n_active);
         if (dt2 < dt) {
             reject = true;
-            c->dt_retry = 0.9 * dt2;
+            retry = 0.9 * dt2;
         } else {
             c->rhs_evals += 1;
         }
     }
+    if (d.morpho && !reject) {
+        if (c->mref) {
+            reject = true;
+            retry = 0.5 * dt;
+        } else {
+            c->rhs_evals += 1;
+            const double adv = 0.9 * advised_from_slot(p, c->slot[3], c->n_active);
+            if (adv < dt) {
+                reject = true;
+                retry = ((dt - adv) / dt < 0.1) ? 0.9 * dt : adv;
+            } else {
+                c->rhs_evals += 1;
+                const double e1 = advised_from_slot(p, c->slot[4], c->n_active);
+                if (e1 < dt) {
+                    reject = true;
+                    retry = 0.9 * e1;
+                } else {
+                    c->rhs_evals += 1;
+                    const double e2 = advised_from_slot(p, c->slot[5], c->n_active);
+                    if (e2 < dt) {
+                        reject = true;
+                        retry = 0.9 * e2;
+                    } else {
+                        c->rhs_evals += 1;
+                    }
+                }
+            }
+        }
+    }
     const unsigned long long huge = dbits(kHuge);
+    c->mref = 0;
+    c->ncand = 0;
+    for (int k = 1; k < 6; ++k) c->slot[k] = huge;
     if (reject) {
         c->rejected++;
         c->status = ST_RETRY;
-        c->slot[1] = huge;
-        c->slot[2] = huge;
+        c->dt_retry = retry;
         return;
     }
-    c->t = c->t + dt;
+    c->t = d.morpho ? c->t + 2.0 * dt : c->t + dt;
     c->steps++;
     c->cell_updates += (long long)c->n_active * p.NC;
     c->last_dt = dt;
     if (c->steps == 1 || dt < c->min_dt) c->min_dt = dt;
     if (dt > c->max_dt) c->max_dt = dt;
-    c->slot[0] = huge; c->slot[1] = huge; c->slot[2] = huge;
+    c->slot[0] = huge;
     c->status = ST_NEW;
+    c->commit = d.morpho ? 1 : 0;
     if (c->mode == 0) {
         if (c->t >= c->tend) c->status = ST_DONE;
     } else {
@@ -909,140 +1283,21 @@ __global__ void finalize_kernel(Dev d, Params p) {
     }
 }
 
-// Cell and face topography from the tile's vertex elevations
-// (ComputeCellCentredTopographicData / ComputeInterfacialTopographicData,
-// src/MorphodynamicRHS.f90:308-578; face b0 of GetHeights, src/dem.f90:378-386).
-__global__ __launch_bounds__(256) void topo_kernel(Dev d, Params p) {
-    const int t = blockIdx.x;
-    const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
-    const int ti = t % p.ntx, tj = t / p.ntx;
-    // neighbour tiles (clamped at the domain edge: their values are never read)
-    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
-    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
-    const size_t NV = (size_t)(nx + 1) * (ny + 1);
-    const double* b = d.vb0 + (size_t)t * NV;
-    const double* bt = d.vbt + (size_t)t * NV;
-    const double* bW = d.vb0 + (size_t)tW * NV; const double* btW = d.vbt + (size_t)tW * NV;
-    const double* bE = d.vb0 + (size_t)tE * NV; const double* btE = d.vbt + (size_t)tE * NV;
-    const double* bS = d.vb0 + (size_t)tS * NV; const double* btS = d.vbt + (size_t)tS * NV;
-    const double* bN = d.vb0 + (size_t)tN * NV; const double* btN = d.vbt + (size_t)tN * NV;
-#define V(a, i, j) a[(j) * NV1 + (i)]
-    for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
-        const int i = c % nx, j = c / nx;
-        Kahan k;
-        k.add(V(b, i, j)); k.add(V(b, i + 1, j)); k.add(V(b, i, j + 1)); k.add(V(b, i + 1, j + 1));
-        const double b0c = 0.25 * k.s;
-        Kahan kt;
-        kt.add(V(bt, i, j)); kt.add(V(bt, i + 1, j)); kt.add(V(bt, i, j + 1)); kt.add(V(bt, i + 1, j + 1));
-        const double btc = 0.25 * kt.s;
-        Kahan kx;
-        kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
-        kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1)); kx.add(-V(b, i, j + 1)); kx.add(-V(bt, i, j + 1));
-        const double dbdx = 0.5 * p.dxr * kx.s;
-        Kahan ky;
-        ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1)); ky.add(-V(b, i, j)); ky.add(-V(bt, i, j));
-        ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(-V(b, i + 1, j)); ky.add(-V(bt, i + 1, j));
-        const double dbdy = 0.5 * p.dyr * ky.s;
-        const size_t gi = (size_t)t * p.NC + c;
-        d.b0c[gi] = b0c; d.btc[gi] = btc; d.cbx[gi] = dbdx; d.cby[gi] = dbdy;
-        d.cgam[gi] = gamma_of(p.geometric, dbdx, dbdy);
-    }
-    // x-faces: i in [0, nx], j in [0, ny)
-    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
-        const int i = f % (nx + 1), j = f / (nx + 1);
-        Kahan kx;
-        if (i >= 1 && i <= nx - 1) {
-            kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1));
-            kx.add(-V(b, i - 1, j)); kx.add(-V(bt, i - 1, j)); kx.add(-V(b, i - 1, j + 1)); kx.add(-V(bt, i - 1, j + 1));
-        } else if (i == 0) {
-            kx.add(V(b, 1, j)); kx.add(V(bt, 1, j)); kx.add(V(b, 1, j + 1)); kx.add(V(bt, 1, j + 1));
-            kx.add(-V(bW, nx - 1, j)); kx.add(-V(btW, nx - 1, j)); kx.add(-V(bW, nx - 1, j + 1)); kx.add(-V(btW, nx - 1, j + 1));
-        } else {
-            kx.add(V(bE, 1, j)); kx.add(V(btE, 1, j)); kx.add(V(bE, 1, j + 1)); kx.add(V(btE, 1, j + 1));
-            kx.add(-V(b, nx - 1, j)); kx.add(-V(bt, nx - 1, j)); kx.add(-V(b, nx - 1, j + 1)); kx.add(-V(bt, nx - 1, j + 1));
-        }
-        const double dbdx = 0.25 * p.dxr * kx.s;
-        const double btf = 0.5 * (V(bt, i, j) + V(bt, i, j + 1));
-        Kahan ky;
-        ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1)); ky.add(-V(b, i, j)); ky.add(-V(bt, i, j));
-        const double dbdy = p.dyr * ky.s;
-        const size_t fi = (size_t)t * p.NFX + f;
-        d.xb0[fi] = 0.5 * (V(b, i, j) + V(b, i, j + 1));
-        d.xbt[fi] = btf;
-        d.xby[fi] = dbdy;
-        d.xgam[fi] = gamma_of(p.geometric, dbdx, dbdy);
-    }
-    // y-faces: i in [0, nx), j in [0, ny]
-    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
-        const int i = f % nx, j = f / nx;
-        Kahan kx;
-        kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
-        const double dbdx = p.dxr * kx.s;
-        const double btf = 0.5 * (V(bt, i, j) + V(bt, i + 1, j));
-        Kahan ky;
-        if (j >= 1 && j <= ny - 1) {
-            ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1));
-            ky.add(-V(b, i + 1, j - 1)); ky.add(-V(bt, i + 1, j - 1)); ky.add(-V(b, i, j - 1)); ky.add(-V(bt, i, j - 1));
-        } else if (j == 0) {
-            ky.add(V(b, i + 1, 1)); ky.add(V(bt, i + 1, 1)); ky.add(V(b, i, 1)); ky.add(V(bt, i, 1));
-            ky.add(-V(bS, i + 1, ny - 1)); ky.add(-V(btS, i + 1, ny - 1)); ky.add(-V(bS, i, ny - 1)); ky.add(-V(btS, i, ny - 1));
-        } else {
-            ky.add(V(bN, i + 1, 1)); ky.add(V(btN, i + 1, 1)); ky.add(V(bN, i, 1)); ky.add(V(btN, i, 1));
-            ky.add(-V(b, i + 1, ny - 1)); ky.add(-V(bt, i + 1, ny - 1)); ky.add(-V(b, i, ny - 1)); ky.add(-V(bt, i, ny - 1));
-        }
-        const double dbdy = 0.25 * p.dyr * ky.s;
-        const size_t fi = (size_t)t * p.NFY + f;
-        d.yb0[fi] = 0.5 * (V(b, i, j) + V(b, i + 1, j));
-        d.ybt[fi] = btf;
-        d.ybx[fi] = dbdx;
-        d.ygam[fi] = gamma_of(p.geometric, dbdx, dbdy);
-    }
-#undef V
+// Topography of every tile of a set (after kst_set_topography / kst_set_state).
+__global__ __launch_bounds__(256) void topo_kernel(Dev d, Params p, Topo T) {
+    tile_cell_topo(d, p, T, blockIdx.x, false);
+    __syncthreads();
+    tile_face_topo(d, p, T, blockIdx.x);
 }
 
-// Static per-face CFL numerators and per-cell surface-gradient prefactors.
-// gam_ratio = min(gam(cell)/gam(face), 1) of the cell on each side of a face
-// (src/HydraulicRHS.f90:983-1006) enters the CFL as (gr*gr)*dx; the
-// prefactors of ConstructHydraulicRHS (src/HydraulicRHS.f90:1233-1243).
-// Needs cgam of the neighbour tiles, so it runs after topo_kernel.
-__global__ __launch_bounds__(256) void topo2_kernel(Dev d, Params p) {
-    const int t = blockIdx.x;
-    const int nx = p.nx, ny = p.ny;
-    const int ti = t % p.ntx, tj = t / p.ntx;
-    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
-    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
-    for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
-        const int gi = t * p.NC + c;
-        const double bx = d.cbx[gi], by = d.cby[gi], gam = d.cgam[gi];
-        if (p.geometric) {
-            d.pxu[gi] = (1.0 + by * by) / gam;
-            d.pxv[gi] = -(bx * by / gam);
-            d.pyv[gi] = (1.0 + bx * bx) / gam;
-        } else {
-            d.pxu[gi] = 1.0; d.pxv[gi] = 0.0; d.pyv[gi] = 1.0;
-        }
-    }
-    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
-        const int i = f % (nx + 1), j = f / (nx + 1);
-        const double fg = d.xgam[t * p.NFX + f];
-        const double gl = (i == 0) ? d.cgam[tW * p.NC + j * nx + nx - 1] : d.cgam[t * p.NC + j * nx + i - 1];
-        const double gr_ = (i == nx) ? d.cgam[tE * p.NC + j * nx] : d.cgam[t * p.NC + j * nx + i];
-        const double a = fmin(gl / fg, 1.0), b = fmin(gr_ / fg, 1.0);
-        d.xnp[t * p.NFX + f] = a * a * p.dx;
-        d.xnm[t * p.NFX + f] = b * b * p.dx;
-    }
-    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
-        const int i = f % nx, j = f / nx;
-        const double fg = d.ygam[t * p.NFY + f];
-        const double gl = (j == 0) ? d.cgam[tS * p.NC + (ny - 1) * nx + i] : d.cgam[t * p.NC + (j - 1) * nx + i];
-        const double gr_ = (j == ny) ? d.cgam[tN * p.NC + i] : d.cgam[t * p.NC + j * nx + i];
-        const double a = fmin(gl / fg, 1.0), b = fmin(gr_ / fg, 1.0);
-        d.ynp[t * p.NFY + f] = a * a * p.dy;
-        d.ynm[t * p.NFY + f] = b * b * p.dy;
-    }
+// Needs the cell gamma of the neighbour tiles, so it runs after topo_kernel.
+__global__ __launch_bounds__(256) void topo2_kernel(Dev d, Params p, Topo T) {
+    tile_static_terms(d, p, T, blockIdx.x);
 }
 
 }  // namespace kst
+
+#include "kst_morpho.hpp"
 
 // ===========================================================================
 // host side: the C-ABI (include/kestrel_step.h)
@@ -1052,7 +1307,8 @@ using namespace kst;
 struct kst_handle {
     kst_config cfg;
     Params p;
-    Dev d;
+    Dev d;          // launch arguments of the first (or only) hydraulic operator
+    Dev d2;         // of the second hydraulic operator of a Strang step
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
@@ -1089,6 +1345,35 @@ int dalloc(kst_handle* h, T** ptr, size_t n) {
     HIPCHK(h, hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), h->stream));
     h->allocs.push_back(p);
     *ptr = static_cast<T*>(p);
+    return 0;
+}
+
+int alloc_topo(kst_handle* h, Topo& T) {
+    const size_t FS = (size_t)h->ntiles * h->NC;
+    const size_t NV = (size_t)(h->nx + 1) * (h->ny + 1) * h->ntiles;
+    const size_t FX = (size_t)h->ntiles * h->p.NFX, FY = (size_t)h->ntiles * h->p.NFY;
+    int rc = 0;
+    rc |= dalloc(h, &T.vbt, NV);
+    rc |= dalloc(h, &T.btc, FS); rc |= dalloc(h, &T.cbx, FS); rc |= dalloc(h, &T.cby, FS);
+    rc |= dalloc(h, &T.cgam, FS); rc |= dalloc(h, &T.pxu, FS); rc |= dalloc(h, &T.pxv, FS);
+    rc |= dalloc(h, &T.pyv, FS);
+    rc |= dalloc(h, &T.xbt, FX); rc |= dalloc(h, &T.xby, FX); rc |= dalloc(h, &T.xgam, FX);
+    rc |= dalloc(h, &T.xib, FX); rc |= dalloc(h, &T.xnp, FX); rc |= dalloc(h, &T.xnm, FX);
+    rc |= dalloc(h, &T.ybt, FY); rc |= dalloc(h, &T.ybx, FY); rc |= dalloc(h, &T.ygam, FY);
+    rc |= dalloc(h, &T.yib, FY); rc |= dalloc(h, &T.ynp, FY); rc |= dalloc(h, &T.ynm, FY);
+    return rc;
+}
+
+// Topography of every tile in every set the handle holds, from the vertex
+// b0 and each set's vertex bt.
+int refresh_topography(kst_handle* h) {
+    Topo* sets[4] = {&h->d.TA, &h->d.TB, &h->d.TM[0], &h->d.TM[1]};
+    const int nsets = h->d.morpho ? 4 : 1;
+    for (int q = 0; q < nsets; ++q)
+        hipLaunchKernelGGL(topo_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p, *sets[q]);
+    for (int q = 0; q < nsets; ++q)
+        hipLaunchKernelGGL(topo2_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p, *sets[q]);
+    HIPCHK(h, hipGetLastError());
     return 0;
 }
 
@@ -1179,7 +1464,19 @@ int launch_iteration(kst_handle* h, int slot) {
         EVREC(2 * q + 1);
     }
 #undef EVREC
+    if (d.morpho) {
+        // M(2dt): three SSP stages, redistribution, face topography of the new bed
+        hipLaunchKernelGGL(morpho_kernel<1>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
+        hipLaunchKernelGGL(morpho_kernel<2>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
+        hipLaunchKernelGGL(morpho_kernel<3>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
+        hipLaunchKernelGGL(redistribute_kernel, dim3(1), dim3(256), 0, h->stream, d, p);
+        hipLaunchKernelGGL(face_topo_kernel, dim3(p.ntiles), dim3(256), 0, h->stream, d, p);
+        // second hydraulic operator
+        for (int q = 0; q < 4; ++q)
+            hipLaunchKernelGGL(k.fn[q], dim3(nblk), dim3(NTHREADS), 0, h->stream, h->d2, p);
+    }
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1), 0, h->stream, d, p);
+    if (d.morpho) hipLaunchKernelGGL(commit_kernel, dim3(p.ntiles), dim3(256), 0, h->stream, d, p);
     HIPCHK(h, hipGetLastError());
     return 0;
 }
@@ -1266,6 +1563,7 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
         st->steps = c.steps; st->rejected = c.rejected; st->rhs_evals = c.rhs_evals;
         st->cell_updates = c.cell_updates; st->active_tiles = c.n_active; st->ghost_tiles = c.n_ghost;
         st->t = c.t; st->last_dt = c.last_dt; st->min_dt = c.min_dt; st->max_dt = c.max_dt;
+        st->redist_cells = c.redist_cells; st->redist_ghost = c.redist_ghost;
     }
     return 0;
 }
@@ -1318,8 +1616,8 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     *out = nullptr;
     if (cfg->nx_per_tile < 4 || cfg->ny_per_tile < 4 || cfg->nx_tiles < 3 || cfg->ny_tiles < 3)
         return KST_ERR_CONFIG;
-    if (cfg->bc != KST_BC_HALT || cfg->morphodynamics || cfg->precision != KST_PREC_FP64)
-        return KST_ERR_CONFIG;
+    if (cfg->bc != KST_BC_HALT || cfg->precision != KST_PREC_FP64) return KST_ERR_CONFIG;
+    if (cfg->n_sources > 0 && !cfg->sources) return KST_ERR_CONFIG;
     kst_handle* h = new kst_handle();
     h->cfg = *cfg;
     h->cfg.sources = nullptr;
@@ -1337,19 +1635,29 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     const size_t FS = (size_t)h->ntiles * h->NC;
     const size_t NV = (size_t)(h->nx + 1) * (h->ny + 1);
     d.FS = (long long)FS;
+    d.morpho = cfg->morphodynamics ? 1 : 0;
     rc |= dalloc(h, &d.S0, 4 * FS); rc |= dalloc(h, &d.S2, 4 * FS); rc |= dalloc(h, &d.S3, 4 * FS);
     rc |= dalloc(h, &d.R, 5 * FS);
-    rc |= dalloc(h, &d.b0c, FS); rc |= dalloc(h, &d.btc, FS); rc |= dalloc(h, &d.cbx, FS);
-    rc |= dalloc(h, &d.cby, FS); rc |= dalloc(h, &d.cgam, FS);
-    rc |= dalloc(h, &d.xb0, (size_t)h->ntiles * h->p.NFX); rc |= dalloc(h, &d.xbt, (size_t)h->ntiles * h->p.NFX);
-    rc |= dalloc(h, &d.xby, (size_t)h->ntiles * h->p.NFX); rc |= dalloc(h, &d.xgam, (size_t)h->ntiles * h->p.NFX);
-    rc |= dalloc(h, &d.yb0, (size_t)h->ntiles * h->p.NFY); rc |= dalloc(h, &d.ybt, (size_t)h->ntiles * h->p.NFY);
-    rc |= dalloc(h, &d.ybx, (size_t)h->ntiles * h->p.NFY); rc |= dalloc(h, &d.ygam, (size_t)h->ntiles * h->p.NFY);
-    rc |= dalloc(h, &d.vb0, (size_t)h->ntiles * NV); rc |= dalloc(h, &d.vbt, (size_t)h->ntiles * NV);
-    rc |= dalloc(h, &d.xnp, (size_t)h->ntiles * h->p.NFX); rc |= dalloc(h, &d.xnm, (size_t)h->ntiles * h->p.NFX);
-    rc |= dalloc(h, &d.ynp, (size_t)h->ntiles * h->p.NFY); rc |= dalloc(h, &d.ynm, (size_t)h->ntiles * h->p.NFY);
-    rc |= dalloc(h, &d.pxu, FS); rc |= dalloc(h, &d.pxv, FS); rc |= dalloc(h, &d.pyv, FS);
+    rc |= dalloc(h, &d.b0c, FS);
+    rc |= dalloc(h, &d.xb0, (size_t)h->ntiles * h->p.NFX);
+    rc |= dalloc(h, &d.yb0, (size_t)h->ntiles * h->p.NFY);
+    rc |= dalloc(h, &d.vb0, (size_t)h->ntiles * NV);
+    rc |= alloc_topo(h, d.TA);
     rc |= dalloc(h, &d.gsx, FS); rc |= dalloc(h, &d.gsy, FS);
+    if (d.morpho) {
+        rc |= alloc_topo(h, d.TB); rc |= alloc_topo(h, d.TM[0]); rc |= alloc_topo(h, d.TM[1]);
+        rc |= dalloc(h, &d.SH, 4 * FS);
+        for (int q = 0; q < 3; ++q) rc |= dalloc(h, &d.SM[q], 2 * FS);
+        rc |= dalloc(h, &d.UV, 2 * FS);
+        rc |= dalloc(h, &d.R2, 5 * FS);
+        // every cell may be a candidate; the sort pads to a power of two
+        size_t cap = 1;
+        while (cap < FS) cap <<= 1;
+        rc |= dalloc(h, &d.cand_key, cap); rc |= dalloc(h, &d.cand_ex, cap);
+        d.cand_cap = (int)cap;
+        d.dirty_cap = 1 << 16;
+        rc |= dalloc(h, &d.dirty, (size_t)d.dirty_cap);
+    }
     rc |= dalloc(h, &d.maxima, 11 * FS);
     rc |= dalloc(h, &d.tstate, (size_t)h->ntiles); rc |= dalloc(h, &d.has_src, (size_t)h->ntiles);
     rc |= dalloc(h, &d.active, (size_t)h->ntiles + 1); rc |= dalloc(h, &d.ghosts, (size_t)h->ntiles + 1);
@@ -1361,6 +1669,12 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     rc |= dalloc(h, &d.src_n, ns); rc |= dalloc(h, &d.src_off, ns); rc |= dalloc(h, &d.src_cells, ns);
     rc |= dalloc(h, &d.src_t, nk); rc |= dalloc(h, &d.src_q, nk); rc |= dalloc(h, &d.src_p, nk);
     if (rc) { kst_destroy(h); return KST_ERR_HIP; }
+    // the operators' launch arguments
+    d.T = d.TA;
+    d.Uw = d.S0; d.Uu = d.S0 + FS; d.Uv = d.S0 + 2 * FS; d.Uh = d.S0 + 3 * FS;
+    d.Sout = d.morpho ? d.SH : d.S0;
+    d.second = 0;
+    d.last = d.morpho ? 0 : 1;
     if (ns > 0) {
         std::vector<double> x(ns), y(ns), r(ns), tt, q, ps;
         std::vector<int> n(ns), off(ns), cells(ns);
@@ -1370,15 +1684,25 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
             off[i] = (int)tt.size();
             for (int k = 0; k < s.n_series; ++k) { tt.push_back(s.time[k]); q.push_back(s.flux[k]); ps.push_back(s.psi[k]); }
         }
-        hipMemcpy(d.src_x, x.data(), 8 * ns, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_y, y.data(), 8 * ns, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_r, r.data(), 8 * ns, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_n, n.data(), 4 * ns, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_off, off.data(), 4 * ns, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_cells, cells.data(), 4 * ns, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_t, tt.data(), 8 * nk, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_q, q.data(), 8 * nk, hipMemcpyHostToDevice);
-        hipMemcpy(d.src_p, ps.data(), 8 * nk, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_x, x.data(), 8 * ns, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_y, y.data(), 8 * ns, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_r, r.data(), 8 * ns, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_n, n.data(), 4 * ns, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_off, off.data(), 4 * ns, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_cells, cells.data(), 4 * ns, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_t, tt.data(), 8 * nk, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_q, q.data(), 8 * nk, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d.src_p, ps.data(), 8 * nk, hipMemcpyHostToDevice);
+    }
+    h->d2 = d;
+    if (d.morpho) {
+        Dev& e = h->d2;
+        e.T = d.TB;
+        e.Uw = d.SM[2]; e.Uu = d.SH + FS; e.Uv = d.SH + 2 * FS; e.Uh = d.SM[2] + FS;
+        e.R = d.R2;
+        e.Sout = d.S0;
+        e.second = 1;
+        e.last = 1;
     }
     *out = h;
     return KST_OK;
@@ -1397,9 +1721,8 @@ int kst_set_topography(kst_handle* h, const double* b0v) {
                     dst[j * NV1 + i] = b0v[(size_t)(tj * ny + j) * (h->NX + 1) + ti * nx + i];
         }
     HIPCHK(h, hipMemcpy(h->d.vb0, tv.data(), tv.size() * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(topo_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p);
-    hipLaunchKernelGGL(topo2_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p);
-    HIPCHK(h, hipGetLastError());
+    int rc = refresh_topography(h);
+    if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->have_topo = true;
     return KST_OK;
@@ -1410,7 +1733,6 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
                   const double* hnmax0) {
     if (!h || !w || !rhu || !rhv || !hnpsi || !ts) return KST_ERR_CONFIG;
     if (!h->have_topo) return set_err(h, KST_ERR_STATE, "kst_set_topography must come first");
-    (void)btv;
     const size_t FS = (size_t)h->ntiles * h->NC;
     // tile lists: active sorted by id; ghosts = non-active 8-neighbours of active tiles
     std::vector<uint8_t> st(h->ntiles, KST_TILE_OFF);
@@ -1430,6 +1752,28 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
                 const int n = (gi + di) + (gj + dj) * h->ntx;
                 if (st[n] == KST_TILE_OFF) { st[n] = KST_TILE_GHOST; gh.push_back(n); }
             }
+    }
+    // vertex bt: each active tile takes its own copy of the global values
+    // (ghost tiles keep bt = 0, as AllocateTopographicData leaves them),
+    // then the topography of every set follows
+    {
+        const int nx = h->nx, ny = h->ny, NV1 = nx + 1;
+        const size_t NV = (size_t)(nx + 1) * (ny + 1);
+        std::vector<double> tv((size_t)h->ntiles * NV, 0.0);
+        if (btv)
+            for (int k : act) {
+                const int ti = k % h->ntx, tj = k / h->ntx;
+                double* dst = &tv[(size_t)k * NV];
+                for (int j = 0; j <= ny; ++j)
+                    for (int i = 0; i <= nx; ++i)
+                        dst[j * NV1 + i] = btv[(size_t)(tj * ny + j) * (h->NX + 1) + ti * nx + i];
+            }
+        Topo* sets[4] = {&h->d.TA, &h->d.TB, &h->d.TM[0], &h->d.TM[1]};
+        for (int q = 0; q < (h->d.morpho ? 4 : 1); ++q)
+            HIPCHK(h, hipMemcpy(sets[q]->vbt, tv.data(), tv.size() * 8, hipMemcpyHostToDevice));
+        int rc = refresh_topography(h);
+        if (rc) return rc;
+        HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     // fields: active tiles take the given state, ghosts take dry data
     std::vector<double> b0c(FS);
@@ -1451,6 +1795,15 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     HIPCHK(h, hipMemcpy(h->d.S2, S.data(), S.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d.S3, S.data(), S.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemset(h->d.R, 0, 5 * FS * 8));
+    if (h->d.morpho) {
+        HIPCHK(h, hipMemcpy(h->d.SH, S.data(), S.size() * 8, hipMemcpyHostToDevice));
+        for (int q = 0; q < 3; ++q) {
+            HIPCHK(h, hipMemcpy(h->d.SM[q], S.data(), FS * 8, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->d.SM[q] + FS, S.data() + 3 * FS, FS * 8, hipMemcpyHostToDevice));
+        }
+        HIPCHK(h, hipMemset(h->d.UV, 0, 2 * FS * 8));
+        HIPCHK(h, hipMemset(h->d.R2, 0, 5 * FS * 8));
+    }
     HIPCHK(h, hipMemset(h->d.gsx, 0, FS * 8));
     HIPCHK(h, hipMemset(h->d.gsy, 0, FS * 8));
     // maxima
@@ -1469,9 +1822,9 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     // (src/SetSources.f90:357-375)
     for (int q = 0; q < h->cfg.n_sources; ++q) {
         double sx, sy, sr;
-        hipMemcpy(&sx, h->d.src_x + q, 8, hipMemcpyDeviceToHost);
-        hipMemcpy(&sy, h->d.src_y + q, 8, hipMemcpyDeviceToHost);
-        hipMemcpy(&sr, h->d.src_r + q, 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&sx, h->d.src_x + q, 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&sy, h->d.src_y + q, 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&sr, h->d.src_r + q, 8, hipMemcpyDeviceToHost);
         for (int J = 0; J < h->NY; ++J)
             for (int I = 0; I < h->NX; ++I) {
                 const int ti = I / h->nx, tj = J / h->ny, i = I % h->nx, j = J % h->ny;
@@ -1486,7 +1839,7 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     c.t = t;
     c.tend = t;
     const unsigned long long huge = 0x7FEFFFFFFFFFFFFFull;
-    c.slot[0] = c.slot[1] = c.slot[2] = huge;
+    for (int k = 0; k < 6; ++k) c.slot[k] = huge;
     c.status = ST_DONE;
     c.n_active = (int)act.size();
     c.n_ghost = (int)gh.size();
@@ -1515,6 +1868,8 @@ static int start_run(kst_handle* h, int mode, double tend, long long nsteps, kst
     c.tend = tend;
     c.steps_left = nsteps;
     c.steps = c.rejected = c.rhs_evals = c.cell_updates = 0;
+    c.redist_cells = c.redist_ghost = 0;
+    c.mref = 0; c.ncand = 0; c.commit = 0;
     c.min_dt = 0.0; c.max_dt = 0.0;
     rc = write_ctrl(h, &c);
     if (rc) return rc;
@@ -1557,19 +1912,34 @@ int kst_get_state(kst_handle* h, double* w, double* rhu, double* rhv, double* hn
     if (rhv) from_tiles(h, S.data() + 2 * FS, rhv);
     if (hnpsi) from_tiles(h, S.data() + 3 * FS, hnpsi);
     if (hn) {
-        std::vector<double> b0c(FS), g(FS), H(FS, 0.0);
+        std::vector<double> b0c(FS), bt(FS), g(FS), H(FS, 0.0);
         HIPCHK(h, hipMemcpy(b0c.data(), h->d.b0c, FS * 8, hipMemcpyDeviceToHost));
-        HIPCHK(h, hipMemcpy(g.data(), h->d.cgam, FS * 8, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(bt.data(), h->d.TA.btc, FS * 8, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(g.data(), h->d.TA.cgam, FS * 8, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < FS; ++i) {
             if (st[i / h->NC] == KST_TILE_OFF) continue;
-            double x = -0.0;
+            double x = -bt[i];
             x = x + (S[i] - b0c[i]);
             x = x * g[i];
             H[i] = x < 0.0 ? 0.0 : x;
         }
         from_tiles(h, H.data(), hn);
     }
-    if (btv) std::fill_n(btv, (size_t)(h->NX + 1) * (h->NY + 1), 0.0);
+    if (btv) {
+        // vertex (I, J) from the copy of the tile holding cell (min(I, NX-1), min(J, NY-1))
+        const int nx = h->nx, ny = h->ny, NV1 = nx + 1;
+        const size_t NV = (size_t)(nx + 1) * (ny + 1);
+        std::vector<double> tv((size_t)h->ntiles * NV);
+        HIPCHK(h, hipMemcpy(tv.data(), h->d.TA.vbt, tv.size() * 8, hipMemcpyDeviceToHost));
+        for (int J = 0; J <= h->NY; ++J)
+            for (int I = 0; I <= h->NX; ++I) {
+                const int ci = std::min(I, h->NX - 1), cj = std::min(J, h->NY - 1);
+                const int k = (cj / ny) * h->ntx + ci / nx;
+                double v = 0.0;
+                if (st[k] != KST_TILE_OFF) v = tv[(size_t)k * NV + (J - (cj / ny) * ny) * NV1 + (I - (ci / nx) * nx)];
+                btv[(size_t)J * (h->NX + 1) + I] = v;
+            }
+    }
     if (ts) std::memcpy(ts, st.data(), st.size());
     Ctrl c;
     int rc = read_ctrl(h, &c);
@@ -1586,6 +1956,29 @@ int kst_get_maxima(kst_handle* h, double* const mx[11]) {
     HIPCHK(h, hipMemcpy(M.data(), h->d.maxima, M.size() * 8, hipMemcpyDeviceToHost));
     for (int q = 0; q < 11; ++q)
         if (mx[q]) from_tiles(h, M.data() + q * FS, mx[q]);
+    return KST_OK;
+}
+
+int kst_get_tile_bt(kst_handle* h, double* bt_tiles) {
+    if (!h || !bt_tiles) return KST_ERR_CONFIG;
+    const size_t n = (size_t)(h->nx + 1) * (h->ny + 1) * h->ntiles;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(bt_tiles, h->d.TA.vbt, n * 8, hipMemcpyDeviceToHost));
+    return KST_OK;
+}
+
+int kst_get_cell_topography(kst_handle* h, double* b0c, double* btc, double* dbdx, double* dbdy) {
+    if (!h) return KST_ERR_CONFIG;
+    const size_t FS = (size_t)h->ntiles * h->NC;
+    std::vector<double> v(FS);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double* outs[4] = {b0c, btc, dbdx, dbdy};
+    const double* srcs[4] = {h->d.b0c, h->d.TA.btc, h->d.TA.cbx, h->d.TA.cby};
+    for (int q = 0; q < 4; ++q) {
+        if (!outs[q]) continue;
+        HIPCHK(h, hipMemcpy(v.data(), srcs[q], FS * 8, hipMemcpyDeviceToHost));
+        from_tiles(h, v.data(), outs[q]);
+    }
     return KST_OK;
 }
 
@@ -1617,9 +2010,9 @@ void kst_destroy(kst_handle* h) {
     if (!h) return;
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
-    if (h->stream) hipStreamSynchronize(h->stream);
-    for (void* p : h->allocs) hipFree(p);
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 

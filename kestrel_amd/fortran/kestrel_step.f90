@@ -46,6 +46,7 @@ module kestrel_step_binding
       integer(c_int64_t) :: steps, rejected, rhs_evals, cell_updates
       integer(c_int32_t) :: active_tiles, ghost_tiles
       real(c_double) :: t, last_dt, min_dt, max_dt
+      integer(c_int64_t) :: redist_cells, redist_ghost
    end type kst_stats
 
    interface
@@ -108,6 +109,18 @@ module kestrel_step_binding
          import :: c_ptr, c_int
          type(c_ptr), value :: h
          type(c_ptr), intent(in) :: maxima(11)
+         integer(c_int) :: rc
+      end function
+      function kst_get_tile_bt(h, bt_tiles) result(rc) bind(C, name="kst_get_tile_bt")
+         import :: c_ptr, c_double, c_int
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: bt_tiles(*)
+         integer(c_int) :: rc
+      end function
+      function kst_get_cell_topography(h, b0c, btc, dbdx, dbdy) result(rc) &
+            bind(C, name="kst_get_cell_topography")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: h, b0c, btc, dbdx, dbdy
          integer(c_int) :: rc
       end function
       function kst_last_error(h, buf, buflen) result(rc) bind(C, name="kst_last_error")

@@ -23,7 +23,8 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_topography",
            "kst_set_state", "kst_integrate_to", "kst_step_n", "kst_get_state", "kst_get_maxima",
-           "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read"]
+           "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read", "kst_get_tile_bt",
+           "kst_get_cell_topography"]
 
 
 class KstStats(ctypes.Structure):
@@ -31,7 +32,8 @@ class KstStats(ctypes.Structure):
                 ("rhs_evals", ctypes.c_int64), ("cell_updates", ctypes.c_int64),
                 ("active_tiles", ctypes.c_int32), ("ghost_tiles", ctypes.c_int32),
                 ("t", ctypes.c_double), ("last_dt", ctypes.c_double),
-                ("min_dt", ctypes.c_double), ("max_dt", ctypes.c_double)]
+                ("min_dt", ctypes.c_double), ("max_dt", ctypes.c_double),
+                ("redist_cells", ctypes.c_int64), ("redist_ghost", ctypes.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -66,10 +68,13 @@ def lib():
         L.kst_profile.restype = ctypes.c_int
         L.kst_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.kst_profile_read.restype = ctypes.c_int
+        L.kst_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
+        L.kst_get_cell_topography.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.kst_destroy.argtypes = [ctypes.c_void_p]
         L.kst_destroy.restype = None
         for n in ["kst_create", "kst_set_topography", "kst_set_state", "kst_integrate_to",
-                  "kst_step_n", "kst_get_state", "kst_get_maxima", "kst_last_error"]:
+                  "kst_step_n", "kst_get_state", "kst_get_maxima", "kst_last_error",
+                  "kst_get_tile_bt", "kst_get_cell_topography"]:
             getattr(L, n).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -112,11 +117,12 @@ class Stepper:
         self._b0v = np.ascontiguousarray(b0v, dtype=np.float64)
         self._check(self.L.kst_set_topography(self.h, _ptr(self._b0v)))
 
-    def set_state(self, init, t: float = 0.0):
+    def set_state(self, init, t: float = 0.0, btv: np.ndarray | None = None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (init.w, init.rhoHnu, init.rhoHnv, init.Hnpsi)]
         ts = np.ascontiguousarray(init.tile_state, dtype=np.uint8)
         hm = None if init.hnmax0 is None else np.ascontiguousarray(init.hnmax0, dtype=np.float64)
-        self._check(self.L.kst_set_state(self.h, *[_ptr(a) for a in arrs], None,
+        bt = None if btv is None else np.ascontiguousarray(btv, dtype=np.float64)
+        self._check(self.L.kst_set_state(self.h, *[_ptr(a) for a in arrs], _ptr(bt),
                                          ts.ctypes.data_as(_u8p), float(t), _ptr(hm)))
 
     def integrate_to(self, tend: float) -> dict:
@@ -132,13 +138,26 @@ class Stepper:
     def state(self) -> dict:
         c = self.cfg
         out = {k: np.zeros((c.NY, c.NX)) for k in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "Hn")}
+        out["btv"] = np.zeros((c.NY + 1, c.NX + 1))
         ts = np.zeros(c.ntiles, dtype=np.uint8)
         t = ctypes.c_double()
         self._check(self.L.kst_get_state(self.h, _ptr(out["w"]), _ptr(out["rhoHnu"]), _ptr(out["rhoHnv"]),
-                                         _ptr(out["Hnpsi"]), _ptr(out["Hn"]), None,
+                                         _ptr(out["Hnpsi"]), _ptr(out["Hn"]), _ptr(out["btv"]),
                                          ts.ctypes.data_as(_u8p), ctypes.byref(t)))
         out["tile_state"] = ts
         out["t"] = t.value
+        return out
+
+    def tile_bt(self) -> np.ndarray:
+        c = self.cfg
+        out = np.zeros((c.ntiles, c.ny_per_tile + 1, c.nx_per_tile + 1))
+        self._check(self.L.kst_get_tile_bt(self.h, _ptr(out)))
+        return out
+
+    def cell_topography(self) -> dict:
+        c = self.cfg
+        out = {k: np.zeros((c.NY, c.NX)) for k in ("b0c", "btc", "dbdx", "dbdy")}
+        self._check(self.L.kst_get_cell_topography(self.h, *[_ptr(out[k]) for k in ("b0c", "btc", "dbdx", "dbdy")]))
         return out
 
     def profile(self, enable: bool = True):

@@ -81,6 +81,8 @@ struct ko_sim {
     tdata* probe[3];          /* I0, I3 before and I3 after redistribution */
     void* probe_list; int probe_n;
     uint8_t* probe_ts;        /* tile states at the probe */
+    int64_t redist_cells;     /* RedistributeCell calls that succeeded */
+    int64_t redist_ghost;     /* their writes into ghost tiles (vertex copies, 3x3 cells) */
     double probe_dt, probe_dt_pending;
 };
 
@@ -1844,6 +1846,22 @@ static int redistribute_cell(ko_sim* s, int k, int i, int j, double corr) {
         double nv = BTV(s, &I3[k], di[q], dj[q]) - delta * bd[q];
         BTV(s, &I3[k], di[q], dj[q]) = nv;
         tmeta* m = &s->meta[k];
+        if (di[q] == 0 || di[q] == nx || dj[q] == 0 || dj[q] == ny) {
+            int nb[3], nn = 0;
+            if (di[q] == 0) nb[nn++] = m->nb[0];
+            else if (di[q] == nx) nb[nn++] = m->nb[1];
+            if (dj[q] == 0) {
+                nb[nn++] = m->nb[3];
+                if (di[q] == 0) nb[nn++] = s->meta[m->nb[3]].nb[0];
+                else if (di[q] == nx) nb[nn++] = s->meta[m->nb[3]].nb[1];
+            } else if (dj[q] == ny) {
+                nb[nn++] = m->nb[2];
+                if (di[q] == 0) nb[nn++] = s->meta[m->nb[2]].nb[0];
+                else if (di[q] == nx) nb[nn++] = s->meta[m->nb[2]].nb[1];
+            }
+            for (int r = 0; r < nn; ++r)
+                if (!s->meta[nb[r]].on) s->redist_ghost++;
+        }
         if (di[q] == 0) BTV(s, &I3[m->nb[0]], nx, dj[q]) = nv;
         else if (di[q] == nx) BTV(s, &I3[m->nb[1]], 0, dj[q]) = nv;
         if (dj[q] == 0) {
@@ -1867,6 +1885,7 @@ static int redistribute_cell(ko_sim* s, int k, int i, int j, double corr) {
         for (int l = -1; l <= 1; ++l) {
             tdata* A = &I3[ct];
             tdata* Bo = &I0[ct];
+            if (!s->meta[ct].on) s->redist_ghost++;
             cell_topo_ij(s, A, ci, cj);
             double dbb = UF(s, A, BT, ci, cj) - UF(s, Bo, BT, ci, cj);
             double go = gam_cell(s, Bo, ci, cj);
@@ -1921,7 +1940,10 @@ static int redistribute_grid(ko_sim* s, const rnode* list, int nl) {
     for (int q = 0; q < nl; ++q) {
         double ex, dbt, psio;
         excess_deposition(s, list[q].k, list[q].i, list[q].j, &ex, &dbt, &psio);
-        if (ex > EPS_WP && !redistribute_cell(s, list[q].k, list[q].i, list[q].j, ex)) return 0;
+        if (ex > EPS_WP) {
+            if (!redistribute_cell(s, list[q].k, list[q].i, list[q].j, ex)) return 0;
+            s->redist_cells++;
+        }
     }
     return 1;
 }
@@ -2098,6 +2120,8 @@ static int one_step(ko_sim* s, double tend, kst_stats* st) {
 }
 
 static void finish_stats(ko_sim* s, kst_stats* st) {
+    st->redist_cells = s->redist_cells;
+    st->redist_ghost = s->redist_ghost;
     st->active_tiles = s->nactive;
     st->ghost_tiles = s->nghost;
     st->t = s->t;
@@ -2108,6 +2132,7 @@ int ko_integrate_to(ko_sim* s, double tend, kst_stats* st) {
     if (!st) st = &local;
     memset(st, 0, sizeof *st);
     if (s->status) return s->status;
+    s->redist_cells = s->redist_ghost = 0;
     if (tend <= s->t) { finish_stats(s, st); return 0; }
     for (;;) {
         int rc = one_step(s, tend, st);
@@ -2123,6 +2148,7 @@ int ko_step_n(ko_sim* s, int64_t n, kst_stats* st) {
     if (!st) st = &local;
     memset(st, 0, sizeof *st);
     if (s->status) return s->status;
+    s->redist_cells = s->redist_ghost = 0;
     for (int64_t q = 0; q < n; ++q) {
         int rc = one_step(s, HUGE_WP, st);
         if (rc) return rc;
@@ -2330,11 +2356,12 @@ int ko_set_state(ko_sim* s, const double* w, const double* rhu, const double* rh
                 }
     }
     if (btv) {
-        /* every allocated tile takes its vertex bt from the global array; the
-         * cell (and, for active tiles, face) topography follows */
+        /* every active tile takes its vertex bt from the global array (ghost
+         * tiles keep bt = 0, as AllocateTopographicData leaves them); the
+         * cell and face topography follow */
         for (int k = 0; k < s->ntiles; ++k) {
             tdata* T = &s->cont[C_TC][k];
-            if (!T->alloc) continue;
+            if (!T->alloc || !s->meta[k].on) continue;
             int gi, gj;
             tile_coords(s, k, &gi, &gj);
             for (int j = 0; j <= ny; ++j)

@@ -30,7 +30,8 @@ class KstStats(ctypes.Structure):
                 ("rhs_evals", ctypes.c_int64), ("cell_updates", ctypes.c_int64),
                 ("active_tiles", ctypes.c_int32), ("ghost_tiles", ctypes.c_int32),
                 ("t", ctypes.c_double), ("last_dt", ctypes.c_double),
-                ("min_dt", ctypes.c_double), ("max_dt", ctypes.c_double)]
+                ("min_dt", ctypes.c_double), ("max_dt", ctypes.c_double),
+                ("redist_cells", ctypes.c_int64), ("redist_ghost", ctypes.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -1,0 +1,149 @@
+"""Parity of the HIP morphodynamic path (Strang step H(dt) M(2dt) H(dt)) with
+the oracle, through the C-ABI.
+
+Decks whose closures are algebraic (fluid/simple erosion, simple
+deposition, step or no erosion transition, rat3 or no damping) must agree
+bit for bit: both sides follow the reference's operation order with no FMA
+contraction and only IEEE division and square roots.  Decks that use tanh,
+log or pow (Mixed erosion, Spearman-Manning settling, smooth transition,
+tanh damping) differ in the last bits of those functions between glibc and
+the device library; they are held to TOL_TRANSCENDENTAL.
+"""
+import numpy as np
+import pytest
+
+from kestrel_amd import workloads
+from tests.helpers import FIELDS, active_mask, oracle_for, rel_inf, stepper_for
+
+TOL = 1e-12
+TOL_TRANSCENDENTAL = 1e-9
+
+pytestmark = pytest.mark.gpu
+
+EXACT = {
+    "simple_rat3": dict(erosion="simple", deposition="simple", transition="step", damping="rat3"),
+    "fluid_none": dict(erosion="fluid", deposition="simple", transition="none", damping="none"),
+    "settling": dict(erosion="fluid", deposition="simple", transition="step", damping="rat3",
+                     settling_speed=0.2, ero_critical_height=0.05),
+}
+
+
+def _compare(wl, g, o, label, tol=TOL):
+    gs, os_ = g.state(), o.state()
+    assert np.array_equal(gs["tile_state"], os_["tile_state"]), f"{label}: tile states differ"
+    assert gs["t"] == os_["t"] or abs(gs["t"] - os_["t"]) <= tol * abs(os_["t"]), \
+        f"{label}: t {gs['t']} != {os_['t']}"
+    m = active_mask(wl.cfg, os_["tile_state"])
+    worst = {}
+    for f in FIELDS:
+        r = rel_inf(gs[f], os_[f], m)
+        worst[f] = r
+        assert r <= tol, f"{label}: field {f} rel err {r:.3e}"
+    bt_g, bt_o = g.tile_bt(), o.tile_bt()
+    ts = os_["tile_state"]
+    on = ts == 1
+    rb = rel_inf(bt_g[on], bt_o[on])
+    worst["bt"] = rb
+    assert rb <= tol, f"{label}: bt rel err {rb:.3e}"
+    return worst
+
+
+@pytest.mark.parametrize("case", list(EXACT))
+def test_morpho_exact_decks_bitwise(gpu, case):
+    wl = workloads.morpho_cap(128, tile=16, **EXACT[case])
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    for chunk, n in enumerate((1, 9, 30)):
+        so = o.step_n(n)
+        sg = g.step_n(n)
+        label = f"{case} chunk {chunk}"
+        assert sg["steps"] == so["steps"] == n, label
+        assert sg["rejected"] == so["rejected"], f"{label}: rejected {sg['rejected']} vs {so['rejected']}"
+        assert sg["active_tiles"] == so["active_tiles"], label
+        assert sg["redist_cells"] == so["redist_cells"], label
+        assert sg["redist_ghost"] == so["redist_ghost"], label
+        worst = _compare(wl, g, o, label)
+        print(f"{label}: {n} steps, rejected {so['rejected']}, redistributed {sg['redist_cells']} "
+              f"(ghost writes {sg['redist_ghost']}), worst {max(worst.values()):.3e}")
+        gs, os_ = g.state(), o.state()
+        m = active_mask(wl.cfg, os_["tile_state"])
+        for f in FIELDS:
+            assert np.array_equal(gs[f][m], os_[f][m]), f"{label}: {f} not bitwise"
+        assert np.array_equal(g.tile_bt()[os_["tile_state"] == 1], o.tile_bt()[os_["tile_state"] == 1])
+    assert np.abs(o.tile_bt()).max() > 1e-4, "the bed must have changed"
+
+
+def test_morpho_redistribution_into_ghost_tiles(gpu):
+    """Without damping, redistribution at the flow edge writes ghost tiles'
+    part of intermed3 (vertex copies, cell w and Hnpsi), which the last RHS
+    of each hydraulic operator reads until the next step refreshes it; the
+    device keeps the same overlay and agrees bit for bit."""
+    wl = workloads.morpho_cap(128, tile=16, **EXACT["fluid_none"])
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    ghost = 0
+    for chunk in range(3):
+        so = o.step_n(40)
+        sg = g.step_n(40)
+        assert sg["redist_ghost"] == so["redist_ghost"]
+        ghost += so["redist_ghost"]
+        _compare(wl, g, o, f"ghost overlay chunk {chunk}")
+    assert ghost > 0, "the deck must exercise redistribution into ghost tiles"
+
+
+def test_morpho_cell_topography_matches(gpu):
+    wl = workloads.morpho_cap(128, tile=16, **EXACT["simple_rat3"])
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    o.step_n(20)
+    g.step_n(20)
+    f13 = o.fields13()
+    topo = g.cell_topography()
+    m = active_mask(wl.cfg, o.state()["tile_state"])
+    for k, idx in (("b0c", 9), ("btc", 10), ("dbdx", 11), ("dbdy", 12)):
+        assert np.array_equal(topo[k][m], f13[idx][m]), k
+
+
+@pytest.mark.parametrize("case", ["mixed", "granular"])
+def test_morpho_transcendental_decks(gpu, case):
+    """Default closures (Mixed erosion with the tanh switch, Spearman-Manning
+    settling, smooth transition, tanh damping): within TOL_TRANSCENDENTAL."""
+    kw = {} if case == "mixed" else dict(erosion="granular", transition="step")
+    wl = workloads.morpho_cap(128, tile=16, **kw)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    so = o.step_n(20)
+    sg = g.step_n(20)
+    assert sg["rejected"] == so["rejected"]
+    worst = _compare(wl, g, o, f"{case} x 20", tol=TOL_TRANSCENDENTAL)
+    print(f"{case}: worst rel {max(worst.values()):.3e}")
+
+
+def test_morpho_maxima(gpu):
+    """Erosion and deposition maxima (UpdateMaximumErosion/Deposit on the
+    pre-step state) with the other running maxima."""
+    wl = workloads.morpho_cap(128, tile=16, **EXACT["simple_rat3"])
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    o.step_n(25)
+    g.step_n(25)
+    mo, mg = o.maxima(), g.maxima()
+    m = active_mask(wl.cfg, o.state()["tile_state"])
+    for k in range(11):
+        assert np.array_equal(mg[k][m], mo[k][m]), f"maxima array {k}"
+    assert mo[6][m].max() > 0
+
+
+def test_morpho_activation_and_integrate_to(gpu):
+    """The flow crosses tile edges during morphodynamic steps; integrate_to
+    clamps the Strang step (2 dt) to the output time."""
+    wl = workloads.morpho_cap(128, tile=16, **EXACT["fluid_none"])
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    for tend in (0.3, 0.75, 1.2):
+        so = o.integrate_to(tend)
+        sg = g.integrate_to(tend)
+        assert sg["steps"] == so["steps"] and sg["rejected"] == so["rejected"]
+        _compare(wl, g, o, f"integrate_to {tend}")
+        assert g.state()["t"] == o.state()["t"] == tend
+    assert so["active_tiles"] > 4
