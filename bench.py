@@ -1,17 +1,26 @@
 """Throughput of the MI355X time step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-plain]
 
 One "step" is one accepted outer step of IntegrateTo (src/TimeStepper.f90:151-275)
-over the active tiles of the workload; the unit of work is the cell-update
-(one active cell advanced through one accepted step).  The timed region holds
-exactly K steps with the state resident in HBM; the host only enqueues the
-step kernels and reads the device's step counter back.
+over the active tiles of the workload -- with morphodynamics the Strang step
+H(dt) M(2dt) H(dt) -- and the unit of work is the cell-update (one active cell
+advanced through one accepted step).
 
-Printed: one JSON line with value = cell-updates/s, the roofline of the
-dominant kernel (the stage-2 kernel, timed with HIP events on the library's
-stream) and the CPU baseline (the oracle, a C restatement, on a bounded sample
-of the same workload).
+The default workload is the full morphodynamic run: 1024x1024 cells of 5 m,
+32 seeded Gaussian hills on a 5 degree ramp, a one-cell flux source of
+q*dx = 100 m^3/s with psi = 0.3, Pouliquen drag, Mixed erosion,
+Spearman-Manning deposition, eddy viscosity 1e-2, FP64, 2000 steps from the
+initial state.  W warm-up steps run on a separate handle; the K timed steps
+start from the initial state with everything resident in HBM, launched as the
+library runs them (hipGraph), bracketed by device synchronisation.
+
+Printed: one JSON line with value = cell-updates/s of those K steps, the
+roofline of the dominant kernel (the hydraulic stage-2 kernel; its mean launch
+time comes from HIP events on the library's stream in a second, profiled pass
+over the same K steps), the plain FP64 hydraulic path at 2048x2048 with its
+own roofline, and the CPU baseline (the oracle, a C restatement, 1 thread, on
+the first steps of the same workload).
 """
 from __future__ import annotations
 
@@ -27,10 +36,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of the stage-2 pass per cell (u1 4 + u^n 4 + b0 1 words
-# read, u2 4 written; FP64) -- the survey's stage-pass model.
+# Algorithmic bytes of the hydraulic stage-2 pass per cell (u1 4 + u^n 4 +
+# b0 1 words read, u2 4 written; FP64) -- the stage-pass model.
 P2_WORDS = 13
-STEP_WORDS = 42
+# Whole step per cell-update: hydraulic 42 words; Strang step 134 words.
+STEP_WORDS = {"hydraulic": 42, "morpho": 134}
 
 
 def cpu_model() -> str:
@@ -44,12 +54,10 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def cpu_baseline(wl_name: str, budget_s: float = 20.0) -> dict:
+def cpu_baseline(wl, budget_s: float = 20.0) -> dict:
     """The oracle (C restatement, 1 thread) on a bounded sample of the
-    workload: the first steps from the same initial state, until budget_s."""
-    from kestrel_amd import workloads
+    workload: its first steps from the same initial state, until budget_s."""
     from oracle.oracle import Oracle
-    wl = getattr(workloads, wl_name)()
     o = Oracle(wl.cfg)
     o.set_topography(wl.b0v)
     o.set_state(wl.init)
@@ -61,21 +69,66 @@ def cpu_baseline(wl_name: str, budget_s: float = 20.0) -> dict:
         cells += s["cell_updates"]
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 1000:
+        if el >= budget_s or steps >= wl.steps:
             break
     o.close()
     return {"value": cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{wl_name}: first {steps} steps ({cells} cell-updates, {el:.1f} s) "
-                      f"on {cpu_model()}, 1 thread"}
+            "sample": f"{wl.name}: first {steps} of {wl.steps} steps ({cells} cell-updates, "
+                      f"{el:.1f} s) on {cpu_model()}, 1 thread"}
+
+
+def run(wl, steps: int, warmup: int):
+    """Warm-up on its own handle, then K graph-launched steps from the initial
+    state (value), then the same K steps again on a fresh handle with every
+    stage kernel bracketed by HIP events (kernel times)."""
+    import torch
+    from kestrel_amd.runtime import Stepper
+
+    def fresh():
+        g = Stepper(wl.cfg)
+        g.set_topography(wl.b0v)
+        g.set_state(wl.init)
+        return g
+
+    if warmup:
+        w = fresh()
+        w.step_n(warmup)
+        w.close()
+    g = fresh()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = g.step_n(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    g.close()
+    g = fresh()
+    g.profile(True)
+    sp = g.step_n(steps)
+    torch.cuda.synchronize()
+    stage_ms, launches = g.profile_read()
+    g.close()
+    assert sp["cell_updates"] == st["cell_updates"], "profiled pass diverged from the timed pass"
+    return st, el, stage_ms, launches
+
+
+def roofline(cells_per_step: float, stage_ms: list) -> dict:
+    p2_s = stage_ms[1] * 1e-3
+    achieved = cells_per_step * P2_WORDS * 8 / p2_s / 1e9 if p2_s > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "stage_kernel<2> (hydraulic RK stage 2)",
+            "algorithmic_bytes_per_cell": P2_WORDS * 8,
+            "cells_per_launch": cells_per_step, "mean_launch_ms": stage_ms[1]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="plain_cap_2048")
+    ap.add_argument("--workload", default="morpho_hills_1024")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -87,35 +140,14 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     from kestrel_amd import workloads
-    from kestrel_amd.runtime import Stepper
 
     wl = getattr(workloads, args.workload)()
-    g = Stepper(wl.cfg)
-    g.set_topography(wl.b0v)
-    g.set_state(wl.init)
-    if args.warmup:
-        g.step_n(args.warmup)
-    torch.cuda.synchronize()
-    # timed region: K steps; every stage kernel is bracketed by HIP events on
-    # the library's stream (direct launches)
-    g.profile(True)
-    t0 = time.perf_counter()
-    st = g.step_n(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    stage_ms, launches = g.profile_read()
-    g.profile(False)
-    # the same K steps again without events, as the library runs them (hipGraph)
-    t1 = time.perf_counter()
-    st2 = g.step_n(args.steps)
-    torch.cuda.synchronize()
-    el2 = time.perf_counter() - t1
-
+    wl.steps = args.steps
+    st, el, stage_ms, launches = run(wl, args.steps, args.warmup)
     cells = st["cell_updates"]
     value = cells / el
     cells_per_step = cells / max(1, st["steps"])
-    p2_s = stage_ms[1] * 1e-3
-    achieved = cells_per_step * P2_WORDS * 8 / p2_s / 1e9 if p2_s > 0 else 0.0
+    mode = "morpho" if wl.cfg.morphodynamics else "hydraulic"
     res = {
         "metric": "cell-updates/s vs HBM roofline (FP64 plain vs batched); p50/p99 scenario latency",
         "value": value,
@@ -131,19 +163,26 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": f"{wl.cfg.NX}x{wl.cfg.NY}", "dx_m": wl.cfg.dx,
                    "tiles": f"{wl.cfg.nx_tiles}x{wl.cfg.ny_tiles} of {wl.cfg.nx_per_tile}x{wl.cfg.ny_per_tile}",
-                   "active_cells_per_step": cells_per_step, "precision": "fp64 plain",
+                   "operator": "Strang H(dt) M(2dt) H(dt)" if mode == "morpho" else "hydraulic",
+                   "active_cells_per_step": cells_per_step, "final_active_tiles": st["active_tiles"],
+                   "simulated_time_s": st["t"], "precision": "fp64 plain",
                    "rhs_evals": st["rhs_evals"], "rejected": st["rejected"],
-                   "stage_kernel_ms": stage_ms},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<2> (RK stage 2)",
-                     "algorithmic_bytes_per_cell": P2_WORDS * 8},
-        "step_roofline_frac": value * STEP_WORDS * 8 / (HBM_PEAK_GBS * 1e9),
-        "graph_run": {"value": st2["cell_updates"] / el2, "ms_per_step": el2 * 1e3 / args.steps,
-                      "note": "next K steps, hipGraph launches, no events"},
+                   "redistributed_cells": st["redist_cells"],
+                   "stage_kernel_ms": stage_ms, "stage_kernel_launches": launches},
+        "roofline": roofline(cells_per_step, stage_ms),
+        "step_roofline_frac": value * STEP_WORDS[mode] * 8 / (HBM_PEAK_GBS * 1e9),
     }
+    if not args.no_plain and args.workload != "plain_cap_2048":
+        pw = workloads.plain_cap_2048(200)
+        pst, pel, pms, _ = run(pw, 200, 10)
+        pcells = pst["cell_updates"] / max(1, pst["steps"])
+        res["plain_fp64"] = {"workload": pw.name, "grid": "2048x2048", "steps": 200,
+                             "value": pst["cell_updates"] / pel, "unit": "cell-updates/s",
+                             "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pcells, pms),
+                             "step_roofline_frac": pst["cell_updates"] / pel * STEP_WORDS["hydraulic"] * 8
+                             / (HBM_PEAK_GBS * 1e9)}
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline("plain_cap_256_sample", args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
     print(json.dumps(res), flush=True)
 
 

@@ -260,52 +260,27 @@ __global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
     }
 }
 
-// ComputeCellCentredTopographicData_ij (src/MorphodynamicRHS.f90:363-416) on
-// the new bed TB: stored for active tiles, returned (bt, gamma) always.
-__device__ void cell_topo_ij(const Dev& d, const Params& p, int t, int i, int j, bool store,
-                             double& btc, double& gam) {
-    const int NV1 = p.nx + 1;
-    const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
-    const double* b = d.vb0 + (size_t)t * NV;
-    const double* bt = d.TB.vbt + (size_t)t * NV;
-#define V(a, ii, jj) a[(jj) * NV1 + (ii)]
-    Kahan kt;
-    kt.add(V(bt, i, j)); kt.add(V(bt, i + 1, j)); kt.add(V(bt, i, j + 1)); kt.add(V(bt, i + 1, j + 1));
-    Kahan kx;
-    kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
-    kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1)); kx.add(-V(b, i, j + 1)); kx.add(-V(bt, i, j + 1));
-    Kahan ky;
-    ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1)); ky.add(-V(b, i, j)); ky.add(-V(bt, i, j));
-    ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(-V(b, i + 1, j)); ky.add(-V(bt, i + 1, j));
-#undef V
-    const size_t gi = (size_t)t * p.NC + j * p.nx + i;
-    const double dbdx = 0.5 * p.dxr * kx.s, dbdy = 0.5 * p.dyr * ky.s;
-    btc = 0.25 * kt.s;
-    gam = gamma_of(p.geometric, dbdx, dbdy);
-    if (store) {
-        d.TB.btc[gi] = btc;
-        d.TB.cbx[gi] = dbdx;
-        d.TB.cby[gi] = dbdy;
-        d.TB.cgam[gi] = gam;
-    }
-}
-
 // A write into a ghost tile's part of intermed3, listed for the next step's refresh.
-__device__ __forceinline__ void note_dirty(const Dev& d, long long e, long long* ghost) {
+__device__ __forceinline__ void note_dirty(const Dev& d, long long e) {
     Ctrl* c = d.ctrl;
-    const long long q = c->ndirty++;
-    if (q < d.dirty_cap) d.dirty[q] = e;
-    ++*ghost;
+    const unsigned long long q = atomicAdd((unsigned long long*)&c->ndirty, 1ULL);
+    if (q < (unsigned long long)d.dirty_cap) d.dirty[q] = e;
 }
 
-// RedistributeCell (src/Redistribute.f90:249-475), 2D, on intermed0 =
-// (SH, TA) and intermed3 = (SM[2], TB) for active tiles.  In ghost tiles the
-// reference writes intermed3 too: vertex copies go to TB.vbt, cell w and
-// Hnpsi to S3 (the stage state the last RHS of each hydraulic operator
-// reads); the cell topography of a ghost cell is recomputed where it is used
-// and not stored, since only intermed3 would hold it.
+// RedistributeCell (src/Redistribute.f90:249-475), 2D, for candidate cell
+// (k, i, j) with correction corr, executed by one wavefront on intermed0 =
+// (SH, TA) and intermed3 = (SM[2], TB).  Every lane evaluates the vertex
+// correction identically; lane 0 writes the vertex copies of every tile that
+// holds them; lanes 0..8 each update one of the 3x3 cells around (i, j),
+// taking the corrected vertices from registers (the cells do not depend on
+// one another).  In ghost tiles the reference writes intermed3 too: vertex
+// copies go to TB.vbt, cell w and Hnpsi to S3 (the stage state the last RHS
+// of each hydraulic operator reads); a ghost cell's topography is recomputed
+// where it is used and not stored, since only intermed3 would hold it.
+// Returns false if the cell borders no depositional vertex (then nothing is
+// written).  *ghost counts writes into ghost tiles.
 __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, double corr,
-                                  long long* ghost) {
+                                  int lane, unsigned long long* ghost) {
     const long long FS = d.FS;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
@@ -314,14 +289,17 @@ __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, i
     double bd[4] = {0.0, 0.0, 0.0, 0.0};
     int di[4] = {0, 0, 0, 0}, dj[4] = {0, 0, 0, 0};
     const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
+    double b3[4];
     int N = 0;
     double sum = 0.0;
     for (int q = 0; q < 4; ++q) {
         const size_t v = (size_t)k * NV + vj[q] * NV1 + vi[q];
-        bd[N] = bt3[v] - bt0[v];
+        const double x3 = bt3[v];
+        bd[N] = x3 - bt0[v];
         if (bd[N] > 0.0) {
             di[N] = vi[q];
             dj[N] = vj[q];
+            b3[N] = x3;
             sum = sum + bd[N];
             N++;
         }
@@ -362,80 +340,108 @@ __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, i
         }
     }
     delta = delta + adj;
-    auto put = [&](int tile, int ii, int jj, double val) {
-        const size_t v = (size_t)tile * NV + jj * NV1 + ii;
-        bt3[v] = val;
-        if (d.tstate[tile] != KST_TILE_ACTIVE) note_dirty(d, (long long)v | (1LL << 62), ghost);
-    };
-    for (int q = 0; q < N; ++q) {
-        const size_t v = (size_t)k * NV + dj[q] * NV1 + di[q];
-        const double nv = bt3[v] - delta * bd[q];
-        bt3[v] = nv;
-        if (di[q] == 0) put(k - 1, nx, dj[q], nv);
-        else if (di[q] == nx) put(k + 1, 0, dj[q], nv);
-        if (dj[q] == 0) {
-            put(k - ntx, di[q], ny, nv);
-            if (di[q] == 0) put(k - ntx - 1, nx, ny, nv);
-            if (di[q] == nx) put(k - ntx + 1, 0, ny, nv);
-        }
-        if (dj[q] == ny) {
-            put(k + ntx, di[q], 0, nv);
-            if (di[q] == 0) put(k + ntx - 1, nx, 0, nv);
-            if (di[q] == nx) put(k + ntx + 1, 0, 0, nv);
+    double nv[4];
+    for (int q = 0; q < N; ++q) nv[q] = b3[q] - delta * bd[q];
+    const int kx0 = (k % ntx) * nx, ky0 = (k / ntx) * ny;   // global origin of tile k
+    if (lane == 0) {
+        auto put = [&](int tile, int ii, int jj, double val) {
+            const size_t v = (size_t)tile * NV + jj * NV1 + ii;
+            bt3[v] = val;
+            if (d.tstate[tile] != KST_TILE_ACTIVE) {
+                note_dirty(d, (long long)v | (1LL << 62));
+                ++*ghost;
+            }
+        };
+        for (int q = 0; q < N; ++q) {
+            bt3[(size_t)k * NV + dj[q] * NV1 + di[q]] = nv[q];
+            if (di[q] == 0) put(k - 1, nx, dj[q], nv[q]);
+            else if (di[q] == nx) put(k + 1, 0, dj[q], nv[q]);
+            if (dj[q] == 0) {
+                put(k - ntx, di[q], ny, nv[q]);
+                if (di[q] == 0) put(k - ntx - 1, nx, ny, nv[q]);
+                if (di[q] == nx) put(k - ntx + 1, 0, ny, nv[q]);
+            }
+            if (dj[q] == ny) {
+                put(k + ntx, di[q], 0, nv[q]);
+                if (di[q] == 0) put(k + ntx - 1, nx, 0, nv[q]);
+                if (di[q] == nx) put(k + ntx + 1, 0, 0, nv[q]);
+            }
         }
     }
-    // the 3x3 cells around (i, j), walking tiles as the reference does
+    if (lane >= 9) return true;
+    // the lane's cell of the 3x3 block (the reference walks the same cells)
     int ct, ci, cj;
-    if (i == 0) { ct = k - 1; ci = nx - 1; }
-    else { ct = k; ci = i - 1; }
-    for (int kk = -1; kk <= 1; ++kk) {
-        if (j == 0) { ct = ct - ntx; cj = ny - 1; }
-        else cj = j - 1;
-        for (int l = -1; l <= 1; ++l) {
-            {
-                const bool act = d.tstate[ct] == KST_TILE_ACTIVE;
-                const size_t g = (size_t)ct * p.NC + cj * nx + ci;
-                double btc, gn;
-                cell_topo_ij(d, p, ct, ci, cj, act, btc, gn);
-                const double dbb = btc - d.TA.btc[g];
-                const double go = d.TA.cgam[g];
-                // intermed0's Hn (ComputeDesingularisedVariables; zero in ghost tiles)
-                double ho = 0.0;
-                if (act) {
-                    ho = compute_hn(d.SH[g], d.b0c[g], d.TA.btc[g], go);
-                    if (ho < 0.0) ho = 0.0;
-                }
-                double w = btc;
-                w = w + (ho * go / gn - dbb / gn) / gn;
-                w = w + d.b0c[g];
-                const double hp = d.SH[3 * FS + g] * go / gn - (1.0 - p.porosity) * dbb / gn;
-                if (act) {
-                    d.SM[2][g] = w;
-                    d.SM[2][FS + g] = hp;
-                } else {
-                    d.S3[g] = w;
-                    d.S3[3 * FS + g] = hp;
-                    note_dirty(d, (long long)g, ghost);
-                }
-            }
-            cj = cj + 1;
-            if (cj == ny && l < 1) { ct = ct + ntx; cj = 0; }
-        }
-        if (j + 1 >= ny) ct = ct - ntx;
-        ci = ci + 1;
-        if (ci == nx) { ct = ct + 1; ci = 0; }
+    map_cell(p, k, i + lane / 3 - 1, j + lane % 3 - 1, ct, ci, cj);
+    const int cx0 = (ct % ntx) * nx, cy0 = (ct / ntx) * ny;
+    const double* vb = d.vb0 + (size_t)ct * NV;
+    auto vbt = [&](int a, int b) -> double {   // tile ct's copy after this correction
+        const int X = cx0 + a, Y = cy0 + b;
+        for (int q = 0; q < N; ++q)
+            if (kx0 + di[q] == X && ky0 + dj[q] == Y) return nv[q];
+        return bt3[(size_t)ct * NV + b * NV1 + a];
+    };
+    const double t00 = vbt(ci, cj), t10 = vbt(ci + 1, cj), t01 = vbt(ci, cj + 1), t11 = vbt(ci + 1, cj + 1);
+#define V0(a, b) vb[(b) * NV1 + (a)]
+    Kahan kt;
+    kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
+    Kahan kx;
+    kx.add(V0(ci + 1, cj)); kx.add(t10); kx.add(-V0(ci, cj)); kx.add(-t00);
+    kx.add(V0(ci + 1, cj + 1)); kx.add(t11); kx.add(-V0(ci, cj + 1)); kx.add(-t01);
+    Kahan ky;
+    ky.add(V0(ci, cj + 1)); ky.add(t01); ky.add(-V0(ci, cj)); ky.add(-t00);
+    ky.add(V0(ci + 1, cj + 1)); ky.add(t11); ky.add(-V0(ci + 1, cj)); ky.add(-t10);
+#undef V0
+    const double btc = 0.25 * kt.s;
+    const double dbdx = 0.5 * p.dxr * kx.s, dbdy = 0.5 * p.dyr * ky.s;
+    const double gn = gamma_of(p.geometric, dbdx, dbdy);
+    const bool act = d.tstate[ct] == KST_TILE_ACTIVE;
+    const size_t g = (size_t)ct * p.NC + cj * nx + ci;
+    const double dbb = btc - d.TA.btc[g];
+    const double go = d.TA.cgam[g];
+    // intermed0's Hn (ComputeDesingularisedVariables; zero in ghost tiles)
+    double ho = 0.0;
+    if (act) {
+        ho = compute_hn(d.SH[g], d.b0c[g], d.TA.btc[g], go);
+        if (ho < 0.0) ho = 0.0;
+    }
+    double w = btc;
+    w = w + (ho * go / gn - dbb / gn) / gn;
+    w = w + d.b0c[g];
+    const double hp = d.SH[3 * FS + g] * go / gn - (1.0 - p.porosity) * dbb / gn;
+    if (act) {
+        d.TB.btc[g] = btc; d.TB.cbx[g] = dbdx; d.TB.cby[g] = dbdy; d.TB.cgam[g] = gn;
+        d.SM[2][g] = w;
+        d.SM[2][FS + g] = hp;
+    } else {
+        d.S3[g] = w;
+        d.S3[3 * FS + g] = hp;
+        note_dirty(d, (long long)g);
+        ++*ghost;
     }
     return true;
 }
 
+constexpr int RD_THREADS = 1024;
+constexpr int RD_WAVES = RD_THREADS / 64;
+constexpr int RD_MAXREADY = 4096;
+
 // AddToRedistList + RedistributeGrid (src/Redistribute.f90:69-244): the
 // candidates the checks marked, sorted by excess deposition (ties in the
 // check loop's order: active-list position, row, column), each re-evaluated
-// and corrected in turn.  One workgroup; the list is bitonic-sorted in place.
-__global__ __launch_bounds__(256) void redistribute_kernel(Dev d, Params p) {
-    __shared__ int sh_n, sh_skip;
+// and corrected in turn.  One workgroup.  Two corrections interact only if
+// their cells are within two cells of each other (a correction reads the 4x4
+// vertices and writes the vertices and 3x3 cells around its cell), so the
+// sorted list is processed in rounds: a candidate whose earlier neighbours
+// within that distance are all done is corrected in the current round, one
+// wavefront per candidate.  Every correction sees exactly the state the
+// sequential walk would give it.
+__global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params p) {
+    __shared__ int sh_n, sh_skip, sh_nready, sh_fail;
+    __shared__ unsigned long long sh_done, sh_ghost;
+    __shared__ int ready[RD_MAXREADY];
     Ctrl* c = d.ctrl;
-    if (threadIdx.x == 0) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
         int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
             const double dt = step_dt(d, p);
@@ -444,6 +450,9 @@ __global__ __launch_bounds__(256) void redistribute_kernel(Dev d, Params p) {
         if (!skip && c->ncand <= 0) skip = 1;
         sh_skip = skip;
         sh_n = c->ncand;
+        sh_fail = 0;
+        sh_done = 0;
+        sh_ghost = 0;
     }
     __syncthreads();
     if (sh_skip) return;
@@ -452,14 +461,14 @@ __global__ __launch_bounds__(256) void redistribute_kernel(Dev d, Params p) {
     while (m2 < n) m2 <<= 1;
     long long* K = d.cand_key;
     double* X = d.cand_ex;
-    for (int q = n + threadIdx.x; q < m2; q += blockDim.x) {
+    for (int q = n + tid; q < m2; q += RD_THREADS) {
         K[q] = 0x7fffffffffffffffLL;
         X[q] = kHuge;
     }
     __syncthreads();
     for (int kk = 2; kk <= m2; kk <<= 1) {
         for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-            for (int q = threadIdx.x; q < m2; q += blockDim.x) {
+            for (int q = tid; q < m2; q += RD_THREADS) {
                 const int l = q ^ jj;
                 if (l <= q) continue;
                 const double xq = X[q], xl = X[l];
@@ -474,24 +483,70 @@ __global__ __launch_bounds__(256) void redistribute_kernel(Dev d, Params p) {
             __syncthreads();
         }
     }
-    if (threadIdx.x != 0) return;
-    long long done = 0, ghost = 0;
-    bool fail = false;
-    for (int q = 0; q < n && !fail; ++q) {
+    // candidate cell -> sorted position, and the done flags
+    const int nx = p.nx, ny = p.ny, NX = p.ntx * nx, NY = p.nty * ny;
+    auto cell_of = [&](int q) -> long long {
         const long long key = K[q];
-        const int tq = (int)(key / p.NC), cell = (int)(key % p.NC);
-        const int k = d.active[tq], i = cell % p.nx, j = cell / p.nx;
-        const size_t gi = (size_t)k * p.NC + cell;
-        double ho, dbt, psio;
-        const double ex = excess_deposition(d, p, gi, d.TB.btc[gi], ho, dbt, psio);
-        if (ex > kEps) {
-            if (!redistribute_cell(d, p, k, i, j, ex, &ghost)) fail = true;
-            else ++done;
-        }
+        return (long long)d.active[key / p.NC] * p.NC + key % p.NC;
+    };
+    for (int q = tid; q < n; q += RD_THREADS) {
+        d.cand_pos[cell_of(q)] = q;
+        d.cand_done[q] = 0;
     }
-    c->redist_cells += done;
-    c->redist_ghost += ghost;
-    if (fail) c->mref |= 2;
+    __syncthreads();
+    const int wave = tid / 64, lane = tid % 64;
+    unsigned long long ghost = 0, done = 0;
+    for (;;) {
+        if (tid == 0) sh_nready = 0;
+        __syncthreads();
+        for (int q = tid; q < n; q += RD_THREADS) {
+            if (d.cand_done[q]) continue;
+            const long long g = cell_of(q);
+            const int t = (int)(g / p.NC), cl = (int)(g % p.NC);
+            const int I = (t % p.ntx) * nx + cl % nx, J = (t / p.ntx) * ny + cl / nx;
+            bool ok = true;
+            for (int dJ = -2; dJ <= 2 && ok; ++dJ)
+                for (int dI = -2; dI <= 2; ++dI) {
+                    const int I2 = I + dI, J2 = J + dJ;
+                    if ((dI == 0 && dJ == 0) || I2 < 0 || J2 < 0 || I2 >= NX || J2 >= NY) continue;
+                    const long long g2 = (long long)((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx;
+                    const int r = d.cand_pos[g2];
+                    if (r >= 0 && r < q && !d.cand_done[r]) { ok = false; break; }
+                }
+            if (ok) {
+                const int s = atomicAdd(&sh_nready, 1);
+                if (s < RD_MAXREADY) ready[s] = q;
+            }
+        }
+        __syncthreads();
+        const int nr = min(sh_nready, RD_MAXREADY);
+        if (nr == 0) break;
+        for (int r = wave; r < nr; r += RD_WAVES) {
+            const long long g = cell_of(ready[r]);
+            const int k = (int)(g / p.NC), cl = (int)(g % p.NC);
+            double ho, dbt, psio;
+            const double ex = excess_deposition(d, p, (size_t)g, d.TB.btc[g], ho, dbt, psio);
+            if (ex > kEps) {
+                if (!redistribute_cell(d, p, k, cl % nx, cl / nx, ex, lane, &ghost)) {
+                    if (lane == 0) sh_fail = 1;
+                } else if (lane == 0) {
+                    ++done;
+                }
+            }
+        }
+        __syncthreads();
+        for (int r = tid; r < nr; r += RD_THREADS) d.cand_done[ready[r]] = 1;
+        __syncthreads();
+    }
+    for (int q = tid; q < n; q += RD_THREADS) d.cand_pos[cell_of(q)] = -1;
+    if (ghost) atomicAdd(&sh_ghost, ghost);
+    if (done) atomicAdd(&sh_done, done);
+    __syncthreads();
+    if (tid == 0) {
+        c->redist_cells += (long long)sh_done;
+        c->redist_ghost += (long long)sh_ghost;
+        if (sh_fail) c->mref |= 2;
+    }
 }
 
 // ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),

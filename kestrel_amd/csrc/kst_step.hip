@@ -101,6 +101,8 @@ struct Dev {
     Topo TB;                          // bed after the morphodynamic operator
     long long* cand_key;              // redistribution candidates: scan-order key
     double* cand_ex;                  // and excess deposition
+    int* cand_pos;                    // cell -> sorted candidate position (-1: none) [tile][NC]
+    int* cand_done;                   // candidate corrected
     int cand_cap;
     int morpho;
     // Redistribution may write ghost tiles' copies of shared vertices and
@@ -1469,7 +1471,7 @@ int launch_iteration(kst_handle* h, int slot) {
         hipLaunchKernelGGL(morpho_kernel<1>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
         hipLaunchKernelGGL(morpho_kernel<2>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
         hipLaunchKernelGGL(morpho_kernel<3>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        hipLaunchKernelGGL(redistribute_kernel, dim3(1), dim3(256), 0, h->stream, d, p);
+        hipLaunchKernelGGL(redistribute_kernel, dim3(1), dim3(RD_THREADS), 0, h->stream, d, p);
         hipLaunchKernelGGL(face_topo_kernel, dim3(p.ntiles), dim3(256), 0, h->stream, d, p);
         // second hydraulic operator
         for (int q = 0; q < 4; ++q)
@@ -1654,6 +1656,8 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
         size_t cap = 1;
         while (cap < FS) cap <<= 1;
         rc |= dalloc(h, &d.cand_key, cap); rc |= dalloc(h, &d.cand_ex, cap);
+        rc |= dalloc(h, &d.cand_pos, FS); rc |= dalloc(h, &d.cand_done, cap);
+        if (!rc) rc |= hipMemsetAsync(d.cand_pos, 0xFF, FS * sizeof(int), h->stream) != hipSuccess;
         d.cand_cap = (int)cap;
         d.dirty_cap = 1 << 16;
         rc |= dalloc(h, &d.dirty, (size_t)d.dirty_cap);

@@ -134,6 +134,12 @@ def morpho_hills(n: int = 1024, steps: int = 2000, tile: int = 32, dx: float = 5
     return Workload(f"morpho_hills_{n}", cfg, b0v, init, steps, [], [src])
 
 
+def morpho_hills_1024(steps: int = 2000) -> Workload:
+    """The full morphodynamic run: 1024x1024 cells of 5 m (32x32 tiles of 32),
+    2000 Strang steps."""
+    return morpho_hills(1024, steps)
+
+
 def morpho_cap(n: int = 128, steps: int = 100, tile: int = 16, dx: float = 1.0,
                psi: float = 0.3, radius: float = 12.0, height: float = 1.0,
                slope_deg: float = 10.0, u: float = 0.0, drag: str = "chezy",

@@ -147,3 +147,22 @@ def test_morpho_activation_and_integrate_to(gpu):
         _compare(wl, g, o, f"integrate_to {tend}")
         assert g.state()["t"] == o.state()["t"] == tend
     assert so["active_tiles"] > 4
+
+
+def test_morpho_hills_flux_source(gpu):
+    """The measured workload's physics (Gaussian hills on a 5 degree ramp, a
+    one-cell flux source with psi = 0.3, Pouliquen drag, Mixed erosion,
+    Spearman-Manning settling, eddy viscosity) on a 256x256 grid: 150
+    Strang steps within TOL_TRANSCENDENTAL of the oracle."""
+    wl = workloads.morpho_hills(256, 150)
+    o = oracle_for(wl)
+    g = stepper_for(wl)
+    for chunk in range(3):
+        so = o.step_n(50)
+        sg = g.step_n(50)
+        assert sg["rejected"] == so["rejected"], f"chunk {chunk}: rejected {sg['rejected']} vs {so['rejected']}"
+        assert sg["redist_cells"] == so["redist_cells"]
+        worst = _compare(wl, g, o, f"hills chunk {chunk}", tol=TOL_TRANSCENDENTAL)
+        print(f"hills chunk {chunk}: rejected {so['rejected']}, active {so['active_tiles']}, "
+              f"worst {max(worst.values()):.3e}")
+    assert np.abs(o.tile_bt()).max() > 0.0
