@@ -267,51 +267,88 @@ __device__ __forceinline__ void note_dirty(const Dev& d, long long e) {
     if (q < (unsigned long long)d.dirty_cap) d.dirty[q] = e;
 }
 
-// RedistributeCell (src/Redistribute.f90:249-475), 2D, for candidate cell
-// (k, i, j) with correction corr, executed by one wavefront on intermed0 =
-// (SH, TA) and intermed3 = (SM[2], TB).  Every lane evaluates the vertex
-// correction identically; lane 0 writes the vertex copies of every tile that
-// holds them; lanes 0..8 each update one of the 3x3 cells around (i, j),
-// taking the corrected vertices from registers (the cells do not depend on
-// one another).  In ghost tiles the reference writes intermed3 too: vertex
-// copies go to TB.vbt, cell w and Hnpsi to S3 (the stage state the last RHS
-// of each hydraulic operator reads); a ghost cell's topography is recomputed
-// where it is used and not stored, since only intermed3 would hold it.
-// Returns false if the cell borders no depositional vertex (then nothing is
-// written).  *ghost counts writes into ghost tiles.
-__device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, double corr,
-                                  int lane, unsigned long long* ghost) {
+// ExcessDeposition + RedistributeCell (src/Redistribute.f90:158-198,249-475),
+// 2D, for candidate cell (k, i, j), executed by one wavefront on intermed0 =
+// (SH, TA) and intermed3 = (SM[2], TB).  Every load is issued up front:
+// lanes 0..8 read the 3x3 cell around (i, j) they will update (cells do not
+// depend on one another; a corrected vertex is taken from registers), every
+// lane reads the candidate cell and its four vertices and evaluates the
+// excess and the vertex correction identically, lane 0 writes the vertex
+// copies of every tile that holds them.  In ghost tiles the reference writes
+// intermed3 too: vertex copies go to TB.vbt, cell w and Hnpsi to S3 (the
+// stage state the last RHS of each hydraulic operator reads); a ghost cell's
+// topography is recomputed where it is used and not stored, since only
+// intermed3 would hold it.  Returns 0 if no correction was needed, 1 if one
+// was made, -1 if the cell borders no depositional vertex (nothing written).
+// *ghost counts writes into ghost tiles.
+__device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, int lane,
+                                 unsigned long long* ghost) {
     const long long FS = d.FS;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     double* bt3 = d.TB.vbt;
     const double* bt0 = d.TA.vbt;
-    double bd[4] = {0.0, 0.0, 0.0, 0.0};
-    int di[4] = {0, 0, 0, 0}, dj[4] = {0, 0, 0, 0};
+    // ---- loads: the lane's 3x3 cell
+    int ct = k, ci = i, cj = j;
+    if (lane < 9) map_cell(p, k, i + lane / 3 - 1, j + lane % 3 - 1, ct, ci, cj);
+    const size_t g = (size_t)ct * p.NC + cj * nx + ci;
+    const double* vb = d.vb0 + (size_t)ct * NV;
+    const double* vt = bt3 + (size_t)ct * NV;
+    double c00 = 0, c10 = 0, c01 = 0, c11 = 0, e00 = 0, e10 = 0, e01 = 0, e11 = 0;
+    double cbt0 = 0, cgo = 1, cw0 = 0, ch0 = 0, cb0 = 0;
+    bool cact = false;
+    if (lane < 9) {
+        c00 = vt[cj * NV1 + ci]; c10 = vt[cj * NV1 + ci + 1];
+        c01 = vt[(cj + 1) * NV1 + ci]; c11 = vt[(cj + 1) * NV1 + ci + 1];
+        e00 = vb[cj * NV1 + ci]; e10 = vb[cj * NV1 + ci + 1];
+        e01 = vb[(cj + 1) * NV1 + ci]; e11 = vb[(cj + 1) * NV1 + ci + 1];
+        cbt0 = d.TA.btc[g]; cgo = d.TA.cgam[g]; cw0 = d.SH[g]; ch0 = d.SH[3 * FS + g]; cb0 = d.b0c[g];
+        cact = d.tstate[ct] == KST_TILE_ACTIVE;
+    }
+    // ---- loads: the candidate cell and its vertices (all lanes)
+    const size_t gc = (size_t)k * p.NC + j * nx + i;
     const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
-    double b3[4];
-    int N = 0;
-    double sum = 0.0;
+    double x3[4], x0[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
         const size_t v = (size_t)k * NV + vj[q] * NV1 + vi[q];
-        const double x3 = bt3[v];
-        bd[N] = x3 - bt0[v];
-        if (bd[N] > 0.0) {
-            di[N] = vi[q];
-            dj[N] = vj[q];
-            b3[N] = x3;
-            sum = sum + bd[N];
+        x3[q] = bt3[v];
+        x0[q] = bt0[v];
+    }
+    const double gamold = d.TA.cgam[gc], btc0 = d.TA.btc[gc], btc3 = d.TB.btc[gc];
+    const double w0 = d.SH[gc], h0 = d.SH[3 * FS + gc], b0c = d.b0c[gc], w3 = d.SM[2][gc];
+    // ---- ExcessDeposition (re-evaluated: earlier corrections may have changed bt)
+    const double ho_raw = compute_hn(w0, b0c, btc0, gamold);
+    const double db = btc3 - btc0;
+    double corr;
+    {
+        const double e = -(h0 * gamold / (1.0 - p.porosity) - db);
+        const double lim = -(ho_raw * gamold - db);
+        corr = (e > lim) ? e : lim;
+    }
+    if (!(corr > kEps)) return 0;
+    // ---- the depositional vertices and the correction
+    double bd[4], nv[4];
+    int di[4], dj[4];
+    int N = 0;
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double bq = x3[q] - x0[q];
+        bd[q] = 0.0; di[q] = -10; dj[q] = -10; nv[q] = 0.0;
+        if (bq > 0.0) {
+            // compact the depositional vertices in order (N is at most q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (r == N) { bd[r] = bq; di[r] = vi[q]; dj[r] = vj[q]; nv[r] = x3[q]; }
+            sum = sum + bq;
             N++;
         }
     }
-    if (N == 0) return false;
+    if (N == 0) return -1;
     double delta = 4.0 * corr / sum;
-    const size_t gc = (size_t)k * p.NC + j * nx + i;
-    const double gamold = d.TA.cgam[gc];
-    double hnold = compute_hn(d.SH[gc], d.b0c[gc], d.TA.btc[gc], gamold);
-    if (hnold < 0.0) hnold = 0.0;
-    const double db = d.TB.btc[gc] - d.TA.btc[gc];
-    double tol = kEps * d.SM[2][gc] * 10.0;
+    const double hnold = ho_raw < 0.0 ? 0.0 : ho_raw;   // intermed0's stored Hn
+    double tol = kEps * w3 * 10.0;
     double adj = 0.0;
     {
         Kahan ks;
@@ -325,7 +362,7 @@ __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, i
             adj = dmax(adj, 0.0);
         }
     }
-    const double hgpo = d.SH[3 * FS + gc] * gamold / (1.0 - p.porosity);
+    const double hgpo = h0 * gamold / (1.0 - p.porosity);
     tol = kEps * 10.0;
     {
         Kahan ks;
@@ -340,9 +377,9 @@ __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, i
         }
     }
     delta = delta + adj;
-    double nv[4];
-    for (int q = 0; q < N; ++q) nv[q] = b3[q] - delta * bd[q];
-    const int kx0 = (k % ntx) * nx, ky0 = (k / ntx) * ny;   // global origin of tile k
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q < N) nv[q] = nv[q] - delta * bd[q];
     if (lane == 0) {
         auto put = [&](int tile, int ii, int jj, double val) {
             const size_t v = (size_t)tile * NV + jj * NV1 + ii;
@@ -368,47 +405,42 @@ __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, i
             }
         }
     }
-    if (lane >= 9) return true;
-    // the lane's cell of the 3x3 block (the reference walks the same cells)
-    int ct, ci, cj;
-    map_cell(p, k, i + lane / 3 - 1, j + lane % 3 - 1, ct, ci, cj);
+    if (lane >= 9) return 1;
+    // ---- the lane's cell: vertex copies of tile ct after the correction
+    const int kx0 = (k % ntx) * nx, ky0 = (k / ntx) * ny;
     const int cx0 = (ct % ntx) * nx, cy0 = (ct / ntx) * ny;
-    const double* vb = d.vb0 + (size_t)ct * NV;
-    auto vbt = [&](int a, int b) -> double {   // tile ct's copy after this correction
+    auto fix = [&](double val, int a, int b) -> double {
         const int X = cx0 + a, Y = cy0 + b;
-        for (int q = 0; q < N; ++q)
-            if (kx0 + di[q] == X && ky0 + dj[q] == Y) return nv[q];
-        return bt3[(size_t)ct * NV + b * NV1 + a];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < N && kx0 + di[q] == X && ky0 + dj[q] == Y) val = nv[q];
+        return val;
     };
-    const double t00 = vbt(ci, cj), t10 = vbt(ci + 1, cj), t01 = vbt(ci, cj + 1), t11 = vbt(ci + 1, cj + 1);
-#define V0(a, b) vb[(b) * NV1 + (a)]
+    const double t00 = fix(c00, ci, cj), t10 = fix(c10, ci + 1, cj);
+    const double t01 = fix(c01, ci, cj + 1), t11 = fix(c11, ci + 1, cj + 1);
     Kahan kt;
     kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
     Kahan kx;
-    kx.add(V0(ci + 1, cj)); kx.add(t10); kx.add(-V0(ci, cj)); kx.add(-t00);
-    kx.add(V0(ci + 1, cj + 1)); kx.add(t11); kx.add(-V0(ci, cj + 1)); kx.add(-t01);
+    kx.add(e10); kx.add(t10); kx.add(-e00); kx.add(-t00);
+    kx.add(e11); kx.add(t11); kx.add(-e01); kx.add(-t01);
     Kahan ky;
-    ky.add(V0(ci, cj + 1)); ky.add(t01); ky.add(-V0(ci, cj)); ky.add(-t00);
-    ky.add(V0(ci + 1, cj + 1)); ky.add(t11); ky.add(-V0(ci + 1, cj)); ky.add(-t10);
-#undef V0
+    ky.add(e01); ky.add(t01); ky.add(-e00); ky.add(-t00);
+    ky.add(e11); ky.add(t11); ky.add(-e10); ky.add(-t10);
     const double btc = 0.25 * kt.s;
     const double dbdx = 0.5 * p.dxr * kx.s, dbdy = 0.5 * p.dyr * ky.s;
     const double gn = gamma_of(p.geometric, dbdx, dbdy);
-    const bool act = d.tstate[ct] == KST_TILE_ACTIVE;
-    const size_t g = (size_t)ct * p.NC + cj * nx + ci;
-    const double dbb = btc - d.TA.btc[g];
-    const double go = d.TA.cgam[g];
+    const double dbb = btc - cbt0;
     // intermed0's Hn (ComputeDesingularisedVariables; zero in ghost tiles)
     double ho = 0.0;
-    if (act) {
-        ho = compute_hn(d.SH[g], d.b0c[g], d.TA.btc[g], go);
+    if (cact) {
+        ho = compute_hn(cw0, cb0, cbt0, cgo);
         if (ho < 0.0) ho = 0.0;
     }
     double w = btc;
-    w = w + (ho * go / gn - dbb / gn) / gn;
-    w = w + d.b0c[g];
-    const double hp = d.SH[3 * FS + g] * go / gn - (1.0 - p.porosity) * dbb / gn;
-    if (act) {
+    w = w + (ho * cgo / gn - dbb / gn) / gn;
+    w = w + cb0;
+    const double hp = ch0 * cgo / gn - (1.0 - p.porosity) * dbb / gn;
+    if (cact) {
         d.TB.btc[g] = btc; d.TB.cbx[g] = dbdx; d.TB.cby[g] = dbdy; d.TB.cgam[g] = gn;
         d.SM[2][g] = w;
         d.SM[2][FS + g] = hp;
@@ -418,27 +450,30 @@ __device__ bool redistribute_cell(const Dev& d, const Params& p, int k, int i, i
         note_dirty(d, (long long)g);
         ++*ghost;
     }
-    return true;
+    return 1;
 }
 
 constexpr int RD_THREADS = 1024;
 constexpr int RD_WAVES = RD_THREADS / 64;
-constexpr int RD_MAXREADY = 4096;
+constexpr int RD_LDS = 2048;          // candidates sorted and levelled in LDS
 
 // AddToRedistList + RedistributeGrid (src/Redistribute.f90:69-244): the
 // candidates the checks marked, sorted by excess deposition (ties in the
 // check loop's order: active-list position, row, column), each re-evaluated
 // and corrected in turn.  One workgroup.  Two corrections interact only if
 // their cells are within two cells of each other (a correction reads the 4x4
-// vertices and writes the vertices and 3x3 cells around its cell), so the
-// sorted list is processed in rounds: a candidate whose earlier neighbours
-// within that distance are all done is corrected in the current round, one
-// wavefront per candidate.  Every correction sees exactly the state the
-// sequential walk would give it.
+// vertices and writes the vertices and 3x3 cells around its cell), so each
+// candidate gets a level one above the highest level of the earlier
+// candidates within that distance; the levels are corrected in order, the
+// candidates of a level concurrently (one wavefront each), and every
+// correction sees exactly the state the sequential walk would give it.
 __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params p) {
-    __shared__ int sh_n, sh_skip, sh_nready, sh_fail;
+    __shared__ int sh_n, sh_skip, sh_fail, sh_maxlev, sh_cnt, sh_changed;
     __shared__ unsigned long long sh_done, sh_ghost;
-    __shared__ int ready[RD_MAXREADY];
+    __shared__ double lx[RD_LDS];
+    __shared__ long long lk[RD_LDS];
+    __shared__ int lev[RD_LDS];
+    __shared__ int list[RD_LDS];
     Ctrl* c = d.ctrl;
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -453,6 +488,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
         sh_fail = 0;
         sh_done = 0;
         sh_ghost = 0;
+        sh_maxlev = 0;
     }
     __syncthreads();
     if (sh_skip) return;
@@ -461,81 +497,98 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
     while (m2 < n) m2 <<= 1;
     long long* K = d.cand_key;
     double* X = d.cand_ex;
-    for (int q = n + tid; q < m2; q += RD_THREADS) {
-        K[q] = 0x7fffffffffffffffLL;
-        X[q] = kHuge;
+    const bool in_lds = m2 <= RD_LDS;
+    // ---- sort by (excess, scan order)
+    if (in_lds) {
+        for (int q = tid; q < m2; q += RD_THREADS) {
+            lk[q] = q < n ? K[q] : 0x7fffffffffffffffLL;
+            lx[q] = q < n ? X[q] : kHuge;
+        }
+    } else {
+        for (int q = n + tid; q < m2; q += RD_THREADS) {
+            K[q] = 0x7fffffffffffffffLL;
+            X[q] = kHuge;
+        }
     }
     __syncthreads();
+    long long* SK = in_lds ? lk : K;
+    double* SX = in_lds ? lx : X;
     for (int kk = 2; kk <= m2; kk <<= 1) {
         for (int jj = kk >> 1; jj > 0; jj >>= 1) {
             for (int q = tid; q < m2; q += RD_THREADS) {
                 const int l = q ^ jj;
                 if (l <= q) continue;
-                const double xq = X[q], xl = X[l];
-                const long long kq = K[q], kl = K[l];
+                const double xq = SX[q], xl = SX[l];
+                const long long kq = SK[q], kl = SK[l];
                 const bool l_less = (xl < xq) || (xl == xq && kl < kq);
                 const bool up = (q & kk) == 0;
                 if (l_less == up) {
-                    X[q] = xl; X[l] = xq;
-                    K[q] = kl; K[l] = kq;
+                    SX[q] = xl; SX[l] = xq;
+                    SK[q] = kl; SK[l] = kq;
                 }
             }
             __syncthreads();
         }
     }
-    // candidate cell -> sorted position, and the done flags
+    // ---- candidate cell -> sorted position
     const int nx = p.nx, ny = p.ny, NX = p.ntx * nx, NY = p.nty * ny;
-    auto cell_of = [&](int q) -> long long {
-        const long long key = K[q];
-        return (long long)d.active[key / p.NC] * p.NC + key % p.NC;
+    auto cell_of = [&](int q) -> int {
+        const long long key = SK[q];
+        return d.active[(int)(key / p.NC)] * p.NC + (int)(key % p.NC);
     };
+    int* LEV = in_lds ? lev : d.cand_done;
     for (int q = tid; q < n; q += RD_THREADS) {
         d.cand_pos[cell_of(q)] = q;
-        d.cand_done[q] = 0;
+        LEV[q] = 1;
     }
     __syncthreads();
-    const int wave = tid / 64, lane = tid % 64;
-    unsigned long long ghost = 0, done = 0;
+    // ---- levels: relax level(q) = 1 + max level of its earlier neighbours
     for (;;) {
-        if (tid == 0) sh_nready = 0;
+        if (tid == 0) sh_changed = 0;
         __syncthreads();
         for (int q = tid; q < n; q += RD_THREADS) {
-            if (d.cand_done[q]) continue;
-            const long long g = cell_of(q);
-            const int t = (int)(g / p.NC), cl = (int)(g % p.NC);
+            const int g = cell_of(q);
+            const int t = g / p.NC, cl = g % p.NC;
             const int I = (t % p.ntx) * nx + cl % nx, J = (t / p.ntx) * ny + cl / nx;
-            bool ok = true;
-            for (int dJ = -2; dJ <= 2 && ok; ++dJ)
+            int l = 1;
+            for (int dJ = -2; dJ <= 2; ++dJ)
                 for (int dI = -2; dI <= 2; ++dI) {
                     const int I2 = I + dI, J2 = J + dJ;
                     if ((dI == 0 && dJ == 0) || I2 < 0 || J2 < 0 || I2 >= NX || J2 >= NY) continue;
-                    const long long g2 = (long long)((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx;
+                    const int g2 = ((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx;
                     const int r = d.cand_pos[g2];
-                    if (r >= 0 && r < q && !d.cand_done[r]) { ok = false; break; }
+                    if (r >= 0 && r < q) l = max(l, LEV[r] + 1);
                 }
-            if (ok) {
-                const int s = atomicAdd(&sh_nready, 1);
-                if (s < RD_MAXREADY) ready[s] = q;
+            if (l != LEV[q]) {
+                LEV[q] = l;
+                sh_changed = 1;
             }
         }
         __syncthreads();
-        const int nr = min(sh_nready, RD_MAXREADY);
-        if (nr == 0) break;
-        for (int r = wave; r < nr; r += RD_WAVES) {
-            const long long g = cell_of(ready[r]);
-            const int k = (int)(g / p.NC), cl = (int)(g % p.NC);
-            double ho, dbt, psio;
-            const double ex = excess_deposition(d, p, (size_t)g, d.TB.btc[g], ho, dbt, psio);
-            if (ex > kEps) {
-                if (!redistribute_cell(d, p, k, cl % nx, cl / nx, ex, lane, &ghost)) {
-                    if (lane == 0) sh_fail = 1;
-                } else if (lane == 0) {
-                    ++done;
-                }
+        if (!sh_changed) break;
+    }
+    for (int q = tid; q < n; q += RD_THREADS) atomicMax(&sh_maxlev, LEV[q]);
+    __syncthreads();
+    // ---- corrections, level by level
+    const int wave = tid / 64, lane = tid % 64;
+    unsigned long long ghost = 0, done = 0;
+    int* L = in_lds ? list : d.cand_done + d.cand_cap;
+    for (int level = 1; level <= sh_maxlev; ++level) {
+        if (tid == 0) sh_cnt = 0;
+        __syncthreads();
+        for (int q = tid; q < n; q += RD_THREADS)
+            if (LEV[q] == level) L[atomicAdd(&sh_cnt, 1)] = q;
+        __syncthreads();
+        const int cnt = sh_cnt;
+        for (int r = wave; r < cnt; r += RD_WAVES) {
+            const int g = cell_of(L[r]);
+            const int k = g / p.NC, cl = g % p.NC;
+            const int res = redistribute_cell(d, p, k, cl % nx, cl / nx, lane, &ghost);
+            if (lane == 0) {
+                if (res < 0) sh_fail = 1;
+                else if (res > 0) ++done;
             }
         }
-        __syncthreads();
-        for (int r = tid; r < nr; r += RD_THREADS) d.cand_done[ready[r]] = 1;
         __syncthreads();
     }
     for (int q = tid; q < n; q += RD_THREADS) d.cand_pos[cell_of(q)] = -1;

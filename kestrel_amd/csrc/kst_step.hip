@@ -102,7 +102,7 @@ struct Dev {
     long long* cand_key;              // redistribution candidates: scan-order key
     double* cand_ex;                  // and excess deposition
     int* cand_pos;                    // cell -> sorted candidate position (-1: none) [tile][NC]
-    int* cand_done;                   // candidate corrected
+    int* cand_done;                   // levels and level lists of large candidate sets (2 x cap)
     int cand_cap;
     int morpho;
     // Redistribution may write ghost tiles' copies of shared vertices and
@@ -1656,7 +1656,7 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
         size_t cap = 1;
         while (cap < FS) cap <<= 1;
         rc |= dalloc(h, &d.cand_key, cap); rc |= dalloc(h, &d.cand_ex, cap);
-        rc |= dalloc(h, &d.cand_pos, FS); rc |= dalloc(h, &d.cand_done, cap);
+        rc |= dalloc(h, &d.cand_pos, FS); rc |= dalloc(h, &d.cand_done, 2 * cap);
         if (!rc) rc |= hipMemsetAsync(d.cand_pos, 0xFF, FS * sizeof(int), h->stream) != hipSuccess;
         d.cand_cap = (int)cap;
         d.dirty_cap = 1 << 16;
