@@ -15,12 +15,13 @@ initial state.  W warm-up steps run on a separate handle; the K timed steps
 start from the initial state with everything resident in HBM, launched as the
 library runs them (hipGraph), bracketed by device synchronisation.
 
-Printed: one JSON line with value = cell-updates/s of those K steps, the
-roofline of the dominant kernel (the hydraulic stage-2 kernel; its mean launch
-time comes from HIP events on the library's stream in a second, profiled pass
-over the same K steps), the plain FP64 hydraulic path at 2048x2048 with its
-own roofline, and the CPU baseline (the oracle, a C restatement, 1 thread, on
-the first steps of the same workload).
+Printed: one JSON line with value = cell-updates/s of those K steps; the
+per-kernel launch times of a second, profiled pass over the same K steps (HIP
+events on the library's stream around every launch, launches that did work
+only); the roofline of the dominant kernel (the most total time); the plain
+FP64 hydraulic path at 2048x2048 with its own roofline; and the CPU baseline
+(the oracle, a C restatement, 1 thread, on the first steps of the same
+workload).
 """
 from __future__ import annotations
 
@@ -36,11 +37,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of the hydraulic stage-2 pass per cell (u1 4 + u^n 4 +
-# b0 1 words read, u2 4 written; FP64) -- the stage-pass model.
-P2_WORDS = 13
+# Algorithmic FP64 words per cell of one launch, stage-pass model: P1 reads
+# u^n 4 + b0 1, writes u^1 4; P2 and P3 read their stage 4 + u^n 4 + b0 1,
+# write 4; P4 reads u^3 4 + b0 1, writes momentum 2.  Over a changing bed each
+# pass also reads the cell bt (1).  A morphodynamic SSP stage reads u_k 4,
+# b0 1, bt_k 1, bt_0 1, w and Hnpsi of u^0 2 and writes bt, w, Hnpsi 3.
+STAGE_WORDS = {"hydraulic": [9, 13, 13, 7], "morpho": [10, 14, 14, 8]}
+MORPHO_STAGE_WORDS = 12
+# One RedistributeCell: 13 words read for each of the 9 cells of its 3x3
+# block (4 vertex bt, 4 vertex b0, bt, gamma, w, Hnpsi, b0 of intermed0), 13
+# for the corrected cell itself, 6 written per cell of the block and up to 4
+# vertices: 188 words per corrected cell.
+REDIST_WORDS = 188
 # Whole step per cell-update: hydraulic 42 words; Strang step 134 words.
 STEP_WORDS = {"hydraulic": 42, "morpho": 134}
+# HBM traffic of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script (mean KiB per productive dispatch, active
+# cells per launch of that run), scaled by the counters' reading for a plain
+# copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Summaries and commands:
+# profiles/stage_kernel_plain_cap_2048.md, profiles/kernels_morpho_hills_1024.md.
+PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
+PMC_TRAFFIC = {
+    ("plain_cap_2048", "stage_kernel<2>"): (211837.66, 51255.44, 1633792.0),
+}
 
 
 def cpu_model() -> str:
@@ -80,7 +99,7 @@ def cpu_baseline(wl, budget_s: float = 20.0) -> dict:
 def run(wl, steps: int, warmup: int):
     """Warm-up on its own handle, then K graph-launched steps from the initial
     state (value), then the same K steps again on a fresh handle with every
-    stage kernel bracketed by HIP events (kernel times)."""
+    kernel launch bracketed by HIP events (per-kernel launch times)."""
     import torch
     from kestrel_amd.runtime import Stepper
 
@@ -105,20 +124,51 @@ def run(wl, steps: int, warmup: int):
     g.profile(True)
     sp = g.step_n(steps)
     torch.cuda.synchronize()
-    stage_ms, launches = g.profile_read()
+    prof = g.profile_read()
     g.close()
     assert sp["cell_updates"] == st["cell_updates"], "profiled pass diverged from the timed pass"
-    return st, el, stage_ms, launches
+    return st, el, prof
 
 
-def roofline(cells_per_step: float, stage_ms: list) -> dict:
-    p2_s = stage_ms[1] * 1e-3
-    achieved = cells_per_step * P2_WORDS * 8 / p2_s / 1e9 if p2_s > 0 else 0.0
+def algorithmic_bytes(name: str, mode: str, cells: float, redist_per_launch: float) -> float:
+    """Algorithmic bytes of one launch of kernel `name` (FP64 words x 8)."""
+    if name.startswith("stage_kernel<"):
+        return STAGE_WORDS[mode][int(name[13]) - 1] * 8 * cells
+    if name.startswith("morpho_kernel<"):
+        return MORPHO_STAGE_WORDS * 8 * cells
+    if name == "redistribute_kernel":
+        return REDIST_WORDS * 8 * redist_per_launch
+    return 0.0
+
+
+def kernel_table(prof: dict, mode: str, cells: float, st: dict) -> list:
+    nred = prof.get("redistribute_kernel", (0.0, 0))[1]
+    rpl = st["redist_cells"] / nred if nred else 0.0
+    total = sum(ms * n for ms, n in prof.values())
+    rows = []
+    for name, (ms, n) in sorted(prof.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        rows.append({"kernel": name, "launches": n, "mean_us": ms * 1e3,
+                     "share": ms * n / total if total else 0.0,
+                     "algorithmic_bytes": algorithmic_bytes(name, mode, cells, rpl)})
+    return rows
+
+
+def roofline(wl_name: str, rows: list, cells: float) -> dict:
+    """HBM roofline of the dominant kernel (the most total time of the launches
+    that did work) of the profiled pass."""
+    top = rows[0]
+    s = top["mean_us"] * 1e-6
+    achieved = top["algorithmic_bytes"] / s / 1e9 if s > 0 else 0.0
+    traffic = None
+    base = top["kernel"].replace(" (H2)", "")
+    key = (wl_name, base)
+    if key in PMC_TRAFFIC:
+        f, w, c0 = PMC_TRAFFIC[key]
+        traffic = (f * PMC_FETCH_SCALE + w * PMC_WRITE_SCALE) * 1024 * cells / c0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "stage_kernel<2> (hydraulic RK stage 2)",
-            "algorithmic_bytes_per_cell": P2_WORDS * 8,
-            "cells_per_launch": cells_per_step, "mean_launch_ms": stage_ms[1]}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": top["kernel"], "algorithmic_bytes_per_launch": top["algorithmic_bytes"],
+            "mean_launch_us": top["mean_us"], "cells_per_launch": cells}
 
 
 def main():
@@ -143,11 +193,12 @@ def main():
 
     wl = getattr(workloads, args.workload)()
     wl.steps = args.steps
-    st, el, stage_ms, launches = run(wl, args.steps, args.warmup)
+    st, el, prof = run(wl, args.steps, args.warmup)
     cells = st["cell_updates"]
     value = cells / el
     cells_per_step = cells / max(1, st["steps"])
     mode = "morpho" if wl.cfg.morphodynamics else "hydraulic"
+    rows = kernel_table(prof, mode, cells_per_step, st)
     res = {
         "metric": "cell-updates/s vs HBM roofline (FP64 plain vs batched); p50/p99 scenario latency",
         "value": value,
@@ -167,18 +218,19 @@ def main():
                    "active_cells_per_step": cells_per_step, "final_active_tiles": st["active_tiles"],
                    "simulated_time_s": st["t"], "precision": "fp64 plain",
                    "rhs_evals": st["rhs_evals"], "rejected": st["rejected"],
-                   "redistributed_cells": st["redist_cells"],
-                   "stage_kernel_ms": stage_ms, "stage_kernel_launches": launches},
-        "roofline": roofline(cells_per_step, stage_ms),
+                   "redistributed_cells": st["redist_cells"]},
+        "roofline": roofline(wl.name, rows, cells_per_step),
+        "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
         "step_roofline_frac": value * STEP_WORDS[mode] * 8 / (HBM_PEAK_GBS * 1e9),
     }
     if not args.no_plain and args.workload != "plain_cap_2048":
         pw = workloads.plain_cap_2048(200)
-        pst, pel, pms, _ = run(pw, 200, 10)
+        pst, pel, pprof = run(pw, 200, 10)
         pcells = pst["cell_updates"] / max(1, pst["steps"])
+        prows = kernel_table(pprof, "hydraulic", pcells, pst)
         res["plain_fp64"] = {"workload": pw.name, "grid": "2048x2048", "steps": 200,
                              "value": pst["cell_updates"] / pel, "unit": "cell-updates/s",
-                             "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pcells, pms),
+                             "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pw.name, prows, pcells),
                              "step_roofline_frac": pst["cell_updates"] / pel * STEP_WORDS["hydraulic"] * 8
                              / (HBM_PEAK_GBS * 1e9)}
     if not args.no_cpu:

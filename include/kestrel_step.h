@@ -184,12 +184,26 @@ int kst_last_error(const kst_handle* h, char* buf, int buflen);
 void kst_destroy(kst_handle* h);
 
 /* Diagnostics (no counterpart in the reference, which has no timers;
- * src/Messages.f90:275-283 prints only t and dt).  While profiling is on,
- * every RK-stage kernel launch is bracketed by HIP events on the handle's
- * stream; kst_profile_read returns the mean duration in ms of the launches
- * that did work, per stage kernel (P1..P4), and their counts, then resets. */
+ * src/Messages.f90:275-283 prints only t and dt).  While profiling is on, the
+ * library launches one iteration at a time and brackets every kernel launch
+ * with HIP events on the handle's stream; kst_profile_read returns, per
+ * kernel id below, the mean duration in ms of the launches that did work
+ * (launches that only skip after a rejected check or the end of a run are
+ * left out) and their counts, then resets. */
+#define KST_PROFILE_KERNELS 16
+enum kst_profile_kernel {
+    KST_K_STAGE1 = 0,       /* hydraulic RK stages 1..4: ids 0..3 */
+    KST_K_STAGE1_H2 = 4,    /* the second hydraulic operator of a Strang step: 4..7 */
+    KST_K_MORPHO1 = 8,      /* morphodynamic SSP stages 1..3: 8..10 */
+    KST_K_REDISTRIBUTE = 11,
+    KST_K_FACE_TOPO = 12,
+    KST_K_COMMIT = 13,
+    KST_K_ACTIVATE = 14,
+    KST_K_FINALIZE = 15
+};
 int kst_profile(kst_handle* h, int enable);
-int kst_profile_read(kst_handle* h, double stage_ms[4], int64_t launches[4]);
+int kst_profile_read(kst_handle* h, double ms[KST_PROFILE_KERNELS],
+                     int64_t launches[KST_PROFILE_KERNELS]);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,7 @@ struct MWork {
     double dt;
 };
 
-__device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_no_refine) {
+__device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_no_refine, int kw) {
     const Ctrl* c = d.ctrl;
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
@@ -35,6 +35,7 @@ __device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_n
         if (!first_operator_ok(c, p, dt)) skip = 1;
         if (need_no_refine && c->mref) skip = 1;
     }
+    if (!skip) count_work(c, kw);
     const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
     const int nwork = c->n_active * spt;
     const int per = (nwork + 7) / 8;
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
     __shared__ double vb[MV * MV];         // new vertex bt of this tile's copy
     __shared__ MWork m;
     const int tid = threadIdx.x;
-    if (tid == 0) morpho_work(d, p, m, false);
+    if (tid == 0) morpho_work(d, p, m, false, KW_MORPHO + S - 1);
     __syncthreads();
     if (m.skip) return;
     const long long FS = d.FS;
@@ -267,57 +268,87 @@ __device__ __forceinline__ void note_dirty(const Dev& d, long long e) {
     if (q < (unsigned long long)d.dirty_cap) d.dirty[q] = e;
 }
 
-// ExcessDeposition + RedistributeCell (src/Redistribute.f90:158-198,249-475),
-// 2D, for candidate cell (k, i, j), executed by one wavefront on intermed0 =
-// (SH, TA) and intermed3 = (SM[2], TB).  Every load is issued up front:
-// lanes 0..8 read the 3x3 cell around (i, j) they will update (cells do not
-// depend on one another; a corrected vertex is taken from registers), every
-// lane reads the candidate cell and its four vertices and evaluates the
-// excess and the vertex correction identically, lane 0 writes the vertex
-// copies of every tile that holds them.  In ghost tiles the reference writes
-// intermed3 too: vertex copies go to TB.vbt, cell w and Hnpsi to S3 (the
-// stage state the last RHS of each hydraulic operator reads); a ghost cell's
-// topography is recomputed where it is used and not stored, since only
-// intermed3 would hold it.  Returns 0 if no correction was needed, 1 if one
-// was made, -1 if the cell borders no depositional vertex (nothing written).
-// *ghost counts writes into ghost tiles.
-__device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, int lane,
-                                 unsigned long long* ghost) {
-    const long long FS = d.FS;
+// The 3x3 update of RedistributeCell (src/Redistribute.f90:402-470) for one
+// cell: topography from its four vertex bt (t) and b0 (e), then w and Hnpsi
+// from intermed0's Hn, Hnpsi and gamma (cbt0, cgo, cw0, ch0, cb0; Hn is zero
+// in ghost tiles, whose derived fields the reference never computes).
+struct CellRe {
+    double btc, dbdx, dbdy, gn, w, hp;
+};
+
+__device__ __forceinline__ CellRe recompute_cell(const Params& p, double t00, double t10, double t01,
+                                                 double t11, double e00, double e10, double e01,
+                                                 double e11, double cbt0, double cgo, double cw0,
+                                                 double ch0, double cb0, bool cact) {
+    Kahan kt;
+    kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
+    Kahan kx;
+    kx.add(e10); kx.add(t10); kx.add(-e00); kx.add(-t00);
+    kx.add(e11); kx.add(t11); kx.add(-e01); kx.add(-t01);
+    Kahan ky;
+    ky.add(e01); ky.add(t01); ky.add(-e00); ky.add(-t00);
+    ky.add(e11); ky.add(t11); ky.add(-e10); ky.add(-t10);
+    CellRe r;
+    r.btc = 0.25 * kt.s;
+    r.dbdx = 0.5 * p.dxr * kx.s;
+    r.dbdy = 0.5 * p.dyr * ky.s;
+    r.gn = gamma_of(p.geometric, r.dbdx, r.dbdy);
+    const double dbb = r.btc - cbt0;
+    double ho = 0.0;
+    if (cact) {
+        ho = compute_hn(cw0, cb0, cbt0, cgo);
+        if (ho < 0.0) ho = 0.0;
+    }
+    double w = r.btc;
+    w = w + (ho * cgo / r.gn - dbb / r.gn) / r.gn;
+    r.w = w + cb0;
+    r.hp = ch0 * cgo / r.gn - (1.0 - p.porosity) * dbb / r.gn;
+    return r;
+}
+
+// One corrected vertex (di, dj) of tile k: the tile's own copy and those of
+// the tiles that share it (src/Redistribute.f90:330-398).  Copies in ghost
+// tiles are listed for the next step's refresh and counted.
+// With write = false the copies are only counted.
+__device__ __forceinline__ void put_vertex(const Dev& d, const Params& p, int k, int di, int dj,
+                                           double val, unsigned long long* ghost, bool write = true) {
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     double* bt3 = d.TB.vbt;
-    const double* bt0 = d.TA.vbt;
-    // ---- loads: the lane's 3x3 cell
-    int ct = k, ci = i, cj = j;
-    if (lane < 9) map_cell(p, k, i + lane / 3 - 1, j + lane % 3 - 1, ct, ci, cj);
-    const size_t g = (size_t)ct * p.NC + cj * nx + ci;
-    const double* vb = d.vb0 + (size_t)ct * NV;
-    const double* vt = bt3 + (size_t)ct * NV;
-    double c00 = 0, c10 = 0, c01 = 0, c11 = 0, e00 = 0, e10 = 0, e01 = 0, e11 = 0;
-    double cbt0 = 0, cgo = 1, cw0 = 0, ch0 = 0, cb0 = 0;
-    bool cact = false;
-    if (lane < 9) {
-        c00 = vt[cj * NV1 + ci]; c10 = vt[cj * NV1 + ci + 1];
-        c01 = vt[(cj + 1) * NV1 + ci]; c11 = vt[(cj + 1) * NV1 + ci + 1];
-        e00 = vb[cj * NV1 + ci]; e10 = vb[cj * NV1 + ci + 1];
-        e01 = vb[(cj + 1) * NV1 + ci]; e11 = vb[(cj + 1) * NV1 + ci + 1];
-        cbt0 = d.TA.btc[g]; cgo = d.TA.cgam[g]; cw0 = d.SH[g]; ch0 = d.SH[3 * FS + g]; cb0 = d.b0c[g];
-        cact = d.tstate[ct] == KST_TILE_ACTIVE;
+    auto put = [&](int tile, int ii, int jj) {
+        const size_t v = (size_t)tile * NV + jj * NV1 + ii;
+        if (write) bt3[v] = val;
+        if (d.tstate[tile] != KST_TILE_ACTIVE) {
+            if (write) note_dirty(d, (long long)v | (1LL << 62));
+            ++*ghost;
+        }
+    };
+    if (write) bt3[(size_t)k * NV + dj * NV1 + di] = val;
+    if (di == 0) put(k - 1, nx, dj);
+    else if (di == nx) put(k + 1, 0, dj);
+    if (dj == 0) {
+        put(k - ntx, di, ny);
+        if (di == 0) put(k - ntx - 1, nx, ny);
+        if (di == nx) put(k - ntx + 1, 0, ny);
     }
-    // ---- loads: the candidate cell and its vertices (all lanes)
-    const size_t gc = (size_t)k * p.NC + j * nx + i;
+    if (dj == ny) {
+        put(k + ntx, di, 0);
+        if (di == 0) put(k + ntx - 1, nx, 0);
+        if (di == nx) put(k + ntx + 1, 0, 0);
+    }
+}
+
+// The excess deposition of a candidate re-evaluated against the current bed
+// and the vertex correction that removes it (src/Redistribute.f90:158-198,
+// 280-330): returns 0 if no correction is needed, -1 if the cell borders no
+// depositional vertex, else 1 with the new values nv of the N depositional
+// vertices (di, dj) in vertex order.  x3/x0: the cell's vertex bt now and in
+// intermed0; btc3/w3: its bt and w now.
+__device__ __forceinline__ int vertex_correction(const Params& p, int i, int j, const double* x3,
+                                                 const double* x0, double gamold, double btc0,
+                                                 double btc3, double w0, double h0, double b0c,
+                                                 double w3, double* nv, int* di, int* dj, int& N) {
     const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
-    double x3[4], x0[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const size_t v = (size_t)k * NV + vj[q] * NV1 + vi[q];
-        x3[q] = bt3[v];
-        x0[q] = bt0[v];
-    }
-    const double gamold = d.TA.cgam[gc], btc0 = d.TA.btc[gc], btc3 = d.TB.btc[gc];
-    const double w0 = d.SH[gc], h0 = d.SH[3 * FS + gc], b0c = d.b0c[gc], w3 = d.SM[2][gc];
-    // ---- ExcessDeposition (re-evaluated: earlier corrections may have changed bt)
     const double ho_raw = compute_hn(w0, b0c, btc0, gamold);
     const double db = btc3 - btc0;
     double corr;
@@ -326,11 +357,9 @@ __device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, in
         const double lim = -(ho_raw * gamold - db);
         corr = (e > lim) ? e : lim;
     }
+    N = 0;
     if (!(corr > kEps)) return 0;
-    // ---- the depositional vertices and the correction
-    double bd[4], nv[4];
-    int di[4], dj[4];
-    int N = 0;
+    double bd[4];
     double sum = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -380,35 +409,69 @@ __device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, in
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         if (q < N) nv[q] = nv[q] - delta * bd[q];
-    if (lane == 0) {
-        auto put = [&](int tile, int ii, int jj, double val) {
-            const size_t v = (size_t)tile * NV + jj * NV1 + ii;
-            bt3[v] = val;
-            if (d.tstate[tile] != KST_TILE_ACTIVE) {
-                note_dirty(d, (long long)v | (1LL << 62));
-                ++*ghost;
-            }
-        };
-        for (int q = 0; q < N; ++q) {
-            bt3[(size_t)k * NV + dj[q] * NV1 + di[q]] = nv[q];
-            if (di[q] == 0) put(k - 1, nx, dj[q], nv[q]);
-            else if (di[q] == nx) put(k + 1, 0, dj[q], nv[q]);
-            if (dj[q] == 0) {
-                put(k - ntx, di[q], ny, nv[q]);
-                if (di[q] == 0) put(k - ntx - 1, nx, ny, nv[q]);
-                if (di[q] == nx) put(k - ntx + 1, 0, ny, nv[q]);
-            }
-            if (dj[q] == ny) {
-                put(k + ntx, di[q], 0, nv[q]);
-                if (di[q] == 0) put(k + ntx - 1, nx, 0, nv[q]);
-                if (di[q] == nx) put(k + ntx + 1, 0, 0, nv[q]);
-            }
-        }
+    return 1;
+}
+
+// ExcessDeposition + RedistributeCell (src/Redistribute.f90:158-198,249-475),
+// 2D, for candidate cell (k, i, j), executed by one wavefront on intermed0 =
+// (SH, TA) and intermed3 = (SM[2], TB).  Every load is issued up front:
+// lanes 0..8 read the 3x3 cell around (i, j) they will update (cells do not
+// depend on one another; a corrected vertex is taken from registers), every
+// lane reads the candidate cell and its four vertices and evaluates the
+// excess and the vertex correction identically, lane 0 writes the vertex
+// copies of every tile that holds them.  In ghost tiles the reference writes
+// intermed3 too: vertex copies go to TB.vbt, cell w and Hnpsi to S3 (the
+// stage state the last RHS of each hydraulic operator reads); a ghost cell's
+// topography is recomputed where it is used and not stored, since only
+// intermed3 would hold it.  Returns 0 if no correction was needed, 1 if one
+// was made, -1 if the cell borders no depositional vertex (nothing written).
+// *ghost counts writes into ghost tiles.
+__device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, int lane,
+                                 unsigned long long* ghost) {
+    const long long FS = d.FS;
+    const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
+    const size_t NV = (size_t)(nx + 1) * (ny + 1);
+    double* bt3 = d.TB.vbt;
+    const double* bt0 = d.TA.vbt;
+    // ---- loads: the lane's 3x3 cell
+    int ct = k, ci = i, cj = j;
+    if (lane < 9) map_cell(p, k, i + lane / 3 - 1, j + lane % 3 - 1, ct, ci, cj);
+    const size_t g = (size_t)ct * p.NC + cj * nx + ci;
+    const double* vb = d.vb0 + (size_t)ct * NV;
+    const double* vt = bt3 + (size_t)ct * NV;
+    double c00 = 0, c10 = 0, c01 = 0, c11 = 0, e00 = 0, e10 = 0, e01 = 0, e11 = 0;
+    double cbt0 = 0, cgo = 1, cw0 = 0, ch0 = 0, cb0 = 0;
+    bool cact = false;
+    if (lane < 9) {
+        c00 = vt[cj * NV1 + ci]; c10 = vt[cj * NV1 + ci + 1];
+        c01 = vt[(cj + 1) * NV1 + ci]; c11 = vt[(cj + 1) * NV1 + ci + 1];
+        e00 = vb[cj * NV1 + ci]; e10 = vb[cj * NV1 + ci + 1];
+        e01 = vb[(cj + 1) * NV1 + ci]; e11 = vb[(cj + 1) * NV1 + ci + 1];
+        cbt0 = d.TA.btc[g]; cgo = d.TA.cgam[g]; cw0 = d.SH[g]; ch0 = d.SH[3 * FS + g]; cb0 = d.b0c[g];
+        cact = d.tstate[ct] == KST_TILE_ACTIVE;
     }
+    // ---- loads: the candidate cell and its vertices (all lanes)
+    const size_t gc = (size_t)k * p.NC + j * nx + i;
+    const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
+    double x3[4], x0[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const size_t v = (size_t)k * NV + vj[q] * NV1 + vi[q];
+        x3[q] = bt3[v];
+        x0[q] = bt0[v];
+    }
+    const double gamold = d.TA.cgam[gc], btc0 = d.TA.btc[gc], btc3 = d.TB.btc[gc];
+    const double w0 = d.SH[gc], h0 = d.SH[3 * FS + gc], b0c = d.b0c[gc], w3 = d.SM[2][gc];
+    double nv[4];
+    int di[4], dj[4], N;
+    const int res = vertex_correction(p, i, j, x3, x0, gamold, btc0, btc3, w0, h0, b0c, w3, nv, di, dj, N);
+    if (res <= 0) return res;
+    if (lane == 0)
+        for (int q = 0; q < N; ++q) put_vertex(d, p, k, di[q], dj[q], nv[q], ghost);
     if (lane >= 9) return 1;
     // ---- the lane's cell: vertex copies of tile ct after the correction
-    const int kx0 = (k % ntx) * nx, ky0 = (k / ntx) * ny;
-    const int cx0 = (ct % ntx) * nx, cy0 = (ct / ntx) * ny;
+    const int kx0 = (k % p.ntx) * nx, ky0 = (k / p.ntx) * ny;
+    const int cx0 = (ct % p.ntx) * nx, cy0 = (ct / p.ntx) * ny;
     auto fix = [&](double val, int a, int b) -> double {
         const int X = cx0 + a, Y = cy0 + b;
 #pragma unroll
@@ -416,37 +479,16 @@ __device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, in
             if (q < N && kx0 + di[q] == X && ky0 + dj[q] == Y) val = nv[q];
         return val;
     };
-    const double t00 = fix(c00, ci, cj), t10 = fix(c10, ci + 1, cj);
-    const double t01 = fix(c01, ci, cj + 1), t11 = fix(c11, ci + 1, cj + 1);
-    Kahan kt;
-    kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
-    Kahan kx;
-    kx.add(e10); kx.add(t10); kx.add(-e00); kx.add(-t00);
-    kx.add(e11); kx.add(t11); kx.add(-e01); kx.add(-t01);
-    Kahan ky;
-    ky.add(e01); ky.add(t01); ky.add(-e00); ky.add(-t00);
-    ky.add(e11); ky.add(t11); ky.add(-e10); ky.add(-t10);
-    const double btc = 0.25 * kt.s;
-    const double dbdx = 0.5 * p.dxr * kx.s, dbdy = 0.5 * p.dyr * ky.s;
-    const double gn = gamma_of(p.geometric, dbdx, dbdy);
-    const double dbb = btc - cbt0;
-    // intermed0's Hn (ComputeDesingularisedVariables; zero in ghost tiles)
-    double ho = 0.0;
+    const CellRe r = recompute_cell(p, fix(c00, ci, cj), fix(c10, ci + 1, cj), fix(c01, ci, cj + 1),
+                                    fix(c11, ci + 1, cj + 1), e00, e10, e01, e11, cbt0, cgo, cw0,
+                                    ch0, cb0, cact);
     if (cact) {
-        ho = compute_hn(cw0, cb0, cbt0, cgo);
-        if (ho < 0.0) ho = 0.0;
-    }
-    double w = btc;
-    w = w + (ho * cgo / gn - dbb / gn) / gn;
-    w = w + cb0;
-    const double hp = ch0 * cgo / gn - (1.0 - p.porosity) * dbb / gn;
-    if (cact) {
-        d.TB.btc[g] = btc; d.TB.cbx[g] = dbdx; d.TB.cby[g] = dbdy; d.TB.cgam[g] = gn;
-        d.SM[2][g] = w;
-        d.SM[2][FS + g] = hp;
+        d.TB.btc[g] = r.btc; d.TB.cbx[g] = r.dbdx; d.TB.cby[g] = r.dbdy; d.TB.cgam[g] = r.gn;
+        d.SM[2][g] = r.w;
+        d.SM[2][FS + g] = r.hp;
     } else {
-        d.S3[g] = w;
-        d.S3[3 * FS + g] = hp;
+        d.S3[g] = r.w;
+        d.S3[3 * FS + g] = r.hp;
         note_dirty(d, (long long)g);
         ++*ghost;
     }
@@ -457,16 +499,220 @@ constexpr int RD_THREADS = 1024;
 constexpr int RD_WAVES = RD_THREADS / 64;
 constexpr int RD_LDS = 2048;          // candidates sorted and levelled in LDS
 
+// Neighbour k (0..7) of a cell at offset (kDI[k], kDJ[k]).
+__constant__ const int kDI[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+__constant__ const int kDJ[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+
+__device__ void redistribute_small(const Dev& d, const Params& p, int n, double* lx, long long* lk,
+                                   int* lev, int* cellid) {
+    __shared__ double slot[RD_THREADS][4];     // vertex bt of each candidate's cell
+    __shared__ short nbr[RD_THREADS][8];       // sorted position of the candidate at neighbour k
+    __shared__ unsigned char rec[RD_THREADS];  // its bt and w were recomputed by a correction
+    __shared__ unsigned char res[RD_THREADS];  // 0 none, 1 corrected, 2 no depositional vertex
+    __shared__ unsigned char dmask[RD_THREADS];
+    __shared__ int s_stop;
+    __shared__ unsigned long long s_done, s_ghost;
+    Ctrl* c = d.ctrl;
+    const int tid = threadIdx.x;
+    const long long FS = d.FS;
+    const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
+    const int NX = p.ntx * nx, NY = p.nty * ny;
+    const size_t NV = (size_t)(nx + 1) * (ny + 1);
+    if (tid == 0) { s_stop = n; s_done = 0; s_ghost = 0; }
+    if (tid < n) { lk[tid] = d.cand_key[tid]; lx[tid] = d.cand_ex[tid]; }
+    __syncthreads();
+    // ---- rank of each candidate in (excess, scan order)
+    if (tid < n) {
+        const double x = lx[tid];
+        const long long kk = lk[tid];
+        int r = 0;
+#pragma unroll 16
+        for (int u = 0; u < n; ++u) {
+            const double xu = lx[u];
+            const long long ku = lk[u];
+            r += (int)((xu < x) | ((xu == x) & (ku < kk)));
+        }
+        cellid[r] = d.active[(int)(kk / p.NC)] * p.NC + (int)(kk % p.NC);
+    }
+    __syncthreads();
+    int k = 0, i = 0, j = 0, I = 0, J = 0;
+    if (tid < n) {
+        const int g = cellid[tid];
+        k = g / p.NC;
+        i = (g % p.NC) % nx;
+        j = (g % p.NC) / nx;
+        I = (k % p.ntx) * nx + i;
+        J = (k / p.ntx) * ny + j;
+        d.cand_pos[g] = tid;
+    }
+    __syncthreads();
+    // ---- neighbours within one cell, and the per-candidate state
+    double x0[4], e[4];
+    double gamold = 1.0, btc0 = 0.0, w0 = 0.0, h0 = 0.0, b0c = 0.0, btc3s = 0.0, w3s = 0.0;
+    if (tid < n) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int I2 = I + kDI[q], J2 = J + kDJ[q];
+            int r = -1;
+            if (I2 >= 0 && J2 >= 0 && I2 < NX && J2 < NY)
+                r = d.cand_pos[((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx];
+            nbr[tid][q] = (short)r;
+        }
+        const size_t gc = (size_t)k * p.NC + j * nx + i;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const size_t v = (size_t)k * NV + (j + (q >> 1)) * NV1 + i + (q & 1);
+            slot[tid][q] = d.TB.vbt[v];
+            x0[q] = d.TA.vbt[v];
+            e[q] = d.vb0[v];
+        }
+        gamold = d.TA.cgam[gc]; btc0 = d.TA.btc[gc]; btc3s = d.TB.btc[gc];
+        w0 = d.SH[gc]; h0 = d.SH[3 * FS + gc]; b0c = d.b0c[gc]; w3s = d.SM[2][gc];
+        rec[tid] = 0;
+        res[tid] = 0;
+        dmask[tid] = 0;
+        lev[tid] = 0;
+    }
+    __syncthreads();
+    // ---- decisions in rounds: a candidate is decided once every earlier
+    // neighbour has been; lev records the round (0: not yet)
+    for (int round = 1;; ++round) {
+        int pending = 0;
+        if (tid < n && lev[tid] == 0) {
+            bool ready = true;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int r = nbr[tid][m];
+                if (r >= 0 && r < tid && (lev[r] == 0 || lev[r] == round)) ready = false;
+            }
+            if (!ready) {
+                pending = 1;
+            } else {
+                double x3[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) x3[q] = slot[tid][q];
+                double btc3 = btc3s, w3 = w3s;
+                if (rec[tid]) {
+                    const CellRe r = recompute_cell(p, x3[0], x3[1], x3[2], x3[3], e[0], e[1], e[2],
+                                                    e[3], btc0, gamold, w0, h0, b0c, true);
+                    btc3 = r.btc;
+                    w3 = r.w;
+                }
+                double nv[4];
+                int di[4], dj[4], N;
+                const int out = vertex_correction(p, i, j, x3, x0, gamold, btc0, btc3, w0, h0, b0c,
+                                                  w3, nv, di, dj, N);
+                if (out < 0) {
+                    res[tid] = 2;
+                    atomicMin(&s_stop, tid);
+                } else if (out > 0) {
+                    res[tid] = 1;
+                    int mask = 0;
+                    for (int q = 0; q < N; ++q) {
+                        const int a = di[q] - i, b = dj[q] - j;     // vertex (a, b) of the cell
+                        mask |= 1 << (a + 2 * b);
+                        slot[tid][a + 2 * b] = nv[q];
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) {
+                            const int r = nbr[tid][m];
+                            if (r <= tid) continue;
+                            rec[r] = 1;
+                            const int a2 = a - kDI[m], b2 = b - kDJ[m];
+                            if (a2 >= 0 && a2 <= 1 && b2 >= 0 && b2 <= 1) slot[r][a2 + 2 * b2] = nv[q];
+                        }
+                    }
+                    dmask[tid] = (unsigned char)mask;
+                }
+                lev[tid] = round;
+            }
+        }
+        if (__syncthreads_count(pending) == 0) break;
+    }
+    const int stop = s_stop;
+    // ---- vertex copies of the corrections before the stop: each vertex is
+    // written by the last correction that moves it; every correction's
+    // copies in ghost tiles are counted
+    unsigned long long ghost = 0;
+    if (tid < stop && res[tid] == 1) {
+        const int mask = dmask[tid];
+        for (int q = 0; q < 4; ++q) {
+            if (!(mask & (1 << q))) continue;
+            const int a = q & 1, b = q >> 1;
+            bool last = true;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int r = nbr[tid][m];
+                if (r <= tid || r >= stop || res[r] != 1) continue;
+                const int a2 = a - kDI[m], b2 = b - kDJ[m];
+                if (a2 >= 0 && a2 <= 1 && b2 >= 0 && b2 <= 1 && (dmask[r] >> (a2 + 2 * b2)) & 1) last = false;
+            }
+            put_vertex(d, p, k, i + a, j + b, slot[tid][q], &ghost, last);
+        }
+    }
+    __syncthreads();
+    // ---- the 3x3 cells of those corrections from the final vertices
+    for (int u = tid; u < 9 * stop; u += RD_THREADS) {
+        const int q = u / 9, lane = u % 9;
+        if (res[q] != 1) continue;
+        const int g = cellid[q];
+        const int kq = g / p.NC, iq = (g % p.NC) % nx, jq = (g % p.NC) / nx;
+        int ct, ci, cj;
+        map_cell(p, kq, iq + lane / 3 - 1, jq + lane % 3 - 1, ct, ci, cj);
+        const size_t gcl = (size_t)ct * p.NC + cj * nx + ci;
+        const double* vb = d.vb0 + (size_t)ct * NV;
+        const double* vt = d.TB.vbt + (size_t)ct * NV;
+        const bool cact = d.tstate[ct] == KST_TILE_ACTIVE;
+        const CellRe r = recompute_cell(p, vt[cj * NV1 + ci], vt[cj * NV1 + ci + 1], vt[(cj + 1) * NV1 + ci],
+                                        vt[(cj + 1) * NV1 + ci + 1], vb[cj * NV1 + ci], vb[cj * NV1 + ci + 1],
+                                        vb[(cj + 1) * NV1 + ci], vb[(cj + 1) * NV1 + ci + 1], d.TA.btc[gcl],
+                                        d.TA.cgam[gcl], d.SH[gcl], d.SH[3 * FS + gcl], d.b0c[gcl], cact);
+        if (cact) {
+            d.TB.btc[gcl] = r.btc; d.TB.cbx[gcl] = r.dbdx; d.TB.cby[gcl] = r.dbdy; d.TB.cgam[gcl] = r.gn;
+            d.SM[2][gcl] = r.w;
+            d.SM[2][FS + gcl] = r.hp;
+        } else {
+            d.S3[gcl] = r.w;
+            d.S3[3 * FS + gcl] = r.hp;
+            note_dirty(d, (long long)gcl);
+            ++ghost;
+        }
+    }
+    if (tid < n) d.cand_pos[cellid[tid]] = -1;
+    if (tid < stop && res[tid] == 1) atomicAdd(&s_done, 1ULL);
+    if (ghost) atomicAdd(&s_ghost, ghost);
+    __syncthreads();
+    if (tid == 0) {
+        c->redist_cells += (long long)s_done;
+        c->redist_ghost += (long long)s_ghost;
+        if (stop < n) c->mref |= 2;
+    }
+}
+
 // AddToRedistList + RedistributeGrid (src/Redistribute.f90:69-244): the
 // candidates the checks marked, sorted by excess deposition (ties in the
 // check loop's order: active-list position, row, column), each re-evaluated
-// and corrected in turn.  One workgroup.  Two corrections interact only if
-// their cells are within two cells of each other (a correction reads the 4x4
-// vertices and writes the vertices and 3x3 cells around its cell), so each
-// candidate gets a level one above the highest level of the earlier
-// candidates within that distance; the levels are corrected in order, the
-// candidates of a level concurrently (one wavefront each), and every
-// correction sees exactly the state the sequential walk would give it.
+// and corrected in turn.  One workgroup.
+//
+// Up to RD_THREADS candidates (the common case) are handled with the bed
+// state in LDS.  A correction's decision reads only the cell's four vertex bt
+// and its bt and w, which are the stored stage values until a correction of
+// a cell within one cell (a shared vertex) recomputes them from those
+// vertices.  So thread q owns sorted candidate q: each candidate gets a level
+// one above the highest level of the earlier candidates within one cell, the
+// levels are decided in order from the candidates' vertex values held in
+// LDS (a correction updates the copies held by the later candidates that
+// share the vertex), and every decision sees the state the sequential walk
+// gives it.  The walk stops at the first candidate that borders no
+// depositional vertex (the attempt is then refined); only the corrections
+// before it are written: the vertex copies level by level, then the 3x3
+// cells around each correction from the final vertices, which is the value
+// the walk leaves in them (a later correction that moves one of a cell's
+// vertices recomputes that cell).
+//
+// Beyond RD_THREADS candidates the kernel sorts in LDS or in global memory
+// and corrects level by level with one wavefront per candidate, two
+// corrections interacting within two cells (the 3x3 update reads the 4x4
+// vertices); there every level is processed even after a failing candidate.
 __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params p) {
     __shared__ int sh_n, sh_skip, sh_fail, sh_maxlev, sh_cnt, sh_changed;
     __shared__ unsigned long long sh_done, sh_ghost;
@@ -483,6 +729,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
             if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
         if (!skip && c->ncand <= 0) skip = 1;
+        if (!skip) count_work(c, KW_REDIST);
         sh_skip = skip;
         sh_n = c->ncand;
         sh_fail = 0;
@@ -493,6 +740,10 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
     __syncthreads();
     if (sh_skip) return;
     const int n = min(sh_n, d.cand_cap);
+    if (n <= RD_THREADS) {
+        redistribute_small(d, p, n, lx, lk, lev, list);
+        return;
+    }
     int m2 = 1;
     while (m2 < n) m2 <<= 1;
     long long* K = d.cand_key;
@@ -613,6 +864,7 @@ __global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
             const double dt = step_dt(d, p);
             if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
+        if (!skip) count_work(c, KW_FACETOPO);
         if ((int)blockIdx.x >= c->n_active) skip = 1;
         sh_skip = skip;
     }
@@ -629,6 +881,7 @@ __global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
 __global__ __launch_bounds__(256) void commit_kernel(Dev d, Params p) {
     const Ctrl* c = d.ctrl;
     if (!c->commit || (int)blockIdx.x >= c->n_active) return;
+    if (threadIdx.x == 0) count_work(c, KW_COMMIT);
     const int t = d.active[blockIdx.x];
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     for (size_t q = threadIdx.x; q < NV; q += blockDim.x) d.TA.vbt[t * NV + q] = d.TB.vbt[t * NV + q];

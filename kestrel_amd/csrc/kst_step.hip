@@ -13,6 +13,7 @@
 // Control (accept/reject, dt, tile activation) lives in device memory; the
 // host enqueues whole steps and only reads back status every few steps.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -20,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/kestrel_step.h"
@@ -51,7 +53,27 @@ struct Ctrl {
     int mref;                  // morphodynamic step refused: 1 checks, 2 redistribution
     int ncand;                 // redistribution candidates of this attempt
     int commit;                // the step was accepted: the new bed becomes current
+    long long work[16];        // launches that did work, per kernel (KW_*)
+    int cur;                   // mixed precision: u^n is in S0 (0) or S0b (1)
 };
+
+// Kernel ids of Ctrl::work.  Block 0 of a launch that does work counts it, so
+// a profiler can tell productive launches from the no-ops that follow a
+// rejected check or the end of a run.
+enum {
+    KW_STAGE = 0,      // + STAGE - 1, + 4 for the second hydraulic operator
+    KW_MORPHO = 8,     // + S - 1
+    KW_REDIST = 11,
+    KW_FACETOPO = 12,
+    KW_COMMIT = 13,
+    KW_ACTIVATE = 14,
+    KW_FINALIZE = 15,
+    KW_N = 16
+};
+
+__device__ __forceinline__ void count_work(const Ctrl* c, int id) {
+    if (blockIdx.x == 0) const_cast<Ctrl*>(c)->work[id] += 1;
+}
 
 // Bed-dependent topography of one generation, per tile.  Each tile holds its
 // own copy of the vertices it shares with its neighbours (TileType%bt,
@@ -78,6 +100,13 @@ struct Dev {
     Topo T;                           // the topography it sees
     // ---- step state and static data
     double* S0;                       // u^n of the step (tileContainer), 4 fields
+    // Mixed precision keeps u^n in FP64 and the stages as FP32: R holds the
+    // RHS of u^n, S2 and S3 the stages' increments over u^n, all float; the
+    // last stage writes u^{n+1} to the other FP64 buffer (S0 and S0b swap
+    // roles with Ctrl::cur when a step is accepted), since neighbouring
+    // blocks still read u^n + S3 around their cells.
+    double* S0b;
+    int mixed;
     double *b0c, *xb0, *yb0, *vb0;    // b0 at cells, x-/y-faces, vertices [tile][...]
     double *gsx, *gsy;                // ghost w slopes [tile][NC]
     double* maxima;                   // [11][tile][NC]
@@ -210,23 +239,28 @@ __device__ void flux_sources(const Dev& d, const Params& p, double t, double gri
 // Limiters at compile time (src/Limiters.f90:83-184).  MinMod1/2 use the
 // identity max(-x,-y) = -min(x,y): negation and |.| are exact, so the value
 // equals the reference's branchy form.
-template <int LIM>
-__device__ __forceinline__ double lim(double a, double b) {
+template <int LIM, typename T>
+__device__ __forceinline__ T lim(T a, T b) {
     if (LIM == KST_LIMITER_MINMOD2) {
-        const double th = 1.3;
-        const double m = fmin(th * fabs(a), fmin(th * fabs(b), 0.5 * fabs(a + b)));
-        return (a * b <= 0.0) ? 0.0 : copysign(m, a);
+        const T th = T(1.3);
+        const T m = fmin(th * fabs(a), fmin(th * fabs(b), T(0.5) * fabs(a + b)));
+        return (a * b <= T(0)) ? T(0) : copysign(m, a);
     } else if (LIM == KST_LIMITER_MINMOD1) {
-        const double m = fmin(fabs(a), fabs(b));
-        return (a * b <= 0.0) ? 0.0 : copysign(m, a);
+        const T m = fmin(fabs(a), fabs(b));
+        return (a * b <= T(0)) ? T(0) : copysign(m, a);
     } else {
-        return limiter(LIM, a, b);
+        return (T)limiter(LIM, (double)a, (double)b);
     }
 }
 
-template <int NQ>
+// Working set of one sub-block.  With MIX the per-cell Hnpsi, u, v and the
+// fluxes other than the hydrostatic one (q = 4, in fx4/fy4) are float.
+template <int NQ, bool MIX = false>
 struct SmemT {
-    double w[NREG], h[NREG], u[NREG], v[NREG], rho[NREG];
+    using FT = typename std::conditional<MIX, float, double>::type;
+    double w[NREG];
+    FT h[NREG], u[NREG], v[NREG];
+    double rho[NREG];
     double wst[NREG];        // w behind the ghost tiles' stored face values (their tileContainer w)
     double xbed[B][B + 3];   // CorrectSlopes bed of x-faces -1..B+1 of each row (relative)
     double ybed[B][B + 3];   // CorrectSlopes bed of y-faces -1..B+1 of each column
@@ -234,10 +268,11 @@ struct SmemT {
     double yside[2][B];      // the S / N tile's own bed at this tile's edge y-faces
     int gidx[NREG];
     uint8_t act[NREG];
-    double fx[NQ][NFACE];    // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
-    double fy[NQ][NFACE];
+    FT fx[NQ][NFACE];        // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
+    FT fy[NQ][NFACE];
+    double fx4[MIX ? NFACE : 1], fy4[MIX ? NFACE : 1];
     double red[NTHREADS / 64];
-    int tile, bx0, by0, bxa, bya, skip;
+    int tile, bx0, by0, bxa, bya, skip, cur;
     double dt, tstage, tgrid;
 };
 
@@ -408,6 +443,151 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     }
 }
 
+// face_flux in mixed precision.  The free surface stays FP64 end to end --
+// w and its limited slopes, CorrectSlopes against the bed, the face Hn, the
+// density and the hydrostatic flux 0.5 g rho hp^2 -- so a lake at rest
+// balances its bed-slope source exactly as in FP64; Hnpsi, u, v, their
+// slopes and face values, the wave speeds and the central-upwind
+// combination of the other fluxes are float.
+__device__ __forceinline__ float wave_root_f(const Params& p, float hn, float bperp, float gam) {
+    if (hn <= 0.f) hn = 0.f;
+    if (p.geometric) return __builtin_sqrtf((float)p.g * hn * (1.f + bperp * bperp) / (gam * gam * gam));
+    return __builtin_sqrtf((float)p.g * hn);
+}
+
+template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem>
+__device__ __forceinline__ void face_flux_mixed(const Dev& d, const Params& p, Smem& s, int fi, int fj,
+                                                double& cfl_min) {
+    int iLL, iL, iR, iRR;
+    if (!YDIR) { iLL = rg(fi - 2, fj); iL = rg(fi - 1, fj); iR = rg(fi, fj); iRR = rg(fi + 1, fj); }
+    else       { iLL = rg(fj, fi - 2); iL = rg(fj, fi - 1); iR = rg(fj, fi); iRR = rg(fj, fi + 1); }
+    const double dxr = YDIR ? p.dyr : p.dxr;
+    const double dx = YDIR ? p.dy : p.dx;
+    const float dxrf = (float)dxr, hdx = (float)(0.5 * dx);
+    const bool actL = s.act[iL], actR = s.act[iR];
+    const int nn = YDIR ? p.ny : p.nx;
+    const int fpos = (YDIR ? s.by0 : s.bx0) + fi;
+
+    // FP64: w slopes (non-active tiles keep their stored slope), density slopes
+    const double wLL = s.w[iLL], wL = s.w[iL], wR = s.w[iR], wRR = s.w[iRR];
+    const double* gs = YDIR ? d.gsy : d.gsx;
+    const double sLw = actL ? dxr * lim<LIM>(wR - wL, wL - wLL) : gs[s.gidx[iL]];
+    const double sRw = actR ? dxr * lim<LIM>(wRR - wR, wR - wL) : gs[s.gidx[iR]];
+    const double rLL = s.rho[iLL], rL = s.rho[iL], rR = s.rho[iR], rRR = s.rho[iRR];
+    const double sLr = actL ? dxr * lim<LIM>(rR - rL, rL - rLL) : 0.0;
+    const double sRr = actR ? dxr * lim<LIM>(rRR - rR, rR - rL) : 0.0;
+    // FP32: Hnpsi, u, v slopes
+    const float hLL = s.h[iLL], hL = s.h[iL], hR = s.h[iR], hRR = s.h[iRR];
+    const float sLh = actL ? dxrf * lim<LIM>(hR - hL, hL - hLL) : 0.f;
+    const float sRh = actR ? dxrf * lim<LIM>(hRR - hR, hR - hL) : 0.f;
+    const float uLL = s.u[iLL], uL = s.u[iL], uR = s.u[iR], uRR = s.u[iRR];
+    const float sLu = actL ? dxrf * lim<LIM>(uR - uL, uL - uLL) : 0.f;
+    const float sRu = actR ? dxrf * lim<LIM>(uRR - uR, uR - uL) : 0.f;
+    const float vLL = s.v[iLL], vL = s.v[iL], vR = s.v[iR], vRR = s.v[iRR];
+    const float sLv = actL ? dxrf * lim<LIM>(vR - vL, vL - vLL) : 0.f;
+    const float sRv = actR ? dxrf * lim<LIM>(vRR - vR, vR - vL) : 0.f;
+
+    // FP64: w face values, beds and CorrectSlopes (as face_flux)
+    const double wEL = wL + sLw * 0.5 * dx;
+    const double wWR = wR - sRw * 0.5 * dx;
+    const double* bed = YDIR ? &s.ybed[fj][0] : &s.xbed[fj][0];
+    const double bWL = bed[fi], bER = bed[fi + 2];
+    const double fbL = (fpos == 0) ? (YDIR ? s.yside[0][fj] : s.xside[0][fj]) : bed[fi + 1];
+    const double fbR = (fpos == nn) ? (YDIR ? s.yside[1][fj] : s.xside[1][fj]) : bed[fi + 1];
+    const double wLc = s.wst[iL], wRc = s.wst[iR];
+    double wFL = wEL, wFR = wWR;
+    if ((wLc + sLw * 0.5 * dx < fbL) || (wLc - sLw * 0.5 * dx < bWL)) wFL = wL + 0.5 * (fbL - bWL);
+    if ((wRc + sRw * 0.5 * dx < bER) || (wRc - sRw * 0.5 * dx < fbR)) wFR = wR + 0.5 * (fbR - bER);
+    float hFL = hL + sLh * hdx, hFR = hR - sRh * hdx;
+    if (fpos == 0) {
+        if (hFL < 0.f) hFL = hL;
+    } else {
+        const float hWL = hL - sLh * hdx;
+        if ((hFL < 0.f) || (hWL < 0.f)) hFL = hL;
+    }
+    if (fpos == nn) {
+        if (hFR < 0.f) hFR = hR;
+    } else {
+        const float hER = hR + sRh * hdx;
+        if ((hFR < 0.f) || (hER < 0.f)) hFR = hR;
+    }
+    const float uFL = uL + sLu * hdx, uFR = uR - sRu * hdx;
+    const float vFL = vL + sLv * hdx, vFR = vR - sRv * hdx;
+    const double rFL = rL + sLr * 0.5 * dx, rFR = rR - sRr * 0.5 * dx;
+
+    const int t = s.tile;
+    int fidx;
+    double fbp, fg, fb0, fbt;
+    if (!YDIR) {
+        fidx = t * p.NFX + (s.by0 + fj) * (p.nx + 1) + fpos;
+        fbp = d.T.xby[fidx]; fg = d.T.xgam[fidx]; fb0 = d.xb0[fidx]; fbt = d.T.xbt[fidx];
+    } else {
+        fidx = t * p.NFY + fpos * p.nx + (s.bx0 + fj);
+        fbp = d.T.ybx[fidx]; fg = d.T.ygam[fidx]; fb0 = d.yb0[fidx]; fbt = d.T.ybt[fidx];
+    }
+    const double HnL = compute_hn(wFL, fb0, fbt, fg);
+    const double HnR = compute_hn(wFR, fb0, fbt, fg);
+    const float HnLf = (float)HnL, HnRf = (float)HnR, fgf = (float)fg;
+    const float rFLf = (float)rFL, rFRf = (float)rFR;
+    const float unL = YDIR ? vFL : uFL, unR = YDIR ? vFR : uFR;
+    const float ruL = rFLf * HnLf * uFL, ruR = rFRf * HnRf * uFR;
+    const float rvL = rFLf * HnLf * vFL, rvR = rFRf * HnRf * vFR;
+
+    const float rP = wave_root_f(p, HnRf, (float)fbp, fgf);
+    const float rM = wave_root_f(p, HnLf, (float)fbp, fgf);
+    float wsP = unR + rP, wsM = unL + rM;
+    float aPos = (wsP > wsM) ? wsP : wsM;
+    if (aPos < 0.f) aPos = 0.f;
+    wsP = unR - rP; wsM = unL - rM;
+    float aNeg = (wsP < wsM) ? wsP : wsM;
+    if (aNeg > 0.f) aNeg = 0.f;
+    if (CFL) {
+        if ((double)aPos > kEps)
+            cfl_min = fmin((double)((float)(YDIR ? d.T.ynp : d.T.xnp)[fidx] / aPos), cfl_min);
+        if ((double)fabsf(aNeg) > kEps)
+            cfl_min = fmin((double)((float)(YDIR ? d.T.ynm : d.T.xnm)[fidx] / fabsf(aNeg)), cfl_min);
+    }
+
+    float* F = YDIR ? &s.fy[0][0] : &s.fx[0][0];
+    double* F4 = YDIR ? s.fy4 : s.fx4;
+    const int fslot = YDIR ? (fi * B + fj) : (fj * (B + 1) + fi);
+    const float dif = aPos - aNeg;
+    if (dif < 1e-10f) {
+        for (int q = 0; q < (VISC ? 7 : 5); ++q) F[q * NFACE + fslot] = 0.f;
+        F4[fslot] = 0.0;
+        return;
+    }
+    const float rd = __builtin_amdgcn_rcpf(dif);
+    const float cPw = HnRf * unR * fgf, cMw = HnLf * unL * fgf;
+    const float cPh = hFR * unR * fgf, cMh = hFL * unL * fgf;
+    const float cPu = ruR * unR, cMu = ruL * unL;
+    const float cPv = rvR * unR, cMv = rvL * unL;
+    const float ab = aPos * aNeg;
+    F[0 * NFACE + fslot] = ((HnRf * fgf - HnLf * fgf) * ab + (aPos * cMw - aNeg * cPw)) * rd;
+    F[1 * NFACE + fslot] = ((ruR - ruL) * ab + (aPos * cMu - aNeg * cPu)) * rd;
+    F[2 * NFACE + fslot] = ((rvR - rvL) * ab + (aPos * cMv - aNeg * cPv)) * rd;
+    F[3 * NFACE + fslot] = ((hFR * fgf - hFL * fgf) * ab + (aPos * cMh - aNeg * cPh)) * rd;
+    // FP64 hydrostatic flux
+    double hp = -fbt;
+    hp = hp + (wFR - fb0);
+    const double hydP = 0.5 * p.g * rFR * hp * hp;
+    hp = -fbt;
+    hp = hp + (wFL - fb0);
+    const double hydM = 0.5 * p.g * rFL * hp * hp;
+    const double aP = aPos, aN = aNeg;
+    F4[fslot] = (aP * hydM - aN * hydP) / (aP - aN);
+    if (VISC) {
+        const float nu = (float)p.nu;
+        float pPu, pPv, pMu, pMv;
+        if (HnRf < 0.f) { pPu = 0.f; pPv = 0.f; }
+        else { pPu = nu * rFRf * HnRf * sRu; pPv = nu * rFRf * HnRf * sRv; }
+        if (HnLf < 0.f) { pMu = 0.f; pMv = 0.f; }
+        else { pMu = nu * rFLf * HnLf * sLu; pMv = nu * rFLf * HnLf * sLv; }
+        F[5 * NFACE + fslot] = 0.5f * (pPu + pMu);
+        F[6 * NFACE + fslot] = 0.5f * (pPv + pMv);
+    }
+}
+
 // Block-wide minimum of a positive double into a control slot.
 template <typename Smem>
 __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
@@ -457,13 +637,29 @@ __device__ __forceinline__ double face_bed(const Dev& d, const Params& p, int t,
 // STAGE 3: RHS(u2); u3 -> S3.                   (src/TimeStepper.f90:447-498)
 // STAGE 4: RHS(u3); implicit drag -> Sout; maxima; activation flags.
 //                                               (src/TimeStepper.f90:501-524)
+template <bool YDIR, bool VISC, int LIM, bool CFL, bool MIX, typename Smem>
+__device__ __forceinline__ void face_work(const Dev& d, const Params& p, Smem& s, int fi, int fj,
+                                          double& cfl_min) {
+    if constexpr (MIX) face_flux_mixed<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
+    else face_flux<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
+}
+
+// Flux q of face slot i as FP64.
+template <bool MIX, bool YDIR, typename Smem>
+__device__ __forceinline__ double fluxv(const Smem& s, int q, int i) {
+    if constexpr (MIX) {
+        if (q == 4) return YDIR ? s.fy4[i] : s.fx4[i];
+    }
+    return YDIR ? (double)s.fy[q][i] : (double)s.fx[q][i];
+}
+
 // With morphodynamics the same kernels evaluate both hydraulic operators of
 // the Strang step H(dt) M(2dt) H(dt); the second starts at t + dt from the
 // morphodynamic result and first checks its substep-1 advice
 // (src/TimeStepper.f90:214-258).
-template <int STAGE, bool VISC, int LIM>
+template <int STAGE, bool VISC, int LIM, bool MIX = false>
 __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
-    __shared__ SmemT<VISC ? 7 : 5> s;
+    __shared__ SmemT<VISC ? 7 : 5, MIX> s;
     const int tid = threadIdx.x;
     if (tid == 0) {
         const Ctrl* c = d.ctrl;
@@ -479,7 +675,9 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1], c->n_active) < dt) skip = 1;
             if (STAGE >= 4 && advised_from_slot(p, c->slot[sb + 2], c->n_active) < dt) skip = 1;
         }
+        if (!skip) count_work(c, KW_STAGE + STAGE - 1 + (d.second ? 4 : 0));
         s.dt = dt;
+        s.cur = c->cur;
         const double tb = d.second ? c->t + dt : c->t;   // grid%t of this operator
         s.tgrid = tb;
         s.tstage = (STAGE == 1) ? tb : (STAGE == 3 ? tb + 0.5 * dt : tb + dt);
@@ -513,8 +711,9 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     const bool own = (tx < bxa) && (ty < bya);
     double ow = 0, oru = 0, orv = 0, oh = 0, ohn = 0, ou = 0, ov = 0, opsi = 0, orho = 0;
     double ob0 = 0, obt = 0, og = 1;
-    double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stage 2)
+    double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stage 2; mixed: 2..4)
     int oidx = 0;
+    const double* Un = (MIX && s.cur) ? d.S0b : d.S0;   // u^n of the step
     for (int round = 0; round < 2; ++round) {
         int rx, ry;
         bool valid;
@@ -533,7 +732,25 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         map_cell(p, t, s.bx0 + rx, s.by0 + ry, tc, ci, cj);
         const int gi = tc * p.NC + cj * p.nx + ci;
         double w, ru, rv, h;
-        if (STAGE == 1 || STAGE == 2) {
+        if constexpr (MIX) {
+            w = Un[gi]; ru = Un[FS + gi]; rv = Un[2 * FS + gi]; h = Un[3 * FS + gi];
+            if (round == 0) { pw = w; pru = ru; prv = rv; ph = h; }
+            if (STAGE == 2) {
+                const float* Rf = (const float*)d.R;
+                const double ew = Rf[gi], eu = Rf[FS + gi], ev = Rf[2 * FS + gi];
+                const double eh = Rf[3 * FS + gi], im = Rf[4 * FS + gi];
+                ru = (ru + dt * eu) / (1.0 - dt * im);
+                rv = (rv + dt * ev) / (1.0 - dt * im);
+                w = w + dt * ew;
+                h = h + dt * eh;
+            } else if (STAGE >= 3) {
+                const float* Sf = (const float*)((STAGE == 3) ? d.S2 : d.S3);
+                w = w + (double)Sf[gi];
+                ru = ru + (double)Sf[FS + gi];
+                rv = rv + (double)Sf[2 * FS + gi];
+                h = h + (double)Sf[3 * FS + gi];
+            }
+        } else if (STAGE == 1 || STAGE == 2) {
             w = d.Uw[gi]; ru = d.Uu[gi]; rv = d.Uv[gi]; h = d.Uh[gi];
             if (STAGE == 2) {
                 if (round == 0) { pw = w; pru = ru; prv = rv; ph = h; }
@@ -567,7 +784,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         }
         const int r = rg(rx, ry);
         s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho;
-        s.wst[r] = act ? w : d.S0[gi];
+        s.wst[r] = act ? w : Un[gi];
         s.gidx[r] = gi;
         s.act[r] = act ? 1 : 0;
         if (round == 0) {
@@ -625,14 +842,14 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     // x-faces: consecutive lanes walk along a row; y-faces: along a row too
     if (bxa == B && bya == B) {
         for (int f = tid; f < (B + 1) * B; f += NTHREADS)
-            face_flux<false, VISC, LIM, STAGE != 4>(d, p, s, f % (B + 1), f / (B + 1), cflm);
+            face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (B + 1), f / (B + 1), cflm);
         for (int f = tid; f < (B + 1) * B; f += NTHREADS)
-            face_flux<true, VISC, LIM, STAGE != 4>(d, p, s, f / B, f % B, cflm);
+            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
     } else {
         for (int f = tid; f < (bxa + 1) * bya; f += NTHREADS)
-            face_flux<false, VISC, LIM, STAGE != 4>(d, p, s, f % (bxa + 1), f / (bxa + 1), cflm);
+            face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (bxa + 1), f / (bxa + 1), cflm);
         for (int f = tid; f < (bya + 1) * bxa; f += NTHREADS)
-            face_flux<true, VISC, LIM, STAGE != 4>(d, p, s, f / bxa, f % bxa, cflm);
+            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / bxa, f % bxa, cflm);
     }
     if (STAGE != 4) {
         reduce_min_to(s, cflm, &d.ctrl->slot[(d.second ? 3 : 0) + STAGE - 1]);
@@ -643,10 +860,8 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     // ---- phase C: RHS of the owned cell and the stage update -------------
     if (!own) return;
     const int xs = ty * (B + 1) + tx, ys = ty * B + tx;
-    const double* FXp = &s.fx[0][0];
-    const double* FYp = &s.fy[0][0];
-#define FX(q, o) FXp[(q) * NFACE + xs + (o)]
-#define FY(q, o) FYp[(q) * NFACE + ys + (o)]
+#define FX(q, o) fluxv<MIX, false>(s, q, xs + (o))
+#define FY(q, o) fluxv<MIX, true>(s, q, ys + (o))
     const double gam = og;
     const double bx = d.T.cbx[oidx], by = d.T.cby[oidx];
     // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are per-bed topography
@@ -712,8 +927,14 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     }
 
     if (STAGE == 1) {
-        d.R[oidx] = Ew; d.R[FS + oidx] = Eu; d.R[2 * FS + oidx] = Ev; d.R[3 * FS + oidx] = Eh;
-        d.R[4 * FS + oidx] = Im;
+        if constexpr (MIX) {
+            float* Rf = (float*)d.R;
+            Rf[oidx] = (float)Ew; Rf[FS + oidx] = (float)Eu; Rf[2 * FS + oidx] = (float)Ev;
+            Rf[3 * FS + oidx] = (float)Eh; Rf[4 * FS + oidx] = (float)Im;
+        } else {
+            d.R[oidx] = Ew; d.R[FS + oidx] = Eu; d.R[2 * FS + oidx] = Ev; d.R[3 * FS + oidx] = Eh;
+            d.R[4 * FS + oidx] = Im;
+        }
     } else if (STAGE == 2) {
         // second SSP substep (src/TimeStepper.f90:407-444)
         const double u2 = 0.75 * pru + 0.25 * (oru + dt * Eu) / (1.0 - dt * Im);
@@ -728,11 +949,17 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         wu = wu + 0.25 * dt * Ew;
         wu = wu + ob0;
         const double h2 = 0.75 * ph + 0.25 * (oh + dt * Eh);
-        d.S2[oidx] = wu; d.S2[FS + oidx] = u2; d.S2[2 * FS + oidx] = v2; d.S2[3 * FS + oidx] = h2;
+        if constexpr (MIX) {
+            float* Sf = (float*)d.S2;
+            Sf[oidx] = (float)(wu - pw); Sf[FS + oidx] = (float)(u2 - pru);
+            Sf[2 * FS + oidx] = (float)(v2 - prv); Sf[3 * FS + oidx] = (float)(h2 - ph);
+        } else {
+            d.S2[oidx] = wu; d.S2[FS + oidx] = u2; d.S2[2 * FS + oidx] = v2; d.S2[3 * FS + oidx] = h2;
+        }
     } else if (STAGE == 3) {
         // third SSP substep (src/TimeStepper.f90:466-498)
-        const double w0 = d.Uw[oidx], u0 = d.Uu[oidx], v0 = d.Uv[oidx];
-        const double h0 = d.Uh[oidx];
+        const double w0 = MIX ? pw : d.Uw[oidx], u0 = MIX ? pru : d.Uu[oidx], v0 = MIX ? prv : d.Uv[oidx];
+        const double h0 = MIX ? ph : d.Uh[oidx];
         const double third = 1.0 / 3.0, twothird = 2.0 / 3.0;
         const double u3 = third * u0 + twothird * (oru + dt * Eu) / (1.0 - dt * Im);
         const double v3 = third * v0 + twothird * (orv + dt * Ev) / (1.0 - dt * Im);
@@ -746,7 +973,13 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         wu = wu + twothird * dt * Ew;
         wu = wu + ob0;
         const double h3 = third * h0 + twothird * (oh + dt * Eh);
-        d.S3[oidx] = wu; d.S3[FS + oidx] = u3; d.S3[2 * FS + oidx] = v3; d.S3[3 * FS + oidx] = h3;
+        if constexpr (MIX) {
+            float* Sf = (float*)d.S3;
+            Sf[oidx] = (float)(wu - w0); Sf[FS + oidx] = (float)(u3 - u0);
+            Sf[2 * FS + oidx] = (float)(v3 - v0); Sf[3 * FS + oidx] = (float)(h3 - h0);
+        } else {
+            d.S3[oidx] = wu; d.S3[FS + oidx] = u3; d.S3[2 * FS + oidx] = v3; d.S3[3 * FS + oidx] = h3;
+        }
     } else {
         // final implicit substep (src/TimeStepper.f90:504-518)
         const double un = (oru - dt * dt * Eu * Im) / (1.0 + dt * dt * Im * Im);
@@ -755,8 +988,8 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             // UpdateMaximum* on the pre-step state at nextT (src/TimeStepper.f90:1155-1303);
             // the second operator of a Strang step repeats the first one's
             // update with the same state, which changes nothing
-            const double w0 = d.S0[oidx], u0 = d.S0[FS + oidx], v0 = d.S0[2 * FS + oidx];
-            const double h0 = d.S0[3 * FS + oidx];
+            const double w0 = Un[oidx], u0 = Un[FS + oidx], v0 = Un[2 * FS + oidx];
+            const double h0 = Un[3 * FS + oidx];
             const double tn = d.ctrl->t + dt;
             double* M = d.maxima;
             if (obt < 0.0 && -obt > M[4 * FS + oidx]) { M[4 * FS + oidx] = -obt; M[5 * FS + oidx] = tn; }
@@ -779,7 +1012,8 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
                 if (psi > M[8 * FS + oidx]) { M[8 * FS + oidx] = psi; M[9 * FS + oidx] = tn; }
             }
         }
-        d.Sout[oidx] = ow; d.Sout[FS + oidx] = un; d.Sout[2 * FS + oidx] = vn; d.Sout[3 * FS + oidx] = oh;
+        double* Out = MIX ? (s.cur ? d.S0 : d.S0b) : d.Sout;
+        Out[oidx] = ow; Out[FS + oidx] = un; Out[2 * FS + oidx] = vn; Out[3 * FS + oidx] = oh;
         // activation flags from this state's Hn (CheckIfNearBoundaries,
         // src/TimeStepper.f90:946-1150): bit 0 N, 1 S, 2 E, 3 W
         if (d.last && ohn > p.hthr) {
@@ -955,7 +1189,7 @@ __device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, 
 // (ReconstructwAtEdges, src/UpdateTiles.f90:669-750).
 __device__ void ghost_slopes(const Dev& d, const Params& p, int g) {
     const int nx = p.nx, ny = p.ny, NC = p.NC;
-    const double* w = d.S0;
+    const double* w = (d.mixed && d.ctrl->cur) ? d.S0b : d.S0;
     const size_t base = (size_t)g * NC;
     auto on = [&](int k) { return k >= 0 && k < p.ntiles && d.tstate[k] == KST_TILE_ACTIVE; };
     int nb = g - 1;
@@ -995,6 +1229,22 @@ __device__ void dry_tile(const Dev& d, const Params& p, int k, bool active) {
     const long long FS = d.FS;
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
         const double b = d.b0c[base + c];
+        if (d.mixed) {
+            // both FP64 states hold the dry data; increments and RHS are zero
+            double* S[2] = {d.S0, d.S0b};
+            for (int q = 0; q < 2; ++q) {
+                S[q][base + c] = b;
+                S[q][FS + base + c] = 0.0;
+                S[q][2 * FS + base + c] = 0.0;
+                S[q][3 * FS + base + c] = 0.0;
+            }
+            float* S2f = (float*)d.S2;
+            float* S3f = (float*)d.S3;
+            float* Rf = (float*)d.R;
+            for (int q = 0; q < 4; ++q) { S2f[q * FS + base + c] = 0.f; S3f[q * FS + base + c] = 0.f; }
+            for (int q = 0; q < 5; ++q) Rf[q * FS + base + c] = 0.f;
+            continue;
+        }
         double* S[4] = {d.S0, d.S2, d.S3, d.SH};
         for (int q = 0; q < (d.morpho ? 4 : 3); ++q) {
             S[q][base + c] = b;
@@ -1059,6 +1309,7 @@ __global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) {
     __shared__ int nnew, bad, need;
     Ctrl* c = d.ctrl;
     if (c->status != ST_NEW) return;
+    if (threadIdx.x == 0) count_work(c, KW_ACTIVATE);
     if (d.morpho) refresh_ghost_overlay(d, p);
     // fast path: does any flagged edge face a tile that is not active?
     if (threadIdx.x == 0) need = 0;
@@ -1189,7 +1440,8 @@ __global__ __launch_bounds__(256) void init_flags_kernel(Dev d, Params p) {
     const int tb = p.tile_buffer;
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
         const size_t gi = (size_t)t * p.NC + c;
-        double hn = compute_hn(d.S0[gi], d.b0c[gi], d.T.btc[gi], d.T.cgam[gi]);
+        const double* w = (d.mixed && d.ctrl->cur) ? d.S0b : d.S0;
+        double hn = compute_hn(w[gi], d.b0c[gi], d.T.btc[gi], d.T.cgam[gi]);
         if (hn > p.hthr) {
             const int i = c % p.nx, j = c / p.nx;
             if (j + 1 > p.ny - tb) f |= 1u;
@@ -1211,6 +1463,7 @@ __global__ void finalize_kernel(Dev d, Params p) {
     Ctrl* c = d.ctrl;
     c->commit = 0;
     if (c->status == ST_DONE || c->status == ST_ERROR) return;
+    c->work[KW_FINALIZE] += 1;
     const bool ran_p1 = (c->status == ST_NEW);
     const double dt = step_dt(d, p);
     c->rhs_evals += (ran_p1 ? 1 : 0) + 1;
@@ -1278,6 +1531,7 @@ __global__ void finalize_kernel(Dev d, Params p) {
     c->slot[0] = huge;
     c->status = ST_NEW;
     c->commit = d.morpho ? 1 : 0;
+    if (d.mixed) c->cur ^= 1;
     if (c->mode == 0) {
         if (c->t >= c->tend) c->status = ST_DONE;
     } else {
@@ -1322,8 +1576,8 @@ struct kst_handle {
     bool profiling = false;
     hipGraphExec_t gexec = nullptr;      // graph of kGraphIters whole iterations
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
-    double prof_ms[4] = {0, 0, 0, 0};
-    long long prof_n[4] = {0, 0, 0, 0};
+    double prof_ms[KW_N] = {};
+    long long prof_n[KW_N] = {};
 };
 
 namespace {
@@ -1436,49 +1690,62 @@ void from_tiles(const kst_handle* h, const double* t, double* g) {
 typedef void (*StageFn)(Dev, Params);
 struct StageSet { StageFn fn[4]; };
 
-template <bool VISC, int LIM>
+template <bool VISC, int LIM, bool MIX>
 StageSet make_set() {
-    return StageSet{{stage_kernel<1, VISC, LIM>, stage_kernel<2, VISC, LIM>,
-                     stage_kernel<3, VISC, LIM>, stage_kernel<4, VISC, LIM>}};
+    return StageSet{{stage_kernel<1, VISC, LIM, MIX>, stage_kernel<2, VISC, LIM, MIX>,
+                     stage_kernel<3, VISC, LIM, MIX>, stage_kernel<4, VISC, LIM, MIX>}};
 }
 
-const StageSet& stage_set(bool visc, int limiter) {
+template <bool MIX>
+const StageSet& stage_set_of(bool visc, int limiter) {
     static const StageSet sets[2][5] = {
-        {make_set<false, 0>(), make_set<false, 1>(), make_set<false, 2>(), make_set<false, 3>(), make_set<false, 4>()},
-        {make_set<true, 0>(), make_set<true, 1>(), make_set<true, 2>(), make_set<true, 3>(), make_set<true, 4>()}};
+        {make_set<false, 0, MIX>(), make_set<false, 1, MIX>(), make_set<false, 2, MIX>(),
+         make_set<false, 3, MIX>(), make_set<false, 4, MIX>()},
+        {make_set<true, 0, MIX>(), make_set<true, 1, MIX>(), make_set<true, 2, MIX>(),
+         make_set<true, 3, MIX>(), make_set<true, 4, MIX>()}};
     return sets[visc ? 1 : 0][(limiter >= 0 && limiter < 5) ? limiter : 1];
 }
 
-int launch_iteration(kst_handle* h, int slot) {
+const StageSet& stage_set(bool visc, int limiter, bool mixed) {
+    return mixed ? stage_set_of<true>(visc, limiter) : stage_set_of<false>(visc, limiter);
+}
+
+int launch_iteration(kst_handle* h, bool profiled) {
     const Params& p = h->p;
     const Dev& d = h->d;
-    hipEvent_t* E = (h->profiling && slot >= 0) ? &h->ev[8 * slot] : nullptr;
-#define EVREC(k) if (E) HIPCHK(h, hipEventRecord(E[k], h->stream))
+    // while profiling, the dispatch of each launch carries the event pair of
+    // its kernel id, which time-stamps the kernel's own start and end
+    hipEvent_t* E = profiled ? h->ev.data() : nullptr;
+#define LAUNCH(id, kern, grid, block, ...)                                                  \
+    do {                                                                                    \
+        if (E)                                                                              \
+            hipExtLaunchKernelGGL(kern, grid, block, 0, h->stream, E[2 * (id)],             \
+                                  E[2 * (id) + 1], 0, __VA_ARGS__);                          \
+        else                                                                                \
+            hipLaunchKernelGGL(kern, grid, block, 0, h->stream, __VA_ARGS__);                \
+    } while (0)
     const int spt = ((p.nx + B - 1) / B) * ((p.ny + B - 1) / B);
     // 8 * ceil(work/8) blocks so the XCD-chunked mapping is a bijection
     const int nblk = 8 * ((p.ntiles * spt + 7) / 8);
     const bool visc = p.nu != 0.0;
-    hipLaunchKernelGGL(activate_kernel, dim3(1), dim3(256), 0, h->stream, d, p);
-    const StageSet& k = stage_set(visc, p.limiter);
-    for (int q = 0; q < 4; ++q) {
-        EVREC(2 * q);
-        hipLaunchKernelGGL(k.fn[q], dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        EVREC(2 * q + 1);
-    }
-#undef EVREC
+    LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), d, p);
+    const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0);
+    for (int q = 0; q < 4; ++q)
+        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk), dim3(NTHREADS), d, p);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
-        hipLaunchKernelGGL(morpho_kernel<1>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        hipLaunchKernelGGL(morpho_kernel<2>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        hipLaunchKernelGGL(morpho_kernel<3>, dim3(nblk), dim3(NTHREADS), 0, h->stream, d, p);
-        hipLaunchKernelGGL(redistribute_kernel, dim3(1), dim3(RD_THREADS), 0, h->stream, d, p);
-        hipLaunchKernelGGL(face_topo_kernel, dim3(p.ntiles), dim3(256), 0, h->stream, d, p);
+        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(NTHREADS), d, p);
+        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(NTHREADS), d, p);
+        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(NTHREADS), d, p);
+        LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), d, p);
+        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles), dim3(256), d, p);
         // second hydraulic operator
         for (int q = 0; q < 4; ++q)
-            hipLaunchKernelGGL(k.fn[q], dim3(nblk), dim3(NTHREADS), 0, h->stream, h->d2, p);
+            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk), dim3(NTHREADS), h->d2, p);
     }
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1), 0, h->stream, d, p);
-    if (d.morpho) hipLaunchKernelGGL(commit_kernel, dim3(p.ntiles), dim3(256), 0, h->stream, d, p);
+    LAUNCH(KW_FINALIZE, finalize_kernel, dim3(1), dim3(1), d, p);
+    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles), dim3(256), d, p);
+#undef LAUNCH
     HIPCHK(h, hipGetLastError());
     return 0;
 }
@@ -1492,7 +1759,7 @@ int ensure_graph(kst_handle* h) {
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int q = 0; q < kGraphIters; ++q) {
-        int rc = launch_iteration(h, -1);
+        int rc = launch_iteration(h, false);
         if (rc) {
             hipGraph_t dummy = nullptr;
             (void)hipStreamEndCapture(h->stream, &dummy);
@@ -1510,24 +1777,24 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
     Ctrl c;
     int rc = read_ctrl(h, &c);
     if (rc) return rc;
-    int batch = h->profiling ? 4 : kGraphIters;
+    // profiling runs one iteration per control read-back, so that the
+    // per-kernel work counters name the launches of the iteration that did work
+    int batch = h->profiling ? 1 : kGraphIters;
     long long launched = 0;
+    if (h->profiling && (int)h->ev.size() < 2 * KW_N) {
+        const size_t old = h->ev.size();
+        h->ev.resize(2 * KW_N);
+        for (size_t q = old; q < h->ev.size(); ++q) HIPCHK(h, hipEventCreate(&h->ev[q]));
+    }
     for (;;) {
         if (launched > max_iters)
             return set_err(h, KST_ERR_TOO_MANY_STEPS, "step limit exceeded before reaching the target");
         launched += batch;
-        const long long done_before = c.steps + c.rejected;
-        const long long new_before = c.steps;
-        if (h->profiling && (int)h->ev.size() < 8 * batch) {
-            const size_t old = h->ev.size();
-            h->ev.resize(8 * batch);
-            for (size_t q = old; q < h->ev.size(); ++q) HIPCHK(h, hipEventCreate(&h->ev[q]));
-        }
+        long long work_before[KW_N];
+        std::memcpy(work_before, c.work, sizeof work_before);
         if (h->profiling) {
-            for (int q = 0; q < batch; ++q) {
-                rc = launch_iteration(h, q);
-                if (rc) return rc;
-            }
+            rc = launch_iteration(h, true);
+            if (rc) return rc;
         } else {
             rc = ensure_graph(h);
             if (rc) return rc;
@@ -1536,19 +1803,13 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
         rc = read_ctrl(h, &c);
         if (rc) return rc;
         if (h->profiling) {
-            // the first (steps + rejected) iterations of the batch did work;
-            // P1 ran only in iterations that started a new step
-            const long long productive = c.steps + c.rejected - done_before;
-            long long p1 = c.steps - new_before;
-            for (long long q = 0; q < productive && q < batch; ++q)
-                for (int k = 0; k < 4; ++k) {
-                    if (k == 0 && p1 <= 0) continue;
-                    float ms = 0.f;
-                    HIPCHK(h, hipEventElapsedTime(&ms, h->ev[8 * q + 2 * k], h->ev[8 * q + 2 * k + 1]));
-                    h->prof_ms[k] += ms;
-                    h->prof_n[k] += 1;
-                    if (k == 0) --p1;
-                }
+            for (int k = 0; k < KW_N; ++k) {
+                if (c.work[k] == work_before[k]) continue;
+                float ms = 0.f;
+                HIPCHK(h, hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]));
+                h->prof_ms[k] += ms;
+                h->prof_n[k] += 1;
+            }
         }
         if (c.status == ST_ERROR) {
             if (c.err == KST_ERR_HALT)
@@ -1558,7 +1819,7 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
             return set_err(h, c.err ? c.err : KST_ERR_STATE, "device error");
         }
         if (c.status == ST_DONE) break;
-        batch = std::min(batch * 2, 64);
+        if (!h->profiling) batch = std::min(batch * 2, 64);
     }
     h->t = c.t;
     if (st) {
@@ -1618,7 +1879,10 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     *out = nullptr;
     if (cfg->nx_per_tile < 4 || cfg->ny_per_tile < 4 || cfg->nx_tiles < 3 || cfg->ny_tiles < 3)
         return KST_ERR_CONFIG;
-    if (cfg->bc != KST_BC_HALT || cfg->precision != KST_PREC_FP64) return KST_ERR_CONFIG;
+    if (cfg->bc != KST_BC_HALT) return KST_ERR_CONFIG;
+    if (cfg->precision != KST_PREC_FP64 && cfg->precision != KST_PREC_MIXED) return KST_ERR_CONFIG;
+    // mixed precision covers the hydraulic operator only
+    if (cfg->precision == KST_PREC_MIXED && cfg->morphodynamics) return KST_ERR_CONFIG;
     if (cfg->n_sources > 0 && !cfg->sources) return KST_ERR_CONFIG;
     kst_handle* h = new kst_handle();
     h->cfg = *cfg;
@@ -1638,8 +1902,17 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     const size_t NV = (size_t)(h->nx + 1) * (h->ny + 1);
     d.FS = (long long)FS;
     d.morpho = cfg->morphodynamics ? 1 : 0;
-    rc |= dalloc(h, &d.S0, 4 * FS); rc |= dalloc(h, &d.S2, 4 * FS); rc |= dalloc(h, &d.S3, 4 * FS);
-    rc |= dalloc(h, &d.R, 5 * FS);
+    d.mixed = cfg->precision == KST_PREC_MIXED ? 1 : 0;
+    rc |= dalloc(h, &d.S0, 4 * FS);
+    if (d.mixed) {
+        // S2, S3 (4 fields) and R (5 fields) hold floats
+        rc |= dalloc(h, &d.S0b, 4 * FS);
+        rc |= dalloc(h, &d.S2, 2 * FS); rc |= dalloc(h, &d.S3, 2 * FS);
+        rc |= dalloc(h, &d.R, (5 * FS + 1) / 2);
+    } else {
+        rc |= dalloc(h, &d.S2, 4 * FS); rc |= dalloc(h, &d.S3, 4 * FS);
+        rc |= dalloc(h, &d.R, 5 * FS);
+    }
     rc |= dalloc(h, &d.b0c, FS);
     rc |= dalloc(h, &d.xb0, (size_t)h->ntiles * h->p.NFX);
     rc |= dalloc(h, &d.yb0, (size_t)h->ntiles * h->p.NFY);
@@ -1796,9 +2069,16 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
         }
     }
     HIPCHK(h, hipMemcpy(h->d.S0, S.data(), S.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d.S2, S.data(), S.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d.S3, S.data(), S.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemset(h->d.R, 0, 5 * FS * 8));
+    if (h->d.mixed) {
+        HIPCHK(h, hipMemcpy(h->d.S0b, S.data(), S.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemset(h->d.S2, 0, 4 * FS * 4));
+        HIPCHK(h, hipMemset(h->d.S3, 0, 4 * FS * 4));
+        HIPCHK(h, hipMemset(h->d.R, 0, 5 * FS * 4));
+    } else {
+        HIPCHK(h, hipMemcpy(h->d.S2, S.data(), S.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->d.S3, S.data(), S.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemset(h->d.R, 0, 5 * FS * 8));
+    }
     if (h->d.morpho) {
         HIPCHK(h, hipMemcpy(h->d.SH, S.data(), S.size() * 8, hipMemcpyHostToDevice));
         for (int q = 0; q < 3; ++q) {
@@ -1904,7 +2184,14 @@ int kst_get_state(kst_handle* h, double* w, double* rhu, double* rhv, double* hn
     const size_t FS = (size_t)h->ntiles * h->NC;
     std::vector<double> S(4 * FS);
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    HIPCHK(h, hipMemcpy(S.data(), h->d.S0, S.size() * 8, hipMemcpyDeviceToHost));
+    const double* Sn = h->d.S0;
+    if (h->d.mixed) {
+        Ctrl c;
+        int rc = read_ctrl(h, &c);
+        if (rc) return rc;
+        if (c.cur) Sn = h->d.S0b;
+    }
+    HIPCHK(h, hipMemcpy(S.data(), Sn, S.size() * 8, hipMemcpyDeviceToHost));
     std::vector<uint8_t> st(h->ntiles);
     HIPCHK(h, hipMemcpy(st.data(), h->d.tstate, st.size(), hipMemcpyDeviceToHost));
     // Fields outside allocated (active or ghost) tiles read as zero.
@@ -1995,14 +2282,14 @@ int kst_last_error(const kst_handle* h, char* buf, int buflen) {
 int kst_profile(kst_handle* h, int enable) {
     if (!h) return KST_ERR_CONFIG;
     h->profiling = enable != 0;
-    for (int k = 0; k < 4; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
+    for (int k = 0; k < KW_N; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
     return KST_OK;
 }
 
-int kst_profile_read(kst_handle* h, double stage_ms[4], int64_t launches[4]) {
+int kst_profile_read(kst_handle* h, double ms[KST_PROFILE_KERNELS], int64_t launches[KST_PROFILE_KERNELS]) {
     if (!h) return KST_ERR_CONFIG;
-    for (int k = 0; k < 4; ++k) {
-        if (stage_ms) stage_ms[k] = h->prof_n[k] ? h->prof_ms[k] / h->prof_n[k] : 0.0;
+    for (int k = 0; k < KW_N; ++k) {
+        if (ms) ms[k] = h->prof_n[k] ? h->prof_ms[k] / h->prof_n[k] : 0.0;
         if (launches) launches[k] = h->prof_n[k];
         h->prof_ms[k] = 0.0;
         h->prof_n[k] = 0;

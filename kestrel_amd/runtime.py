@@ -19,6 +19,14 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkestrel_amd.so")
 _LIB = None
 _dp = ctypes.POINTER(ctypes.c_double)
+
+# Kernel ids of kst_profile_read (enum kst_profile_kernel).
+PROFILE_KERNELS = 16
+PROFILE_NAMES = ["stage_kernel<1>", "stage_kernel<2>", "stage_kernel<3>", "stage_kernel<4>",
+                 "stage_kernel<1> (H2)", "stage_kernel<2> (H2)", "stage_kernel<3> (H2)",
+                 "stage_kernel<4> (H2)", "morpho_kernel<1>", "morpho_kernel<2>", "morpho_kernel<3>",
+                 "redistribute_kernel", "face_topo_kernel", "commit_kernel", "activate_kernel",
+                 "finalize_kernel"]
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_topography",
@@ -163,11 +171,12 @@ class Stepper:
     def profile(self, enable: bool = True):
         self._check(self.L.kst_profile(self.h, int(enable)))
 
-    def profile_read(self) -> tuple[list, list]:
-        ms = (ctypes.c_double * 4)()
-        n = (ctypes.c_int64 * 4)()
+    def profile_read(self) -> dict:
+        """{kernel name: (mean ms of the launches that did work, their count)}."""
+        ms = (ctypes.c_double * PROFILE_KERNELS)()
+        n = (ctypes.c_int64 * PROFILE_KERNELS)()
         self._check(self.L.kst_profile_read(self.h, ms, n))
-        return list(ms), list(n)
+        return {name: (ms[k], n[k]) for k, name in enumerate(PROFILE_NAMES) if n[k]}
 
     def maxima(self) -> list:
         c = self.cfg
