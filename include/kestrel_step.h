@@ -205,6 +205,38 @@ int kst_profile(kst_handle* h, int enable);
 int kst_profile_read(kst_handle* h, double ms[KST_PROFILE_KERNELS],
                      int64_t launches[KST_PROFILE_KERNELS]);
 
+/* ---- Batches --------------------------------------------------------------
+ * No counterpart in the reference, which advances one scenario per process
+ * (src/main.f90:41-133).  A batch holds up to max_slots scenario handles
+ * (each created, given its topography and state as above) resident on the
+ * device and advances all of them with one launch per kernel: every running
+ * slot keeps its own tiles, dt, retries and time; each iteration walks one
+ * work list of the active sub-blocks of all running slots.  Handles join
+ * (attach) and leave (detach) between runs without rebuilding the batch's
+ * hipGraph.  A batch holds hydraulic scenarios with the MinMod2 limiter, all
+ * of one precision and one eddy-viscosity setting.  While attached, a handle
+ * is advanced only through its batch; kst_get_state / kst_get_maxima read it
+ * after kst_batch_run returns.  Results are the same bits as running the
+ * handle alone. */
+typedef struct kst_batch kst_batch;
+enum { KST_SLOT_FREE = -1, KST_SLOT_RUNNING = 0, KST_SLOT_DONE = 1, KST_SLOT_ERROR = 2 };
+
+int kst_batch_create(int max_slots, kst_batch** out);
+/* Attach a handle to a free slot (returned in *slot). */
+int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot);
+int kst_batch_detach(kst_batch* b, int slot);
+/* Give a slot its next target: nsteps > 0 accepted steps (as kst_step_n),
+ * else integrate to tend (as kst_integrate_to). */
+int kst_batch_start(kst_batch* b, int slot, double tend, int64_t nsteps);
+/* Advance every running slot until none is running or about max_iters
+ * iterations have run; status[max_slots] receives KST_SLOT_* per slot. */
+int kst_batch_run(kst_batch* b, int64_t max_iters, int* status);
+/* Counters of the slot's last target (as kst_integrate_to's stats); returns
+ * the scenario's error code if it stopped on one. */
+int kst_batch_stats(kst_batch* b, int slot, kst_stats* out);
+int kst_batch_last_error(const kst_batch* b, char* buf, int buflen);
+void kst_batch_destroy(kst_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

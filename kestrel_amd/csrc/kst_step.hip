@@ -657,9 +657,12 @@ __device__ __forceinline__ double fluxv(const Smem& s, int q, int i) {
 // the Strang step H(dt) M(2dt) H(dt); the second starts at t + dt from the
 // morphodynamic result and first checks its substep-1 advice
 // (src/TimeStepper.f90:214-258).
-template <int STAGE, bool VISC, int LIM, bool MIX = false>
-__global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
-    __shared__ SmemT<VISC ? 7 : 5, MIX> s;
+//
+// stage_body does one work item (one sub-block of an active tile): witem is
+// its index in the scenario's active-list order, or -1 to derive it from the
+// block index with the XCD-chunked map of a one-scenario launch.
+template <int STAGE, bool VISC, int LIM, bool MIX, typename Smem>
+__device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& s, int witem) {
     const int tid = threadIdx.x;
     if (tid == 0) {
         const Ctrl* c = d.ctrl;
@@ -675,7 +678,6 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
             if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1], c->n_active) < dt) skip = 1;
             if (STAGE >= 4 && advised_from_slot(p, c->slot[sb + 2], c->n_active) < dt) skip = 1;
         }
-        if (!skip) count_work(c, KW_STAGE + STAGE - 1 + (d.second ? 4 : 0));
         s.dt = dt;
         s.cur = c->cur;
         const double tb = d.second ? c->t + dt : c->t;   // grid%t of this operator
@@ -685,10 +687,14 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
         // list go to the same XCD (blocks are dealt round-robin to 8 XCDs)
         const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
         const int nwork = c->n_active * spt;
-        const int per = (nwork + 7) / 8;
-        const int b = blockIdx.x;
-        const int w = (b % 8) * per + b / 8;
-        if (b / 8 >= per || w >= nwork) skip = 1;
+        int w = witem;
+        if (witem < 0) {
+            const int per = (nwork + 7) / 8;
+            const int b = blockIdx.x;
+            w = (b / 8 >= per) ? nwork : (b % 8) * per + b / 8;
+        }
+        if (!skip && w == 0) const_cast<Ctrl*>(c)->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)] += 1;
+        if (w >= nwork) skip = 1;
         if (!skip) {
             s.tile = d.active[w / spt];
             const int sub = w % spt;
@@ -1028,6 +1034,12 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
     }
 }
 
+template <int STAGE, bool VISC, int LIM, bool MIX = false>
+__global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
+    __shared__ SmemT<VISC ? 7 : 5, MIX> s;
+    stage_body<STAGE, VISC, LIM, MIX>(d, p, s, -1);
+}
+
 // ---------------------------------------------------------------------------
 // topography
 // ---------------------------------------------------------------------------
@@ -1304,7 +1316,7 @@ __device__ void refresh_ghost_overlay(const Dev& d, const Params& p) {
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) {
+__device__ void activate_body(const Dev& d, const Params& p) {
     __shared__ int newt[MAXNEW];
     __shared__ int nnew, bad, need;
     Ctrl* c = d.ctrl;
@@ -1424,6 +1436,8 @@ __global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) {
     }
 }
 
+__global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) { activate_body(d, p); }
+
 // Rebuild every ghost's edge slopes (end of LoadSourceConditions,
 // src/SetSources.f90:388-392) and the activation flags of every active tile.
 __global__ __launch_bounds__(256) void init_ghosts_kernel(Dev d, Params p) {
@@ -1459,7 +1473,7 @@ __global__ __launch_bounds__(256) void init_flags_kernel(Dev d, Params p) {
 // morphodynamic checks or redistribution (dt <- dt/2), the second
 // operator's substep-1 advice (dt <- 0.9 dt or the advice), or its CFL
 // re-checks (dt <- 0.9 dt_k).
-__global__ void finalize_kernel(Dev d, Params p) {
+__device__ void finalize_body(const Dev& d, const Params& p) {
     Ctrl* c = d.ctrl;
     c->commit = 0;
     if (c->status == ST_DONE || c->status == ST_ERROR) return;
@@ -1539,6 +1553,97 @@ __global__ void finalize_kernel(Dev d, Params p) {
     }
 }
 
+__global__ void finalize_kernel(Dev d, Params p) { finalize_body(d, p); }
+
+// ---------------------------------------------------------------------------
+// batches: many scenarios advanced by one launch per kernel
+// ---------------------------------------------------------------------------
+
+// A slot table of resident scenarios (each with its own state, tiles, dt and
+// control block) and the work list of one iteration: off[s] is the first
+// work item (sub-block of an active tile) of slot s, off[nslot] the total.
+struct Batch {
+    const Dev* dev;
+    const Params* par;
+    const int* on;        // slot holds a scenario
+    int* off;             // [nslot + 1]
+    int* status;          // [nslot] the scenario's status after each iteration
+    int nslot;
+};
+
+constexpr int BATCH_PLAN_THREADS = 1024;
+
+// Activation of every slot (one block per slot).
+__global__ __launch_bounds__(256) void batch_activate_kernel(Batch bt) {
+    const int s = blockIdx.x;
+    if (!bt.on[s]) return;
+    activate_body(bt.dev[s], bt.par[s]);
+}
+
+// Work list of the iteration: the running slots' active sub-blocks.
+__global__ __launch_bounds__(BATCH_PLAN_THREADS) void batch_plan_kernel(Batch bt) {
+    __shared__ int sc[BATCH_PLAN_THREADS];
+    int carry = 0;
+    for (int base = 0; base < bt.nslot; base += BATCH_PLAN_THREADS) {
+        const int s = base + (int)threadIdx.x;
+        int n = 0;
+        if (s < bt.nslot && bt.on[s]) {
+            const Dev& d = bt.dev[s];
+            const Params& p = bt.par[s];
+            const Ctrl* c = d.ctrl;
+            if (c->status == ST_NEW || c->status == ST_RETRY)
+                n = c->n_active * (((p.nx + B - 1) / B) * ((p.ny + B - 1) / B));
+        }
+        sc[threadIdx.x] = n;
+        __syncthreads();
+        for (int o = 1; o < BATCH_PLAN_THREADS; o <<= 1) {
+            const int v = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (s < bt.nslot) bt.off[s] = carry + sc[threadIdx.x] - n;
+        carry += sc[BATCH_PLAN_THREADS - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bt.off[bt.nslot] = carry;
+}
+
+// One RK stage over the work list.  The grid is a multiple of 8 blocks that
+// stay resident and walk the list; XCD x (blocks x, x+8, ...) takes the x-th
+// contiguous eighth of it, so neighbouring sub-blocks share an L2.
+template <int STAGE, bool VISC, int LIM, bool MIX>
+__global__ __launch_bounds__(NTHREADS) void batch_stage_kernel(Batch bt) {
+    __shared__ SmemT<VISC ? 7 : 5, MIX> s;
+    __shared__ int sh_slot;
+    const int total = bt.off[bt.nslot];
+    const int m = gridDim.x / 8, x = blockIdx.x % 8;
+    const int per = (total + 7) / 8;
+    const int end = min((x + 1) * per, total);
+    for (int w = x * per + (int)(blockIdx.x / 8); w < end; w += m) {
+        if (threadIdx.x == 0) {
+            int lo = 0, hi = bt.nslot - 1;       // last slot with off <= w
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) / 2;
+                if (bt.off[mid] <= w) lo = mid; else hi = mid - 1;
+            }
+            sh_slot = lo;
+        }
+        __syncthreads();
+        const int sl = __builtin_amdgcn_readfirstlane(sh_slot);
+        stage_body<STAGE, VISC, LIM, MIX>(bt.dev[sl], bt.par[sl], s, w - bt.off[sl]);
+        __syncthreads();
+    }
+}
+
+// Step control of every slot, and the status table the host polls.
+__global__ void batch_finalize_kernel(Batch bt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= bt.nslot || !bt.on[s]) return;
+    finalize_body(bt.dev[s], bt.par[s]);
+    bt.status[s] = bt.dev[s].ctrl->status;
+}
+
 // Topography of every tile of a set (after kst_set_topography / kst_set_state).
 __global__ __launch_bounds__(256) void topo_kernel(Dev d, Params p, Topo T) {
     tile_cell_topo(d, p, T, blockIdx.x, false);
@@ -1578,6 +1683,26 @@ struct kst_handle {
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[KW_N] = {};
     long long prof_n[KW_N] = {};
+};
+
+// A batch: a fixed table of slots, each holding a scenario handle whose
+// device state stays resident; one hipGraph of kBatchIters iterations walks
+// every running slot (kernel arguments point at the table, so scenarios join
+// and leave between launches without a new graph).
+struct kst_batch {
+    int nslot = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<kst_handle*> slots;
+    std::vector<int> on;
+    Batch bt{};
+    Dev* d_dev = nullptr;
+    Params* d_par = nullptr;
+    int *d_on = nullptr, *d_off = nullptr, *d_status = nullptr;
+    int visc = -1, mixed = -1;     // fixed by the first scenario attached
+    int grid = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
 };
 
 namespace {
@@ -2295,6 +2420,244 @@ int kst_profile_read(kst_handle* h, double ms[KST_PROFILE_KERNELS], int64_t laun
         h->prof_n[k] = 0;
     }
     return KST_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+static int batch_err(kst_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BCHK(b, call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return batch_err(b, KST_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBatchIters = 8;
+
+typedef void (*BatchStageFn)(Batch);
+struct BatchSet { BatchStageFn fn[4]; };
+
+template <bool VISC, bool MIX>
+BatchSet make_batch_set() {
+    constexpr int L = KST_LIMITER_MINMOD2;
+    return BatchSet{{batch_stage_kernel<1, VISC, L, MIX>, batch_stage_kernel<2, VISC, L, MIX>,
+                     batch_stage_kernel<3, VISC, L, MIX>, batch_stage_kernel<4, VISC, L, MIX>}};
+}
+
+static const BatchSet& batch_set(bool visc, bool mixed) {
+    static const BatchSet sets[2][2] = {{make_batch_set<false, false>(), make_batch_set<false, true>()},
+                                        {make_batch_set<true, false>(), make_batch_set<true, true>()}};
+    return sets[visc ? 1 : 0][mixed ? 1 : 0];
+}
+
+static int batch_launch_iteration(kst_batch* b) {
+    const Batch& bt = b->bt;
+    const BatchSet& k = batch_set(b->visc == 1, b->mixed == 1);
+    hipLaunchKernelGGL(batch_activate_kernel, dim3(b->nslot), dim3(256), 0, b->stream, bt);
+    hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(BATCH_PLAN_THREADS), 0, b->stream, bt);
+    for (int q = 0; q < 4; ++q)
+        hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(NTHREADS), 0, b->stream, bt);
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((b->nslot + 63) / 64), dim3(64), 0, b->stream, bt);
+    BCHK(b, hipGetLastError());
+    return 0;
+}
+
+static int batch_ensure_graph(kst_batch* b) {
+    if (b->gexec) return 0;
+    hipGraph_t g = nullptr;
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < kBatchIters; ++q) {
+        int rc = batch_launch_iteration(b);
+        if (rc) {
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(b->stream, &dummy);
+            if (dummy) (void)hipGraphDestroy(dummy);
+            return rc;
+        }
+    }
+    BCHK(b, hipStreamEndCapture(b->stream, &g));
+    BCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    BCHK(b, hipGraphDestroy(g));
+    return 0;
+}
+
+static int slot_state(int st) {
+    if (st == ST_NEW || st == ST_RETRY) return KST_SLOT_RUNNING;
+    if (st == ST_DONE) return KST_SLOT_DONE;
+    return KST_SLOT_ERROR;
+}
+
+extern "C" {
+
+int kst_batch_create(int max_slots, kst_batch** out) {
+    if (!out || max_slots <= 0 || max_slots > 65536) return KST_ERR_CONFIG;
+    *out = nullptr;
+    kst_batch* b = new kst_batch();
+    b->nslot = max_slots;
+    b->slots.assign(max_slots, nullptr);
+    b->on.assign(max_slots, 0);
+    auto fail = [&](int code) { kst_batch_destroy(b); return code; };
+    if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess)
+        return fail(KST_ERR_HIP);
+    if (hipMalloc(&b->d_dev, sizeof(Dev) * max_slots) != hipSuccess ||
+        hipMalloc(&b->d_par, sizeof(Params) * max_slots) != hipSuccess ||
+        hipMalloc(&b->d_on, sizeof(int) * max_slots) != hipSuccess ||
+        hipMalloc(&b->d_off, sizeof(int) * (max_slots + 1)) != hipSuccess ||
+        hipMalloc(&b->d_status, sizeof(int) * max_slots) != hipSuccess)
+        return fail(KST_ERR_HIP);
+    if (hipMemset(b->d_on, 0, sizeof(int) * max_slots) != hipSuccess ||
+        hipMemset(b->d_off, 0, sizeof(int) * (max_slots + 1)) != hipSuccess ||
+        hipMemset(b->d_status, 0, sizeof(int) * max_slots) != hipSuccess)
+        return fail(KST_ERR_HIP);
+    b->bt = Batch{b->d_dev, b->d_par, b->d_on, b->d_off, b->d_status, max_slots};
+    // a resident grid: the stage kernel's occupancy on every CU, a multiple of 8
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, batch_stage_kernel<2, true, KST_LIMITER_MINMOD2, false>, NTHREADS, 0) != hipSuccess)
+        return fail(KST_ERR_HIP);
+    b->grid = std::max(8, (cus * std::max(per_cu, 1)) / 8 * 8);
+    *out = b;
+    return KST_OK;
+}
+
+int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot) {
+    if (!b || !h || !slot) return KST_ERR_CONFIG;
+    if (!h->have_state) return batch_err(b, KST_ERR_STATE, "kst_set_state must come before kst_batch_attach");
+    if (h->d.morpho) return batch_err(b, KST_ERR_CONFIG, "batches run the hydraulic operator only");
+    if (h->p.limiter != KST_LIMITER_MINMOD2)
+        return batch_err(b, KST_ERR_CONFIG, "batches use the MinMod2 limiter");
+    const int visc = h->p.nu != 0.0 ? 1 : 0, mixed = h->d.mixed ? 1 : 0;
+    bool any = false;
+    for (int q = 0; q < b->nslot; ++q) any = any || b->on[q];
+    if (any && (visc != b->visc || mixed != b->mixed))
+        return batch_err(b, KST_ERR_CONFIG, "a batch holds scenarios of one precision and one eddy-viscosity setting");
+    int s = -1;
+    for (int q = 0; q < b->nslot && s < 0; ++q)
+        if (!b->slots[q]) s = q;
+    if (s < 0) return batch_err(b, KST_ERR_CONFIG, "no free slot");
+    if (b->visc != visc || b->mixed != mixed) {
+        // the graph's kernels depend on the setting
+        if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        b->visc = visc;
+        b->mixed = mixed;
+    }
+    BCHK(b, hipStreamSynchronize(h->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    const int one = 1, st = ST_DONE;
+    BCHK(b, hipMemcpy(b->d_dev + s, &h->d, sizeof(Dev), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->d_par + s, &h->p, sizeof(Params), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->d_status + s, &st, sizeof(int), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->d_on + s, &one, sizeof(int), hipMemcpyHostToDevice));
+    b->slots[s] = h;
+    b->on[s] = 1;
+    *slot = s;
+    return KST_OK;
+}
+
+int kst_batch_detach(kst_batch* b, int slot) {
+    if (!b || slot < 0 || slot >= b->nslot || !b->slots[slot]) return KST_ERR_CONFIG;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    const int zero = 0;
+    BCHK(b, hipMemcpy(b->d_on + slot, &zero, sizeof(int), hipMemcpyHostToDevice));
+    b->slots[slot] = nullptr;
+    b->on[slot] = 0;
+    return KST_OK;
+}
+
+int kst_batch_start(kst_batch* b, int slot, double tend, int64_t nsteps) {
+    if (!b || slot < 0 || slot >= b->nslot || !b->slots[slot]) return KST_ERR_CONFIG;
+    kst_handle* h = b->slots[slot];
+    BCHK(b, hipStreamSynchronize(b->stream));
+    Ctrl c;
+    if (read_ctrl(h, &c)) return batch_err(b, KST_ERR_HIP, h->err);
+    if (c.status == ST_ERROR) return batch_err(b, c.err, "the scenario is in an error state");
+    const bool by_steps = nsteps > 0;
+    if (!by_steps && !(tend > c.t)) return batch_err(b, KST_ERR_CONFIG, "tend must lie after the scenario's time");
+    c.status = ST_NEW;
+    c.mode = by_steps ? 1 : 0;
+    c.tend = by_steps ? DBL_MAX : tend;
+    c.steps_left = by_steps ? nsteps : 0;
+    c.steps = c.rejected = c.rhs_evals = c.cell_updates = 0;
+    c.redist_cells = c.redist_ghost = 0;
+    c.mref = 0; c.ncand = 0; c.commit = 0;
+    c.min_dt = 0.0; c.max_dt = 0.0;
+    if (write_ctrl(h, &c)) return batch_err(b, KST_ERR_HIP, h->err);
+    const int st = ST_NEW;
+    BCHK(b, hipMemcpy(b->d_status + slot, &st, sizeof(int), hipMemcpyHostToDevice));
+    return KST_OK;
+}
+
+int kst_batch_run(kst_batch* b, int64_t max_iters, int* status) {
+    if (!b) return KST_ERR_CONFIG;
+    std::vector<int> st(b->nslot, ST_DONE);
+    int64_t iters = 0;
+    bool running = false;
+    for (int q = 0; q < b->nslot; ++q) running = running || b->on[q];
+    if (running) {
+        int rc = batch_ensure_graph(b);
+        if (rc) return rc;
+    }
+    while (running && iters < max_iters) {
+        BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+        iters += kBatchIters;
+        BCHK(b, hipMemcpyAsync(st.data(), b->d_status, sizeof(int) * b->nslot, hipMemcpyDeviceToHost, b->stream));
+        BCHK(b, hipStreamSynchronize(b->stream));
+        running = false;
+        for (int q = 0; q < b->nslot; ++q)
+            if (b->on[q] && (st[q] == ST_NEW || st[q] == ST_RETRY)) running = true;
+    }
+    if (!running) {
+        BCHK(b, hipMemcpyAsync(st.data(), b->d_status, sizeof(int) * b->nslot, hipMemcpyDeviceToHost, b->stream));
+        BCHK(b, hipStreamSynchronize(b->stream));
+    }
+    if (status)
+        for (int q = 0; q < b->nslot; ++q) status[q] = b->on[q] ? slot_state(st[q]) : KST_SLOT_FREE;
+    return KST_OK;
+}
+
+int kst_batch_stats(kst_batch* b, int slot, kst_stats* out) {
+    if (!b || !out || slot < 0 || slot >= b->nslot || !b->slots[slot]) return KST_ERR_CONFIG;
+    kst_handle* h = b->slots[slot];
+    BCHK(b, hipStreamSynchronize(b->stream));
+    Ctrl c;
+    if (read_ctrl(h, &c)) return batch_err(b, KST_ERR_HIP, h->err);
+    std::memset(out, 0, sizeof *out);
+    out->steps = c.steps; out->rejected = c.rejected; out->rhs_evals = c.rhs_evals;
+    out->cell_updates = c.cell_updates; out->active_tiles = c.n_active; out->ghost_tiles = c.n_ghost;
+    out->t = c.t; out->last_dt = c.last_dt; out->min_dt = c.min_dt; out->max_dt = c.max_dt;
+    h->t = c.t;
+    if (c.status == ST_ERROR) {
+        if (c.err == KST_ERR_HALT)
+            return set_err(h, KST_ERR_HALT, "Error: tried to add a tile outside the domain. Try increasing the "
+                                            "domain size or repositioning its location.");
+        return set_err(h, c.err ? c.err : KST_ERR_STATE, "device error");
+    }
+    return KST_OK;
+}
+
+int kst_batch_last_error(const kst_batch* b, char* buf, int buflen) {
+    if (!b || !buf || buflen <= 0) return KST_ERR_CONFIG;
+    std::snprintf(buf, buflen, "%s", b->err.c_str());
+    return KST_OK;
+}
+
+void kst_batch_destroy(kst_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    (void)hipFree(b->d_dev); (void)hipFree(b->d_par); (void)hipFree(b->d_on);
+    (void)hipFree(b->d_off); (void)hipFree(b->d_status);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 void kst_destroy(kst_handle* h) {

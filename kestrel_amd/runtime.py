@@ -32,6 +32,8 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_topography",
            "kst_set_state", "kst_integrate_to", "kst_step_n", "kst_get_state", "kst_get_maxima",
            "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read", "kst_get_tile_bt",
+           "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
+           "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
            "kst_get_cell_topography"]
 
 
@@ -80,6 +82,18 @@ def lib():
         L.kst_get_cell_topography.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.kst_destroy.argtypes = [ctypes.c_void_p]
         L.kst_destroy.restype = None
+        L.kst_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.kst_batch_attach.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.kst_batch_detach.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.kst_batch_start.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64]
+        L.kst_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int)]
+        L.kst_batch_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(KstStats)]
+        L.kst_batch_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        L.kst_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.kst_batch_destroy.restype = None
+        for n in ["kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
+                  "kst_batch_run", "kst_batch_stats", "kst_batch_last_error"]:
+            getattr(L, n).restype = ctypes.c_int
         for n in ["kst_create", "kst_set_topography", "kst_set_state", "kst_integrate_to",
                   "kst_step_n", "kst_get_state", "kst_get_maxima", "kst_last_error",
                   "kst_get_tile_bt", "kst_get_cell_topography"]:
@@ -184,3 +198,63 @@ class Stepper:
         ptrs = (_dp * 11)(*[_ptr(a) for a in arrs])
         self._check(self.L.kst_get_maxima(self.h, ptrs))
         return arrs
+
+
+SLOT_FREE, SLOT_RUNNING, SLOT_DONE, SLOT_ERROR = -1, 0, 1, 2
+
+
+class Batch:
+    """Scenarios resident together and advanced by one launch per kernel
+    (kst_batch_*).  Attached Steppers are advanced only through the batch."""
+
+    def __init__(self, max_slots: int):
+        self.L = lib()
+        b = ctypes.c_void_p()
+        rc = self.L.kst_batch_create(int(max_slots), ctypes.byref(b))
+        if rc != 0:
+            raise KestrelError(rc, "kst_batch_create failed")
+        self.b = b
+        self.max_slots = int(max_slots)
+        self.members: dict[int, Stepper] = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            buf = ctypes.create_string_buffer(512)
+            self.L.kst_batch_last_error(self.b, buf, 512)
+            raise KestrelError(rc, buf.value.decode())
+
+    def attach(self, stepper: Stepper) -> int:
+        slot = ctypes.c_int()
+        self._check(self.L.kst_batch_attach(self.b, stepper.h, ctypes.byref(slot)))
+        self.members[slot.value] = stepper
+        return slot.value
+
+    def detach(self, slot: int):
+        self._check(self.L.kst_batch_detach(self.b, int(slot)))
+        self.members.pop(slot, None)
+
+    def start(self, slot: int, tend: float = 0.0, nsteps: int = 0):
+        self._check(self.L.kst_batch_start(self.b, int(slot), float(tend), int(nsteps)))
+
+    def run(self, max_iters: int = 1 << 40) -> list:
+        st = (ctypes.c_int * self.max_slots)()
+        self._check(self.L.kst_batch_run(self.b, int(max_iters), st))
+        return list(st)
+
+    def stats(self, slot: int) -> dict:
+        out = KstStats()
+        rc = self.L.kst_batch_stats(self.b, int(slot), ctypes.byref(out))
+        if rc != 0:
+            self.members[slot]._check(rc)
+        return out.as_dict()
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.L.kst_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,6 +11,10 @@ morpho_hills_1024  : 1024x1024, dx = 5 m, seeded 32 Gaussian hills on a
                      5 degree ramp, one-cell flux source q = 20 m^2/s (Q = q*dx),
                      psi = 0.3, Pouliquen drag, Mixed erosion, Spearman-Manning
                      deposition, eddy viscosity 1e-2, 2000 steps.
+ensemble           : 64 hydraulic scenarios, sides log-uniform in [128, 1024]
+                     (multiples of 32), dx = 5 m, seeded Gaussian hills on a
+                     2 degree ramp, a paraboloid cap of volume U[1e3, 1e5] m^3
+                     (radius 50 m), Chezy coefficient U[0.005, 0.05].
 """
 from __future__ import annotations
 
@@ -201,3 +205,41 @@ def morpho_fixture_case(name: str) -> Workload:
 
 
 MORPHO_FIXTURE_CASES = ["mixed", "simple", "fluid", "granular", "fastsettling", "nogeo"]
+
+
+def ensemble_member(k: int, seed: int = 20240612, side_range=(128, 1024), dx: float = 5.0,
+                    tile: int = 16, steps: int = 100, precision: int = C.PREC_MIXED,
+                    n_hills: int = 16) -> Workload:
+    """Scenario k of the batched ensemble: its own seeded draw of grid side
+    (log-uniform in side_range, a multiple of 32), Gaussian-hill DEM on a
+    2 degree x-ramp, cap volume V ~ U[1e3, 1e5] m^3 painted at the domain
+    centre as a paraboloid of radius 0.06 L (height 2 V / (pi r^2), as
+    capVolume sets it, src/InitConds.f90:362-368), and Chezy coefficient
+    U[0.005, 0.05].  Tiles of 16 cells."""
+    rng = np.random.default_rng([seed, k])
+    lo, hi = math.log(side_range[0]), math.log(side_range[1])
+    side = int(round(math.exp(rng.uniform(lo, hi)) / 32)) * 32
+    side = min(max(side, side_range[0]), side_range[1])
+    vol = float(rng.uniform(1.0e3, 1.0e5))
+    chezy = float(rng.uniform(0.005, 0.05))
+    hseed = int(rng.integers(1 << 30))
+    nt = side // tile
+    cfg = C.default_config(nt, nt, tile, tile, dx=dx)
+    C.hydraulic_only(cfg)
+    cfg.drag = C.DRAGS["chezy"]
+    cfg.chezy_co = chezy
+    cfg.cfl = 0.25
+    cfg.precision = precision
+    C.derive(cfg)
+    b0v = S.topo_gaussian_hills(cfg, S.Hills(hseed, n=n_hills, ramp_deg=2.0))
+    L = side * dx
+    cap_radius = 0.06 * L
+    height = 2.0 * vol / (math.pi * cap_radius * cap_radius)
+    caps = [S.Cap(0.0, 0.0, cap_radius, height, shape="para")]
+    init = S.initial_state(cfg, b0v, caps)
+    return Workload(f"ensemble_{k}_side{side}", cfg, b0v, init, steps, caps, [])
+
+
+def ensemble(n: int = 64, steps: int = 100, precision: int = C.PREC_MIXED, **kw) -> list:
+    """The batched ensemble: n independent scenarios (see ensemble_member)."""
+    return [ensemble_member(k, steps=steps, precision=precision, **kw) for k in range(n)]
