@@ -171,6 +171,73 @@ def roofline(wl_name: str, rows: list, cells: float) -> dict:
             "mean_launch_us": top["mean_us"], "cells_per_launch": cells}
 
 
+def ensemble_leg(n: int = 64, steps: int = 100) -> dict:
+    """The batched ensemble: n scenarios of `steps` steps each, advanced
+    (a) one after another on the plain FP64 path, (b) together in one FP64
+    batch, (c) together in one mixed-precision batch.  Setup (create,
+    topography, state) is outside every timed region."""
+    import torch
+    from kestrel_amd import config as C
+    from kestrel_amd import workloads
+    from kestrel_amd.runtime import Batch, Stepper
+
+    wls = workloads.ensemble(n, steps)
+
+    def make(w, prec):
+        w.cfg.precision = prec
+        g = Stepper(w.cfg)
+        g.set_topography(w.b0v)
+        g.set_state(w.init)
+        return g
+
+    cells = 0
+    el = 0.0
+    for w in wls:
+        g = make(w, C.PREC_FP64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = g.step_n(steps)
+        el += time.perf_counter() - t0
+        cells += st["cell_updates"]
+        g.close()
+    out = {"workload": f"ensemble of {n}: sides log-uniform 128-1024, dx 5 m, cap U[1e3,1e5] m^3, "
+                       f"Chezy U[0.005,0.05]", "steps_per_scenario": steps,
+           "grid_cells": sum(w.cfg.NX * w.cfg.NY for w in wls), "cell_updates": cells,
+           "plain_fp64_sequential": cells / el}
+    for prec, name in ((C.PREC_FP64, "batched_fp64"), (C.PREC_MIXED, "batched_mixed")):
+        b = Batch(n)
+        gs = [make(w, prec) for w in wls]
+        slots = [b.attach(g) for g in gs]
+        for sl in slots:
+            b.start(sl, nsteps=steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        status = b.run()
+        el = time.perf_counter() - t0
+        c = sum(b.stats(sl)["cell_updates"] for sl in slots)
+        assert c == cells and all(status[sl] == 1 for sl in slots), f"{name}: batch diverged"
+        out[name] = c / el
+        b.close()
+        for g in gs:
+            g.close()
+    out["unit"] = "cell-updates/s"
+    out["batched_mixed_vs_plain"] = out["batched_mixed"] / out["plain_fp64_sequential"]
+    return out
+
+
+def serving_leg(n: int = 32, rate: float = 4.0) -> dict:
+    """A Poisson trace of n requests at `rate` per second (sides log-uniform
+    64-2048, T ~ U[60, 600] s simulated) served with continuous batching
+    under a 200 GB HBM budget, mixed precision."""
+    from kestrel_amd import serving
+    reqs = serving.poisson_trace(n, rate)
+    srv = serving.Server(256, 200e9)
+    out = srv.run(reqs)
+    srv.close()
+    out["trace"] = f"{n} requests, Poisson rate {rate}/s, sides 64-2048, T U[60,600] s, mixed precision"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +247,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -233,6 +303,10 @@ def main():
                              "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pw.name, prows, pcells),
                              "step_roofline_frac": pst["cell_updates"] / pel * STEP_WORDS["hydraulic"] * 8
                              / (HBM_PEAK_GBS * 1e9)}
+    if not args.no_batch:
+        res["ensemble"] = ensemble_leg()
+    if not args.no_serve:
+        res["serving"] = serving_leg(args.serve_requests)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
     print(json.dumps(res), flush=True)

@@ -52,12 +52,16 @@ class Request:
     state: dict | None = None
 
 
-def trace_request(k: int, seed: int = 20240613, side_range=(64, 2048), dx: float = 5.0,
+def trace_request(k: int, seed: int = 20240613, side_range=(64, 2048),
                   precision: int = C.PREC_MIXED, t_range=(60.0, 600.0)) -> tuple:
     """Request k of the serving trace: a grid side log-uniform in side_range
-    (a multiple of 32), a seeded Gaussian-hill DEM on a 2 degree ramp, a cap
-    (volume U[1e3, 1e5] m^3) or a one-cell flux source (q U[1, 20] m^2/s for
-    the whole run), Chezy U[0.005, 0.05], run to T ~ U[t_range]."""
+    (a multiple of 32) with dx = 5 m, coarsened by powers of two below 512
+    cells so that every domain spans at least 2.5 km; a seeded Gaussian-hill
+    DEM in a shallow bowl (50 m rise to the domain edge) that keeps the flow
+    on the grid; a paraboloid cap of volume V ~ U[1e3, 1e5] m^3 (radius
+    0.06 L) or a one-cell flux source releasing the same kind of volume over
+    the whole run (Q = V / T); Chezy coefficient U[0.005, 0.05]; run to
+    T ~ U[t_range] s."""
     rng = np.random.default_rng([seed, k])
     lo, hi = math.log(side_range[0]), math.log(side_range[1])
     side = int(round(math.exp(rng.uniform(lo, hi)) / 32)) * 32
@@ -66,8 +70,8 @@ def trace_request(k: int, seed: int = 20240613, side_range=(64, 2048), dx: float
     chezy = float(rng.uniform(0.005, 0.05))
     use_flux = bool(rng.uniform() < 0.5)
     vol = float(rng.uniform(1.0e3, 1.0e5))
-    q = float(rng.uniform(1.0, 20.0))
     hseed = int(rng.integers(1 << 30))
+    dx = 5.0 * 2 ** max(0, math.ceil(math.log2(512 / side)))
     tile = 16
     nt = side // tile
     cfg = C.default_config(nt, nt, tile, tile, dx=dx)
@@ -77,13 +81,17 @@ def trace_request(k: int, seed: int = 20240613, side_range=(64, 2048), dx: float
     cfg.cfl = 0.25
     cfg.precision = precision
     C.derive(cfg)
-    b0v = S.topo_gaussian_hills(cfg, S.Hills(hseed, n=16, ramp_deg=2.0))
     L = side * dx
+    b0v = S.topo_gaussian_hills(cfg, S.Hills(hseed, n=16, ramp_deg=0.0))
+    xv, yv = S.vertex_coords(cfg)
+    X, Y = np.meshgrid(xv, yv)
+    b0v = b0v + 50.0 * (X * X + Y * Y) / (0.5 * L) ** 2
     if use_flux:
         xc, yc = S.cell_centres(cfg)
         x0 = float(xc[np.argmin(np.abs(xc))])
         y0 = float(yc[np.argmin(np.abs(yc))])
-        src = C.FluxSource(x0, y0, 0.4 * dx, [0.0, 1.0e9], [q * dx, q * dx], [0.0, 0.0])
+        Q = vol / tend
+        src = C.FluxSource(x0, y0, 0.4 * dx, [0.0, 1.0e9], [Q, Q], [0.0, 0.0])
         init = S.initial_state(cfg, b0v, [], [src])
         C.set_sources(cfg, [src])
         wl = workloads.Workload(f"serve_{k}_side{side}_flux", cfg, b0v, init, 0, [], [src])
@@ -120,6 +128,8 @@ class Server:
         self.resident: dict = {}       # slot -> (Request, Stepper, bytes)
         self.done: list = []
         self.cell_updates = 0
+        self.peak_resident = 0
+        self.peak_bytes = 0
 
     def submit(self, req: Request):
         heapq.heappush(self.queue, (req.arrival, req.rid, req))
@@ -140,6 +150,8 @@ class Server:
             req.status = "running"
             self.resident[slot] = (req, g, need)
             self.used += need
+            self.peak_resident = max(self.peak_resident, len(self.resident))
+            self.peak_bytes = max(self.peak_bytes, self.used)
 
     def _retire(self, status: list, now: float):
         for slot in list(self.resident):
@@ -180,6 +192,7 @@ class Server:
         lat = np.array([r.finished - r.arrival for r in self.done])
         ok = [r for r in self.done if r.status == "done"]
         return {"requests": len(self.done), "completed": len(ok), "wall_s": wall,
+                "peak_resident": self.peak_resident, "peak_device_bytes": self.peak_bytes,
                 "cell_updates": self.cell_updates,
                 "cell_updates_per_s": self.cell_updates / wall if wall > 0 else 0.0,
                 "latency_p50_s": float(np.percentile(lat, 50)) if lat.size else math.nan,
