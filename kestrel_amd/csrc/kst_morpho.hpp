@@ -86,7 +86,7 @@ __device__ __forceinline__ double excess_deposition(const Dev& d, const Params& 
 //   w, Hnpsi from the change of bt (src/TimeStepper.f90:587-610)
 // Stage 3 then runs the checks of src/TimeStepper.f90:713-752.
 template <int S>
-__global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(STAGE_NT) void morpho_kernel(Dev d, Params p) {
     __shared__ double hn[NREG];            // Hn of intermed_{S-1}, 2-cell halo
     __shared__ double emd[MR1 * MR1];      // E - D, 1-cell halo
     __shared__ double sbx[MR1 * MR1], sby[MR1 * MR1];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
     const double eps = p.hthr;
 
     // ---- Hn of the stage state (ComputeDesingularisedVariables) -----------
-    for (int q = tid; q < NREG; q += NTHREADS) {
+    for (int q = tid; q < NREG; q += STAGE_NT) {
         const int rx = q % RW - 2, ry = q / RW - 2;
         if (rx > bxa + 1 || ry > bya + 1) continue;
         int tc, ci, cj;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
     __syncthreads();
 
     // ---- E - D with the dryness check (src/MorphodynamicRHS.f90:90-145) ---
-    for (int q = tid; q < MR1 * MR1; q += NTHREADS) {
+    for (int q = tid; q < MR1 * MR1; q += STAGE_NT) {
         const int rx = q % MR1 - 1, ry = q / MR1 - 1;
         if (rx > bxa || ry > bya) continue;
         int tc, ci, cj;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(NTHREADS) void morpho_kernel(Dev d, Params p) {
     const int NV1 = p.nx + 1;
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     const double psib = 1.0 - p.porosity;
-    for (int q = tid; q < MV * MV; q += NTHREADS) {
+    for (int q = tid; q < MV * MV; q += STAGE_NT) {
         const int a = q % MV, b = q / MV;
         if (a > bxa || b > bya) continue;
         const int vi = bx0 + a, vj = by0 + b;
@@ -854,10 +854,13 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
 }
 
 // ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),
-// plus its CFL numerators and prefactors; one active tile per block.
+// plus its CFL numerators and prefactors; TILE_PARTS blocks per active tile.
+constexpr int TILE_PARTS = 4;
+
 __global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
     __shared__ int sh_skip;
     const Ctrl* c = d.ctrl;
+    const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
     if (threadIdx.x == 0) {
         int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
@@ -865,39 +868,43 @@ __global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
             if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
         if (!skip) count_work(c, KW_FACETOPO);
-        if ((int)blockIdx.x >= c->n_active) skip = 1;
+        if (q >= c->n_active) skip = 1;
         sh_skip = skip;
     }
     __syncthreads();
     if (sh_skip) return;
-    const int t = d.active[blockIdx.x];
-    tile_face_topo(d, p, d.TB, t);
-    __syncthreads();
-    tile_static_terms(d, p, d.TB, t);
+    const int t = d.active[q];
+    // a face's static terms read only that face's topography, which the same
+    // thread computes, and cell gamma, which the morphodynamic stage wrote
+    tile_face_topo(d, p, d.TB, t, part, TILE_PARTS);
+    tile_static_terms(d, p, d.TB, t, part, TILE_PARTS);
 }
 
 // The accepted step's bed becomes the current one (CopySolutionData
-// intermed3 -> tileContainer, src/TimeStepper.f90:263); one active tile per block.
+// intermed3 -> tileContainer, src/TimeStepper.f90:263); TILE_PARTS blocks per
+// active tile.
 __global__ __launch_bounds__(256) void commit_kernel(Dev d, Params p) {
     const Ctrl* c = d.ctrl;
-    if (!c->commit || (int)blockIdx.x >= c->n_active) return;
+    const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
+    if (!c->commit || q >= c->n_active) return;
     if (threadIdx.x == 0) count_work(c, KW_COMMIT);
-    const int t = d.active[blockIdx.x];
+    const int t = d.active[q];
+    const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * TILE_PARTS;
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
-    for (size_t q = threadIdx.x; q < NV; q += blockDim.x) d.TA.vbt[t * NV + q] = d.TB.vbt[t * NV + q];
-    for (int q = threadIdx.x; q < p.NC; q += blockDim.x) {
-        const size_t g = (size_t)t * p.NC + q;
+    for (size_t v = lane0; v < NV; v += stride) d.TA.vbt[t * NV + v] = d.TB.vbt[t * NV + v];
+    for (int k = lane0; k < p.NC; k += stride) {
+        const size_t g = (size_t)t * p.NC + k;
         d.TA.btc[g] = d.TB.btc[g]; d.TA.cbx[g] = d.TB.cbx[g]; d.TA.cby[g] = d.TB.cby[g];
         d.TA.cgam[g] = d.TB.cgam[g]; d.TA.pxu[g] = d.TB.pxu[g]; d.TA.pxv[g] = d.TB.pxv[g];
         d.TA.pyv[g] = d.TB.pyv[g];
     }
-    for (int q = threadIdx.x; q < p.NFX; q += blockDim.x) {
-        const size_t f = (size_t)t * p.NFX + q;
+    for (int k = lane0; k < p.NFX; k += stride) {
+        const size_t f = (size_t)t * p.NFX + k;
         d.TA.xbt[f] = d.TB.xbt[f]; d.TA.xby[f] = d.TB.xby[f]; d.TA.xgam[f] = d.TB.xgam[f];
         d.TA.xib[f] = d.TB.xib[f]; d.TA.xnp[f] = d.TB.xnp[f]; d.TA.xnm[f] = d.TB.xnm[f];
     }
-    for (int q = threadIdx.x; q < p.NFY; q += blockDim.x) {
-        const size_t f = (size_t)t * p.NFY + q;
+    for (int k = lane0; k < p.NFY; k += stride) {
+        const size_t f = (size_t)t * p.NFY + k;
         d.TA.ybt[f] = d.TB.ybt[f]; d.TA.ybx[f] = d.TB.ybx[f]; d.TA.ygam[f] = d.TB.ygam[f];
         d.TA.yib[f] = d.TB.yib[f]; d.TA.ynp[f] = d.TB.ynp[f]; d.TA.ynm[f] = d.TB.ynm[f];
     }

@@ -35,6 +35,11 @@ constexpr int B = 16;            // sub-block edge (cells)
 constexpr int RW = B + 4;        // region width with 2-cell halo
 constexpr int NREG = RW * RW;
 constexpr int NTHREADS = 256;
+// Threads of a stage (and morphodynamic-stage) workgroup: 256 own the
+// sub-block's cells.  With 512, 128 more would load the halo strips alongside
+// them and all would share the faces; measured slower on MI355X (register
+// pressure halves the resident waves), so the halo takes a second round.
+constexpr int STAGE_NT = 256;
 constexpr int NFACE = (B + 1) * B;
 
 enum Status { ST_NEW = 0, ST_RETRY = 1, ST_DONE = 2, ST_ERROR = 3 };
@@ -271,7 +276,7 @@ struct SmemT {
     FT fx[NQ][NFACE];        // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
     FT fy[NQ][NFACE];
     double fx4[MIX ? NFACE : 1], fy4[MIX ? NFACE : 1];
-    double red[NTHREADS / 64];
+    double red[STAGE_NT / 64];
     int tile, bx0, by0, bxa, bya, skip, cur;
     double dt, tstage, tgrid;
 };
@@ -597,7 +602,7 @@ __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = s.red[0];
-        for (int i = 1; i < NTHREADS / 64; ++i) m = fmin(m, s.red[i]);
+        for (int i = 1; i < STAGE_NT / 64; ++i) m = fmin(m, s.red[i]);
         if (m < kHuge) atomicMin(slot, dbits(m));
     }
 }
@@ -720,14 +725,19 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
     double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stage 2; mixed: 2..4)
     int oidx = 0;
     const double* Un = (MIX && s.cur) ? d.S0b : d.S0;   // u^n of the step
-    for (int round = 0; round < 2; ++round) {
+    // threads 0..255: the owned cells; the 2-cell halo strips (4 x 32 cells)
+    // by threads 256..383 in the same round, or by 128..255 in a second one
+    // (waves 0 and 1 take the extra faces of phase B instead)
+    constexpr bool one_round = STAGE_NT >= 384;
+    for (int round = 0; round < (one_round ? 1 : 2); ++round) {
         int rx, ry;
         bool valid;
-        if (round == 0) {
+        const int hq = one_round ? tid - 256 : tid - 128;
+        if (round == 0 && tid < 256) {
             rx = tx; ry = ty; valid = own;
         } else {
-            if (tid >= 128) break;
-            const int strip = tid >> 5, k = tid & 31, dd = k >> 4, pos = k & 15;
+            if (hq < 0 || hq >= 128) break;
+            const int strip = hq >> 5, k = hq & 31, dd = k >> 4, pos = k & 15;
             if (strip == 0) { rx = -2 + dd; ry = pos; valid = pos < bya; }
             else if (strip == 1) { rx = bxa + dd; ry = pos; valid = pos < bya; }
             else if (strip == 2) { rx = pos; ry = -2 + dd; valid = pos < bxa; }
@@ -799,7 +809,7 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
         }
     }
     // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
-    for (int q = tid; q < B * (B + 3); q += NTHREADS) {
+    for (int q = tid; q < B * (B + 3); q += STAGE_NT) {
         const int r = q / (B + 3), k = q % (B + 3) - 1;
         if (r < bya && k <= bxa + 1) s.xbed[r][k + 1] = face_bed<false>(d, p, t, s.bx0 + k, s.by0 + r);
         if (r < bxa && k <= bya + 1) s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
@@ -846,15 +856,27 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
     // ---- phase B: face fluxes --------------------------------------------
     double cflm = kHuge;
     // x-faces: consecutive lanes walk along a row; y-faces: along a row too
-    if (bxa == B && bya == B) {
-        for (int f = tid; f < (B + 1) * B; f += NTHREADS)
+    if (bxa == B && bya == B && STAGE_NT == 256) {
+        // 272 x-faces and 272 y-faces: one of each per thread, the 16 extra
+        // x-faces on wave 0 and the 16 extra y-faces on wave 1
+        face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, tid % (B + 1), tid / (B + 1), cflm);
+        face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, tid / B, tid % B, cflm);
+        if (tid < 16) {
+            const int f = 256 + tid;
             face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (B + 1), f / (B + 1), cflm);
-        for (int f = tid; f < (B + 1) * B; f += NTHREADS)
+        } else if (tid >= 64 && tid < 80) {
+            const int f = 256 + tid - 64;
+            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
+        }
+    } else if (bxa == B && bya == B) {
+        for (int f = tid; f < (B + 1) * B; f += STAGE_NT)
+            face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (B + 1), f / (B + 1), cflm);
+        for (int f = tid; f < (B + 1) * B; f += STAGE_NT)
             face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
     } else {
-        for (int f = tid; f < (bxa + 1) * bya; f += NTHREADS)
+        for (int f = tid; f < (bxa + 1) * bya; f += STAGE_NT)
             face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (bxa + 1), f / (bxa + 1), cflm);
-        for (int f = tid; f < (bya + 1) * bxa; f += NTHREADS)
+        for (int f = tid; f < (bya + 1) * bxa; f += STAGE_NT)
             face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / bxa, f % bxa, cflm);
     }
     if (STAGE != 4) {
@@ -1035,7 +1057,7 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
 }
 
 template <int STAGE, bool VISC, int LIM, bool MIX = false>
-__global__ __launch_bounds__(NTHREADS) void stage_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(STAGE_NT) void stage_kernel(Dev d, Params p) {
     __shared__ SmemT<VISC ? 7 : 5, MIX> s;
     stage_body<STAGE, VISC, LIM, MIX>(d, p, s, -1);
 }
@@ -1076,7 +1098,10 @@ __device__ void tile_cell_topo(const Dev& d, const Params& p, const Topo& T, int
     }
 }
 
-__device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int t) {
+// part/nparts: the share of the tile's faces one block of several computes.
+__device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int t, int part = 0,
+                               int nparts = 1) {
+    const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
     const int ti = t % p.ntx, tj = t / p.ntx;
     // neighbour tiles (clamped at the domain edge: their values are never read)
@@ -1090,7 +1115,7 @@ __device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int
     const double* bS = d.vb0 + (size_t)tS * NV; const double* btS = T.vbt + (size_t)tS * NV;
     const double* bN = d.vb0 + (size_t)tN * NV; const double* btN = T.vbt + (size_t)tN * NV;
     // x-faces: i in [0, nx], j in [0, ny)
-    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
+    for (int f = lane0; f < (nx + 1) * ny; f += stride) {
         const int i = f % (nx + 1), j = f / (nx + 1);
         Kahan kx;
         if (i >= 1 && i <= nx - 1) {
@@ -1120,7 +1145,7 @@ __device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int
         T.xib[fi] = ib;
     }
     // y-faces: i in [0, nx), j in [0, ny]
-    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
+    for (int f = lane0; f < nx * (ny + 1); f += stride) {
         const int i = f % nx, j = f / nx;
         Kahan kx;
         kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
@@ -1157,12 +1182,14 @@ __device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int
 // (src/HydraulicRHS.f90:983-1006) enters the CFL as (gr*gr)*dx; the
 // prefactors of ConstructHydraulicRHS (src/HydraulicRHS.f90:1233-1243).
 // Needs the cell gamma of the neighbour tiles.
-__device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, int t) {
+__device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, int t, int part = 0,
+                                  int nparts = 1) {
+    const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
     const int nx = p.nx, ny = p.ny;
     const int ti = t % p.ntx, tj = t / p.ntx;
     const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
     const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
-    for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
+    for (int c = lane0; c < p.NC; c += stride) {
         const int gi = t * p.NC + c;
         const double bx = T.cbx[gi], by = T.cby[gi], gam = T.cgam[gi];
         if (p.geometric) {
@@ -1173,7 +1200,7 @@ __device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, 
             T.pxu[gi] = 1.0; T.pxv[gi] = 0.0; T.pyv[gi] = 1.0;
         }
     }
-    for (int f = threadIdx.x; f < (nx + 1) * ny; f += blockDim.x) {
+    for (int f = lane0; f < (nx + 1) * ny; f += stride) {
         const int i = f % (nx + 1), j = f / (nx + 1);
         const double fg = T.xgam[t * p.NFX + f];
         const double gl = (i == 0) ? T.cgam[tW * p.NC + j * nx + nx - 1] : T.cgam[t * p.NC + j * nx + i - 1];
@@ -1182,7 +1209,7 @@ __device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, 
         T.xnp[t * p.NFX + f] = a * a * p.dx;
         T.xnm[t * p.NFX + f] = b * b * p.dx;
     }
-    for (int f = threadIdx.x; f < nx * (ny + 1); f += blockDim.x) {
+    for (int f = lane0; f < nx * (ny + 1); f += stride) {
         const int i = f % nx, j = f / nx;
         const double fg = T.ygam[t * p.NFY + f];
         const double gl = (j == 0) ? T.cgam[tS * p.NC + (ny - 1) * nx + i] : T.cgam[t * p.NC + (j - 1) * nx + i];
@@ -1613,7 +1640,7 @@ __global__ __launch_bounds__(BATCH_PLAN_THREADS) void batch_plan_kernel(Batch bt
 // stay resident and walk the list; XCD x (blocks x, x+8, ...) takes the x-th
 // contiguous eighth of it, so neighbouring sub-blocks share an L2.
 template <int STAGE, bool VISC, int LIM, bool MIX>
-__global__ __launch_bounds__(NTHREADS) void batch_stage_kernel(Batch bt) {
+__global__ __launch_bounds__(STAGE_NT) void batch_stage_kernel(Batch bt) {
     __shared__ SmemT<VISC ? 7 : 5, MIX> s;
     __shared__ int sh_slot;
     const int total = bt.off[bt.nslot];
@@ -1856,20 +1883,20 @@ int launch_iteration(kst_handle* h, bool profiled) {
     LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), d, p);
     const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0);
     for (int q = 0; q < 4; ++q)
-        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk), dim3(NTHREADS), d, p);
+        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk), dim3(STAGE_NT), d, p);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
-        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(NTHREADS), d, p);
-        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(NTHREADS), d, p);
-        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(NTHREADS), d, p);
+        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(STAGE_NT), d, p);
+        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(STAGE_NT), d, p);
+        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(STAGE_NT), d, p);
         LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), d, p);
-        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles), dim3(256), d, p);
+        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), d, p);
         // second hydraulic operator
         for (int q = 0; q < 4; ++q)
-            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk), dim3(NTHREADS), h->d2, p);
+            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk), dim3(STAGE_NT), h->d2, p);
     }
     LAUNCH(KW_FINALIZE, finalize_kernel, dim3(1), dim3(1), d, p);
-    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles), dim3(256), d, p);
+    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), d, p);
 #undef LAUNCH
     HIPCHK(h, hipGetLastError());
     return 0;
@@ -2464,7 +2491,7 @@ static int batch_launch_iteration(kst_batch* b) {
     hipLaunchKernelGGL(batch_activate_kernel, dim3(b->nslot), dim3(256), 0, b->stream, bt);
     hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(BATCH_PLAN_THREADS), 0, b->stream, bt);
     for (int q = 0; q < 4; ++q)
-        hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(NTHREADS), 0, b->stream, bt);
+        hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(STAGE_NT), 0, b->stream, bt);
     hipLaunchKernelGGL(batch_finalize_kernel, dim3((b->nslot + 63) / 64), dim3(64), 0, b->stream, bt);
     BCHK(b, hipGetLastError());
     return 0;
@@ -2522,7 +2549,7 @@ int kst_batch_create(int max_slots, kst_batch** out) {
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, batch_stage_kernel<2, true, KST_LIMITER_MINMOD2, false>, NTHREADS, 0) != hipSuccess)
+            &per_cu, batch_stage_kernel<2, true, KST_LIMITER_MINMOD2, false>, STAGE_NT, 0) != hipSuccess)
         return fail(KST_ERR_HIP);
     b->grid = std::max(8, (cus * std::max(per_cu, 1)) / 8 * 8);
     *out = b;
