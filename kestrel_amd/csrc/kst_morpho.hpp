@@ -506,7 +506,6 @@ __constant__ const int kDJ[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 __device__ void redistribute_small(const Dev& d, const Params& p, int n, double* lx, long long* lk,
                                    int* lev, int* cellid) {
     __shared__ double slot[RD_THREADS][4];     // vertex bt of each candidate's cell
-    __shared__ short nbr[RD_THREADS][8];       // sorted position of the candidate at neighbour k
     __shared__ unsigned char rec[RD_THREADS];  // its bt and w were recomputed by a correction
     __shared__ unsigned char res[RD_THREADS];  // 0 none, 1 corrected, 2 no depositional vertex
     __shared__ unsigned char dmask[RD_THREADS];
@@ -519,20 +518,31 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
     const int NX = p.ntx * nx, NY = p.nty * ny;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     if (tid == 0) { s_stop = n; s_done = 0; s_ghost = 0; }
-    if (tid < n) { lk[tid] = d.cand_key[tid]; lx[tid] = d.cand_ex[tid]; }
+    if (tid < n) { lk[tid] = d.cand_key[tid]; lx[tid] = d.cand_ex[tid]; lev[tid] = 0; }
     __syncthreads();
-    // ---- rank of each candidate in (excess, scan order)
-    if (tid < n) {
-        const double x = lx[tid];
-        const long long kk = lk[tid];
-        int r = 0;
-#pragma unroll 16
-        for (int u = 0; u < n; ++u) {
-            const double xu = lx[u];
-            const long long ku = lk[u];
-            r += (int)((xu < x) | ((xu == x) & (ku < kk)));
+    // ---- rank of each candidate in (excess, scan order): P threads per
+    // candidate count over interleaved parts of the list
+    {
+        int P = 1;
+        while (P < 16 && 2 * P * n <= RD_THREADS) P <<= 1;
+        const int q = tid % n, part = tid / n;
+        if (n > 0 && part < P) {
+            const double x = lx[q];
+            const long long kk = lk[q];
+            int r = 0;
+#pragma unroll 8
+            for (int u = part; u < n; u += P) {
+                const double xu = lx[u];
+                const long long ku = lk[u];
+                r += (int)((xu < x) | ((xu == x) & (ku < kk)));
+            }
+            atomicAdd(&lev[q], r);
         }
-        cellid[r] = d.active[(int)(kk / p.NC)] * p.NC + (int)(kk % p.NC);
+        __syncthreads();
+        if (tid < n) {
+            const long long kk = lk[tid];
+            cellid[lev[tid]] = d.active[(int)(kk / p.NC)] * p.NC + (int)(kk % p.NC);
+        }
     }
     __syncthreads();
     int k = 0, i = 0, j = 0, I = 0, J = 0;
@@ -549,6 +559,9 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
     // ---- neighbours within one cell, and the per-candidate state
     double x0[4], e[4];
     double gamold = 1.0, btc0 = 0.0, w0 = 0.0, h0 = 0.0, b0c = 0.0, btc3s = 0.0, w3s = 0.0;
+    int nb[8];                                 // this thread's neighbours, kept in registers
+#pragma unroll
+    for (int q = 0; q < 8; ++q) nb[q] = -1;
     if (tid < n) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -556,7 +569,7 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
             int r = -1;
             if (I2 >= 0 && J2 >= 0 && I2 < NX && J2 < NY)
                 r = d.cand_pos[((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx];
-            nbr[tid][q] = (short)r;
+            nb[q] = r;
         }
         const size_t gc = (size_t)k * p.NC + j * nx + i;
 #pragma unroll
@@ -582,7 +595,7 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
             bool ready = true;
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                const int r = nbr[tid][m];
+                const int r = nb[m];
                 if (r >= 0 && r < tid && (lev[r] == 0 || lev[r] == round)) ready = false;
             }
             if (!ready) {
@@ -614,7 +627,7 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
                         slot[tid][a + 2 * b] = nv[q];
 #pragma unroll
                         for (int m = 0; m < 8; ++m) {
-                            const int r = nbr[tid][m];
+                            const int r = nb[m];
                             if (r <= tid) continue;
                             rec[r] = 1;
                             const int a2 = a - kDI[m], b2 = b - kDJ[m];
@@ -641,7 +654,7 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
             bool last = true;
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                const int r = nbr[tid][m];
+                const int r = nb[m];
                 if (r <= tid || r >= stop || res[r] != 1) continue;
                 const int a2 = a - kDI[m], b2 = b - kDJ[m];
                 if (a2 >= 0 && a2 <= 1 && b2 >= 0 && b2 <= 1 && (dmask[r] >> (a2 + 2 * b2)) & 1) last = false;
