@@ -1,0 +1,362 @@
+"""Kestrel input decks (the reference's `%%ListType` files) to a run of this library.
+
+    from kestrel_amd import deck
+    run = deck.load("Input_cap_morpho_2d.txt")    # run.workload, run.tend, run.nout
+    s = Stepper(run.workload.cfg); s.set_topography(run.workload.b0v); s.set_state(run.workload.init)
+
+Follows the reference's reader and setters: line classes and blocks of
+parseListInputs (src/Input.f90:91-123,127-365); DomainSettings_Set
+(src/DomainSettings.f90:48-229); Params_Set with its defaults and default-reset
+rules (src/Parameters.f90:82-652); Solver_Set (src/SolverSettings.f90:59-264);
+the Function topographies this path supports (src/TopogSettings.f90:213-342,
+src/TopogFuncs.f90:395-577); Cap_Set (src/InitConds.f90:254-511) and Source_Set
+(46-208).  Labels of the Domain, Parameters, Solver, Output and Topog blocks
+compare in lower case; Source and Cap keys compare as written, as there.
+Settings the device path does not run (periodic, dirichlet or sponge
+boundaries, Cube initial conditions, DEM topographies, restarts) raise
+DeckUnsupported naming the setting.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import config as C
+from . import scenario as S
+from .workloads import Workload
+
+BLOCKS = ("Domain", "Source", "Cap", "Cube", "Parameters", "Solver", "Topog", "Output")
+
+
+class DeckError(ValueError):
+    """The deck is malformed (the reference stops with FatalErrorMessage)."""
+
+
+class DeckUnsupported(NotImplementedError):
+    """A valid deck setting that the device path does not run."""
+
+
+@dataclass
+class Deck:
+    domain: dict = field(default_factory=dict)
+    params: dict = field(default_factory=dict)
+    solver: dict = field(default_factory=dict)
+    output: dict = field(default_factory=dict)
+    topog: dict = field(default_factory=dict)
+    caps: list = field(default_factory=list)
+    sources: list = field(default_factory=list)
+    cubes: list = field(default_factory=list)
+    warnings: list = field(default_factory=list)
+
+
+@dataclass
+class Run:
+    workload: Workload
+    tend: float
+    nout: int
+    tstart: float
+    deck: Deck
+
+
+def _linetype(s: str) -> int:
+    """src/Input.f90:91-123: 0 blank, -1 comment, 1 block name, 2 key = value, 3 other."""
+    if not s:
+        return 0
+    if s[0] in "%#":
+        return -1
+    if ":" in s:
+        return 1
+    if "=" in s:
+        return 2
+    return 3
+
+
+def parse(text: str) -> Deck:
+    d = Deck()
+    current = None
+    for raw in text.splitlines():
+        line = raw.strip()
+        if "#" in line:                       # drop_comments
+            line = line[:line.index("#")].rstrip()
+        t = _linetype(line)
+        if t == 1:
+            name = line[:line.index(":")].strip()
+            if name in BLOCKS:
+                current = name
+                if name == "Source":
+                    d.sources.append({})
+                elif name == "Cap":
+                    d.caps.append({})
+                elif name == "Cube":
+                    d.cubes.append({})
+        elif t == 2:
+            k, v = line.split("=", 1)
+            k, v = k.strip(), v.strip()
+            if current == "Domain":
+                d.domain[k.lower()] = v
+            elif current == "Parameters":
+                d.params[k.lower()] = v
+            elif current == "Solver":
+                d.solver[k.lower()] = v
+            elif current == "Output":
+                d.output[k.lower()] = v
+            elif current == "Topog":
+                d.topog[k.lower()] = v
+            elif current == "Source":
+                d.sources[-1][k] = v
+            elif current == "Cap":
+                d.caps[-1][k] = v
+            elif current == "Cube":
+                d.cubes[-1][k] = v
+    return d
+
+
+def _real(v: str) -> float:
+    return float(v.replace("d", "e").replace("D", "e"))
+
+
+def _set(v: str) -> list:
+    """A '(a, b, ...)' value (varString read_set)."""
+    if "(" not in v or ")" not in v:
+        raise DeckError(f"varString {v} is not recognized as a set")
+    inner = v[v.index("(") + 1:v.rindex(")")]
+    return [_real(x) for x in inner.split(",") if x.strip()]
+
+
+# ---------------------------------------------------------------------------
+# blocks
+# ---------------------------------------------------------------------------
+
+def _domain(d: Deck):
+    dom = d.domain
+    for key in ("lat", "lon", "nxtiles", "nytiles", "nxpertile", "nypertile"):
+        alias = {"lat": ("lat", "latitude"), "lon": ("lon", "longitude")}.get(key, (key,))
+        if not any(a in dom for a in alias):
+            raise DeckError(f"The 'Domain' block does not contain the required '{key}' variable.")
+    ntx, nty = int(dom["nxtiles"]), int(dom["nytiles"])
+    nx, ny = int(dom["nxpertile"]), int(dom["nypertile"])
+    # the first of Xtilesize / Ytilesize given wins (src/DomainSettings.f90:114-127)
+    order = [k for k in dom if k in ("xtilesize", "ytilesize")]
+    if not order:
+        raise DeckError("The 'Domain' block contains neither Xtilesize nor Ytilesize.")
+    if order[0] == "xtilesize":
+        xt = _real(dom["xtilesize"])
+        yt = xt * float(ny) / float(nx)
+    else:
+        yt = _real(dom["ytilesize"])
+        xt = yt * float(nx) / float(ny)
+    bcs = dom.get("boundary conditions", "halt").lower()
+    if bcs != "halt":
+        raise DeckUnsupported(f"Boundary Conditions = {bcs}: the device path runs 'halt' only")
+    if nty * ny == 1:
+        raise DeckUnsupported("one-dimensional domains (nYtiles * nYpertile = 1)")
+    return ntx, nty, nx, ny, xt, yt
+
+
+_SWITCH = {"tanh": "tanh", "rat3": "rat3", "cos": "cos", "linear": "linear", "equal": "equal",
+           "0.5": "equal", "off": "zero", "0": "zero", "zero": "zero", "1": "one", "one": "one",
+           "step": "step"}
+_DRAG = {"chezy", "coulomb", "voellmy", "pouliquen", "edwards2019", "variable", "manning"}
+_EROSION = {"mixed": "mixed", "on": "mixed", "simple": "simple", "fluid": "fluid",
+            "granular": "granular", "off": "off"}
+_LIMITER = {"minmod1": "minmod1", "minmod2": "minmod2", "van albada": "vanalbada",
+            "albada": "vanalbada", "weno": "weno", "none": "none"}
+
+
+def _params(d: Deck, c: C.KstConfig):
+    """Params_Set (src/Parameters.f90:82-652)."""
+    P = d.params
+    settling = None
+    reals = {"g": "g", "chezy co": "chezy_co", "manning co": "manning_co", "coulomb co": "coulomb_co",
+             "pouliquen min": "pouliquen_min", "pouliquen max": "pouliquen_max",
+             "pouliquen intermediate": "pouliquen_intermediate", "pouliquen beta": "pouliquen_beta",
+             "edwards2019 betastar": "edwards_betastar", "edwards2019 kappa": "edwards_kappa",
+             "edwards2019 gamma": "edwards_gamma", "voellmy switch rate": "voellmy_switch_rate",
+             "voellmy switch value": "voellmy_switch_value", "erosion rate": "ero_rate",
+             "granular erosion rate": "ero_rate_granular", "erosion depth": "ero_depth",
+             "erosion critical height": "ero_critical_height", "bed porosity": "bed_porosity",
+             "rhow": "rhow", "rhos": "rhos", "maxpack": "max_pack", "max pack": "max_pack",
+             "solid diameter": "solid_diameter", "eddy viscosity": "eddy_viscosity"}
+    for k, v in P.items():
+        lv = v.strip().lower()
+        if k in reals:
+            setattr(c, reals[k], _real(v))
+        elif k == "switch function":
+            c.fswitch = C.SWITCHES[_SWITCH.get(lv, "tanh")]
+        elif k == "deposition":
+            c.deposition = C.DEPOSITIONS[lv if lv in ("none", "simple", "spearman manning") else "spearman manning"]
+        elif k == "morphodynamic damping":
+            c.morpho_damping = C.DAMPINGS[{"none": "none", "off": "none", "tanh": "tanh",
+                                           "rat3": "rat3"}.get(lv, "tanh")]
+        elif k in ("iverson", "geometric factors"):
+            if lv not in ("on", "off"):
+                raise DeckError(f"'{k}' must be 'on' or 'off'")
+            c.geometric_factors = 1 if lv == "on" else 0
+        elif k == "drag":
+            if lv not in _DRAG:
+                raise DeckError(f"The 'Drag' value {v} is not recognized.")
+            c.drag = C.DRAGS[lv]
+        elif k == "erosion":
+            if lv not in _EROSION:
+                raise DeckError(f"The 'Erosion' value {v} is not recognized.")
+            c.erosion = C.EROSIONS[_EROSION[lv]]
+            c.morphodynamics = 0 if lv == "off" else 1
+        elif k == "erosion transition":
+            c.ero_transition = C.TRANSITIONS[{"smooth": "smooth", "step": "step",
+                                              "off": "off"}.get(lv, "smooth")]
+        elif k == "settling speed":
+            settling = _real(v)
+        else:
+            d.warnings.append(f"Parameters: label '{k}' not recognized")
+    # an unset switch function also resets the damping to its default
+    # (src/Parameters.f90:486-490)
+    if "switch function" not in P:
+        c.fswitch = C.SWITCHES["tanh"]
+        c.morpho_damping = C.DAMPINGS["tanh"]
+    if "drag" not in P:
+        c.drag = C.DRAGS["chezy"]
+    if "erosion" not in P:
+        c.erosion = C.EROSIONS["mixed"]
+        c.morphodynamics = 1
+    return settling
+
+
+def _solver(d: Deck, c: C.KstConfig):
+    """Solver_Set (src/SolverSettings.f90:59-264)."""
+    Sv = d.solver
+    tend, tstart = None, 0.0
+    for k, v in Sv.items():
+        lv = v.strip().lower()
+        if k == "t end":
+            tend = _real(v)
+        elif k == "limiter":
+            c.limiter = C.LIMITERS[_LIMITER.get(lv, "minmod2")]
+        elif k == "height threshold":
+            c.height_threshold = _real(v)
+        elif k == "tile buffer":
+            c.tile_buffer = int(v)
+        elif k == "cfl":
+            c.cfl = _real(v)
+        elif k == "max dt":
+            c.max_dt = _real(v)
+        elif k == "t start":
+            tstart = _real(v)
+        elif k == "restart":
+            if lv == "on":
+                raise DeckUnsupported("Restart = on")
+        elif k in ("sponge strength", "initial condition"):
+            pass
+        else:
+            d.warnings.append(f"Solver: label '{k}' not recognized")
+    if tend is None:
+        raise DeckError("The 'Solver' block does not contain the required 'T end' variable.")
+    return tend, tstart
+
+
+def _topography(d: Deck, c: C.KstConfig) -> np.ndarray:
+    """Function topographies at the vertices (GetHeights with TopogFunc,
+    src/dem.f90:360-393; src/TopogFuncs.f90:395-577)."""
+    T = d.topog
+    kind = T.get("type", "").strip().lower()
+    if kind != "function":
+        raise DeckUnsupported(f"Topog Type = {T.get('type')}: only 'Function' topographies run here")
+    name = T.get("topog function", "").strip().lower()
+    prm = _set(T["topog params"]) if "topog params" in T else []
+    xv, yv = S.vertex_coords(c)
+    X, Y = np.meshgrid(xv, yv)
+    Lx, Ly = c.x_size, c.y_size
+    need = {"flat": 0, "xslope": 1, "yslope": 1, "xyslope": 2, "xsinslope": 1, "xysinslope": 1,
+            "xparab": 1}
+    if name not in need:
+        raise DeckUnsupported(f"Topog function = {name}")
+    if len(prm) < need[name]:
+        raise DeckError(f"{need[name]} 'Topog params' needed for '{name}'; received {len(prm)}")
+    if name == "flat":
+        b = np.zeros_like(X)
+    elif name == "xslope":
+        b = prm[0] * X
+    elif name == "yslope":
+        b = prm[0] * Y
+    elif name == "xyslope":
+        b = prm[0] * X + prm[1] * Y
+    elif name == "xsinslope":
+        b = prm[0] * np.sin(X * (2.0 * math.pi / Lx))
+    elif name == "xysinslope":
+        b = prm[0] * np.sin(X * (2.0 * math.pi / Lx)) * np.sin(Y * (2.0 * math.pi / Ly))
+    else:  # xparab
+        b = prm[0] * X * X
+    return np.ascontiguousarray(b, dtype=np.float64)
+
+
+def _caps(d: Deck, c: C.KstConfig) -> list:
+    """Cap_Set (src/InitConds.f90:254-425)."""
+    out = []
+    for k, cp in enumerate(d.caps):
+        if "capLat" in cp or "capLon" in cp:
+            raise DeckUnsupported("caps placed by latitude/longitude")
+        if "capX" not in cp or "capY" not in cp:
+            raise DeckError(f"Cap {k + 1}: no position (capX, capY)")
+        shape = {"parabolic": "para", "para": "para", "flat": "flat",
+                 "level": "level"}.get(cp.get("capShape", "para").strip().lower(), "para")
+        r = _real(cp["capRadius"]) if "capRadius" in cp else None
+        h = _real(cp["capHeight"]) if "capHeight" in cp else None
+        vol = _real(cp["capVolume"]) if "capVolume" in cp else None
+        if h is None and vol is not None and r is not None:
+            h = vol / math.pi / r / r if shape == "flat" else 2.0 * vol / math.pi / r / r
+        elif r is None and vol is not None and h is not None:
+            r = math.sqrt(vol / math.pi / h) if shape == "flat" else math.sqrt(2.0 * vol / math.pi / h)
+        if r is None or h is None:
+            raise DeckError(f"Cap {k + 1}: two of capRadius, capHeight, capVolume are needed")
+        out.append(S.Cap(_real(cp["capX"]), _real(cp["capY"]), r, h,
+                         psi=_real(cp.get("capConc", "0")), u=_real(cp.get("capU", "0")),
+                         v=_real(cp.get("capV", "0")), shape=shape))
+    return out
+
+
+def _sources(d: Deck) -> list:
+    """Source_Set (src/InitConds.f90:46-208)."""
+    out = []
+    for k, s in enumerate(d.sources):
+        if "sourceLat" in s or "sourceLon" in s:
+            raise DeckUnsupported("sources placed by latitude/longitude")
+        if "sourceX" not in s or "sourceY" not in s:
+            raise DeckError(f"Source {k + 1}: no position (sourceX and sourceY)")
+        if "sourceRadius" not in s:
+            raise DeckError(f"Source {k + 1}: sourceRadius is not given")
+        t = _set(s["sourceTime"]) if "sourceTime" in s else [0.0]
+        q = _set(s["sourceFlux"]) if "sourceFlux" in s else [0.0] * len(t)
+        p = _set(s["sourceConc"]) if "sourceConc" in s else [0.0] * len(t)
+        out.append(C.FluxSource(_real(s["sourceX"]), _real(s["sourceY"]), _real(s["sourceRadius"]),
+                                t, q, p))
+    return out
+
+
+def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64) -> Run:
+    if d.cubes:
+        raise DeckUnsupported("Cube initial conditions")
+    ntx, nty, nx, ny, xt, yt = _domain(d)
+    c = C.default_config(ntx, nty, nx, ny, dx=xt / float(nx), dy=yt / float(ny))
+    c.x_size = ntx * xt
+    c.y_size = nty * yt
+    settling = _params(d, c)
+    tend, tstart = _solver(d, c)
+    c.precision = precision
+    C.derive(c, settling_speed=settling)
+    b0v = _topography(d, c)
+    caps = _caps(d, c)
+    srcs = _sources(d)
+    if not caps and not srcs:
+        raise DeckError("An 'Initial Conditions' block (Cap or Source) is needed")
+    init = S.initial_state(c, b0v, caps, srcs)
+    if srcs:
+        C.set_sources(c, srcs)
+    nout = int(d.output.get("n out", "1"))
+    return Run(Workload(name, c, b0v, init, 0, caps, srcs), tend, nout, tstart, d)
+
+
+def load(path: str, precision: int = C.PREC_FP64) -> Run:
+    import os
+    with open(path) as f:
+        return build(parse(f.read()), os.path.splitext(os.path.basename(path))[0], precision)
