@@ -60,6 +60,7 @@ struct Ctrl {
     int commit;                // the step was accepted: the new bed becomes current
     long long work[16];        // launches that did work, per kernel (KW_*)
     int cur;                   // mixed precision: u^n is in S0 (0) or S0b (1)
+    double next_t;             // NextFluxSeriesTime of t, kept with t
 };
 
 // Kernel ids of Ctrl::work.  Block 0 of a launch that does work counts it, so
@@ -175,11 +176,12 @@ __device__ double advised_from_slot(const Params& p, unsigned long long slot, in
 // The step's (hydraulic) dt: substep-1 advice clamped to the next
 // output/flux-series time, or to half of it with morphodynamics
 // (src/TimeStepper.f90:155-168), or the refined dt of a retried step.
-__device__ double step_dt(const Dev& d, const Params& p) {
-    const Ctrl* c = d.ctrl;
+// c: the control block, or a copy of it a prologue loaded at once.
+__device__ double step_dt(const Dev& d, const Params& p, const Ctrl* c = nullptr) {
+    if (!c) c = d.ctrl;
     if (c->status == ST_RETRY) return c->dt_retry;
     double adv = 0.9 * advised_from_slot(p, c->slot[0], c->n_active);
-    double tmax = dmin(c->tend, next_flux_time(d, p, c->t));
+    double tmax = dmin(c->tend, c->next_t);
     if (!d.morpho) return dmin(adv, tmax - c->t);
     return dmin(adv, 0.5 * (tmax - c->t));
 }
@@ -404,11 +406,10 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     // --- fluxes (src/HydraulicRHS.f90:1027-1067, src/Equations.f90:53-245)
     double* F = YDIR ? &s.fy[0][0] : &s.fx[0][0];
     const int fslot = YDIR ? (fi * B + fj) : (fj * (B + 1) + fi);
+    // zero flux where the wave speeds close (src/HydraulicRHS.f90:1028-1031);
+    // computed without a branch so the two faces of a thread interleave
     const double dif = aPos - aNeg;
-    if (dif < 1e-10) {
-        for (int q = 0; q < (VISC ? 7 : 5); ++q) F[q * NFACE + fslot] = 0.0;
-        return;
-    }
+    const bool none = dif < 1e-10;
     const double cPw = HnR * unR * fg, cMw = HnL * unL * fg;
     const double cPh = hFR * unR * fg, cMh = hFL * unL * fg;
     const double cPu = ruR * unR, cMu = ruL * unL;
@@ -423,28 +424,27 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
     double x = HnR * fg - HnL * fg;
     x = x * aPos * aNeg;
     x = x + (aPos * cMw - aNeg * cPw);
-    F[0 * NFACE + fslot] = x / dif;
+    F[0 * NFACE + fslot] = none ? 0.0 : x / dif;
     x = ruR - ruL;
     x = x * aPos * aNeg;
     x = x + (aPos * cMu - aNeg * cPu);
-    F[1 * NFACE + fslot] = x / dif;
+    F[1 * NFACE + fslot] = none ? 0.0 : x / dif;
     x = rvR - rvL;
     x = x * aPos * aNeg;
     x = x + (aPos * cMv - aNeg * cPv);
-    F[2 * NFACE + fslot] = x / dif;
+    F[2 * NFACE + fslot] = none ? 0.0 : x / dif;
     x = hFR * fg - hFL * fg;
     x = x * aPos * aNeg;
     x = x + (aPos * cMh - aNeg * cPh);
-    F[3 * NFACE + fslot] = x / dif;
-    F[4 * NFACE + fslot] = (aPos * hydM - aNeg * hydP) / dif;
+    F[3 * NFACE + fslot] = none ? 0.0 : x / dif;
+    F[4 * NFACE + fslot] = none ? 0.0 : (aPos * hydM - aNeg * hydP) / dif;
     if (VISC) {
-        double pPu, pPv, pMu, pMv;
-        if (HnR < 0.0) { pPu = 0.0; pPv = 0.0; }
-        else { pPu = p.nu * rFR * HnR * sRu; pPv = p.nu * rFR * HnR * sRv; }
-        if (HnL < 0.0) { pMu = 0.0; pMv = 0.0; }
-        else { pMu = p.nu * rFL * HnL * sLu; pMv = p.nu * rFL * HnL * sLv; }
-        F[5 * NFACE + fslot] = 0.5 * (pPu + pMu);
-        F[6 * NFACE + fslot] = 0.5 * (pPv + pMv);
+        const double pPu = (HnR < 0.0) ? 0.0 : p.nu * rFR * HnR * sRu;
+        const double pPv = (HnR < 0.0) ? 0.0 : p.nu * rFR * HnR * sRv;
+        const double pMu = (HnL < 0.0) ? 0.0 : p.nu * rFL * HnL * sLu;
+        const double pMv = (HnL < 0.0) ? 0.0 : p.nu * rFL * HnL * sLv;
+        F[5 * NFACE + fslot] = none ? 0.0 : 0.5 * (pPu + pMu);
+        F[6 * NFACE + fslot] = none ? 0.0 : 0.5 * (pPv + pMv);
     }
 }
 
@@ -670,14 +670,16 @@ template <int STAGE, bool VISC, int LIM, bool MIX, typename Smem>
 __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& s, int witem) {
     const int tid = threadIdx.x;
     if (tid == 0) {
-        const Ctrl* c = d.ctrl;
+        // every control field this prologue reads, loaded together
+        const Ctrl cl = *d.ctrl;
+        const Ctrl* c = &cl;
         int skip = 0;
         if (c->status == ST_DONE || c->status == ST_ERROR) skip = 1;
         if (STAGE == 1 && !d.second && c->status != ST_NEW) skip = 1;
         const int sb = d.second ? 3 : 0;
         double dt = 0.0;
         if (!skip && (STAGE >= 2 || d.second)) {
-            dt = step_dt(d, p);
+            dt = step_dt(d, p, c);
             if (d.second && (!first_operator_ok(c, p, dt) || c->mref)) skip = 1;
             if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3], c->n_active) < dt) skip = 1;
             if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1], c->n_active) < dt) skip = 1;
@@ -698,7 +700,7 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
             const int b = blockIdx.x;
             w = (b / 8 >= per) ? nwork : (b % 8) * per + b / 8;
         }
-        if (!skip && w == 0) const_cast<Ctrl*>(c)->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)] += 1;
+        if (!skip && w == 0) d.ctrl->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)] += 1;
         if (w >= nwork) skip = 1;
         if (!skip) {
             s.tile = d.active[w / spt];
@@ -1564,6 +1566,7 @@ __device__ void finalize_body(const Dev& d, const Params& p) {
         return;
     }
     c->t = d.morpho ? c->t + 2.0 * dt : c->t + dt;
+    c->next_t = next_flux_time(d, p, c->t);
     c->steps++;
     c->cell_updates += (long long)c->n_active * p.NC;
     c->last_dt = dt;
@@ -1703,6 +1706,7 @@ struct kst_handle {
     std::string err;
     int nx, ny, ntx, nty, ntiles, NX, NY, NC;
     bool have_topo = false, have_state = false;
+    std::vector<std::vector<double>> src_times;   // flux-source knot times (host copy)
     double t = 0.0;
     // profiling: per-iteration event pairs around the four stage kernels
     bool profiling = false;
@@ -2112,6 +2116,7 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
             x[i] = s.x; y[i] = s.y; r[i] = s.radius; n[i] = s.n_series; cells[i] = s.num_cells;
             off[i] = (int)tt.size();
             for (int k = 0; k < s.n_series; ++k) { tt.push_back(s.time[k]); q.push_back(s.flux[k]); ps.push_back(s.psi[k]); }
+            h->src_times.emplace_back(s.time, s.time + s.n_series);
         }
         (void)hipMemcpy(d.src_x, x.data(), 8 * ns, hipMemcpyHostToDevice);
         (void)hipMemcpy(d.src_y, y.data(), 8 * ns, hipMemcpyHostToDevice);
@@ -2155,6 +2160,17 @@ int kst_set_topography(kst_handle* h, const double* b0v) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->have_topo = true;
     return KST_OK;
+}
+
+// NextFluxSeriesTime (src/TimeStepper.f90:281-305) on the host copy of the knots.
+static double host_next_flux_time(const kst_handle* h, double t) {
+    double nextT = kHuge, tdiff = kHuge;
+    for (const auto& knots : h->src_times)
+        for (double tt : knots) {
+            const double tmp = tt - t;
+            if (tmp > 0.0 && tmp < tdiff) { tdiff = tmp; nextT = tt; }
+        }
+    return nextT;
 }
 
 int kst_set_state(kst_handle* h, const double* w, const double* rhu, const double* rhv,
@@ -2274,6 +2290,7 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     std::memset(&c, 0, sizeof c);
     c.t = t;
     c.tend = t;
+    c.next_t = host_next_flux_time(h, t);
     const unsigned long long huge = 0x7FEFFFFFFFFFFFFFull;
     for (int k = 0; k < 6; ++k) c.slot[k] = huge;
     c.status = ST_DONE;
