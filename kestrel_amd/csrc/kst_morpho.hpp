@@ -27,12 +27,11 @@ struct MWork {
 };
 
 __device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_no_refine, int kw) {
-    const Ctrl cl = *d.ctrl;   // loaded together
-    const Ctrl* c = &cl;
+    const Ctrl* c = d.ctrl;
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
     if (!skip) {
-        dt = step_dt(d, p, c);
+        dt = step_dt(d, p);
         if (!first_operator_ok(c, p, dt)) skip = 1;
         if (need_no_refine && c->mref) skip = 1;
     }
@@ -737,16 +736,15 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
     Ctrl* c = d.ctrl;
     const int tid = threadIdx.x;
     if (tid == 0) {
-        const Ctrl cl = *c;   // loaded together
-        int skip = (cl.status != ST_NEW && cl.status != ST_RETRY);
+        int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
-            const double dt = step_dt(d, p, &cl);
-            if (!first_operator_ok(&cl, p, dt) || cl.mref) skip = 1;
+            const double dt = step_dt(d, p);
+            if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
-        if (!skip && cl.ncand <= 0) skip = 1;
+        if (!skip && c->ncand <= 0) skip = 1;
         if (!skip) count_work(c, KW_REDIST);
         sh_skip = skip;
-        sh_n = cl.ncand;
+        sh_n = c->ncand;
         sh_fail = 0;
         sh_done = 0;
         sh_ghost = 0;
@@ -877,14 +875,13 @@ __global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
     const Ctrl* c = d.ctrl;
     const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
     if (threadIdx.x == 0) {
-        const Ctrl cl = *c;   // loaded together
-        int skip = (cl.status != ST_NEW && cl.status != ST_RETRY);
+        int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
-            const double dt = step_dt(d, p, &cl);
-            if (!first_operator_ok(&cl, p, dt) || cl.mref) skip = 1;
+            const double dt = step_dt(d, p);
+            if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
         if (!skip) count_work(c, KW_FACETOPO);
-        if (q >= cl.n_active) skip = 1;
+        if (q >= c->n_active) skip = 1;
         sh_skip = skip;
     }
     __syncthreads();

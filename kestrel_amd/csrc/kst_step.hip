@@ -40,7 +40,6 @@ constexpr int NTHREADS = 256;
 // them and all would share the faces; measured slower on MI355X (register
 // pressure halves the resident waves), so the halo takes a second round.
 constexpr int STAGE_NT = 256;
-constexpr int NFACE = (B + 1) * B;
 
 enum Status { ST_NEW = 0, ST_RETRY = 1, ST_DONE = 2, ST_ERROR = 3 };
 
@@ -262,22 +261,27 @@ __device__ __forceinline__ T lim(T a, T b) {
 
 // Working set of one sub-block.  With MIX the per-cell Hnpsi, u, v and the
 // fluxes other than the hydrostatic one (q = 4, in fx4/fy4) are float.
-template <int NQ, bool MIX = false>
+// BY: the sub-block's height (B or B/2); the region is RW x (BY + 4) cells.
+template <int NQ, bool MIX = false, int BY = B>
 struct SmemT {
     using FT = typename std::conditional<MIX, float, double>::type;
-    double w[NREG];
-    FT h[NREG], u[NREG], v[NREG];
-    double rho[NREG];
-    double wst[NREG];        // w behind the ghost tiles' stored face values (their tileContainer w)
-    double xbed[B][B + 3];   // CorrectSlopes bed of x-faces -1..B+1 of each row (relative)
-    double ybed[B][B + 3];   // CorrectSlopes bed of y-faces -1..B+1 of each column
-    double xside[2][B];      // the W / E tile's own bed at this tile's edge x-faces
+    static constexpr int by = BY;
+    static constexpr int NR = RW * (BY + 4);       // region cells
+    static constexpr int NFX = (B + 1) * BY;       // x-faces
+    static constexpr int NFY = B * (BY + 1);       // y-faces
+    double w[NR];
+    FT h[NR], u[NR], v[NR];
+    double rho[NR];
+    double wst[NR];          // w behind the ghost tiles' stored face values (their tileContainer w)
+    double xbed[BY][B + 3];  // CorrectSlopes bed of x-faces -1..B+1 of each row (relative)
+    double ybed[B][BY + 3];  // CorrectSlopes bed of y-faces -1..BY+1 of each column
+    double xside[2][BY];     // the W / E tile's own bed at this tile's edge x-faces
     double yside[2][B];      // the S / N tile's own bed at this tile's edge y-faces
-    int gidx[NREG];
-    uint8_t act[NREG];
-    FT fx[NQ][NFACE];        // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
-    FT fy[NQ][NFACE];
-    double fx4[MIX ? NFACE : 1], fy4[MIX ? NFACE : 1];
+    int gidx[NR];
+    uint8_t act[NR];
+    FT fx[NQ][NFX];          // hw, hru, hrv, hh, g (, pu, pv with eddy viscosity)
+    FT fy[NQ][NFY];
+    double fx4[MIX ? NFX : 1], fy4[MIX ? NFY : 1];
     double red[STAGE_NT / 64];
     int tile, bx0, by0, bxa, bya, skip, cur;
     double dt, tstage, tgrid;
@@ -405,6 +409,7 @@ __device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s
 
     // --- fluxes (src/HydraulicRHS.f90:1027-1067, src/Equations.f90:53-245)
     double* F = YDIR ? &s.fy[0][0] : &s.fx[0][0];
+    constexpr int NFACE = YDIR ? Smem::NFY : Smem::NFX;
     const int fslot = YDIR ? (fi * B + fj) : (fj * (B + 1) + fi);
     // zero flux where the wave speeds close (src/HydraulicRHS.f90:1028-1031);
     // computed without a branch so the two faces of a thread interleave
@@ -554,6 +559,7 @@ __device__ __forceinline__ void face_flux_mixed(const Dev& d, const Params& p, S
     }
 
     float* F = YDIR ? &s.fy[0][0] : &s.fx[0][0];
+    constexpr int NFACE = YDIR ? Smem::NFY : Smem::NFX;
     double* F4 = YDIR ? s.fy4 : s.fx4;
     const int fslot = YDIR ? (fi * B + fj) : (fj * (B + 1) + fi);
     const float dif = aPos - aNeg;
@@ -668,18 +674,17 @@ __device__ __forceinline__ double fluxv(const Smem& s, int q, int i) {
 // block index with the XCD-chunked map of a one-scenario launch.
 template <int STAGE, bool VISC, int LIM, bool MIX, typename Smem>
 __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& s, int witem) {
+    constexpr int BY = Smem::by;       // sub-block height
     const int tid = threadIdx.x;
     if (tid == 0) {
-        // every control field this prologue reads, loaded together
-        const Ctrl cl = *d.ctrl;
-        const Ctrl* c = &cl;
+        const Ctrl* c = d.ctrl;
         int skip = 0;
         if (c->status == ST_DONE || c->status == ST_ERROR) skip = 1;
         if (STAGE == 1 && !d.second && c->status != ST_NEW) skip = 1;
         const int sb = d.second ? 3 : 0;
         double dt = 0.0;
         if (!skip && (STAGE >= 2 || d.second)) {
-            dt = step_dt(d, p, c);
+            dt = step_dt(d, p);
             if (d.second && (!first_operator_ok(c, p, dt) || c->mref)) skip = 1;
             if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3], c->n_active) < dt) skip = 1;
             if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1], c->n_active) < dt) skip = 1;
@@ -692,7 +697,7 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
         s.tstage = (STAGE == 1) ? tb : (STAGE == 3 ? tb + 0.5 * dt : tb + dt);
         // work item -> (tile, sub-block); contiguous chunks of the active
         // list go to the same XCD (blocks are dealt round-robin to 8 XCDs)
-        const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
+        const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
         const int nwork = c->n_active * spt;
         int w = witem;
         if (witem < 0) {
@@ -706,9 +711,9 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
             s.tile = d.active[w / spt];
             const int sub = w % spt;
             s.bx0 = (sub % sbx) * B;
-            s.by0 = (sub / sbx) * B;
+            s.by0 = (sub / sbx) * BY;
             s.bxa = min(B, p.nx - s.bx0);
-            s.bya = min(B, p.ny - s.by0);
+            s.bya = min(BY, p.ny - s.by0);
         }
         s.skip = skip;
     }
@@ -727,15 +732,17 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
     double pw = 0, pru = 0, prv = 0, ph = 0;   // u^n of the owned cell (stage 2; mixed: 2..4)
     int oidx = 0;
     const double* Un = (MIX && s.cur) ? d.S0b : d.S0;   // u^n of the step
-    // threads 0..255: the owned cells; the 2-cell halo strips (4 x 32 cells)
-    // by threads 256..383 in the same round, or by 128..255 in a second one
-    // (waves 0 and 1 take the extra faces of phase B instead)
-    constexpr bool one_round = STAGE_NT >= 384;
+    // threads 0..OWN-1: the owned cells; the 2-cell halo strips (4 strips of
+    // 2 x 16 slots) by the next 128 threads in the same round when there are
+    // enough, else by threads 128..255 in a second round (waves 0 and 1 take
+    // the extra faces of phase B instead)
+    constexpr int OWN = B * BY;
+    constexpr bool one_round = STAGE_NT >= OWN + 128;
     for (int round = 0; round < (one_round ? 1 : 2); ++round) {
         int rx, ry;
         bool valid;
-        const int hq = one_round ? tid - 256 : tid - 128;
-        if (round == 0 && tid < 256) {
+        const int hq = one_round ? tid - OWN : tid - 128;
+        if (round == 0 && tid < OWN) {
             rx = tx; ry = ty; valid = own;
         } else {
             if (hq < 0 || hq >= 128) break;
@@ -858,7 +865,7 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
     // ---- phase B: face fluxes --------------------------------------------
     double cflm = kHuge;
     // x-faces: consecutive lanes walk along a row; y-faces: along a row too
-    if (bxa == B && bya == B && STAGE_NT == 256) {
+    if (bxa == B && bya == BY && BY == B && STAGE_NT == 256) {
         // 272 x-faces and 272 y-faces: one of each per thread, the 16 extra
         // x-faces on wave 0 and the 16 extra y-faces on wave 1
         face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, tid % (B + 1), tid / (B + 1), cflm);
@@ -870,11 +877,16 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
             const int f = 256 + tid - 64;
             face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
         }
-    } else if (bxa == B && bya == B) {
-        for (int f = tid; f < (B + 1) * B; f += STAGE_NT)
-            face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (B + 1), f / (B + 1), cflm);
-        for (int f = tid; f < (B + 1) * B; f += STAGE_NT)
-            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
+    } else if (bxa == B && bya == BY) {
+        // x- and y-faces as one list over the block
+        for (int f = tid; f < Smem::NFX + Smem::NFY; f += STAGE_NT) {
+            if (f < Smem::NFX) {
+                face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (B + 1), f / (B + 1), cflm);
+            } else {
+                const int g = f - Smem::NFX;
+                face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, g / B, g % B, cflm);
+            }
+        }
     } else {
         for (int f = tid; f < (bxa + 1) * bya; f += STAGE_NT)
             face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (bxa + 1), f / (bxa + 1), cflm);
@@ -1058,9 +1070,9 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
     }
 }
 
-template <int STAGE, bool VISC, int LIM, bool MIX = false>
+template <int STAGE, bool VISC, int LIM, bool MIX = false, int BY = B>
 __global__ __launch_bounds__(STAGE_NT) void stage_kernel(Dev d, Params p) {
-    __shared__ SmemT<VISC ? 7 : 5, MIX> s;
+    __shared__ SmemT<VISC ? 7 : 5, MIX, BY> s;
     stage_body<STAGE, VISC, LIM, MIX>(d, p, s, -1);
 }
 
@@ -1710,7 +1722,7 @@ struct kst_handle {
     double t = 0.0;
     // profiling: per-iteration event pairs around the four stage kernels
     bool profiling = false;
-    hipGraphExec_t gexec = nullptr;      // graph of kGraphIters whole iterations
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // kGraphIters whole iterations, per sub-block shape
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[KW_N] = {};
     long long prof_n[KW_N] = {};
@@ -1846,27 +1858,34 @@ void from_tiles(const kst_handle* h, const double* t, double* g) {
 typedef void (*StageFn)(Dev, Params);
 struct StageSet { StageFn fn[4]; };
 
-template <bool VISC, int LIM, bool MIX>
+template <bool VISC, int LIM, bool MIX, int BY>
 StageSet make_set() {
-    return StageSet{{stage_kernel<1, VISC, LIM, MIX>, stage_kernel<2, VISC, LIM, MIX>,
-                     stage_kernel<3, VISC, LIM, MIX>, stage_kernel<4, VISC, LIM, MIX>}};
+    return StageSet{{stage_kernel<1, VISC, LIM, MIX, BY>, stage_kernel<2, VISC, LIM, MIX, BY>,
+                     stage_kernel<3, VISC, LIM, MIX, BY>, stage_kernel<4, VISC, LIM, MIX, BY>}};
 }
 
-template <bool MIX>
+template <bool MIX, int BY>
 const StageSet& stage_set_of(bool visc, int limiter) {
     static const StageSet sets[2][5] = {
-        {make_set<false, 0, MIX>(), make_set<false, 1, MIX>(), make_set<false, 2, MIX>(),
-         make_set<false, 3, MIX>(), make_set<false, 4, MIX>()},
-        {make_set<true, 0, MIX>(), make_set<true, 1, MIX>(), make_set<true, 2, MIX>(),
-         make_set<true, 3, MIX>(), make_set<true, 4, MIX>()}};
+        {make_set<false, 0, MIX, BY>(), make_set<false, 1, MIX, BY>(), make_set<false, 2, MIX, BY>(),
+         make_set<false, 3, MIX, BY>(), make_set<false, 4, MIX, BY>()},
+        {make_set<true, 0, MIX, BY>(), make_set<true, 1, MIX, BY>(), make_set<true, 2, MIX, BY>(),
+         make_set<true, 3, MIX, BY>(), make_set<true, 4, MIX, BY>()}};
     return sets[visc ? 1 : 0][(limiter >= 0 && limiter < 5) ? limiter : 1];
 }
 
-const StageSet& stage_set(bool visc, int limiter, bool mixed) {
-    return mixed ? stage_set_of<true>(visc, limiter) : stage_set_of<false>(visc, limiter);
+// Sub-blocks of 16 x 8 cells when the active set is small: twice the
+// workgroups, each with half the faces and its halo loaded in one round, so
+// the latency of a stage that cannot fill the chip roughly halves.  FP64 only.
+constexpr int BY_SMALL = B / 2;
+constexpr int SMALL_WORK = 128;       // sub-blocks of 16 x 16 below which the small shape is used
+
+const StageSet& stage_set(bool visc, int limiter, bool mixed, bool small) {
+    if (mixed) return stage_set_of<true, B>(visc, limiter);
+    return small ? stage_set_of<false, BY_SMALL>(visc, limiter) : stage_set_of<false, B>(visc, limiter);
 }
 
-int launch_iteration(kst_handle* h, bool profiled) {
+int launch_iteration(kst_handle* h, bool profiled, bool small) {
     const Params& p = h->p;
     const Dev& d = h->d;
     // while profiling, the dispatch of each launch carries the event pair of
@@ -1884,10 +1903,13 @@ int launch_iteration(kst_handle* h, bool profiled) {
     // 8 * ceil(work/8) blocks so the XCD-chunked mapping is a bijection
     const int nblk = 8 * ((p.ntiles * spt + 7) / 8);
     const bool visc = p.nu != 0.0;
+    const bool sm = small && !d.mixed;
+    const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
+    const int nblk_stage = sm ? 8 * ((p.ntiles * spt_s + 7) / 8) : nblk;
     LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), d, p);
-    const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0);
+    const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0, sm);
     for (int q = 0; q < 4; ++q)
-        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk), dim3(STAGE_NT), d, p);
+        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), d, p);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
         LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(STAGE_NT), d, p);
@@ -1897,7 +1919,7 @@ int launch_iteration(kst_handle* h, bool profiled) {
         LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), d, p);
         // second hydraulic operator
         for (int q = 0; q < 4; ++q)
-            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk), dim3(STAGE_NT), h->d2, p);
+            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), h->d2, p);
     }
     LAUNCH(KW_FINALIZE, finalize_kernel, dim3(1), dim3(1), d, p);
     if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), d, p);
@@ -1910,12 +1932,13 @@ constexpr int kGraphIters = 8;
 
 // One hipGraph holding kGraphIters whole iterations (activation, four stage
 // kernels, finalize); kernel arguments never change for a handle.
-int ensure_graph(kst_handle* h) {
-    if (h->gexec) return 0;
+int ensure_graph(kst_handle* h, bool small) {
+    hipGraphExec_t& gx = h->gexec[small ? 1 : 0];
+    if (gx) return 0;
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int q = 0; q < kGraphIters; ++q) {
-        int rc = launch_iteration(h, false);
+        int rc = launch_iteration(h, false, small);
         if (rc) {
             hipGraph_t dummy = nullptr;
             (void)hipStreamEndCapture(h->stream, &dummy);
@@ -1924,7 +1947,7 @@ int ensure_graph(kst_handle* h) {
         }
     }
     HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-    HIPCHK(h, hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    HIPCHK(h, hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
     HIPCHK(h, hipGraphDestroy(g));
     return 0;
 }
@@ -1948,13 +1971,17 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
         launched += batch;
         long long work_before[KW_N];
         std::memcpy(work_before, c.work, sizeof work_before);
+        // the sub-block shape for this batch, from the last known active set
+        const int spt = ((h->p.nx + B - 1) / B) * ((h->p.ny + B - 1) / B);
+        const bool small = (long long)c.n_active * spt < SMALL_WORK;
         if (h->profiling) {
-            rc = launch_iteration(h, true);
+            rc = launch_iteration(h, true, small);
             if (rc) return rc;
         } else {
-            rc = ensure_graph(h);
+            rc = ensure_graph(h, small);
             if (rc) return rc;
-            for (int q = 0; q < batch; q += kGraphIters) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+            for (int q = 0; q < batch; q += kGraphIters)
+                HIPCHK(h, hipGraphLaunch(h->gexec[small ? 1 : 0], h->stream));
         }
         rc = read_ctrl(h, &c);
         if (rc) return rc;
@@ -2706,7 +2733,8 @@ void kst_batch_destroy(kst_batch* b) {
 
 void kst_destroy(kst_handle* h) {
     if (!h) return;
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    for (hipGraphExec_t g : h->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
