@@ -26,7 +26,8 @@ struct MWork {
     double dt;
 };
 
-__device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_no_refine, int kw) {
+template <class DevT>
+__device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw) {
     const Ctrl* c = d.ctrl;
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
@@ -58,7 +59,8 @@ __device__ void morpho_work(const Dev& d, const Params& p, MWork& m, bool need_n
 // ExcessDeposition (src/Redistribute.f90:158-198) of cell gi, between
 // intermed0 = (SH, TA) and intermed3 = (SM[2], TB); also returns Hn_old,
 // deltaBt and psiold.
-__device__ __forceinline__ double excess_deposition(const Dev& d, const Params& p, size_t gi,
+template <class DevT>
+__device__ __forceinline__ double excess_deposition(const DevT& d, const Params& p, size_t gi,
                                                     double btc3, double& ho, double& dbt,
                                                     double& psio) {
     const long long FS = d.FS;
@@ -86,7 +88,9 @@ __device__ __forceinline__ double excess_deposition(const Dev& d, const Params& 
 //   w, Hnpsi from the change of bt (src/TimeStepper.f90:587-610)
 // Stage 3 then runs the checks of src/TimeStepper.f90:713-752.
 template <int S>
-__global__ __launch_bounds__(STAGE_NT) void morpho_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(STAGE_NT) void morpho_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    const auto& d = const_arg(dp);
+    const Params& p = *pp;
     __shared__ double hn[NREG];            // Hn of intermed_{S-1}, 2-cell halo
     __shared__ double emd[MR1 * MR1];      // E - D, 1-cell halo
     __shared__ double sbx[MR1 * MR1], sby[MR1 * MR1];
@@ -101,8 +105,8 @@ __global__ __launch_bounds__(STAGE_NT) void morpho_kernel(Dev d, Params p) {
     const double dtm = 2.0 * m.dt;
     const double* Win = (S == 1) ? d.SH : d.SM[S - 2];
     const double* Hin = (S == 1) ? d.SH + 3 * FS : d.SM[S - 2] + FS;
-    const Topo& Tin = (S == 1) ? d.TA : d.TM[S - 2];
-    const Topo& Tout = (S == 3) ? d.TB : d.TM[S - 1];
+    const auto& Tin = (S == 1) ? d.TA : d.TM[S - 2];
+    const auto& Tout = (S == 3) ? d.TB : d.TM[S - 1];
     double* Wout = d.SM[S - 1];
     double* Hout = d.SM[S - 1] + FS;
     const double eps = p.hthr;
@@ -262,7 +266,8 @@ __global__ __launch_bounds__(STAGE_NT) void morpho_kernel(Dev d, Params p) {
 }
 
 // A write into a ghost tile's part of intermed3, listed for the next step's refresh.
-__device__ __forceinline__ void note_dirty(const Dev& d, long long e) {
+template <class DevT>
+__device__ __forceinline__ void note_dirty(const DevT& d, long long e) {
     Ctrl* c = d.ctrl;
     const unsigned long long q = atomicAdd((unsigned long long*)&c->ndirty, 1ULL);
     if (q < (unsigned long long)d.dirty_cap) d.dirty[q] = e;
@@ -310,7 +315,8 @@ __device__ __forceinline__ CellRe recompute_cell(const Params& p, double t00, do
 // the tiles that share it (src/Redistribute.f90:330-398).  Copies in ghost
 // tiles are listed for the next step's refresh and counted.
 // With write = false the copies are only counted.
-__device__ __forceinline__ void put_vertex(const Dev& d, const Params& p, int k, int di, int dj,
+template <class DevT>
+__device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k, int di, int dj,
                                            double val, unsigned long long* ghost, bool write = true) {
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
@@ -426,7 +432,8 @@ __device__ __forceinline__ int vertex_correction(const Params& p, int i, int j, 
 // intermed3 would hold it.  Returns 0 if no correction was needed, 1 if one
 // was made, -1 if the cell borders no depositional vertex (nothing written).
 // *ghost counts writes into ghost tiles.
-__device__ int redistribute_cell(const Dev& d, const Params& p, int k, int i, int j, int lane,
+template <class DevT>
+__device__ int redistribute_cell(const DevT& d, const Params& p, int k, int i, int j, int lane,
                                  unsigned long long* ghost) {
     const long long FS = d.FS;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
@@ -503,7 +510,8 @@ constexpr int RD_LDS = 2048;          // candidates sorted and levelled in LDS
 __constant__ const int kDI[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ const int kDJ[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 
-__device__ void redistribute_small(const Dev& d, const Params& p, int n, double* lx, long long* lk,
+template <class DevT>
+__device__ void redistribute_small(const DevT& d, const Params& p, int n, double* lx, long long* lk,
                                    int* lev, int* cellid) {
     __shared__ double slot[RD_THREADS][4];     // vertex bt of each candidate's cell
     __shared__ unsigned char rec[RD_THREADS];  // its bt and w were recomputed by a correction
@@ -726,7 +734,9 @@ __device__ void redistribute_small(const Dev& d, const Params& p, int n, double*
 // and corrects level by level with one wavefront per candidate, two
 // corrections interacting within two cells (the 3x3 update reads the 4x4
 // vertices); there every level is processed even after a failing candidate.
-__global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    const auto& d = const_arg(dp);
+    const Params& p = *pp;
     __shared__ int sh_n, sh_skip, sh_fail, sh_maxlev, sh_cnt, sh_changed;
     __shared__ unsigned long long sh_done, sh_ghost;
     __shared__ double lx[RD_LDS];
@@ -870,7 +880,9 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(Dev d, Params 
 // plus its CFL numerators and prefactors; TILE_PARTS blocks per active tile.
 constexpr int TILE_PARTS = 4;
 
-__global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    const auto& d = const_arg(dp);
+    const Params& p = *pp;
     __shared__ int sh_skip;
     const Ctrl* c = d.ctrl;
     const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
@@ -896,7 +908,9 @@ __global__ __launch_bounds__(256) void face_topo_kernel(Dev d, Params p) {
 // The accepted step's bed becomes the current one (CopySolutionData
 // intermed3 -> tileContainer, src/TimeStepper.f90:263); TILE_PARTS blocks per
 // active tile.
-__global__ __launch_bounds__(256) void commit_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(256) void commit_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    const auto& d = const_arg(dp);
+    const Params& p = *pp;
     const Ctrl* c = d.ctrl;
     const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
     if (!c->commit || q >= c->n_active) return;

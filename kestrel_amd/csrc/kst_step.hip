@@ -148,12 +148,27 @@ struct Dev {
     int dirty_cap;
 };
 
+// Kernel arguments (Dev, Params) that live in device memory.  Passed by value, every field
+// a kernel reads is loaded into scalar registers at entry and held to the
+// end, and the ~100 that do not fit are spilled; read through a pointer they
+// are loaded where they are used.  The pointer is taken as one into the
+// constant address space, so that the array pointers read from it are known
+// to address global memory (global rather than flat loads and stores).
+template <class T>
+using ConstArg = const __attribute__((address_space(4))) T;
+
+template <class T>
+__device__ __forceinline__ ConstArg<T>& const_arg(const T* p) {
+    return *(ConstArg<T>*)p;
+}
+
 // ---------------------------------------------------------------------------
 // time-step control shared by kernels
 // ---------------------------------------------------------------------------
 
 // NextFluxSeriesTime (src/TimeStepper.f90:281-305).
-__device__ double next_flux_time(const Dev& d, const Params& p, double t) {
+template <class DevT>
+__device__ double next_flux_time(const DevT& d, const Params& p, double t) {
     double nextT = kHuge, tdiff = kHuge;
     for (int i = 0; i < p.n_sources; ++i)
         for (int j = 0; j < d.src_n[i]; ++j) {
@@ -176,7 +191,8 @@ __device__ double advised_from_slot(const Params& p, unsigned long long slot, in
 // output/flux-series time, or to half of it with morphodynamics
 // (src/TimeStepper.f90:155-168), or the refined dt of a retried step.
 // c: the control block, or a copy of it a prologue loaded at once.
-__device__ double step_dt(const Dev& d, const Params& p, const Ctrl* c = nullptr) {
+template <class DevT>
+__device__ double step_dt(const DevT& d, const Params& p, const Ctrl* c = nullptr) {
     if (!c) c = d.ctrl;
     if (c->status == ST_RETRY) return c->dt_retry;
     double adv = 0.9 * advised_from_slot(p, c->slot[0], c->n_active);
@@ -195,7 +211,8 @@ __device__ __forceinline__ bool first_operator_ok(const Ctrl* c, const Params& p
 // ---------------------------------------------------------------------------
 // explicit source terms (src/Equations.f90:456-620)
 // ---------------------------------------------------------------------------
-__device__ void flux_sources(const Dev& d, const Params& p, double t, double grid_t, double x,
+template <class DevT>
+__device__ void flux_sources(const DevT& d, const Params& p, double t, double grid_t, double x,
                              double y, double& Qt, double& psiQt) {
     double qsum = 0.0, psum = 0.0;
     for (int J = 0; J < p.n_sources; ++J) {
@@ -304,8 +321,8 @@ __device__ __forceinline__ void map_cell(const Params& p, int t, int i, int j, i
 // CorrectSlopes + CalculateFluxes, src/HydraulicRHS.f90:181-1182).
 // Along the face normal the four cells are c[-2], L=c[-1], R=c[0], c[1];
 // fi is the face position along the normal, fj the row (x) / column (y).
-template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem>
-__device__ __forceinline__ void face_flux(const Dev& d, const Params& p, Smem& s, int fi, int fj,
+template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem, class DevT>
+__device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& s, int fi, int fj,
                                           double& cfl_min) {
     int iLL, iL, iR, iRR;
     if (!YDIR) { iLL = rg(fi - 2, fj); iL = rg(fi - 1, fj); iR = rg(fi, fj); iRR = rg(fi + 1, fj); }
@@ -465,8 +482,8 @@ __device__ __forceinline__ float wave_root_f(const Params& p, float hn, float bp
     return __builtin_sqrtf((float)p.g * hn);
 }
 
-template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem>
-__device__ __forceinline__ void face_flux_mixed(const Dev& d, const Params& p, Smem& s, int fi, int fj,
+template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem, class DevT>
+__device__ __forceinline__ void face_flux_mixed(const DevT& d, const Params& p, Smem& s, int fi, int fj,
                                                 double& cfl_min) {
     int iLL, iL, iR, iRR;
     if (!YDIR) { iLL = rg(fi - 2, fj); iL = rg(fi - 1, fj); iR = rg(fi, fj); iRR = rg(fi + 1, fj); }
@@ -616,7 +633,8 @@ __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
 // InterpolateB (src/HydraulicRHS.f90:741-753) of tile tn's copy of vertices
 // (i0, j0) and (i1, j1) in the bed the morphodynamic operator left (TB), which
 // is intermed3's: a ghost tile's copy there may carry redistribution writes.
-__device__ __forceinline__ double ghost_bed(const Dev& d, const Params& p, int tn, int i0, int j0,
+template <class DevT>
+__device__ __forceinline__ double ghost_bed(const DevT& d, const Params& p, int tn, int i0, int j0,
                                            int i1, int j1) {
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     const size_t a = (size_t)tn * NV + j0 * (p.nx + 1) + i0, b = (size_t)tn * NV + j1 * (p.nx + 1) + i1;
@@ -630,8 +648,8 @@ __device__ __forceinline__ double ghost_bed(const Dev& d, const Params& p, int t
 // [-1, nn+1] along the normal and its row/column r, as tile t sees it
 // (faces beyond the tile edge belong to the neighbour; they do not touch the
 // shared vertices, so every tile sees the same value there).
-template <bool YDIR>
-__device__ __forceinline__ double face_bed(const Dev& d, const Params& p, int t, int K, int r) {
+template <bool YDIR, class DevT>
+__device__ __forceinline__ double face_bed(const DevT& d, const Params& p, int t, int K, int r) {
     if (!YDIR) {
         if (K < 0) return d.T.xib[(t - 1) * p.NFX + r * (p.nx + 1) + (p.nx - 1)];
         if (K > p.nx) return d.T.xib[(t + 1) * p.NFX + r * (p.nx + 1) + 1];
@@ -648,8 +666,8 @@ __device__ __forceinline__ double face_bed(const Dev& d, const Params& p, int t,
 // STAGE 3: RHS(u2); u3 -> S3.                   (src/TimeStepper.f90:447-498)
 // STAGE 4: RHS(u3); implicit drag -> Sout; maxima; activation flags.
 //                                               (src/TimeStepper.f90:501-524)
-template <bool YDIR, bool VISC, int LIM, bool CFL, bool MIX, typename Smem>
-__device__ __forceinline__ void face_work(const Dev& d, const Params& p, Smem& s, int fi, int fj,
+template <bool YDIR, bool VISC, int LIM, bool CFL, bool MIX, typename Smem, class DevT>
+__device__ __forceinline__ void face_work(const DevT& d, const Params& p, Smem& s, int fi, int fj,
                                           double& cfl_min) {
     if constexpr (MIX) face_flux_mixed<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
     else face_flux<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
@@ -672,8 +690,8 @@ __device__ __forceinline__ double fluxv(const Smem& s, int q, int i) {
 // stage_body does one work item (one sub-block of an active tile): witem is
 // its index in the scenario's active-list order, or -1 to derive it from the
 // block index with the XCD-chunked map of a one-scenario launch.
-template <int STAGE, bool VISC, int LIM, bool MIX, typename Smem>
-__device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& s, int witem) {
+template <int STAGE, bool VISC, int LIM, bool MIX, typename Smem, class DevT>
+__device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem& s, int witem) {
     constexpr int BY = Smem::by;       // sub-block height
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -1071,9 +1089,10 @@ __device__ __forceinline__ void stage_body(const Dev& d, const Params& p, Smem& 
 }
 
 template <int STAGE, bool VISC, int LIM, bool MIX = false, int BY = B>
-__global__ __launch_bounds__(STAGE_NT) void stage_kernel(Dev d, Params p) {
+__global__ __launch_bounds__(STAGE_NT) void stage_kernel(const Dev* __restrict__ d,
+                                                         const Params* __restrict__ p) {
     __shared__ SmemT<VISC ? 7 : 5, MIX, BY> s;
-    stage_body<STAGE, VISC, LIM, MIX>(d, p, s, -1);
+    stage_body<STAGE, VISC, LIM, MIX>(const_arg(d), *p, s, -1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1085,7 +1104,8 @@ __global__ __launch_bounds__(STAGE_NT) void stage_kernel(Dev d, Params p) {
 // src/MorphodynamicRHS.f90:308-578; face b0 of GetHeights, src/dem.f90:378-386;
 // CorrectSlopes beds of InterpolateB, src/HydraulicRHS.f90:741-753).  Block-wide.
 #define V(a, i, j) a[(j) * NV1 + (i)]
-__device__ void tile_cell_topo(const Dev& d, const Params& p, const Topo& T, int t, bool zero_btc) {
+template <class DevT, class TopoT>
+__device__ void tile_cell_topo(const DevT& d, const Params& p, const TopoT& T, int t, bool zero_btc) {
     const int nx = p.nx, NV1 = nx + 1;
     const size_t NV = (size_t)(nx + 1) * (p.ny + 1);
     const double* b = d.vb0 + (size_t)t * NV;
@@ -1113,7 +1133,8 @@ __device__ void tile_cell_topo(const Dev& d, const Params& p, const Topo& T, int
 }
 
 // part/nparts: the share of the tile's faces one block of several computes.
-__device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int t, int part = 0,
+template <class DevT, class TopoT>
+__device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, int t, int part = 0,
                                int nparts = 1) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
@@ -1196,7 +1217,8 @@ __device__ void tile_face_topo(const Dev& d, const Params& p, const Topo& T, int
 // (src/HydraulicRHS.f90:983-1006) enters the CFL as (gr*gr)*dx; the
 // prefactors of ConstructHydraulicRHS (src/HydraulicRHS.f90:1233-1243).
 // Needs the cell gamma of the neighbour tiles.
-__device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, int t, int part = 0,
+template <class DevT, class TopoT>
+__device__ void tile_static_terms(const DevT& d, const Params& p, const TopoT& T, int t, int part = 0,
                                   int nparts = 1) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
     const int nx = p.nx, ny = p.ny;
@@ -1240,7 +1262,8 @@ __device__ void tile_static_terms(const Dev& d, const Params& p, const Topo& T, 
 
 // Ghost w slopes at the edges of tile g that face active tiles
 // (ReconstructwAtEdges, src/UpdateTiles.f90:669-750).
-__device__ void ghost_slopes(const Dev& d, const Params& p, int g) {
+template <class DevT>
+__device__ void ghost_slopes(const DevT& d, const Params& p, int g) {
     const int nx = p.nx, ny = p.ny, NC = p.NC;
     const double* w = (d.mixed && d.ctrl->cur) ? d.S0b : d.S0;
     const size_t base = (size_t)g * NC;
@@ -1277,7 +1300,8 @@ __device__ void ghost_slopes(const Dev& d, const Params& p, int g) {
 
 // Dry data of a freshly activated or ghost tile (ActivateTile sets w = b0,
 // SetDefaultTileData zeroes the rest; src/UpdateTiles.f90:342-384,647-664).
-__device__ void dry_tile(const Dev& d, const Params& p, int k, bool active) {
+template <class DevT>
+__device__ void dry_tile(const DevT& d, const Params& p, int k, bool active) {
     const size_t base = (size_t)k * p.NC;
     const long long FS = d.FS;
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
@@ -1323,7 +1347,8 @@ static constexpr int MAXNEW = 1024;
 // A new step refreshes intermed3's ghost tiles from the tileContainer
 // (InitialiseTimeSteppingArrays, src/TimeStepper.f90:785-808): undo the
 // redistribution writes of the previous step.
-__device__ void refresh_ghost_overlay(const Dev& d, const Params& p) {
+template <class DevT>
+__device__ void refresh_ghost_overlay(const DevT& d, const Params& p) {
     Ctrl* c = d.ctrl;
     const long long n = c->ndirty;
     if (n == 0) return;
@@ -1357,7 +1382,8 @@ __device__ void refresh_ghost_overlay(const Dev& d, const Params& p) {
     __syncthreads();
 }
 
-__device__ void activate_body(const Dev& d, const Params& p) {
+template <class DevT>
+__device__ void activate_body(const DevT& d, const Params& p) {
     __shared__ int newt[MAXNEW];
     __shared__ int nnew, bad, need;
     Ctrl* c = d.ctrl;
@@ -1477,7 +1503,10 @@ __device__ void activate_body(const Dev& d, const Params& p) {
     }
 }
 
-__global__ __launch_bounds__(256) void activate_kernel(Dev d, Params p) { activate_body(d, p); }
+__global__ __launch_bounds__(256) void activate_kernel(const Dev* __restrict__ d,
+                                                       const Params* __restrict__ p) {
+    activate_body(const_arg(d), *p);
+}
 
 // Rebuild every ghost's edge slopes (end of LoadSourceConditions,
 // src/SetSources.f90:388-392) and the activation flags of every active tile.
@@ -1514,7 +1543,8 @@ __global__ __launch_bounds__(256) void init_flags_kernel(Dev d, Params p) {
 // morphodynamic checks or redistribution (dt <- dt/2), the second
 // operator's substep-1 advice (dt <- 0.9 dt or the advice), or its CFL
 // re-checks (dt <- 0.9 dt_k).
-__device__ void finalize_body(const Dev& d, const Params& p) {
+template <class DevT>
+__device__ void finalize_body(const DevT& d, const Params& p) {
     Ctrl* c = d.ctrl;
     c->commit = 0;
     if (c->status == ST_DONE || c->status == ST_ERROR) return;
@@ -1595,7 +1625,9 @@ __device__ void finalize_body(const Dev& d, const Params& p) {
     }
 }
 
-__global__ void finalize_kernel(Dev d, Params p) { finalize_body(d, p); }
+__global__ void finalize_kernel(const Dev* __restrict__ d, const Params* __restrict__ p) {
+    finalize_body(const_arg(d), *p);
+}
 
 // ---------------------------------------------------------------------------
 // batches: many scenarios advanced by one launch per kernel
@@ -1712,6 +1744,8 @@ struct kst_handle {
     Params p;
     Dev d;          // launch arguments of the first (or only) hydraulic operator
     Dev d2;         // of the second hydraulic operator of a Strang step
+    Dev* d_args = nullptr;      // device copies of d and d2
+    Params* d_par = nullptr;    // device copy of p
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
@@ -1855,7 +1889,7 @@ void from_tiles(const kst_handle* h, const double* t, double* g) {
         }
 }
 
-typedef void (*StageFn)(Dev, Params);
+typedef void (*StageFn)(const Dev*, const Params*);
 struct StageSet { StageFn fn[4]; };
 
 template <bool VISC, int LIM, bool MIX, int BY>
@@ -1906,23 +1940,25 @@ int launch_iteration(kst_handle* h, bool profiled, bool small) {
     const bool sm = small && !d.mixed;
     const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
     const int nblk_stage = sm ? 8 * ((p.ntiles * spt_s + 7) / 8) : nblk;
-    LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), d, p);
+    const Dev* const A = h->d_args;
+    const Params* const P = h->d_par;
+    LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), A, P);
     const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0, sm);
     for (int q = 0; q < 4; ++q)
-        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), d, p);
+        LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A, P);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
-        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(STAGE_NT), d, p);
-        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(STAGE_NT), d, p);
-        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(STAGE_NT), d, p);
-        LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), d, p);
-        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), d, p);
+        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(STAGE_NT), A, P);
+        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(STAGE_NT), A, P);
+        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(STAGE_NT), A, P);
+        LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), A, P);
+        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), A, P);
         // second hydraulic operator
         for (int q = 0; q < 4; ++q)
-            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), h->d2, p);
+            LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A + 1, P);
     }
-    LAUNCH(KW_FINALIZE, finalize_kernel, dim3(1), dim3(1), d, p);
-    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), d, p);
+    LAUNCH(KW_FINALIZE, finalize_kernel, dim3(1), dim3(1), A, P);
+    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), A, P);
 #undef LAUNCH
     HIPCHK(h, hipGetLastError());
     return 0;
@@ -2165,6 +2201,15 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
         e.second = 1;
         e.last = 1;
     }
+    rc |= dalloc(h, &h->d_args, 2);
+    rc |= dalloc(h, &h->d_par, 1);
+    if (!rc) {
+        const Dev both[2] = {h->d, h->d2};
+        rc |= hipMemcpyAsync(h->d_args, both, sizeof both, hipMemcpyHostToDevice, h->stream) != hipSuccess;
+        rc |= hipMemcpyAsync(h->d_par, &h->p, sizeof(Params), hipMemcpyHostToDevice, h->stream) != hipSuccess;
+        rc |= hipStreamSynchronize(h->stream) != hipSuccess;
+    }
+    if (rc) { kst_destroy(h); return KST_ERR_HIP; }
     *out = h;
     return KST_OK;
 }
