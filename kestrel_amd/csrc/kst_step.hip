@@ -162,6 +162,13 @@ __device__ __forceinline__ ConstArg<T>& const_arg(const T* p) {
     return *(ConstArg<T>*)p;
 }
 
+// The same object as an ordinary reference (reads through it still become
+// scalar loads once the compiler sees where it came from).
+template <class T>
+__device__ __forceinline__ const T& generic(ConstArg<T>& r) {
+    return *(const T*)&r;
+}
+
 // ---------------------------------------------------------------------------
 // time-step control shared by kernels
 // ---------------------------------------------------------------------------
@@ -1651,7 +1658,7 @@ constexpr int BATCH_PLAN_THREADS = 1024;
 __global__ __launch_bounds__(256) void batch_activate_kernel(Batch bt) {
     const int s = blockIdx.x;
     if (!bt.on[s]) return;
-    activate_body(bt.dev[s], bt.par[s]);
+    activate_body(const_arg(bt.dev + s), generic(const_arg(bt.par + s)));
 }
 
 // Work list of the iteration: the running slots' active sub-blocks.
@@ -1705,7 +1712,8 @@ __global__ __launch_bounds__(STAGE_NT) void batch_stage_kernel(Batch bt) {
         }
         __syncthreads();
         const int sl = __builtin_amdgcn_readfirstlane(sh_slot);
-        stage_body<STAGE, VISC, LIM, MIX>(bt.dev[sl], bt.par[sl], s, w - bt.off[sl]);
+        stage_body<STAGE, VISC, LIM, MIX>(const_arg(bt.dev + sl), generic(const_arg(bt.par + sl)), s,
+                                          w - bt.off[sl]);
         __syncthreads();
     }
 }
