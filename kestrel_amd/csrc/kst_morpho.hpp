@@ -22,12 +22,13 @@ constexpr int MV = B + 1;    // vertices along a sub-block edge
 // first hydraulic operator passed its re-checks), its dt, and the block's
 // work item (sub-block of an active tile, XCD-chunked as the stage kernel).
 struct MWork {
-    int tile, q, bx0, by0, bxa, bya, skip;
+    int tile, q, bx0, by0, bxa, bya, skip, more;
     double dt;
 };
 
 template <class DevT>
-__device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw) {
+__device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw,
+                            int pass) {
     const Ctrl* c = d.ctrl;
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
@@ -39,10 +40,10 @@ __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_
     if (!skip) count_work(d.ctrl, kw);
     const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
     const int nwork = c->n_active * spt;
-    const int per = (nwork + 7) / 8;
-    const int b = blockIdx.x;
-    const int w = (b % 8) * per + b / 8;
-    if (b / 8 >= per || w >= nwork) skip = 1;
+    bool more = false;
+    const int w = xcd_item(nwork, pass, &more);
+    if (w >= nwork) skip = 1;
+    m.more = more;
     if (!skip) {
         m.q = w / spt;
         m.tile = d.active[m.q];
@@ -87,19 +88,22 @@ __device__ __forceinline__ double excess_deposition(const DevT& d, const Params&
 //   cell topography (ComputeCellCentredTopographicData, :308-360)
 //   w, Hnpsi from the change of bt (src/TimeStepper.f90:587-610)
 // Stage 3 then runs the checks of src/TimeStepper.f90:713-752.
-template <int S>
-__global__ __launch_bounds__(STAGE_NT) void morpho_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
-    const auto& d = const_arg(dp);
-    const Params& p = *pp;
+template <int S, class DevT>
+__device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int pass) {
     __shared__ double hn[NREG];            // Hn of intermed_{S-1}, 2-cell halo
     __shared__ double emd[MR1 * MR1];      // E - D, 1-cell halo
     __shared__ double sbx[MR1 * MR1], sby[MR1 * MR1];
     __shared__ double vb[MV * MV];         // new vertex bt of this tile's copy
     __shared__ MWork m;
-    const int tid = threadIdx.x;
-    if (tid == 0) morpho_work(d, p, m, false, KW_MORPHO + S - 1);
+    // a fresh thread index per pass (nothing derived from it is held across
+    // passes); the asm has no side effects
+    unsigned tidu = threadIdx.x;
+    asm("" : "+v"(tidu) : "s"(pass));
+    const int tid = (int)(tidu & (STAGE_NT - 1));
+    if (tid == 0) morpho_work(d, p, m, false, KW_MORPHO + S - 1, pass);
     __syncthreads();
-    if (m.skip) return;
+    if (__builtin_amdgcn_readfirstlane(m.skip)) return false;   // wave-uniform (a uniform loop)
+    const bool more = __builtin_amdgcn_readfirstlane(m.more) != 0;
     const long long FS = d.FS;
     const int t = m.tile, bx0 = m.bx0, by0 = m.by0, bxa = m.bxa, bya = m.bya;
     const double dtm = 2.0 * m.dt;
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(STAGE_NT) void morpho_kernel(const Dev* __restrict_
 
     // ---- cell topography, w and Hnpsi of the owned cells -------------------
     const int tx = tid % B, ty = tid / B;
-    if (tx >= bxa || ty >= bya) return;
+    if (tx >= bxa || ty >= bya) return more;
     const int i = bx0 + tx, j = by0 + ty;
     const double* b0 = d.vb0 + (size_t)t * NV;
 #define VB(a, b) vb[(b) * MV + (a)]
@@ -262,6 +266,22 @@ __global__ __launch_bounds__(STAGE_NT) void morpho_kernel(const Dev* __restrict_
             const double rel = fabs(hnew - ho2) / fabs(ho2);
             if (rel > 0.1) atomicOr(&c->mref, 1);
         }
+    }
+    return more;
+}
+
+// The grid is sized for the active set the host last saw; a block whose
+// share of the list holds more items takes them in turn.
+template <int S>
+__global__ __launch_bounds__(STAGE_NT) void morpho_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    for (int pass = 0;; ++pass) {
+        // fresh argument pointers per pass: loads through them are not
+        // hoisted out of the loop and held in registers across passes
+        const Dev* d = dp;
+        const Params* p = pp;
+        asm("" : "+s"(d), "+s"(p) : "s"(pass));
+        if (!morpho_pass<S>(const_arg(d), *p, pass)) break;
+        __syncthreads();
     }
 }
 
@@ -877,15 +897,14 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
 }
 
 // ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),
-// plus its CFL numerators and prefactors; TILE_PARTS blocks per active tile.
+// plus its CFL numerators and prefactors, in (tile, part) items.
 constexpr int TILE_PARTS = 4;
 
 __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
     const auto& d = const_arg(dp);
     const Params& p = *pp;
-    __shared__ int sh_skip;
+    __shared__ int sh_skip, sh_n;
     const Ctrl* c = d.ctrl;
-    const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
     if (threadIdx.x == 0) {
         int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
@@ -893,29 +912,24 @@ __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ 
             if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
         if (!skip) count_work(c, KW_FACETOPO);
-        if (q >= c->n_active) skip = 1;
         sh_skip = skip;
+        sh_n = c->n_active * TILE_PARTS;
     }
     __syncthreads();
     if (sh_skip) return;
-    const int t = d.active[q];
-    // a face's static terms read only that face's topography, which the same
-    // thread computes, and cell gamma, which the morphodynamic stage wrote
-    tile_face_topo(d, p, d.TB, t, part, TILE_PARTS);
-    tile_static_terms(d, p, d.TB, t, part, TILE_PARTS);
+    // (tile, part) items, the grid sized for the active set the host last saw
+    for (int it = blockIdx.x; it < sh_n; it += gridDim.x) {
+        const int t = d.active[it / TILE_PARTS], part = it % TILE_PARTS;
+        // a face's static terms read only that face's topography, which the
+        // same thread computes, and cell gamma, which the morphodynamic stage wrote
+        tile_face_topo(d, p, d.TB, t, part, TILE_PARTS);
+        tile_static_terms(d, p, d.TB, t, part, TILE_PARTS);
+    }
 }
 
-// The accepted step's bed becomes the current one (CopySolutionData
-// intermed3 -> tileContainer, src/TimeStepper.f90:263); TILE_PARTS blocks per
-// active tile.
-__global__ __launch_bounds__(256) void commit_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
-    const auto& d = const_arg(dp);
-    const Params& p = *pp;
-    const Ctrl* c = d.ctrl;
-    const int q = blockIdx.x / TILE_PARTS, part = blockIdx.x % TILE_PARTS;
-    if (!c->commit || q >= c->n_active) return;
-    if (threadIdx.x == 0) count_work(c, KW_COMMIT);
-    const int t = d.active[q];
+// One of the TILE_PARTS shares of tile t's bed copy.
+template <class DevT>
+__device__ void commit_part(const DevT& d, const Params& p, int t, int part) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * TILE_PARTS;
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     for (size_t v = lane0; v < NV; v += stride) d.TA.vbt[t * NV + v] = d.TB.vbt[t * NV + v];
@@ -935,6 +949,20 @@ __global__ __launch_bounds__(256) void commit_kernel(const Dev* __restrict__ dp,
         d.TA.ybt[f] = d.TB.ybt[f]; d.TA.ybx[f] = d.TB.ybx[f]; d.TA.ygam[f] = d.TB.ygam[f];
         d.TA.yib[f] = d.TB.yib[f]; d.TA.ynp[f] = d.TB.ynp[f]; d.TA.ynm[f] = d.TB.ynm[f];
     }
+}
+
+// The accepted step's bed becomes the current one (CopySolutionData
+// intermed3 -> tileContainer, src/TimeStepper.f90:263), in (tile, part) items
+// over a grid sized for the active set the host last saw.
+__global__ __launch_bounds__(256) void commit_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    const auto& d = const_arg(dp);
+    const Params& p = *pp;
+    const Ctrl* c = d.ctrl;
+    if (!c->commit) return;
+    if (threadIdx.x == 0) count_work(c, KW_COMMIT);
+    const int n = c->n_active * TILE_PARTS;
+    for (int it = blockIdx.x; it < n; it += gridDim.x)
+        commit_part(d, p, d.active[it / TILE_PARTS], it % TILE_PARTS);
 }
 
 }  // namespace kst

@@ -43,11 +43,16 @@ constexpr int STAGE_NT = 256;
 
 enum Status { ST_NEW = 0, ST_RETRY = 1, ST_DONE = 2, ST_ERROR = 3 };
 
+// A CFL minimum (bit pattern of a positive double) on a cache line of its
+// own: every block of a stage kernel updates one with an atomic, and the
+// fields the blocks read at their start must not share its line (reads queue
+// behind the atomics in L2).
+struct alignas(128) CflSlot {
+    unsigned long long bits;
+};
+
 struct Ctrl {
     double t, tend, dt_retry, last_dt, min_dt, max_dt;
-    // CFL minima (bit patterns): first hydraulic operator stages 1..3, then
-    // the second operator of a Strang step stages 1..3
-    unsigned long long slot[6];
     long long steps, rejected, rhs_evals, cell_updates;
     long long steps_left;
     long long redist_cells;    // cells corrected by RedistributeCell
@@ -60,6 +65,9 @@ struct Ctrl {
     long long work[16];        // launches that did work, per kernel (KW_*)
     int cur;                   // mixed precision: u^n is in S0 (0) or S0b (1)
     double next_t;             // NextFluxSeriesTime of t, kept with t
+    // CFL minima: first hydraulic operator stages 1..3, then the second
+    // operator of a Strang step stages 1..3
+    CflSlot slot[6];
 };
 
 // Kernel ids of Ctrl::work.  Block 0 of a launch that does work counts it, so
@@ -202,7 +210,7 @@ template <class DevT>
 __device__ double step_dt(const DevT& d, const Params& p, const Ctrl* c = nullptr) {
     if (!c) c = d.ctrl;
     if (c->status == ST_RETRY) return c->dt_retry;
-    double adv = 0.9 * advised_from_slot(p, c->slot[0], c->n_active);
+    double adv = 0.9 * advised_from_slot(p, c->slot[0].bits, c->n_active);
     double tmax = dmin(c->tend, c->next_t);
     if (!d.morpho) return dmin(adv, tmax - c->t);
     return dmin(adv, 0.5 * (tmax - c->t));
@@ -211,8 +219,8 @@ __device__ double step_dt(const DevT& d, const Params& p, const Ctrl* c = nullpt
 // The first hydraulic operator of the attempt passed both CFL re-checks
 // (src/TimeStepper.f90:393,452).
 __device__ __forceinline__ bool first_operator_ok(const Ctrl* c, const Params& p, double dt) {
-    return advised_from_slot(p, c->slot[1], c->n_active) >= dt &&
-           advised_from_slot(p, c->slot[2], c->n_active) >= dt;
+    return advised_from_slot(p, c->slot[1].bits, c->n_active) >= dt &&
+           advised_from_slot(p, c->slot[2].bits, c->n_active) >= dt;
 }
 
 // ---------------------------------------------------------------------------
@@ -307,7 +315,7 @@ struct SmemT {
     FT fy[NQ][NFY];
     double fx4[MIX ? NFX : 1], fy4[MIX ? NFY : 1];
     double red[STAGE_NT / 64];
-    int tile, bx0, by0, bxa, bya, skip, cur;
+    int tile, bx0, by0, bxa, bya, skip, cur, more;
     double dt, tstage, tgrid;
 };
 
@@ -694,13 +702,36 @@ __device__ __forceinline__ double fluxv(const Smem& s, int q, int i) {
 // morphodynamic result and first checks its substep-1 advice
 // (src/TimeStepper.f90:214-258).
 //
+// The pass-th work item of this block in a one-scenario launch of G blocks
+// (G a multiple of 8) over a list of nwork items.  Blocks are dealt to the 8
+// XCDs round-robin, so XCD x (blocks x, x + 8, ...) takes the x-th contiguous
+// eighth of the list and neighbouring sub-blocks share an L2; within it block
+// b takes items b/8, b/8 + G/8, ...  Returns nwork when there is none; *more
+// tells whether the block has a further item.
+__device__ __forceinline__ int xcd_item(int nwork, int pass, bool* more) {
+    const int G = gridDim.x, b = blockIdx.x, x = b % 8, m = G / 8;
+    const int per = (nwork + 7) / 8;
+    const int end = min((x + 1) * per, nwork);
+    const int w = x * per + b / 8 + pass * m;
+    *more = w + m < end;
+    return w < end ? w : nwork;
+}
+
 // stage_body does one work item (one sub-block of an active tile): witem is
-// its index in the scenario's active-list order, or -1 to derive it from the
-// block index with the XCD-chunked map of a one-scenario launch.
+// its index in the scenario's active-list order, or -1 to take the pass-th
+// item of the block in a one-scenario launch.  Returns whether the block has
+// a further item (false when it had none).
 template <int STAGE, bool VISC, int LIM, bool MIX, typename Smem, class DevT>
-__device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem& s, int witem) {
+__device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem& s, int witem,
+                                           int pass = 0) {
     constexpr int BY = Smem::by;       // sub-block height
-    const int tid = threadIdx.x;
+    // the thread index as a fresh value per pass: in a block's loop over
+    // items nothing derived from it is hoisted and held across items (which
+    // would cost registers, and resident waves, for the common single pass).
+    // The asm has no side effects, so loads are still known unclobbered.
+    unsigned tidu = threadIdx.x;
+    asm("" : "+v"(tidu) : "s"(pass));
+    const int tid = (int)(tidu & (STAGE_NT - 1));
     if (tid == 0) {
         const Ctrl* c = d.ctrl;
         int skip = 0;
@@ -711,9 +742,9 @@ __device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem&
         if (!skip && (STAGE >= 2 || d.second)) {
             dt = step_dt(d, p);
             if (d.second && (!first_operator_ok(c, p, dt) || c->mref)) skip = 1;
-            if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3], c->n_active) < dt) skip = 1;
-            if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1], c->n_active) < dt) skip = 1;
-            if (STAGE >= 4 && advised_from_slot(p, c->slot[sb + 2], c->n_active) < dt) skip = 1;
+            if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3].bits, c->n_active) < dt) skip = 1;
+            if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1].bits, c->n_active) < dt) skip = 1;
+            if (STAGE >= 4 && advised_from_slot(p, c->slot[sb + 2].bits, c->n_active) < dt) skip = 1;
         }
         s.dt = dt;
         s.cur = c->cur;
@@ -725,11 +756,8 @@ __device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem&
         const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
         const int nwork = c->n_active * spt;
         int w = witem;
-        if (witem < 0) {
-            const int per = (nwork + 7) / 8;
-            const int b = blockIdx.x;
-            w = (b / 8 >= per) ? nwork : (b % 8) * per + b / 8;
-        }
+        bool more = false;
+        if (witem < 0) w = xcd_item(nwork, pass, &more);
         if (!skip && w == 0) d.ctrl->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)] += 1;
         if (w >= nwork) skip = 1;
         if (!skip) {
@@ -741,9 +769,13 @@ __device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem&
             s.bya = min(BY, p.ny - s.by0);
         }
         s.skip = skip;
+        s.more = more;
     }
     __syncthreads();
-    if (s.skip) return;
+    // read as wave-uniform values, so that the block's loop over items is a
+    // uniform loop (a loop the compiler must treat as divergent costs far more)
+    if (__builtin_amdgcn_readfirstlane(s.skip)) return false;
+    const bool more = __builtin_amdgcn_readfirstlane(s.more) != 0;
     const int t = s.tile;
     const double dt = s.dt;
     const int FS = (int)d.FS;
@@ -919,13 +951,13 @@ __device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem&
             face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / bxa, f % bxa, cflm);
     }
     if (STAGE != 4) {
-        reduce_min_to(s, cflm, &d.ctrl->slot[(d.second ? 3 : 0) + STAGE - 1]);
+        reduce_min_to(s, cflm, &d.ctrl->slot[(d.second ? 3 : 0) + STAGE - 1].bits);
     } else {
         __syncthreads();
     }
 
     // ---- phase C: RHS of the owned cell and the stage update -------------
-    if (!own) return;
+    if (!own) return more;
     const int xs = ty * (B + 1) + tx, ys = ty * B + tx;
 #define FX(q, o) fluxv<MIX, false>(s, q, xs + (o))
 #define FY(q, o) fluxv<MIX, true>(s, q, ys + (o))
@@ -1093,13 +1125,17 @@ __device__ __forceinline__ void stage_body(const DevT& d, const Params& p, Smem&
             if (f) atomicOr(&d.eflags[t], f);
         }
     }
+    return more;
 }
 
+// The grid is sized for the active set the host last saw (with room to
+// grow); a block whose share of the list holds more items takes them in turn.
 template <int STAGE, bool VISC, int LIM, bool MIX = false, int BY = B>
 __global__ __launch_bounds__(STAGE_NT) void stage_kernel(const Dev* __restrict__ d,
                                                          const Params* __restrict__ p) {
     __shared__ SmemT<VISC ? 7 : 5, MIX, BY> s;
-    stage_body<STAGE, VISC, LIM, MIX>(const_arg(d), *p, s, -1);
+    for (int pass = 0; stage_body<STAGE, VISC, LIM, MIX>(const_arg(d), *p, s, -1, pass); ++pass)
+        __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -1561,13 +1597,13 @@ __device__ void finalize_body(const DevT& d, const Params& p) {
     c->rhs_evals += (ran_p1 ? 1 : 0) + 1;
     bool reject = false;
     double retry = 0.0;
-    const double dt1 = advised_from_slot(p, c->slot[1], c->n_active);
+    const double dt1 = advised_from_slot(p, c->slot[1].bits, c->n_active);
     if (dt1 < dt) {
         reject = true;
         retry = 0.9 * dt1;
     } else {
         c->rhs_evals += 1;
-        const double dt2 = advised_from_slot(p, c->slot[2], c->n_active);
+        const double dt2 = advised_from_slot(p, c->slot[2].bits, c->n_active);
         if (dt2 < dt) {
             reject = true;
             retry = 0.9 * dt2;
@@ -1581,19 +1617,19 @@ __device__ void finalize_body(const DevT& d, const Params& p) {
             retry = 0.5 * dt;
         } else {
             c->rhs_evals += 1;
-            const double adv = 0.9 * advised_from_slot(p, c->slot[3], c->n_active);
+            const double adv = 0.9 * advised_from_slot(p, c->slot[3].bits, c->n_active);
             if (adv < dt) {
                 reject = true;
                 retry = ((dt - adv) / dt < 0.1) ? 0.9 * dt : adv;
             } else {
                 c->rhs_evals += 1;
-                const double e1 = advised_from_slot(p, c->slot[4], c->n_active);
+                const double e1 = advised_from_slot(p, c->slot[4].bits, c->n_active);
                 if (e1 < dt) {
                     reject = true;
                     retry = 0.9 * e1;
                 } else {
                     c->rhs_evals += 1;
-                    const double e2 = advised_from_slot(p, c->slot[5], c->n_active);
+                    const double e2 = advised_from_slot(p, c->slot[5].bits, c->n_active);
                     if (e2 < dt) {
                         reject = true;
                         retry = 0.9 * e2;
@@ -1607,7 +1643,7 @@ __device__ void finalize_body(const DevT& d, const Params& p) {
     const unsigned long long huge = dbits(kHuge);
     c->mref = 0;
     c->ncand = 0;
-    for (int k = 1; k < 6; ++k) c->slot[k] = huge;
+    for (int k = 1; k < 6; ++k) c->slot[k].bits = huge;
     if (reject) {
         c->rejected++;
         c->status = ST_RETRY;
@@ -1621,7 +1657,7 @@ __device__ void finalize_body(const DevT& d, const Params& p) {
     c->last_dt = dt;
     if (c->steps == 1 || dt < c->min_dt) c->min_dt = dt;
     if (dt > c->max_dt) c->max_dt = dt;
-    c->slot[0] = huge;
+    c->slot[0].bits = huge;
     c->status = ST_NEW;
     c->commit = d.morpho ? 1 : 0;
     if (d.mixed) c->cur ^= 1;
@@ -1764,7 +1800,8 @@ struct kst_handle {
     double t = 0.0;
     // profiling: per-iteration event pairs around the four stage kernels
     bool profiling = false;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // kGraphIters whole iterations, per sub-block shape
+    // kGraphIters whole iterations, per (grid cap, sub-block shape)
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[KW_N] = {};
     long long prof_n[KW_N] = {};
@@ -1927,7 +1964,9 @@ const StageSet& stage_set(bool visc, int limiter, bool mixed, bool small) {
     return small ? stage_set_of<false, BY_SMALL>(visc, limiter) : stage_set_of<false, B>(visc, limiter);
 }
 
-int launch_iteration(kst_handle* h, bool profiled, bool small) {
+// Launch grids cover `cap` active tiles (the kernels take further items in
+// turn when the active set has outgrown it).
+int launch_iteration(kst_handle* h, bool profiled, bool small, int cap) {
     const Params& p = h->p;
     const Dev& d = h->d;
     // while profiling, the dispatch of each launch carries the event pair of
@@ -1942,12 +1981,12 @@ int launch_iteration(kst_handle* h, bool profiled, bool small) {
             hipLaunchKernelGGL(kern, grid, block, 0, h->stream, __VA_ARGS__);                \
     } while (0)
     const int spt = ((p.nx + B - 1) / B) * ((p.ny + B - 1) / B);
-    // 8 * ceil(work/8) blocks so the XCD-chunked mapping is a bijection
-    const int nblk = 8 * ((p.ntiles * spt + 7) / 8);
+    // multiples of 8 blocks, as the XCD-chunked mapping assumes
+    const int nblk = 8 * ((cap * spt + 7) / 8);
     const bool visc = p.nu != 0.0;
     const bool sm = small && !d.mixed;
     const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
-    const int nblk_stage = sm ? 8 * ((p.ntiles * spt_s + 7) / 8) : nblk;
+    const int nblk_stage = sm ? 8 * ((cap * spt_s + 7) / 8) : nblk;
     const Dev* const A = h->d_args;
     const Params* const P = h->d_par;
     LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), A, P);
@@ -1960,13 +1999,13 @@ int launch_iteration(kst_handle* h, bool profiled, bool small) {
         LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(STAGE_NT), A, P);
         LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(STAGE_NT), A, P);
         LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), A, P);
-        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), A, P);
+        LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(cap * TILE_PARTS), dim3(256), A, P);
         // second hydraulic operator
         for (int q = 0; q < 4; ++q)
             LAUNCH(KW_STAGE + 4 + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A + 1, P);
     }
     LAUNCH(KW_FINALIZE, finalize_kernel, dim3(1), dim3(1), A, P);
-    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(p.ntiles * TILE_PARTS), dim3(256), A, P);
+    if (d.morpho) LAUNCH(KW_COMMIT, commit_kernel, dim3(cap * TILE_PARTS), dim3(256), A, P);
 #undef LAUNCH
     HIPCHK(h, hipGetLastError());
     return 0;
@@ -1974,15 +2013,25 @@ int launch_iteration(kst_handle* h, bool profiled, bool small) {
 
 constexpr int kGraphIters = 8;
 
+// The grid cap for an active set of n tiles: a power of two with room for
+// the set to double, at least 8, at most every tile.
+int work_cap(const kst_handle* h, int n) {
+    int cap = 8;
+    while (cap < 2 * n && cap < h->ntiles) cap <<= 1;
+    return std::min(cap, h->ntiles);
+}
+
 // One hipGraph holding kGraphIters whole iterations (activation, four stage
-// kernels, finalize); kernel arguments never change for a handle.
-int ensure_graph(kst_handle* h, bool small) {
-    hipGraphExec_t& gx = h->gexec[small ? 1 : 0];
-    if (gx) return 0;
+// kernels, finalize), per grid cap and sub-block shape; kernel arguments
+// never change for a handle.
+int ensure_graph(kst_handle* h, bool small, int cap, hipGraphExec_t* out) {
+    const int key = 2 * cap + (small ? 1 : 0);
+    for (const auto& kv : h->graphs)
+        if (kv.first == key) { *out = kv.second; return 0; }
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int q = 0; q < kGraphIters; ++q) {
-        int rc = launch_iteration(h, false, small);
+        int rc = launch_iteration(h, false, small, cap);
         if (rc) {
             hipGraph_t dummy = nullptr;
             (void)hipStreamEndCapture(h->stream, &dummy);
@@ -1991,8 +2040,12 @@ int ensure_graph(kst_handle* h, bool small) {
         }
     }
     HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-    HIPCHK(h, hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
-    HIPCHK(h, hipGraphDestroy(g));
+    hipGraphExec_t gx = nullptr;
+    const hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(h, e);
+    h->graphs.emplace_back(key, gx);
+    *out = gx;
     return 0;
 }
 
@@ -2015,17 +2068,19 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
         launched += batch;
         long long work_before[KW_N];
         std::memcpy(work_before, c.work, sizeof work_before);
-        // the sub-block shape for this batch, from the last known active set
+        // the sub-block shape and grid cap for this batch, from the last
+        // known active set
         const int spt = ((h->p.nx + B - 1) / B) * ((h->p.ny + B - 1) / B);
         const bool small = (long long)c.n_active * spt < SMALL_WORK;
+        const int cap = work_cap(h, c.n_active);
         if (h->profiling) {
-            rc = launch_iteration(h, true, small);
+            rc = launch_iteration(h, true, small, cap);
             if (rc) return rc;
         } else {
-            rc = ensure_graph(h, small);
+            hipGraphExec_t gx = nullptr;
+            rc = ensure_graph(h, small, cap, &gx);
             if (rc) return rc;
-            for (int q = 0; q < batch; q += kGraphIters)
-                HIPCHK(h, hipGraphLaunch(h->gexec[small ? 1 : 0], h->stream));
+            for (int q = 0; q < batch; q += kGraphIters) HIPCHK(h, hipGraphLaunch(gx, h->stream));
         }
         rc = read_ctrl(h, &c);
         if (rc) return rc;
@@ -2372,7 +2427,7 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     c.tend = t;
     c.next_t = host_next_flux_time(h, t);
     const unsigned long long huge = 0x7FEFFFFFFFFFFFFFull;
-    for (int k = 0; k < 6; ++k) c.slot[k] = huge;
+    for (int k = 0; k < 6; ++k) c.slot[k].bits = huge;
     c.status = ST_DONE;
     c.n_active = (int)act.size();
     c.n_ghost = (int)gh.size();
@@ -2786,8 +2841,8 @@ void kst_batch_destroy(kst_batch* b) {
 
 void kst_destroy(kst_handle* h) {
     if (!h) return;
-    for (hipGraphExec_t g : h->gexec)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (const auto& kv : h->graphs)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
