@@ -29,11 +29,11 @@ struct MWork {
 template <class DevT>
 __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw,
                             int pass) {
-    const Ctrl* c = d.ctrl;
+    const auto* c = ctrl_view(d);
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
     if (!skip) {
-        dt = step_dt(d, p);
+        dt = step_dt(d, p, c);
         if (!first_operator_ok(c, p, dt)) skip = 1;
         if (need_no_refine && c->mref) skip = 1;
     }
@@ -768,7 +768,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
     if (tid == 0) {
         int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
-            const double dt = step_dt(d, p);
+            const double dt = step_dt(d, p, c);
             if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
         if (!skip && c->ncand <= 0) skip = 1;
@@ -904,14 +904,14 @@ __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ 
     const auto& d = const_arg(dp);
     const Params& p = *pp;
     __shared__ int sh_skip, sh_n;
-    const Ctrl* c = d.ctrl;
+    const auto* c = ctrl_view(d);
     if (threadIdx.x == 0) {
         int skip = (c->status != ST_NEW && c->status != ST_RETRY);
         if (!skip) {
-            const double dt = step_dt(d, p);
+            const double dt = step_dt(d, p, c);
             if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
         }
-        if (!skip) count_work(c, KW_FACETOPO);
+        if (!skip) count_work(d.ctrl, KW_FACETOPO);
         sh_skip = skip;
         sh_n = c->n_active * TILE_PARTS;
     }
@@ -957,9 +957,9 @@ __device__ void commit_part(const DevT& d, const Params& p, int t, int part) {
 __global__ __launch_bounds__(256) void commit_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
     const auto& d = const_arg(dp);
     const Params& p = *pp;
-    const Ctrl* c = d.ctrl;
+    const auto* c = ctrl_view(d);
     if (!c->commit) return;
-    if (threadIdx.x == 0) count_work(c, KW_COMMIT);
+    if (threadIdx.x == 0) count_work(d.ctrl, KW_COMMIT);
     const int n = c->n_active * TILE_PARTS;
     for (int it = blockIdx.x; it < n; it += gridDim.x)
         commit_part(d, p, d.active[it / TILE_PARTS], it % TILE_PARTS);

@@ -202,13 +202,21 @@ __device__ double advised_from_slot(const Params& p, unsigned long long slot, in
     return dmin(m, p.max_dt);
 }
 
+// The control block as kernels read it at their start: a pointer into the
+// constant address space, so that the reads are scalar loads.  The fields
+// read this way do not change while a kernel runs; its writes and atomics go
+// through d.ctrl itself.
+template <class DevT>
+__device__ __forceinline__ ConstArg<Ctrl>* ctrl_view(const DevT& d) {
+    return (ConstArg<Ctrl>*)d.ctrl;
+}
+
 // The step's (hydraulic) dt: substep-1 advice clamped to the next
 // output/flux-series time, or to half of it with morphodynamics
 // (src/TimeStepper.f90:155-168), or the refined dt of a retried step.
-// c: the control block, or a copy of it a prologue loaded at once.
-template <class DevT>
-__device__ double step_dt(const DevT& d, const Params& p, const Ctrl* c = nullptr) {
-    if (!c) c = d.ctrl;
+// c: the control block or its constant view.
+template <class DevT, class CtrlP>
+__device__ double step_dt(const DevT& d, const Params& p, CtrlP c) {
     if (c->status == ST_RETRY) return c->dt_retry;
     double adv = 0.9 * advised_from_slot(p, c->slot[0].bits, c->n_active);
     double tmax = dmin(c->tend, c->next_t);
@@ -218,7 +226,8 @@ __device__ double step_dt(const DevT& d, const Params& p, const Ctrl* c = nullpt
 
 // The first hydraulic operator of the attempt passed both CFL re-checks
 // (src/TimeStepper.f90:393,452).
-__device__ __forceinline__ bool first_operator_ok(const Ctrl* c, const Params& p, double dt) {
+template <class CtrlP>
+__device__ __forceinline__ bool first_operator_ok(CtrlP c, const Params& p, double dt) {
     return advised_from_slot(p, c->slot[1].bits, c->n_active) >= dt &&
            advised_from_slot(p, c->slot[2].bits, c->n_active) >= dt;
 }
@@ -733,14 +742,14 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     asm("" : "+v"(tidu) : "s"(pass));
     const int tid = (int)(tidu & (STAGE_NT - 1));
     if (tid == 0) {
-        const Ctrl* c = d.ctrl;
+        const auto* c = ctrl_view(d);
         int skip = 0;
         if (c->status == ST_DONE || c->status == ST_ERROR) skip = 1;
         if (STAGE == 1 && !d.second && c->status != ST_NEW) skip = 1;
         const int sb = d.second ? 3 : 0;
         double dt = 0.0;
         if (!skip && (STAGE >= 2 || d.second)) {
-            dt = step_dt(d, p);
+            dt = step_dt(d, p, c);
             if (d.second && (!first_operator_ok(c, p, dt) || c->mref)) skip = 1;
             if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3].bits, c->n_active) < dt) skip = 1;
             if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1].bits, c->n_active) < dt) skip = 1;
@@ -1593,7 +1602,7 @@ __device__ void finalize_body(const DevT& d, const Params& p) {
     if (c->status == ST_DONE || c->status == ST_ERROR) return;
     c->work[KW_FINALIZE] += 1;
     const bool ran_p1 = (c->status == ST_NEW);
-    const double dt = step_dt(d, p);
+    const double dt = step_dt(d, p, d.ctrl);
     c->rhs_evals += (ran_p1 ? 1 : 0) + 1;
     bool reject = false;
     double retry = 0.0;
