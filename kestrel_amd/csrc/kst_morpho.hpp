@@ -17,6 +17,11 @@ namespace kst {
 
 constexpr int MR1 = B + 2;   // sub-block plus a 1-cell halo
 constexpr int MV = B + 1;    // vertices along a sub-block edge
+// Threads of a morphodynamic block: enough for every phase (Hn on the 20 x 20
+// region, E - D on 18 x 18, 17 x 17 vertices, 16 x 16 cells) to take one
+// round, so the transcendental-heavy E - D is evaluated once on the critical path.
+constexpr int MORPHO_NT = 512;
+static_assert(MORPHO_NT >= NREG && MORPHO_NT >= MR1 * MR1, "one round per phase");
 
 // Thread 0: whether the morphodynamic operator runs in this attempt (the
 // first hydraulic operator passed its re-checks), its dt, and the block's
@@ -99,7 +104,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     // passes); the asm has no side effects
     unsigned tidu = threadIdx.x;
     asm("" : "+v"(tidu) : "s"(pass));
-    const int tid = (int)(tidu & (STAGE_NT - 1));
+    const int tid = (int)(tidu & (MORPHO_NT - 1));
     if (tid == 0) morpho_work(d, p, m, false, KW_MORPHO + S - 1, pass);
     __syncthreads();
     if (__builtin_amdgcn_readfirstlane(m.skip)) return false;   // wave-uniform (a uniform loop)
@@ -116,7 +121,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     const double eps = p.hthr;
 
     // ---- Hn of the stage state (ComputeDesingularisedVariables) -----------
-    for (int q = tid; q < NREG; q += STAGE_NT) {
+    for (int q = tid; q < NREG; q += MORPHO_NT) {
         const int rx = q % RW - 2, ry = q / RW - 2;
         if (rx > bxa + 1 || ry > bya + 1) continue;
         int tc, ci, cj;
@@ -130,7 +135,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     __syncthreads();
 
     // ---- E - D with the dryness check (src/MorphodynamicRHS.f90:90-145) ---
-    for (int q = tid; q < MR1 * MR1; q += STAGE_NT) {
+    for (int q = tid; q < MR1 * MR1; q += MORPHO_NT) {
         const int rx = q % MR1 - 1, ry = q / MR1 - 1;
         if (rx > bxa || ry > bya) continue;
         int tc, ci, cj;
@@ -174,7 +179,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     const int NV1 = p.nx + 1;
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     const double psib = 1.0 - p.porosity;
-    for (int q = tid; q < MV * MV; q += STAGE_NT) {
+    for (int q = tid; q < MV * MV; q += MORPHO_NT) {
         const int a = q % MV, b = q / MV;
         if (a > bxa || b > bya) continue;
         const int vi = bx0 + a, vj = by0 + b;
@@ -273,7 +278,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
 // The grid is sized for the active set the host last saw; a block whose
 // share of the list holds more items takes them in turn.
 template <int S>
-__global__ __launch_bounds__(STAGE_NT) void morpho_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+__global__ __launch_bounds__(MORPHO_NT) void morpho_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
     for (int pass = 0;; ++pass) {
         // fresh argument pointers per pass: loads through them are not
         // hoisted out of the loop and held in registers across passes

@@ -30,6 +30,9 @@ struct Params {
     double vrate, vvalue, max_pack, porosity, diam, nu;
     double cfl, max_dt, hthr, diff_ts;
     double gred, ws0, nsettling, crit_shields;
+    // PouliquenStaticSlope (pmin + t1) / (1 - pmin t1), t1 = tan(1 degree),
+    // of GranularErosion (src/Closures.f90:626-627), computed on the host
+    double gran_stat;
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
@@ -227,14 +230,14 @@ __device__ inline double erosion_fluid(const Params& p, const CellState& c) {
     return 0.0;
 }
 __device__ inline double erosion_granular(const Params& p, const CellState& c) {
-    const double pi = 4.0 * atan(1.0);
     const double m2 = speed2(p.geometric, c.u, c.v, c.bx, c.by);
     const double mu1 = p.pmin;
-    const double t1 = tan(pi / 180.0);
-    const double stat = (mu1 + t1) / (1.0 - mu1 * t1);
+    const double stat = p.gran_stat;
     const double gcos = p.g / gamma_of(p.geometric, c.bx, c.by);
     const double mu = pouliquen_mu(p, gcos, c.hn, sqrt(m2));
-    const double mun = mu1 + (stat - mu1) / (1.0 + pow(c.hn / 25.0 / p.diam, 2.0));
+    // (Hn/25/d)**2.0_wp: the square, as gfortran evaluates a real power 2
+    const double r = c.hn / 25.0 / p.diam;
+    const double mun = mu1 + (stat - mu1) / (1.0 + r * r);
     if (mu > mun) {
         double e = p.ero_rate_granular * (mu - mun);
         return e * particle_speed(p, c);

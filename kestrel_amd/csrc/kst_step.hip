@@ -1913,6 +1913,10 @@ void fill_params(kst_handle* h) {
     p.nu = c.eddy_viscosity; p.cfl = c.cfl; p.max_dt = c.max_dt; p.hthr = c.height_threshold;
     p.diff_ts = c.diffusive_time_scale; p.gred = c.gred; p.ws0 = c.ws0;
     p.nsettling = c.nsettling; p.crit_shields = c.critical_shields;
+    // pi is 4 atan(1) (src/SetPrecision.f90:38)
+    const double pi = 4.0 * std::atan(1.0);
+    const double t1 = std::tan(pi / 180.0);
+    p.gran_stat = (p.pmin + t1) / (1.0 - p.pmin * t1);
 }
 
 int read_ctrl(kst_handle* h, Ctrl* c) {
@@ -2004,9 +2008,9 @@ int launch_iteration(kst_handle* h, bool profiled, bool small, int cap) {
         LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A, P);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
-        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(STAGE_NT), A, P);
-        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(STAGE_NT), A, P);
-        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(STAGE_NT), A, P);
+        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(MORPHO_NT), A, P);
+        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(MORPHO_NT), A, P);
+        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(MORPHO_NT), A, P);
         LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), A, P);
         LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(cap * TILE_PARTS), dim3(256), A, P);
         // second hydraulic operator

@@ -305,7 +305,10 @@ static double erosion_granular(const kst_config* c, const cstate* s) {
     double stat = (mu1 + t1) / (1.0 - mu1 * t1);
     double gcos = c->g / gamma_of(c, s->bx, s->by);
     double mu = pouliquen_mu(c, gcos, hn, sqrt(m2));
-    double mun = mu1 + (stat - mu1) / (1.0 + pow(hn / 25.0 / c->solid_diameter, 2.0));
+    /* (Hn/25/d)**2.0_wp (src/Closures.f90:631-632): gfortran evaluates a
+       real power 2 as the square */
+    double r = hn / 25.0 / c->solid_diameter;
+    double mun = mu1 + (stat - mu1) / (1.0 + r * r);
     if (mu > mun) {
         double e = c->ero_rate_granular * (mu - mun);
         return e * particle_speed(c, s);
