@@ -1966,10 +1966,11 @@ const StageSet& stage_set_of(bool visc, int limiter) {
     return sets[visc ? 1 : 0][(limiter >= 0 && limiter < 5) ? limiter : 1];
 }
 
-// Sub-blocks of 16 x 8 cells when the active set is small: twice the
-// workgroups, each with half the faces and its halo loaded in one round, so
-// the latency of a stage that cannot fill the chip roughly halves.  FP64 only.
-constexpr int BY_SMALL = B / 2;
+// Sub-blocks of 16 x 4 cells when the active set is small: four times the
+// workgroups, each with one face per thread and its cells and halo loaded in
+// one round, so a stage that cannot fill the chip has a shorter critical
+// path (16 x 8 measured between the two).  FP64 only.
+constexpr int BY_SMALL = B / 4;
 constexpr int SMALL_WORK = 128;       // sub-blocks of 16 x 16 below which the small shape is used
 
 const StageSet& stage_set(bool visc, int limiter, bool mixed, bool small) {
