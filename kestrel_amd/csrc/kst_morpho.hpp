@@ -374,11 +374,13 @@ __device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k
 // 280-330): returns 0 if no correction is needed, -1 if the cell borders no
 // depositional vertex, else 1 with the new values nv of the N depositional
 // vertices (di, dj) in vertex order.  x3/x0: the cell's vertex bt now and in
-// intermed0; btc3/w3: its bt and w now.
+// intermed0; btc3/w3: its bt and w now.  *delta_out, when given, receives
+// the factor: vertex k with b_k = x3[k] - x0[k] > 0 moves to x3[k] - delta b_k.
 __device__ __forceinline__ int vertex_correction(const Params& p, int i, int j, const double* x3,
                                                  const double* x0, double gamold, double btc0,
                                                  double btc3, double w0, double h0, double b0c,
-                                                 double w3, double* nv, int* di, int* dj, int& N) {
+                                                 double w3, double* nv, int* di, int* dj, int& N,
+                                                 double* delta_out = nullptr) {
     const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
     const double ho_raw = compute_hn(w0, b0c, btc0, gamold);
     const double db = btc3 - btc0;
@@ -440,6 +442,7 @@ __device__ __forceinline__ int vertex_correction(const Params& p, int i, int j, 
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         if (q < N) nv[q] = nv[q] - delta * bd[q];
+    if (delta_out) *delta_out = delta;
     return 1;
 }
 
@@ -534,6 +537,11 @@ constexpr int RD_LDS = 2048;          // candidates sorted and levelled in LDS
 // Neighbour k (0..7) of a cell at offset (kDI[k], kDJ[k]).
 __constant__ const int kDI[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ const int kDJ[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+// Vertex k = a + 2b of a cell (a, b in 0..1) is shared by the neighbours at
+// offsets (a - 1 .. a, b - 1 .. b) other than the cell: their indices m in
+// the list above, and the index of the vertex among each one's own.
+constexpr int kNbM[4][3] = {{0, 1, 3}, {1, 2, 4}, {3, 5, 6}, {4, 6, 7}};
+constexpr int kNbV[4][3] = {{3, 2, 1}, {3, 2, 0}, {3, 1, 0}, {2, 1, 0}};
 
 template <class DevT>
 __device__ void redistribute_small(const DevT& d, const Params& p, int n, double* lx, long long* lk,
@@ -644,29 +652,39 @@ __device__ void redistribute_small(const DevT& d, const Params& p, int n, double
                     btc3 = r.btc;
                     w3 = r.w;
                 }
-                double nv[4];
+                double nv[4], delta;
                 int di[4], dj[4], N;
                 const int out = vertex_correction(p, i, j, x3, x0, gamold, btc0, btc3, w0, h0, b0c,
-                                                  w3, nv, di, dj, N);
+                                                  w3, nv, di, dj, N, &delta);
                 if (out < 0) {
                     res[tid] = 2;
                     atomicMin(&s_stop, tid);
                 } else if (out > 0) {
                     res[tid] = 1;
+                    // the depositional vertices k (in vertex order, as
+                    // vertex_correction compacts them) move; the later
+                    // candidates around each take the new value into their
+                    // copy: vertex k of this cell is vertex kNbV[k][u] of
+                    // neighbour kNbM[k][u]
                     int mask = 0;
-                    for (int q = 0; q < N; ++q) {
-                        const int a = di[q] - i, b = dj[q] - j;     // vertex (a, b) of the cell
-                        mask |= 1 << (a + 2 * b);
-                        slot[tid][a + 2 * b] = nv[q];
 #pragma unroll
-                        for (int m = 0; m < 8; ++m) {
-                            const int r = nb[m];
-                            if (r <= tid) continue;
-                            rec[r] = 1;
-                            const int a2 = a - kDI[m], b2 = b - kDJ[m];
-                            if (a2 >= 0 && a2 <= 1 && b2 >= 0 && b2 <= 1) slot[r][a2 + 2 * b2] = nv[q];
+                    for (int k = 0; k < 4; ++k) {
+                        const double bq = x3[k] - x0[k];
+                        if (bq > 0.0) {
+                            const double v = x3[k] - delta * bq;
+                            mask |= 1 << k;
+                            slot[tid][k] = v;
+#pragma unroll
+                            for (int u = 0; u < 3; ++u) {
+                                const int r = nb[kNbM[k][u]];
+                                if (r > tid) slot[r][kNbV[k][u]] = v;
+                            }
                         }
                     }
+                    // the 3x3 update recomputes every later neighbour's bt and w
+#pragma unroll
+                    for (int m = 0; m < 8; ++m)
+                        if (nb[m] > tid) rec[nb[m]] = 1;
                     dmask[tid] = (unsigned char)mask;
                 }
                 lev[tid] = round;
