@@ -921,7 +921,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
 
 // ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),
 // plus its CFL numerators and prefactors, in (tile, part) items.
-constexpr int TILE_PARTS = 4;
+constexpr int TILE_PARTS = 8;
 
 __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
     const auto& d = const_arg(dp);
@@ -943,10 +943,10 @@ __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ 
     // (tile, part) items, the grid sized for the active set the host last saw
     for (int it = blockIdx.x; it < sh_n; it += gridDim.x) {
         const int t = d.active[it / TILE_PARTS], part = it % TILE_PARTS;
-        // a face's static terms read only that face's topography, which the
-        // same thread computes, and cell gamma, which the morphodynamic stage wrote
-        tile_face_topo(d, p, d.TB, t, part, TILE_PARTS);
-        tile_static_terms(d, p, d.TB, t, part, TILE_PARTS);
+        // a face's static terms read only that face's topography, computed in
+        // the same pass, and cell gamma, which the morphodynamic stage wrote
+        tile_face_topo<true>(d, p, d.TB, t, part, TILE_PARTS);
+        tile_prefactors(d, p, d.TB, t, part, TILE_PARTS);
     }
 }
 

@@ -1185,7 +1185,9 @@ __device__ void tile_cell_topo(const DevT& d, const Params& p, const TopoT& T, i
 }
 
 // part/nparts: the share of the tile's faces one block of several computes.
-template <class DevT, class TopoT>
+// With STATIC the faces' CFL numerators (tile_static_terms) are computed in
+// the same pass; they read the cell gamma of the tiles on both sides.
+template <bool STATIC = false, class DevT, class TopoT>
 __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, int t, int part = 0,
                                int nparts = 1) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
@@ -1204,6 +1206,11 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
     // x-faces: i in [0, nx], j in [0, ny)
     for (int f = lane0; f < (nx + 1) * ny; f += stride) {
         const int i = f % (nx + 1), j = f / (nx + 1);
+        double gl = 0.0, gr_ = 0.0;
+        if (STATIC) {
+            gl = (i == 0) ? T.cgam[tW * p.NC + j * nx + nx - 1] : T.cgam[t * p.NC + j * nx + i - 1];
+            gr_ = (i == nx) ? T.cgam[tE * p.NC + j * nx] : T.cgam[t * p.NC + j * nx + i];
+        }
         Kahan kx;
         if (i >= 1 && i <= nx - 1) {
             kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1));
@@ -1225,15 +1232,26 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
         ib = ib + V(b, i, j + 1);
         ib = ib * 0.5;
         const size_t fi = (size_t)t * p.NFX + f;
+        const double fg = gamma_of(p.geometric, dbdx, dbdy);
         d.xb0[fi] = 0.5 * (V(b, i, j) + V(b, i, j + 1));
         T.xbt[fi] = btf;
         T.xby[fi] = dbdy;
-        T.xgam[fi] = gamma_of(p.geometric, dbdx, dbdy);
+        T.xgam[fi] = fg;
         T.xib[fi] = ib;
+        if (STATIC) {
+            const double a = fmin(gl / fg, 1.0), c = fmin(gr_ / fg, 1.0);
+            T.xnp[fi] = a * a * p.dx;
+            T.xnm[fi] = c * c * p.dx;
+        }
     }
     // y-faces: i in [0, nx), j in [0, ny]
     for (int f = lane0; f < nx * (ny + 1); f += stride) {
         const int i = f % nx, j = f / nx;
+        double gl = 0.0, gr_ = 0.0;
+        if (STATIC) {
+            gl = (j == 0) ? T.cgam[tS * p.NC + (ny - 1) * nx + i] : T.cgam[t * p.NC + (j - 1) * nx + i];
+            gr_ = (j == ny) ? T.cgam[tN * p.NC + i] : T.cgam[t * p.NC + j * nx + i];
+        }
         Kahan kx;
         kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
         const double dbdx = p.dxr * kx.s;
@@ -1255,11 +1273,17 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
         ib = ib + V(b, i + 1, j);
         ib = ib * 0.5;
         const size_t fi = (size_t)t * p.NFY + f;
+        const double fg = gamma_of(p.geometric, dbdx, dbdy);
         d.yb0[fi] = 0.5 * (V(b, i, j) + V(b, i + 1, j));
         T.ybt[fi] = btf;
         T.ybx[fi] = dbdx;
-        T.ygam[fi] = gamma_of(p.geometric, dbdx, dbdy);
+        T.ygam[fi] = fg;
         T.yib[fi] = ib;
+        if (STATIC) {
+            const double a = fmin(gl / fg, 1.0), c = fmin(gr_ / fg, 1.0);
+            T.ynp[fi] = a * a * p.dy;
+            T.ynm[fi] = c * c * p.dy;
+        }
     }
 }
 #undef V
@@ -1270,13 +1294,9 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
 // prefactors of ConstructHydraulicRHS (src/HydraulicRHS.f90:1233-1243).
 // Needs the cell gamma of the neighbour tiles.
 template <class DevT, class TopoT>
-__device__ void tile_static_terms(const DevT& d, const Params& p, const TopoT& T, int t, int part = 0,
-                                  int nparts = 1) {
+__device__ void tile_prefactors(const DevT& d, const Params& p, const TopoT& T, int t, int part = 0,
+                                int nparts = 1) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
-    const int nx = p.nx, ny = p.ny;
-    const int ti = t % p.ntx, tj = t / p.ntx;
-    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
-    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
     for (int c = lane0; c < p.NC; c += stride) {
         const int gi = t * p.NC + c;
         const double bx = T.cbx[gi], by = T.cby[gi], gam = T.cgam[gi];
@@ -1288,6 +1308,17 @@ __device__ void tile_static_terms(const DevT& d, const Params& p, const TopoT& T
             T.pxu[gi] = 1.0; T.pxv[gi] = 0.0; T.pyv[gi] = 1.0;
         }
     }
+}
+
+template <class DevT, class TopoT>
+__device__ void tile_static_terms(const DevT& d, const Params& p, const TopoT& T, int t, int part = 0,
+                                  int nparts = 1) {
+    const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
+    const int nx = p.nx, ny = p.ny;
+    const int ti = t % p.ntx, tj = t / p.ntx;
+    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
+    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
+    tile_prefactors(d, p, T, t, part, nparts);
     for (int f = lane0; f < (nx + 1) * ny; f += stride) {
         const int i = f % (nx + 1), j = f / (nx + 1);
         const double fg = T.xgam[t * p.NFX + f];
