@@ -341,13 +341,38 @@ __device__ __forceinline__ void map_cell(const Params& p, int t, int i, int j, i
     else if (j >= p.ny) { tile += p.ntx; cj -= p.ny; }
 }
 
+// The face topography a face's flux reads: db/d(along), gamma, b0 and bt of
+// the face, and the CFL numerators of its two cells.
+struct FaceTopo {
+    double bp, g, b0, bt, np, nm;
+};
+
+template <bool YDIR, bool CFL, typename Smem, class DevT>
+__device__ __forceinline__ FaceTopo load_face_topo(const DevT& d, const Params& p, const Smem& s,
+                                                   int fi, int fj) {
+    const int t = s.tile;
+    const int fpos = (YDIR ? s.by0 : s.bx0) + fi;
+    FaceTopo f;
+    int fidx;
+    if (!YDIR) {
+        fidx = t * p.NFX + (s.by0 + fj) * (p.nx + 1) + fpos;
+        f.bp = d.T.xby[fidx]; f.g = d.T.xgam[fidx]; f.b0 = d.xb0[fidx]; f.bt = d.T.xbt[fidx];
+    } else {
+        fidx = t * p.NFY + fpos * p.nx + (s.bx0 + fj);
+        f.bp = d.T.ybx[fidx]; f.g = d.T.ygam[fidx]; f.b0 = d.yb0[fidx]; f.bt = d.T.ybt[fidx];
+    }
+    f.np = CFL ? (YDIR ? d.T.ynp : d.T.xnp)[fidx] : 0.0;
+    f.nm = CFL ? (YDIR ? d.T.ynm : d.T.xnm)[fidx] : 0.0;
+    return f;
+}
+
 // Fused work of one x- or y-face (CalculateLimitedDerivs + Reconstruct +
 // CorrectSlopes + CalculateFluxes, src/HydraulicRHS.f90:181-1182).
 // Along the face normal the four cells are c[-2], L=c[-1], R=c[0], c[1];
 // fi is the face position along the normal, fj the row (x) / column (y).
 template <bool YDIR, bool VISC, int LIM, bool CFL, typename Smem, class DevT>
 __device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& s, int fi, int fj,
-                                          double& cfl_min) {
+                                          double& cfl_min, const FaceTopo& ft) {
     int iLL, iL, iR, iRR;
     if (!YDIR) { iLL = rg(fi - 2, fj); iL = rg(fi - 1, fj); iR = rg(fi, fj); iRR = rg(fi + 1, fj); }
     else       { iLL = rg(fj, fi - 2); iL = rg(fj, fi - 1); iR = rg(fj, fi); iRR = rg(fj, fi + 1); }
@@ -416,16 +441,7 @@ __device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& 
     const double rFL = rL + sLr * 0.5 * dx, rFR = rR - sRr * 0.5 * dx;
 
     // face topography of this tile
-    const int t = s.tile;
-    int fidx;
-    double fbp, fg, fb0, fbt;
-    if (!YDIR) {
-        fidx = t * p.NFX + (s.by0 + fj) * (p.nx + 1) + fpos;
-        fbp = d.T.xby[fidx]; fg = d.T.xgam[fidx]; fb0 = d.xb0[fidx]; fbt = d.T.xbt[fidx];
-    } else {
-        fidx = t * p.NFY + fpos * p.nx + (s.bx0 + fj);
-        fbp = d.T.ybx[fidx]; fg = d.T.ygam[fidx]; fb0 = d.yb0[fidx]; fbt = d.T.ybt[fidx];
-    }
+    const double fbp = ft.bp, fg = ft.g, fb0 = ft.b0, fbt = ft.bt;
     // face Hn from the corrected w (src/HydraulicRHS.f90:492-517)
     const double HnL = compute_hn(wFL, fb0, fbt, fg);
     const double HnR = compute_hn(wFR, fb0, fbt, fg);
@@ -444,8 +460,8 @@ __device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& 
     if (aNeg > 0.0) aNeg = 0.0;
     if (CFL) {
         // numerators (min(gam_cell/gam_face,1))^2 * dx are static topography
-        if (aPos > kEps) cfl_min = fmin((YDIR ? d.T.ynp : d.T.xnp)[fidx] / aPos, cfl_min);
-        if (fabs(aNeg) > kEps) cfl_min = fmin((YDIR ? d.T.ynm : d.T.xnm)[fidx] / fabs(aNeg), cfl_min);
+        if (aPos > kEps) cfl_min = fmin(ft.np / aPos, cfl_min);
+        if (fabs(aNeg) > kEps) cfl_min = fmin(ft.nm / fabs(aNeg), cfl_min);
     }
 
     // --- fluxes (src/HydraulicRHS.f90:1027-1067, src/Equations.f90:53-245)
@@ -690,11 +706,16 @@ __device__ __forceinline__ double face_bed(const DevT& d, const Params& p, int t
 // STAGE 3: RHS(u2); u3 -> S3.                   (src/TimeStepper.f90:447-498)
 // STAGE 4: RHS(u3); implicit drag -> Sout; maxima; activation flags.
 //                                               (src/TimeStepper.f90:501-524)
+// pre: the face's topography when it was loaded ahead (FP64 only).
 template <bool YDIR, bool VISC, int LIM, bool CFL, bool MIX, typename Smem, class DevT>
 __device__ __forceinline__ void face_work(const DevT& d, const Params& p, Smem& s, int fi, int fj,
-                                          double& cfl_min) {
-    if constexpr (MIX) face_flux_mixed<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
-    else face_flux<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
+                                          double& cfl_min, const FaceTopo* pre = nullptr) {
+    if constexpr (MIX) {
+        face_flux_mixed<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min);
+    } else {
+        const FaceTopo ft = pre ? *pre : load_face_topo<YDIR, CFL>(d, p, s, fi, fj);
+        face_flux<YDIR, VISC, LIM, CFL>(d, p, s, fi, fj, cfl_min, ft);
+    }
 }
 
 // Flux q of face slot i as FP64.
@@ -883,6 +904,27 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             ob0 = b0c; obt = btc; og = gam; oidx = gi;
         }
     }
+    // A sub-block with at most one face per thread (the small shape) loads
+    // ahead what phases B and C read from global memory -- its face's
+    // topography, its cell's bed slopes and prefactors (and u^n in stage 3)
+    // -- so that those loads overlap this phase's instead of following it.
+    constexpr bool PRE = !MIX && Smem::NFX + Smem::NFY <= STAGE_NT;
+    const bool pre = PRE && bxa == B && bya == BY;
+    FaceTopo pft{};
+    double pbx = 0, pby = 0, pgxu = 0, pgxv = 0, pgyv = 0, pw0 = 0, pu0 = 0, pv0 = 0, ph0 = 0;
+    if (pre) {
+        if (tid < Smem::NFX) {
+            pft = load_face_topo<false, STAGE != 4>(d, p, s, tid % (B + 1), tid / (B + 1));
+        } else if (tid < Smem::NFX + Smem::NFY) {
+            const int g = tid - Smem::NFX;
+            pft = load_face_topo<true, STAGE != 4>(d, p, s, g / B, g % B);
+        }
+        if (own) {
+            pbx = d.T.cbx[oidx]; pby = d.T.cby[oidx];
+            pgxu = d.T.pxu[oidx]; pgxv = d.T.pxv[oidx]; pgyv = d.T.pyv[oidx];
+            if (STAGE == 3) { pw0 = d.Uw[oidx]; pu0 = d.Uu[oidx]; pv0 = d.Uv[oidx]; ph0 = d.Uh[oidx]; }
+        }
+    }
     // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
     for (int q = tid; q < B * (B + 3); q += STAGE_NT) {
         const int r = q / (B + 3), k = q % (B + 3) - 1;
@@ -943,6 +985,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             const int f = 256 + tid - 64;
             face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
         }
+    } else if (pre) {
+        if (tid < Smem::NFX) {
+            face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, tid % (B + 1), tid / (B + 1), cflm, &pft);
+        } else if (tid < Smem::NFX + Smem::NFY) {
+            const int g = tid - Smem::NFX;
+            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, g / B, g % B, cflm, &pft);
+        }
     } else if (bxa == B && bya == BY) {
         // x- and y-faces as one list over the block
         for (int f = tid; f < Smem::NFX + Smem::NFY; f += STAGE_NT) {
@@ -971,9 +1020,10 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
 #define FX(q, o) fluxv<MIX, false>(s, q, xs + (o))
 #define FY(q, o) fluxv<MIX, true>(s, q, ys + (o))
     const double gam = og;
-    const double bx = d.T.cbx[oidx], by = d.T.cby[oidx];
+    const double bx = pre ? pbx : d.T.cbx[oidx], by = pre ? pby : d.T.cby[oidx];
     // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are per-bed topography
-    const double gxu = d.T.pxu[oidx], gxv = d.T.pxv[oidx], gyu = gxv, gyv = d.T.pyv[oidx];
+    const double gxu = pre ? pgxu : d.T.pxu[oidx], gxv = pre ? pgxv : d.T.pxv[oidx], gyu = gxv;
+    const double gyv = pre ? pgyv : d.T.pyv[oidx];
     // ConstructHydraulicRHS (src/HydraulicRHS.f90:1228-1271)
     const double stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
     const double sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
@@ -1066,8 +1116,8 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         }
     } else if (STAGE == 3) {
         // third SSP substep (src/TimeStepper.f90:466-498)
-        const double w0 = MIX ? pw : d.Uw[oidx], u0 = MIX ? pru : d.Uu[oidx], v0 = MIX ? prv : d.Uv[oidx];
-        const double h0 = MIX ? ph : d.Uh[oidx];
+        const double w0 = MIX ? pw : (pre ? pw0 : d.Uw[oidx]), u0 = MIX ? pru : (pre ? pu0 : d.Uu[oidx]);
+        const double v0 = MIX ? prv : (pre ? pv0 : d.Uv[oidx]), h0 = MIX ? ph : (pre ? ph0 : d.Uh[oidx]);
         const double third = 1.0 / 3.0, twothird = 2.0 / 3.0;
         const double u3 = third * u0 + twothird * (oru + dt * Eu) / (1.0 - dt * Im);
         const double v3 = third * v0 + twothird * (orv + dt * Ev) / (1.0 - dt * Im);
