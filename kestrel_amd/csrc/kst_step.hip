@@ -825,6 +825,23 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     // the extra faces of phase B instead)
     constexpr int OWN = B * BY;
     constexpr bool one_round = STAGE_NT >= OWN + 128;
+    // A sub-block with at most one face per thread (the small shape) issues
+    // its other global loads ahead of the state's so that they overlap: the
+    // CorrectSlopes beds of its faces here (kept in a register until the
+    // state is staged), below what phases B and C read.
+    constexpr bool PRE = !MIX && Smem::NFX + Smem::NFY <= STAGE_NT;
+    const bool pre = PRE && bxa == B && bya == BY;
+    constexpr int NXB = BY * (B + 3), NYB = B * (BY + 3);   // x-face and y-face beds
+    static_assert(!PRE || NXB + NYB <= STAGE_NT, "one bed per thread");
+    double bedv = 0.0;
+    if (pre) {
+        if (tid < NXB) {
+            bedv = face_bed<false>(d, p, t, s.bx0 + tid % (B + 3) - 1, s.by0 + tid / (B + 3));
+        } else if (tid < NXB + NYB) {
+            const int q = tid - NXB;
+            bedv = face_bed<true>(d, p, t, s.by0 + q % (BY + 3) - 1, s.bx0 + q / (BY + 3));
+        }
+    }
     for (int round = 0; round < (one_round ? 1 : 2); ++round) {
         int rx, ry;
         bool valid;
@@ -904,12 +921,8 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             ob0 = b0c; obt = btc; og = gam; oidx = gi;
         }
     }
-    // A sub-block with at most one face per thread (the small shape) loads
-    // ahead what phases B and C read from global memory -- its face's
+    // the small shape's loads of what phases B and C read: its face's
     // topography, its cell's bed slopes and prefactors (and u^n in stage 3)
-    // -- so that those loads overlap this phase's instead of following it.
-    constexpr bool PRE = !MIX && Smem::NFX + Smem::NFY <= STAGE_NT;
-    const bool pre = PRE && bxa == B && bya == BY;
     FaceTopo pft{};
     double pbx = 0, pby = 0, pgxu = 0, pgxv = 0, pgyv = 0, pw0 = 0, pu0 = 0, pv0 = 0, ph0 = 0;
     if (pre) {
@@ -926,10 +939,15 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         }
     }
     // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
-    for (int q = tid; q < B * (B + 3); q += STAGE_NT) {
-        const int r = q / (B + 3), k = q % (B + 3) - 1;
-        if (r < bya && k <= bxa + 1) s.xbed[r][k + 1] = face_bed<false>(d, p, t, s.bx0 + k, s.by0 + r);
-        if (r < bxa && k <= bya + 1) s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
+    if (pre) {
+        if (tid < NXB) s.xbed[tid / (B + 3)][tid % (B + 3)] = bedv;
+        else if (tid < NXB + NYB) s.ybed[(tid - NXB) / (BY + 3)][(tid - NXB) % (BY + 3)] = bedv;
+    } else {
+        for (int q = tid; q < B * (B + 3); q += STAGE_NT) {
+            const int r = q / (B + 3), k = q % (B + 3) - 1;
+            if (r < bya && k <= bxa + 1) s.xbed[r][k + 1] = face_bed<false>(d, p, t, s.bx0 + k, s.by0 + r);
+            if (r < bxa && k <= bya + 1) s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
+        }
     }
     // the neighbour tiles' own beds at the shared edge faces; in the last
     // stage a ghost neighbour's copy of the shared vertices is the one the
