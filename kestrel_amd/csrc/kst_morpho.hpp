@@ -290,11 +290,14 @@ __global__ __launch_bounds__(MORPHO_NT) void morpho_kernel(const Dev* __restrict
     }
 }
 
-// A write into a ghost tile's part of intermed3, listed for the next step's refresh.
+// A write into a ghost tile's part of intermed3, listed for the next step's
+// refresh.  Only the redistribution workgroup writes the list: it counts the
+// entries in LDS (nd, loaded from and stored back to Ctrl::ndirty by the
+// kernel), since a returning device-scope atomic per entry would put a
+// device round trip on the critical path of every phase that writes one.
 template <class DevT>
-__device__ __forceinline__ void note_dirty(const DevT& d, long long e) {
-    Ctrl* c = d.ctrl;
-    const unsigned long long q = atomicAdd((unsigned long long*)&c->ndirty, 1ULL);
+__device__ __forceinline__ void note_dirty(const DevT& d, long long e, unsigned long long* nd) {
+    const unsigned long long q = atomicAdd(nd, 1ULL);
     if (q < (unsigned long long)d.dirty_cap) d.dirty[q] = e;
 }
 
@@ -342,7 +345,8 @@ __device__ __forceinline__ CellRe recompute_cell(const Params& p, double t00, do
 // With write = false the copies are only counted.
 template <class DevT>
 __device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k, int di, int dj,
-                                           double val, unsigned long long* ghost, bool write = true) {
+                                           double val, unsigned long long* ghost,
+                                           unsigned long long* nd, bool write = true) {
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     double* bt3 = d.TB.vbt;
@@ -350,7 +354,7 @@ __device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k
         const size_t v = (size_t)tile * NV + jj * NV1 + ii;
         if (write) bt3[v] = val;
         if (d.tstate[tile] != KST_TILE_ACTIVE) {
-            if (write) note_dirty(d, (long long)v | (1LL << 62));
+            if (write) note_dirty(d, (long long)v | (1LL << 62), nd);
             ++*ghost;
         }
     };
@@ -462,7 +466,7 @@ __device__ __forceinline__ int vertex_correction(const Params& p, int i, int j, 
 // *ghost counts writes into ghost tiles.
 template <class DevT>
 __device__ int redistribute_cell(const DevT& d, const Params& p, int k, int i, int j, int lane,
-                                 unsigned long long* ghost) {
+                                 unsigned long long* ghost, unsigned long long* nd) {
     const long long FS = d.FS;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
@@ -502,7 +506,7 @@ __device__ int redistribute_cell(const DevT& d, const Params& p, int k, int i, i
     const int res = vertex_correction(p, i, j, x3, x0, gamold, btc0, btc3, w0, h0, b0c, w3, nv, di, dj, N);
     if (res <= 0) return res;
     if (lane == 0)
-        for (int q = 0; q < N; ++q) put_vertex(d, p, k, di[q], dj[q], nv[q], ghost);
+        for (int q = 0; q < N; ++q) put_vertex(d, p, k, di[q], dj[q], nv[q], ghost, nd);
     if (lane >= 9) return 1;
     // ---- the lane's cell: vertex copies of tile ct after the correction
     const int kx0 = (k % p.ntx) * nx, ky0 = (k / p.ntx) * ny;
@@ -524,7 +528,7 @@ __device__ int redistribute_cell(const DevT& d, const Params& p, int k, int i, i
     } else {
         d.S3[g] = r.w;
         d.S3[3 * FS + g] = r.hp;
-        note_dirty(d, (long long)g);
+        note_dirty(d, (long long)g, nd);
         ++*ghost;
     }
     return 1;
@@ -545,7 +549,7 @@ constexpr int kNbV[4][3] = {{3, 2, 1}, {3, 2, 0}, {3, 1, 0}, {2, 1, 0}};
 
 template <class DevT>
 __device__ void redistribute_small(const DevT& d, const Params& p, int n, double* lx, long long* lk,
-                                   int* lev, int* cellid) {
+                                   int* lev, int* cellid, unsigned long long* nd) {
     __shared__ double slot[RD_THREADS][4];     // vertex bt of each candidate's cell
     __shared__ unsigned char rec[RD_THREADS];  // its bt and w were recomputed by a correction
     __shared__ unsigned char res[RD_THREADS];  // 0 none, 1 corrected, 2 no depositional vertex
@@ -710,7 +714,7 @@ __device__ void redistribute_small(const DevT& d, const Params& p, int n, double
                 const int a2 = a - kDI[m], b2 = b - kDJ[m];
                 if (a2 >= 0 && a2 <= 1 && b2 >= 0 && b2 <= 1 && (dmask[r] >> (a2 + 2 * b2)) & 1) last = false;
             }
-            put_vertex(d, p, k, i + a, j + b, slot[tid][q], &ghost, last);
+            put_vertex(d, p, k, i + a, j + b, slot[tid][q], &ghost, nd, last);
         }
     }
     __syncthreads();
@@ -737,7 +741,7 @@ __device__ void redistribute_small(const DevT& d, const Params& p, int n, double
         } else {
             d.S3[gcl] = r.w;
             d.S3[3 * FS + gcl] = r.hp;
-            note_dirty(d, (long long)gcl);
+            note_dirty(d, (long long)gcl, nd);
             ++ghost;
         }
     }
@@ -781,7 +785,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
     const auto& d = const_arg(dp);
     const Params& p = *pp;
     __shared__ int sh_n, sh_skip, sh_fail, sh_maxlev, sh_cnt, sh_changed;
-    __shared__ unsigned long long sh_done, sh_ghost;
+    __shared__ unsigned long long sh_done, sh_ghost, sh_ndirty;
     __shared__ double lx[RD_LDS];
     __shared__ long long lk[RD_LDS];
     __shared__ int lev[RD_LDS];
@@ -801,13 +805,16 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
         sh_fail = 0;
         sh_done = 0;
         sh_ghost = 0;
+        sh_ndirty = c->ndirty;
         sh_maxlev = 0;
     }
     __syncthreads();
     if (sh_skip) return;
     const int n = min(sh_n, d.cand_cap);
     if (n <= RD_THREADS) {
-        redistribute_small(d, p, n, lx, lk, lev, list);
+        redistribute_small(d, p, n, lx, lk, lev, list, &sh_ndirty);
+        __syncthreads();
+        if (tid == 0) c->ndirty = (long long)sh_ndirty;
         return;
     }
     int m2 = 1;
@@ -900,7 +907,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
         for (int r = wave; r < cnt; r += RD_WAVES) {
             const int g = cell_of(L[r]);
             const int k = g / p.NC, cl = g % p.NC;
-            const int res = redistribute_cell(d, p, k, cl % nx, cl / nx, lane, &ghost);
+            const int res = redistribute_cell(d, p, k, cl % nx, cl / nx, lane, &ghost, &sh_ndirty);
             if (lane == 0) {
                 if (res < 0) sh_fail = 1;
                 else if (res > 0) ++done;
@@ -915,6 +922,7 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
     if (tid == 0) {
         c->redist_cells += (long long)sh_done;
         c->redist_ghost += (long long)sh_ghost;
+        c->ndirty = (long long)sh_ndirty;
         if (sh_fail) c->mref |= 2;
     }
 }
