@@ -303,7 +303,45 @@ __device__ inline double morpho_damping(const Params& p, double hn) {
 
 // ErosionDepositionTerms, Erosion, Deposition (src/Equations.f90:385-448):
 // returns E - D.
+// E - D for the reference's default laws (Mixed erosion, Spearman-Manning
+// deposition, smooth erosion transition, tanh damping, tanh switch) in
+// straight-line code: the same operations as the general form below, with
+// its data-dependent branches as selects, so that the independent
+// transcendental chains (three tanh, a log, two pow, square roots) overlap
+// instead of running one after another.
+__device__ __forceinline__ double emd_default_laws(const Params& p, const CellState& c) {
+    const double gam = gamma_of(p.geometric, c.bx, c.by);
+    const double m2 = speed2(p.geometric, c.u, c.v, c.bx, c.by);
+    const double gred = p.gred / gam;
+    const double ps = sqrt(gred * p.diam);                          // FlowParticleSpeed
+    // fluid erosion (FlowShieldsNumber)
+    const double sh = (p.chezy * m2) / (gred * p.diam);
+    const double efl = (sh > p.crit_shields) ? (p.ero_rate * (sh - p.crit_shields)) * ps : 0.0;
+    // granular erosion (PouliquenFrictionCoefficient at g cos = g / gamma)
+    const double gcos = p.g / gam;
+    const double fr = sqrt(m2) / sqrt(gcos * c.hn);
+    const double I = fr * p.diam / c.hn;
+    const double mu = (c.hn > p.hthr) ? p.pmin + (p.pmax - p.pmin) * I / (p.pbeta + I) : p.pmin;
+    const double r = c.hn / 25.0 / p.diam;
+    const double mun = p.pmin + (p.gran_stat - p.pmin) / (1.0 + r * r);
+    const double egr = (mu > mun) ? (p.ero_rate_granular * (mu - mun)) * ps : 0.0;
+    const double fc = 0.5 * (1.0 + tanh(p.vrate * (c.psi - p.vvalue)));
+    double e = (1.0 - fc) * efl + fc * egr;
+    e = e * (0.5 * (1.0 + tanh(1e5 * (c.bt + p.ero_depth))));
+    const double a = 2.7 - 0.15 * p.nsettling;
+    const double b = 0.62 * p.nsettling - 1.46;
+    const double sm = c.psi * pow(1.0 - c.psi, a) * pow(1.0 - c.psi / p.max_pack, b);
+    const double alpha = (c.psi >= p.max_pack) ? 0.0 : ((c.psi > 0.0) ? sm : 0.0);
+    const double dep = p.ws0 * alpha;
+    const double damp = 0.5 * (1.0 + tanh(10.0 * log(c.hn / p.ero_hcrit)));
+    return e * damp - dep * damp;
+}
+
 __device__ inline double erosion_minus_deposition(const Params& p, const CellState& c) {
+    if (p.erosion == KST_EROSION_MIXED && p.deposition == KST_DEPOSITION_SPEARMAN_MANNING &&
+        p.ero_transition == KST_TRANSITION_SMOOTH && p.morpho_damping == KST_DAMPING_TANH &&
+        p.fswitch == KST_SWITCH_TANH)
+        return emd_default_laws(p, c);
     double e = erosion_closure(p, c);
     e = e * erosion_transition(p, c.bt);
     double alpha;
