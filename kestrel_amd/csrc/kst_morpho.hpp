@@ -17,11 +17,16 @@ namespace kst {
 
 constexpr int MR1 = B + 2;   // sub-block plus a 1-cell halo
 constexpr int MV = B + 1;    // vertices along a sub-block edge
-// Threads of a morphodynamic block: enough for every phase (Hn on the 20 x 20
-// region, E - D on 18 x 18, 17 x 17 vertices, 16 x 16 cells) to take one
-// round, so the transcendental-heavy E - D is evaluated once on the critical path.
-constexpr int MORPHO_NT = 512;
-static_assert(MORPHO_NT >= NREG && MORPHO_NT >= MR1 * MR1, "one round per phase");
+// Threads of a morphodynamic block of 16 x BY cells: enough for every phase
+// (Hn on the 20 x (BY+4) region, E - D on 18 x (BY+2), 17 x (BY+1) vertices,
+// 16 x BY cells) to take one round, so the transcendental-heavy E - D is
+// evaluated once on the critical path.  Small active sets use BY = 4: the
+// E - D of a 16 x 16 block is ~6 waves of FP64 work on one CU, of a 16 x 4
+// block 2 waves on two SIMDs, and the chip has idle CUs for the extra blocks.
+template <int BY>
+constexpr int morpho_nt() { return BY == B ? 512 : 256; }
+static_assert(morpho_nt<B>() >= NREG && morpho_nt<B>() >= MR1 * MR1, "one round per phase");
+static_assert(morpho_nt<B / 4>() >= RW * (B / 4 + 4), "one round per phase");
 
 // Thread 0: whether the morphodynamic operator runs in this attempt (the
 // first hydraulic operator passed its re-checks), its dt, and the block's
@@ -31,7 +36,7 @@ struct MWork {
     double dt;
 };
 
-template <class DevT>
+template <int BY, class DevT>
 __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw,
                             int pass) {
     const auto* c = ctrl_view(d);
@@ -43,7 +48,7 @@ __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_
         if (need_no_refine && c->mref) skip = 1;
     }
     if (!skip) count_work(d.ctrl, kw);
-    const int sbx = (p.nx + B - 1) / B, sby = (p.ny + B - 1) / B, spt = sbx * sby;
+    const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
     const int nwork = c->n_active * spt;
     bool more = false;
     const int w = xcd_item(nwork, pass, &more);
@@ -54,9 +59,9 @@ __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_
         m.tile = d.active[m.q];
         const int sub = w % spt;
         m.bx0 = (sub % sbx) * B;
-        m.by0 = (sub / sbx) * B;
+        m.by0 = (sub / sbx) * BY;
         m.bxa = min(B, p.nx - m.bx0);
-        m.bya = min(B, p.ny - m.by0);
+        m.bya = min(BY, p.ny - m.by0);
     }
     m.skip = skip;
     m.dt = dt;
@@ -86,26 +91,28 @@ __device__ __forceinline__ double excess_deposition(const DevT& d, const Params&
 }
 
 // One SSP stage S of MorphodynamicTimeStepper (src/TimeStepper.f90:562-710)
-// on one 16x16 sub-block:
+// on one 16 x BY sub-block:
 //   E - D of intermed_{S-1} (CalculateMorphodynamicRHS, src/MorphodynamicRHS.f90:72-149)
 //   vertex rate (BtSourceTerm, src/MorphodynamicRHS.f90:154-305)
 //   bt_S = max(-EroDepth, SSP combination) at this tile's vertices
 //   cell topography (ComputeCellCentredTopographicData, :308-360)
 //   w, Hnpsi from the change of bt (src/TimeStepper.f90:587-610)
 // Stage 3 then runs the checks of src/TimeStepper.f90:713-752.
-template <int S, class DevT>
+template <int S, int BY, class DevT>
 __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int pass) {
-    __shared__ double hn[NREG];            // Hn of intermed_{S-1}, 2-cell halo
-    __shared__ double emd[MR1 * MR1];      // E - D, 1-cell halo
-    __shared__ double sbx[MR1 * MR1], sby[MR1 * MR1];
-    __shared__ double vb[MV * MV];         // new vertex bt of this tile's copy
+    constexpr int NT = morpho_nt<BY>();
+    constexpr int NH = RW * (BY + 4), NE = MR1 * (BY + 2), NVB = MV * (BY + 1);
+    __shared__ double hn[NH];              // Hn of intermed_{S-1}, 2-cell halo
+    __shared__ double emd[NE];             // E - D, 1-cell halo
+    __shared__ double sbx[NE], sby[NE];
+    __shared__ double vb[NVB];             // new vertex bt of this tile's copy
     __shared__ MWork m;
     // a fresh thread index per pass (nothing derived from it is held across
     // passes); the asm has no side effects
     unsigned tidu = threadIdx.x;
     asm("" : "+v"(tidu) : "s"(pass));
-    const int tid = (int)(tidu & (MORPHO_NT - 1));
-    if (tid == 0) morpho_work(d, p, m, false, KW_MORPHO + S - 1, pass);
+    const int tid = (int)(tidu & (NT - 1));
+    if (tid == 0) morpho_work<BY>(d, p, m, false, KW_MORPHO + S - 1, pass);
     __syncthreads();
     if (__builtin_amdgcn_readfirstlane(m.skip)) return false;   // wave-uniform (a uniform loop)
     const bool more = __builtin_amdgcn_readfirstlane(m.more) != 0;
@@ -121,7 +128,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     const double eps = p.hthr;
 
     // ---- Hn of the stage state (ComputeDesingularisedVariables) -----------
-    for (int q = tid; q < NREG; q += MORPHO_NT) {
+    for (int q = tid; q < NH; q += NT) {
         const int rx = q % RW - 2, ry = q / RW - 2;
         if (rx > bxa + 1 || ry > bya + 1) continue;
         int tc, ci, cj;
@@ -135,7 +142,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     __syncthreads();
 
     // ---- E - D with the dryness check (src/MorphodynamicRHS.f90:90-145) ---
-    for (int q = tid; q < MR1 * MR1; q += MORPHO_NT) {
+    for (int q = tid; q < NE; q += NT) {
         const int rx = q % MR1 - 1, ry = q / MR1 - 1;
         if (rx > bxa || ry > bya) continue;
         int tc, ci, cj;
@@ -179,7 +186,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     const int NV1 = p.nx + 1;
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     const double psib = 1.0 - p.porosity;
-    for (int q = tid; q < MV * MV; q += MORPHO_NT) {
+    for (int q = tid; q < NVB; q += NT) {
         const int a = q % MV, b = q / MV;
         if (a > bxa || b > bya) continue;
         const int vi = bx0 + a, vj = by0 + b;
@@ -277,15 +284,16 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
 
 // The grid is sized for the active set the host last saw; a block whose
 // share of the list holds more items takes them in turn.
-template <int S>
-__global__ __launch_bounds__(MORPHO_NT) void morpho_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+template <int S, int BY = B>
+__global__ __launch_bounds__(morpho_nt<BY>()) void morpho_kernel(const Dev* __restrict__ dp,
+                                                               const Params* __restrict__ pp) {
     for (int pass = 0;; ++pass) {
         // fresh argument pointers per pass: loads through them are not
         // hoisted out of the loop and held in registers across passes
         const Dev* d = dp;
         const Params* p = pp;
         asm("" : "+s"(d), "+s"(p) : "s"(pass));
-        if (!morpho_pass<S>(const_arg(d), *p, pass)) break;
+        if (!morpho_pass<S, BY>(const_arg(d), *p, pass)) break;
         __syncthreads();
     }
 }

@@ -2068,7 +2068,8 @@ const StageSet& stage_set_of(bool visc, int limiter) {
 // Sub-blocks of 16 x 4 cells when the active set is small: four times the
 // workgroups, each with one face per thread and its cells and halo loaded in
 // one round, so a stage that cannot fill the chip has a shorter critical
-// path (16 x 8 measured between the two).  FP64 only.
+// path (16 x 8 measured between the two); the morphodynamic kernels use the
+// same shape then.  FP64 only.
 constexpr int BY_SMALL = B / 4;
 constexpr int SMALL_WORK = 128;       // sub-blocks of 16 x 16 below which the small shape is used
 
@@ -2108,9 +2109,17 @@ int launch_iteration(kst_handle* h, bool profiled, bool small, int cap) {
         LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A, P);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
-        LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(MORPHO_NT), A, P);
-        LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(MORPHO_NT), A, P);
-        LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(MORPHO_NT), A, P);
+        if (sm) {
+            constexpr int NT = morpho_nt<BY_SMALL>();
+            LAUNCH(KW_MORPHO + 0, (morpho_kernel<1, BY_SMALL>), dim3(nblk_stage), dim3(NT), A, P);
+            LAUNCH(KW_MORPHO + 1, (morpho_kernel<2, BY_SMALL>), dim3(nblk_stage), dim3(NT), A, P);
+            LAUNCH(KW_MORPHO + 2, (morpho_kernel<3, BY_SMALL>), dim3(nblk_stage), dim3(NT), A, P);
+        } else {
+            constexpr int NT = morpho_nt<B>();
+            LAUNCH(KW_MORPHO + 0, morpho_kernel<1>, dim3(nblk), dim3(NT), A, P);
+            LAUNCH(KW_MORPHO + 1, morpho_kernel<2>, dim3(nblk), dim3(NT), A, P);
+            LAUNCH(KW_MORPHO + 2, morpho_kernel<3>, dim3(nblk), dim3(NT), A, P);
+        }
         LAUNCH(KW_REDIST, redistribute_kernel, dim3(1), dim3(RD_THREADS), A, P);
         LAUNCH(KW_FACETOPO, face_topo_kernel, dim3(cap * TILE_PARTS), dim3(256), A, P);
         // second hydraulic operator
