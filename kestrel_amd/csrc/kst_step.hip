@@ -938,6 +938,16 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             if (STAGE == 3) { pw0 = d.Uw[oidx]; pu0 = d.Uu[oidx]; pv0 = d.Uv[oidx]; ph0 = d.Uh[oidx]; }
         }
     }
+    // the last stage's maxima update reads u^n and six maxima fields of the
+    // owned cell (each written only by its owner): loaded here too
+    const bool trk = STAGE == 4 && p.track_maxima && !d.second;
+    double mxv[6] = {0, 0, 0, 0, 0, 0}, mun[4] = {0, 0, 0, 0};
+    if (trk && own) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) mxv[k] = d.maxima[2 * k * FS + oidx];   // fields 0, 2, ..., 10
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mun[k] = Un[k * FS + oidx];
+    }
     // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
     if (pre) {
         if (tid < NXB) s.xbed[tid / (B + 3)][tid % (B + 3)] = bedv;
@@ -1160,16 +1170,15 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         // final implicit substep (src/TimeStepper.f90:504-518)
         const double un = (oru - dt * dt * Eu * Im) / (1.0 + dt * dt * Im * Im);
         const double vn = (orv - dt * dt * Ev * Im) / (1.0 + dt * dt * Im * Im);
-        if (p.track_maxima && !d.second) {
+        if (trk) {
             // UpdateMaximum* on the pre-step state at nextT (src/TimeStepper.f90:1155-1303);
             // the second operator of a Strang step repeats the first one's
             // update with the same state, which changes nothing
-            const double w0 = Un[oidx], u0 = Un[FS + oidx], v0 = Un[2 * FS + oidx];
-            const double h0 = Un[3 * FS + oidx];
-            const double tn = d.ctrl->t + dt;
+            const double w0 = mun[0], u0 = mun[1], v0 = mun[2], h0 = mun[3];
+            const double tn = s.tgrid + dt;       // the step's t + dt
             double* M = d.maxima;
-            if (obt < 0.0 && -obt > M[4 * FS + oidx]) { M[4 * FS + oidx] = -obt; M[5 * FS + oidx] = tn; }
-            if (obt > 0.0 && obt > M[6 * FS + oidx]) { M[6 * FS + oidx] = obt; M[7 * FS + oidx] = tn; }
+            if (obt < 0.0 && -obt > mxv[2]) { M[4 * FS + oidx] = -obt; M[5 * FS + oidx] = tn; }
+            if (obt > 0.0 && obt > mxv[3]) { M[6 * FS + oidx] = obt; M[7 * FS + oidx] = tn; }
             double hn = compute_hn(w0, ob0, obt, gam);
             if (hn < 0.0) hn = 0.0;
             if (hn > p.hthr) {
@@ -1181,11 +1190,11 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
                 const double rho = density(p, psi);
                 const double uu = 2.0 * hn * u0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
                 const double vv = 2.0 * hn * v0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-                if (M[10 * FS + oidx] == -1) M[10 * FS + oidx] = tn;
-                if (hn > M[oidx]) { M[oidx] = hn; M[FS + oidx] = tn; }
+                if (mxv[5] == -1) M[10 * FS + oidx] = tn;
+                if (hn > mxv[0]) { M[oidx] = hn; M[FS + oidx] = tn; }
                 const double spd = sqrt(speed2(p.geometric, uu, vv, bx, by));
-                if (spd > M[2 * FS + oidx]) { M[2 * FS + oidx] = spd; M[3 * FS + oidx] = tn; }
-                if (psi > M[8 * FS + oidx]) { M[8 * FS + oidx] = psi; M[9 * FS + oidx] = tn; }
+                if (spd > mxv[1]) { M[2 * FS + oidx] = spd; M[3 * FS + oidx] = tn; }
+                if (psi > mxv[4]) { M[8 * FS + oidx] = psi; M[9 * FS + oidx] = tn; }
             }
         }
         double* Out = MIX ? (s.cur ? d.S0 : d.S0b) : d.Sout;
