@@ -923,8 +923,11 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     }
     // the small shape's loads of what phases B and C read: its face's
     // topography, its cell's bed slopes and prefactors (and u^n in stage 3)
+    // (the cell's bed slopes and prefactors also in the first and last stage
+    // of the 16 x 16 shape, which have registers to spare for them)
     FaceTopo pft{};
     double pbx = 0, pby = 0, pgxu = 0, pgxv = 0, pgyv = 0, pw0 = 0, pu0 = 0, pv0 = 0, ph0 = 0;
+    const bool prec = pre || STAGE == 1 || STAGE == 4;
     if (pre) {
         if (tid < Smem::NFX) {
             pft = load_face_topo<false, STAGE != 4>(d, p, s, tid % (B + 1), tid / (B + 1));
@@ -932,11 +935,11 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             const int g = tid - Smem::NFX;
             pft = load_face_topo<true, STAGE != 4>(d, p, s, g / B, g % B);
         }
-        if (own) {
-            pbx = d.T.cbx[oidx]; pby = d.T.cby[oidx];
-            pgxu = d.T.pxu[oidx]; pgxv = d.T.pxv[oidx]; pgyv = d.T.pyv[oidx];
-            if (STAGE == 3) { pw0 = d.Uw[oidx]; pu0 = d.Uu[oidx]; pv0 = d.Uv[oidx]; ph0 = d.Uh[oidx]; }
-        }
+        if (own && STAGE == 3) { pw0 = d.Uw[oidx]; pu0 = d.Uu[oidx]; pv0 = d.Uv[oidx]; ph0 = d.Uh[oidx]; }
+    }
+    if (prec && own) {
+        pbx = d.T.cbx[oidx]; pby = d.T.cby[oidx];
+        pgxu = d.T.pxu[oidx]; pgxv = d.T.pxv[oidx]; pgyv = d.T.pyv[oidx];
     }
     // the last stage's maxima update reads u^n and six maxima fields of the
     // owned cell (each written only by its owner): loaded here too
@@ -1048,10 +1051,10 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
 #define FX(q, o) fluxv<MIX, false>(s, q, xs + (o))
 #define FY(q, o) fluxv<MIX, true>(s, q, ys + (o))
     const double gam = og;
-    const double bx = pre ? pbx : d.T.cbx[oidx], by = pre ? pby : d.T.cby[oidx];
+    const double bx = prec ? pbx : d.T.cbx[oidx], by = prec ? pby : d.T.cby[oidx];
     // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are per-bed topography
-    const double gxu = pre ? pgxu : d.T.pxu[oidx], gxv = pre ? pgxv : d.T.pxv[oidx], gyu = gxv;
-    const double gyv = pre ? pgyv : d.T.pyv[oidx];
+    const double gxu = prec ? pgxu : d.T.pxu[oidx], gxv = prec ? pgxv : d.T.pxv[oidx], gyu = gxv;
+    const double gyv = prec ? pgyv : d.T.pyv[oidx];
     // ConstructHydraulicRHS (src/HydraulicRHS.f90:1228-1271)
     const double stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
     const double sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
