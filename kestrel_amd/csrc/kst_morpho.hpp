@@ -39,6 +39,14 @@ struct MWork {
 template <int BY, class DevT>
 __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw,
                             int pass) {
+    const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
+    // the small shape's item and tile ahead of the control block (cap_item)
+    constexpr bool SPEC = BY < B;
+    int wspec = 0, tspec = 0;
+    if (SPEC) {
+        wspec = cap_item();
+        tspec = d.active[min(wspec / spt, p.ntiles)];   // the list holds ntiles + 1 entries
+    }
     const auto* c = ctrl_view(d);
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
@@ -48,15 +56,15 @@ __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_
         if (need_no_refine && c->mref) skip = 1;
     }
     if (!skip) count_work(d.ctrl, kw);
-    const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
     const int nwork = c->n_active * spt;
+    const bool spec = SPEC && nwork <= (int)gridDim.x;
     bool more = false;
-    const int w = xcd_item(nwork, pass, &more);
+    const int w = spec ? (pass == 0 ? wspec : nwork) : xcd_item(nwork, pass, &more);
     if (w >= nwork) skip = 1;
     m.more = more;
     if (!skip) {
         m.q = w / spt;
-        m.tile = d.active[m.q];
+        m.tile = spec ? tspec : d.active[m.q];
         const int sub = w % spt;
         m.bx0 = (sub % sbx) * B;
         m.by0 = (sub / sbx) * BY;

@@ -747,6 +747,17 @@ __device__ __forceinline__ int xcd_item(int nwork, int pass, bool* more) {
     return w < end ? w : nwork;
 }
 
+// The small shape's block-to-item map, which needs no control-block read: XCD
+// x (blocks x, x + 8, ...) takes items x G/8 .. x G/8 + G/8 - 1 of a launch of
+// G blocks.  It covers the list while nwork <= G (the grid cap leaves room for
+// the active set to double), and a block uses it only then, so that the load
+// of its item's tile can be issued alongside the control-block reads.  (Large
+// sets keep xcd_item: this map would leave XCDs idle when nwork < G.)
+__device__ __forceinline__ int cap_item() {
+    const int G = gridDim.x;
+    return (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+}
+
 // stage_body does one work item (one sub-block of an active tile): witem is
 // its index in the scenario's active-list order, or -1 to take the pass-th
 // item of the block in a one-scenario launch.  Returns whether the block has
@@ -763,6 +774,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     asm("" : "+v"(tidu) : "s"(pass));
     const int tid = (int)(tidu & (STAGE_NT - 1));
     if (tid == 0) {
+        const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
+        constexpr bool SPEC = BY < B;
+        int wspec = 0, tspec = 0;
+        if (SPEC && witem < 0) {
+            wspec = cap_item();
+            tspec = d.active[min(wspec / spt, p.ntiles)];   // the list holds ntiles + 1 entries
+        }
         const auto* c = ctrl_view(d);
         int skip = 0;
         if (c->status == ST_DONE || c->status == ST_ERROR) skip = 1;
@@ -783,15 +801,16 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         s.tstage = (STAGE == 1) ? tb : (STAGE == 3 ? tb + 0.5 * dt : tb + dt);
         // work item -> (tile, sub-block); contiguous chunks of the active
         // list go to the same XCD (blocks are dealt round-robin to 8 XCDs)
-        const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
         const int nwork = c->n_active * spt;
+        const bool spec = SPEC && witem < 0 && nwork <= (int)gridDim.x;
         int w = witem;
         bool more = false;
-        if (witem < 0) w = xcd_item(nwork, pass, &more);
+        if (spec) w = pass == 0 ? wspec : nwork;
+        else if (witem < 0) w = xcd_item(nwork, pass, &more);
         if (!skip && w == 0) d.ctrl->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)] += 1;
         if (w >= nwork) skip = 1;
         if (!skip) {
-            s.tile = d.active[w / spt];
+            s.tile = spec ? tspec : d.active[w / spt];
             const int sub = w % spt;
             s.bx0 = (sub % sbx) * B;
             s.by0 = (sub / sbx) * BY;
