@@ -58,12 +58,12 @@ STEP_WORDS = {"hydraulic": 42, "morpho": 134}
 # profiles/stage_kernel_plain_cap_2048.md, profiles/kernels_morpho_hills_1024.md.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_TRAFFIC = {
-    ("plain_cap_2048", "stage_kernel<2>"): (212930.7, 51263.5, 1634048.0),
-    ("plain_cap_2048", "stage_kernel<3>"): (202269.0, 51263.5, 1634048.0),
-    ("plain_cap_2048", "stage_kernel<4>"): (200296.7, 88000.3, 1634048.0),
-    ("morpho_hills_1024", "redistribute_kernel"): (84.2, 35.6, 5795.84),
-    ("morpho_hills_1024", "stage_kernel<2>"): (1300.5, 181.9, 5795.84),
-    ("morpho_hills_1024", "morpho_kernel<3>"): (693.6, 320.3, 5795.84),
+    ("plain_cap_2048", "stage_kernel<2>"): (212935.2, 51263.5, 1634048.0),
+    ("plain_cap_2048", "stage_kernel<3>"): (202248.1, 51263.5, 1634048.0),
+    ("plain_cap_2048", "stage_kernel<4>"): (214191.2, 88000.3, 1634048.0),
+    ("morpho_hills_1024", "redistribute_kernel"): (83.9, 35.6, 5795.84),
+    ("morpho_hills_1024", "stage_kernel<2>"): (1072.1, 181.9, 5795.84),
+    ("morpho_hills_1024", "morpho_kernel<3>"): (491.7, 317.9, 5795.84),
 }
 
 
