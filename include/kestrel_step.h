@@ -183,6 +183,15 @@ int kst_last_error(const kst_handle* h, char* buf, int buflen);
 
 void kst_destroy(kst_handle* h);
 
+/* Launch grids (no counterpart in the reference, which loops over its active
+ * tiles on the CPU).  By default every launch covers a power-of-two cap of
+ * twice the active set the library last read back, and workgroups take
+ * further work items in turn when the set outgrows it between read-backs.
+ * kst_set_grid_cap fixes the cap at `tiles` active tiles (clamped to the
+ * grid's tile count) so that grids, and the captured graphs, never change;
+ * 0 restores the default.  Results do not depend on the cap. */
+int kst_set_grid_cap(kst_handle* h, int tiles);
+
 /* Diagnostics (no counterpart in the reference, which has no timers;
  * src/Messages.f90:275-283 prints only t and dt).  While profiling is on, the
  * library launches one iteration at a time and brackets every kernel launch

@@ -1941,6 +1941,7 @@ struct kst_handle {
     bool profiling = false;
     // kGraphIters whole iterations, per (grid cap, sub-block shape)
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
+    int fixed_cap = 0;                   // kst_set_grid_cap (0: from the active set)
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[KW_N] = {};
     long long prof_n[KW_N] = {};
@@ -2169,6 +2170,7 @@ constexpr int kGraphIters = 8;
 // The grid cap for an active set of n tiles: a power of two with room for
 // the set to double, at least 8, at most every tile.
 int work_cap(const kst_handle* h, int n) {
+    if (h->fixed_cap > 0) return std::min(h->fixed_cap, h->ntiles);
     int cap = 8;
     while (cap < 2 * n && cap < h->ntiles) cap <<= 1;
     return std::min(cap, h->ntiles);
@@ -2733,6 +2735,12 @@ int kst_get_cell_topography(kst_handle* h, double* b0c, double* btc, double* dbd
 int kst_last_error(const kst_handle* h, char* buf, int buflen) {
     if (!h || !buf || buflen <= 0) return KST_ERR_CONFIG;
     std::snprintf(buf, buflen, "%s", h->err.c_str());
+    return KST_OK;
+}
+
+int kst_set_grid_cap(kst_handle* h, int tiles) {
+    if (!h || tiles < 0) return KST_ERR_CONFIG;
+    h->fixed_cap = tiles;
     return KST_OK;
 }
 
