@@ -19,9 +19,10 @@ Printed: one JSON line with value = cell-updates/s of those K steps; the
 per-kernel launch times of a second, profiled pass over the same K steps (HIP
 events on the library's stream around every launch, launches that did work
 only); the roofline of the dominant kernel (the most total time); the plain
-FP64 hydraulic path at 2048x2048 with its own roofline; and the CPU baseline
-(the oracle, a C restatement, 1 thread, on the first steps of the same
-workload).
+FP64 hydraulic path at 2048x2048 with its own roofline; the wall time of one
+kst_volume call (CalculateVolume on the device) on that grid; and the CPU
+baseline (the oracle, a C restatement, 1 thread, on the first steps of the
+same workload).  --volume-only runs the kst_volume leg alone.
 """
 from __future__ import annotations
 
@@ -243,6 +244,30 @@ def serving_leg(n: int = 32, rate: float = 4.0) -> dict:
     return out
 
 
+def volume_leg(calls: int = 200) -> dict:
+    """kst_volume (CalculateVolume on the device) on the plain 2048x2048 cap
+    after 10 steps: wall time per call, which includes the control-block
+    read-back, both reduction kernels and the copy of the result; the
+    volume kernel's own duration is in the committed rocprofv3 summary."""
+    from kestrel_amd import workloads
+    from kestrel_amd.runtime import Stepper
+    wl = workloads.plain_cap_2048(10)
+    g = Stepper(wl.cfg)
+    g.set_topography(wl.b0v)
+    g.set_state(wl.init)
+    st = g.step_n(10)
+    for _ in range(5):
+        g.volume()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        v = g.volume()
+    el = (time.perf_counter() - t0) / calls
+    g.close()
+    cells = st["active_tiles"] * wl.cfg.nx_per_tile * wl.cfg.ny_per_tile
+    return {"workload": wl.name, "active_cells": cells, "bytes_per_cell": 40,
+            "call_ms": el * 1e3, "volume_m3": float(v[0])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +280,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--volume-only", action="store_true",
+                    help="only the kst_volume leg (for profiling the volume kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -264,6 +291,9 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.volume_only:
+        print(json.dumps({"volume": volume_leg()}), flush=True)
+        return
     from kestrel_amd import workloads
 
     wl = getattr(workloads, args.workload)()
@@ -308,6 +338,8 @@ def main():
                              "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pw.name, prows, pcells),
                              "step_roofline_frac": pst["cell_updates"] / pel * STEP_WORDS["hydraulic"] * 8
                              / (HBM_PEAK_GBS * 1e9)}
+    if not args.no_plain:
+        res["volume"] = volume_leg()
     if not args.no_batch:
         res["ensemble"] = ensemble_leg()
     if not args.no_serve:

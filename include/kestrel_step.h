@@ -178,6 +178,14 @@ int kst_get_cell_topography(kst_handle* h, double* b0c, double* btc, double* dbd
  * psimax, psimax_t, tfirst. */
 int kst_get_maxima(kst_handle* h, double* const maxima[11]);
 
+/* CalculateVolume (src/Output.f90:617-735), computed on the device from the
+ * resident state: out[0..5] = the Volume.txt columns 2-7 -- flow volume,
+ * total bed volume, total mass, bed mass, total solids mass, bed solids
+ * mass -- over the active tiles.  The reference Kahan-sums sequentially; the
+ * device sums are compensated (Sum2), so both lie within an ulp or two of
+ * the exact sums and may differ from each other in the last bits. */
+int kst_volume(kst_handle* h, double out[6]);
+
 /* Copy the last error message (NUL-terminated) into buf. */
 int kst_last_error(const kst_handle* h, char* buf, int buflen);
 

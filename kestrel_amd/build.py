@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libkestrel_amd.so")
 SOURCES = [os.path.join(HERE, "csrc", "kst_step.hip")]
-HEADERS = [os.path.join(HERE, "csrc", "kst_physics.hpp"), os.path.join(ROOT, "include", "kestrel_step.h")]
+HEADERS = [os.path.join(HERE, "csrc", "kst_physics.hpp"), os.path.join(HERE, "csrc", "kst_morpho.hpp"),
+           os.path.join(HERE, "csrc", "kst_output.hpp"), os.path.join(ROOT, "include", "kestrel_step.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

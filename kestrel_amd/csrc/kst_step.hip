@@ -1916,6 +1916,7 @@ __global__ __launch_bounds__(256) void topo2_kernel(Dev d, Params p, Topo T) {
 }  // namespace kst
 
 #include "kst_morpho.hpp"
+#include "kst_output.hpp"
 
 // ===========================================================================
 // host side: the C-ABI (include/kestrel_step.h)
@@ -1942,6 +1943,7 @@ struct kst_handle {
     // kGraphIters whole iterations, per (grid cap, sub-block shape)
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
     int fixed_cap = 0;                   // kst_set_grid_cap (0: from the active set)
+    double* vol_part = nullptr;          // volume partials (VOL_BLOCKS_MAX + 1 groups)
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[KW_N] = {};
     long long prof_n[KW_N] = {};
@@ -2341,6 +2343,7 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     d.morpho = cfg->morphodynamics ? 1 : 0;
     d.mixed = cfg->precision == KST_PREC_MIXED ? 1 : 0;
     rc |= dalloc(h, &d.S0, 4 * FS);
+    rc |= dalloc(h, &h->vol_part, (size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2);
     if (d.mixed) {
         // S2, S3 (4 fields) and R (5 fields) hold floats
         rc |= dalloc(h, &d.S0b, 4 * FS);
@@ -2695,6 +2698,37 @@ int kst_get_state(kst_handle* h, double* w, double* rhu, double* rhv, double* hn
     int rc = read_ctrl(h, &c);
     if (rc) return rc;
     if (t) *t = c.t;
+    return KST_OK;
+}
+
+int kst_volume(kst_handle* h, double out[6]) {
+    if (!h || !out) return KST_ERR_CONFIG;
+    if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    const int nb = std::max(1, std::min(c.n_active, VOL_BLOCKS_MAX));
+    double* fin = h->vol_part + (size_t)VOL_BLOCKS_MAX * VOL_Q * 2;
+    hipLaunchKernelGGL(volume_kernel, dim3(nb), dim3(VOL_NT), 0, h->stream, h->d_args, h->d_par,
+                       h->vol_part);
+    hipLaunchKernelGGL(volume_final_kernel, dim3(1), dim3(VOL_NT), 0, h->stream, h->vol_part, nb, fin);
+    HIPCHK(h, hipGetLastError());
+    double r[VOL_Q * 2];
+    HIPCHK(h, hipMemcpyAsync(r, fin, sizeof r, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const kst_config& g = h->cfg;
+    // each sum times dx dy, then the columns as CalculateVolume writes them
+    // (src/Output.f90:709-730)
+    double v[VOL_Q];
+    for (int q = 0; q < VOL_Q; ++q) v[q] = (r[2 * q] + r[2 * q + 1]) * g.dx * g.dy;
+    const double vol = v[0], mass = v[1], bed = v[2], solids = v[3];
+    const double rhob = g.rhow * g.bed_porosity + g.rhos * (1.0 - g.bed_porosity);
+    out[0] = vol;
+    out[1] = bed;
+    out[2] = mass;
+    out[3] = bed * rhob;
+    out[4] = solids * g.rhos;
+    out[5] = bed * g.rhos * (1.0 - g.bed_porosity);
     return KST_OK;
 }
 

@@ -34,7 +34,7 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read", "kst_get_tile_bt",
            "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
-           "kst_get_cell_topography", "kst_set_grid_cap"]
+           "kst_get_cell_topography", "kst_volume", "kst_set_grid_cap"]
 
 
 class KstStats(ctypes.Structure):
@@ -78,6 +78,8 @@ def lib():
         L.kst_profile.restype = ctypes.c_int
         L.kst_set_grid_cap.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.kst_set_grid_cap.restype = ctypes.c_int
+        L.kst_volume.argtypes = [ctypes.c_void_p, _dp]
+        L.kst_volume.restype = ctypes.c_int
         L.kst_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.kst_profile_read.restype = ctypes.c_int
         L.kst_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
@@ -182,6 +184,13 @@ class Stepper:
         c = self.cfg
         out = {k: np.zeros((c.NY, c.NX)) for k in ("b0c", "btc", "dbdx", "dbdy")}
         self._check(self.L.kst_get_cell_topography(self.h, *[_ptr(out[k]) for k in ("b0c", "btc", "dbdx", "dbdy")]))
+        return out
+
+    def volume(self) -> np.ndarray:
+        """The Volume.txt columns 2-7 (flow volume, total bed volume, total
+        mass, bed mass, total solids mass, bed solids mass), on the device."""
+        out = np.zeros(6)
+        self._check(self.L.kst_volume(self.h, _ptr(out)))
         return out
 
     def set_grid_cap(self, tiles: int):

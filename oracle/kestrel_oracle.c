@@ -2460,6 +2460,70 @@ int ko_get_fields13(ko_sim* s, double* out) {
     return 0;
 }
 
+/* CalculateVolume (src/Output.f90:617-735), 2D: over the active tiles in
+ * list order and their cells row by row (j outer, i inner), Kahan sums
+ * (src/utilities.f90:439-448) of the flow volume (w - b0 - bt) gam^2, the
+ * mass rho Hn gam, the bed change bt and the solids Hnpsi gam, read from
+ * tileContainer's stored u (Hn and rho as ComputeDesingularisedVariables
+ * left them); each times dx dy, then the Volume.txt columns 2-7. */
+int ko_volume(ko_sim* s, double out[6]) {
+    const kst_config* c = &s->cfg;
+    double vol = 0.0, mass = 0.0, bed = 0.0, solids = 0.0;
+    double cv = 0.0, cm = 0.0, cb = 0.0, cs = 0.0;
+    for (int q = 0; q < s->nactive; ++q) {
+        tdata* T = &s->cont[C_TC][s->active[q]];
+        for (int j = 0; j < s->ny; ++j)
+            for (int i = 0; i < s->nx; ++i) {
+                double x[4];
+                const double gam = gam_cell(s, T, i, j);
+                const double hn = UF(s, T, HN, i, j), rho = UF(s, T, RHO, i, j);
+                x[0] = (UF(s, T, W, i, j) - UF(s, T, B0, i, j) - UF(s, T, BT, i, j)) * gam * gam;
+                x[1] = rho * hn * gam;
+                x[2] = UF(s, T, BT, i, j);
+                x[3] = UF(s, T, HNPSI, i, j) * gam;
+                double* acc[4] = {&vol, &mass, &bed, &solids};
+                double* cmp[4] = {&cv, &cm, &cb, &cs};
+                for (int m = 0; m < 4; ++m) {
+                    double y = x[m] - *cmp[m];
+                    double t = *acc[m] + y;
+                    *cmp[m] = (t - *acc[m]) - y;
+                    *acc[m] = t;
+                }
+            }
+    }
+    vol = vol * c->dx * c->dy;
+    mass = mass * c->dx * c->dy;
+    bed = bed * c->dx * c->dy;
+    solids = solids * c->dx * c->dy;
+    const double rhob = c->rhow * c->bed_porosity + c->rhos * (1.0 - c->bed_porosity);
+    out[0] = vol;
+    out[1] = bed;
+    out[2] = mass;
+    out[3] = bed * rhob;
+    out[4] = solids * c->rhos;
+    out[5] = bed * c->rhos * (1.0 - c->bed_porosity);
+    return 0;
+}
+
+/* The four per-cell terms of ko_volume in summation order (4 x NC x nactive). */
+int ko_volume_terms(ko_sim* s, double* terms, int64_t cap) {
+    int64_t n = 0;
+    for (int q = 0; q < s->nactive; ++q) {
+        tdata* T = &s->cont[C_TC][s->active[q]];
+        for (int j = 0; j < s->ny; ++j)
+            for (int i = 0; i < s->nx; ++i) {
+                if (n >= cap) return -1;
+                const double gam = gam_cell(s, T, i, j);
+                terms[4 * n + 0] = (UF(s, T, W, i, j) - UF(s, T, B0, i, j) - UF(s, T, BT, i, j)) * gam * gam;
+                terms[4 * n + 1] = UF(s, T, RHO, i, j) * UF(s, T, HN, i, j) * gam;
+                terms[4 * n + 2] = UF(s, T, BT, i, j);
+                terms[4 * n + 3] = UF(s, T, HNPSI, i, j) * gam;
+                ++n;
+            }
+    }
+    return (int)n;
+}
+
 int ko_get_ghost_slopes(ko_sim* s, double* sx, double* sy) {
     size_t N = (size_t)s->NX * s->NY;
     memset(sx, 0, 8 * N);

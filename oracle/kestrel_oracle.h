@@ -57,6 +57,13 @@ int ko_get_maxima(ko_sim* s, double* const maxima[11]);
  * tile's u (tileContainer), cell-major global layout, NX*NY per field
  * (unallocated tiles give 0). */
 int ko_get_fields13(ko_sim* s, double* out13);
+/* CalculateVolume (src/Output.f90:617-735): the Volume.txt columns 2-7
+ * (volume, total bed volume, total mass, bed mass, total solids mass, bed
+ * solids mass) of tileContainer's active tiles. */
+int ko_volume(ko_sim* s, double out[6]);
+/* Its per-cell terms (flow volume, mass, bed, solids) in summation order;
+ * returns the cell count, or -1 past cap cells. */
+int ko_volume_terms(ko_sim* s, double* terms, int64_t cap);
 /* Ghost-tile w slopes (uLimX/uLimY of field w), global NX*NY each. */
 int ko_get_ghost_slopes(ko_sim* s, double* sx, double* sy);
 

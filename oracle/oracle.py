@@ -69,6 +69,10 @@ def lib():
         L.ko_get_ghost_slopes.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.ko_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.ko_tile_lists.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
+        L.ko_volume.restype = ctypes.c_int
+        L.ko_volume.argtypes = [ctypes.c_void_p, _dp]
+        L.ko_volume_terms.restype = ctypes.c_int
+        L.ko_volume_terms.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
         for name in ["ko_get_tile_bt", "ko_morpho_rhs", "ko_probe_count", "ko_probe_get",
                      "ko_probe_candidates"]:
             getattr(L, name).restype = ctypes.c_int
@@ -186,6 +190,21 @@ class Oracle:
         out = np.zeros((13, c.NY, c.NX))
         self._check(self.L.ko_get_fields13(self.h, _ptr(out)))
         return out
+
+    def volume(self) -> np.ndarray:
+        """CalculateVolume's Volume.txt columns 2-7 (src/Output.f90:617-735)."""
+        out = np.zeros(6)
+        self._check(self.L.ko_volume(self.h, _ptr(out)))
+        return out
+
+    def volume_terms(self) -> np.ndarray:
+        """Per-cell (flow volume, mass, bed, solids) terms in summation order."""
+        cap = self.cfg.NX * self.cfg.NY
+        t = np.zeros((cap, 4))
+        n = self.L.ko_volume_terms(self.h, _ptr(t), cap)
+        if n < 0:
+            raise RuntimeError("ko_volume_terms: more cells than the grid holds")
+        return t[:n]
 
     def ghost_slopes(self):
         c = self.cfg
@@ -355,3 +374,50 @@ def ref_redistribute(cfg: KstConfig, tile_state, b0v, i0, i3, cand):
                         i.astype(np.int32).ctypes.data_as(i32p), j.astype(np.int32).ctypes.data_as(i32p),
                         _ptr(ex.astype(np.float64)), _ptr(fout), _ptr(btout), ctypes.byref(ref))
     return fout, btout, ref.value
+
+
+def _kahan_add(x, s, c):
+    y = x - c
+    t = s + y
+    c = (t - s) - y
+    return t, c
+
+
+def volume_from_state(cfg: KstConfig, b0v: np.ndarray, state: dict) -> dict:
+    """CalculateVolume (src/Output.f90:617-735) restated over a downloaded
+    state: Hn and rho re-derived as ComputeDesingularisedVariables does
+    (src/HydraulicRHS.f90:762-878), then Kahan sums (src/utilities.f90:439-448)
+    over the active tiles in index order, cells row by row.  The cell bed is
+    taken from the state's single-valued vertex bt, so with morphodynamics
+    (where tiles' copies of a shared vertex may differ in the last bits) it
+    can differ from tileContainer's in the last bits."""
+    b0c, btc, bx, by = scenario.cell_topography(cfg, b0v, state.get("btv"))
+    gam = scenario.gamma(cfg, bx, by)
+    w, hnpsi = state["w"], state["Hnpsi"]
+    # derived Hn and rho as ComputeDesingularisedVariables leaves them
+    hn = (-btc + (w - b0c)) * gam
+    hn = np.where(hn < 0.0, 0.0, hn)
+    hp = np.where(hnpsi < 0.0, 0.0, hnpsi)
+    eps = cfg.height_threshold
+    den = hn * hn + np.maximum(hn * hn, eps * eps)
+    psi = np.minimum(2.0 * hn * hp / den, cfg.max_pack)
+    rho = cfg.rhow + (cfg.rhos - cfg.rhow) * psi
+    vol = mass = bed = solids = 0.0
+    cv = cm = cb = cs = 0.0
+    ts = np.asarray(state["tile_state"]).reshape(-1)
+    nx, ny = cfg.nx_per_tile, cfg.ny_per_tile
+    for k in np.nonzero(ts == 1)[0]:
+        ti, tj = k % cfg.nx_tiles, k // cfg.nx_tiles
+        sl = (slice(tj * ny, (tj + 1) * ny), slice(ti * nx, (ti + 1) * nx))
+        dv = ((w[sl] - b0c[sl] - btc[sl]) * gam[sl] * gam[sl]).ravel()
+        dm = (rho[sl] * hn[sl] * gam[sl]).ravel()
+        db = btc[sl].ravel()
+        ds = (hnpsi[sl] * gam[sl]).ravel()
+        for q in range(dv.size):
+            vol, cv = _kahan_add(dv[q], vol, cv)
+            mass, cm = _kahan_add(dm[q], mass, cm)
+            bed, cb = _kahan_add(db[q], bed, cb)
+            solids, cs = _kahan_add(ds[q], solids, cs)
+    a = cfg.dx * cfg.dy
+    return {"volume": vol * cfg.dx * cfg.dy, "mass": mass * cfg.dx * cfg.dy,
+            "bed": bed * cfg.dx * cfg.dy, "solids": solids * cfg.dx * cfg.dy, "cell_area": a}

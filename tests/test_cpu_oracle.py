@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 from kestrel_amd import config as C
-from kestrel_amd import diagnostics, scenario as S, workloads
+from kestrel_amd import scenario as S, workloads
+from oracle.oracle import volume_from_state
 from tests.helpers import FIELDS, active_mask, oracle_for, rel_inf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -150,11 +151,11 @@ def test_mass_conservation_and_positivity():
     depth stays non-negative (tests/testlib.jl:286-302)."""
     wl = workloads.plain_cap(128, 0, tile=16, cap_x=0.0, cap_radius=10.0)
     o = oracle_for(wl)
-    v0 = diagnostics.volume(wl.cfg, wl.b0v, o.state())["volume"]
+    v0 = volume_from_state(wl.cfg, wl.b0v, o.state())["volume"]
     for _ in range(3):
         o.step_n(50)
         st = o.state()
-        v = diagnostics.volume(wl.cfg, wl.b0v, st)["volume"]
+        v = volume_from_state(wl.cfg, wl.b0v, st)["volume"]
         assert abs(v - v0) / v0 < 1e-10, (v, v0)
         m = active_mask(wl.cfg, st["tile_state"])
         b0c, btc, bx, by = S.cell_topography(wl.cfg, wl.b0v)

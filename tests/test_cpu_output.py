@@ -1,0 +1,153 @@
+"""Output diagnostics on the CPU: the oracle's CalculateVolume
+(src/Output.f90:617-735) pinned by an independent restatement over the
+downloaded state, by the exact sum, by a closed form and by the reference's
+own conservation check (tests/testlib.jl:104-158); and the Volume.txt writer
+pinned by the Fortran runtime's own formatting of the reference's format
+strings (src/Output.f90:664-673, 720-727)."""
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from kestrel_amd import config as C, output, scenario as S, workloads
+from oracle.oracle import volume_from_state
+from tests.helpers import oracle_for
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def flat_lake(level=0.5, n_tiles=8, tile=16):
+    """A level lake of depth `level` over a flat bed (b0 = 0), disc of radius 40 m."""
+    cfg = C.default_config(n_tiles, n_tiles, tile, tile, dx=1.0)
+    C.hydraulic_only(cfg)
+    C.derive(cfg)
+    b0v = np.zeros((cfg.NY + 1, cfg.NX + 1))
+    caps = [S.Cap(0.0, 0.0, 40.0, level, shape="level")]
+    init = S.initial_state(cfg, b0v, caps)
+    return workloads.Workload("flat_lake", cfg, b0v, init, 0, caps, [])
+
+
+def test_volume_matches_restatement_over_state():
+    """ko_volume reads tileContainer's stored Hn, rho, bt and slopes; the
+    restatement re-derives them from the downloaded state: same bits."""
+    wl = workloads.plain_cap(128, 0, tile=16, cap_x=0.0, cap_radius=10.0)
+    o = oracle_for(wl)
+    for _ in range(2):
+        o.step_n(40)
+        got = o.volume()
+        ref = volume_from_state(wl.cfg, wl.b0v, o.state())
+        assert got[0] == ref["volume"]
+        assert got[2] == ref["mass"]
+        assert got[1] == ref["bed"]
+        assert got[4] == ref["solids"] * wl.cfg.rhos
+
+
+@pytest.mark.parametrize("morpho", [False, True])
+def test_kahan_sums_are_within_two_ulps_of_the_exact_sums(morpho):
+    wl = (workloads.morpho_cap(64, 0, tile=16, radius=10.0) if morpho
+          else workloads.plain_cap(128, 0, tile=16, cap_x=0.0, cap_radius=10.0))
+    o = oracle_for(wl)
+    o.step_n(30)
+    v = o.volume()
+    terms = o.volume_terms()
+    a = wl.cfg.dx * wl.cfg.dy
+    for col, q in ((0, 0), (2, 1), (4, 3)):
+        exact = math.fsum(terms[:, q])
+        scale = wl.cfg.rhos if col == 4 else 1.0
+        assert abs(v[col] / a / scale - exact) <= 2 * math.ulp(exact), (col, v[col], exact)
+    bed = math.fsum(terms[:, 2])
+    assert abs(v[1] / a - bed) <= 2 * math.ulp(np.abs(terms[:, 2]).sum())
+
+
+def test_flat_lake_volume_is_the_closed_form():
+    wl = flat_lake()
+    o = oracle_for(wl)
+    v = o.volume()
+    st = o.state()
+    n_wet = int((st["w"] > 0.0).sum())
+    assert n_wet > 100
+    assert v[0] == 0.5 * n_wet
+    assert v[2] == wl.cfg.rhow * 0.5 * n_wet
+    assert v[1] == 0.0 and v[4] == 0.0
+
+
+def test_conservation_as_the_reference_checks_it():
+    """tests/testlib.jl:104-158: flow + bed volume and solids mass (flow +
+    bed) are conserved to 1e-10 relative with no flux sources."""
+    wl = workloads.morpho_cap(128, 0, tile=16, radius=10.0)
+    o = oracle_for(wl)
+    v0 = o.volume()
+    o.step_n(60)
+    v1 = o.volume()
+    assert v1[1] != 0.0          # the bed has changed
+    assert abs((v0[0] + v0[1]) - (v1[0] + v1[1])) / (v0[0] + v0[1]) < 1e-10
+    rhos = wl.cfg.rhos
+    s0 = (v0[4] + v0[5]) / rhos
+    s1 = (v1[4] + v1[5]) / rhos
+    assert abs(s0 - s1) / s0 < 1e-10
+
+
+def test_volume_line_rejects_wrong_width():
+    with pytest.raises(ValueError):
+        output.volume_line(0.0, [1.0] * 5)
+
+
+VALUES = [1.5, -2.25e-300, 0.0, 123456.789, 1e100, -7e-5, 2.0 / 3.0, 1e-5, 9.9999999999999995e-1,
+          5e-324, 1.7976931348623157e308, -1234.5678e-99]
+TIMES = [0.0, 0.125, 0.375, 1.005, 12345.675, 99999999.99, -0.5]
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not present")
+def test_volume_format_matches_the_fortran_runtime(tmp_path):
+    """The reference's own format strings, written by flang's runtime, against
+    output.volume_header / volume_line / fortran_es."""
+    src = f"""program fmt
+  implicit none
+  real(8) :: x({len(VALUES)}), t({len(TIMES)})
+  integer :: i
+  read(*,*) x
+  read(*,*) t
+  write(*,fmt="(a12,6(a2,a24))") '        time', ', ', '                  volume', ', ', &
+     '        total_bed_volume', ', ', '              total_mass', ', ', '                bed_mass', ', ', &
+     '       total_solids_mass', ', ', '         bed_solids_mass'
+  do i = 1, {len(TIMES)}
+     write(*,fmt="(f12.2,6(a2, es24.15))") t(i), ', ', x(mod(i, {len(VALUES)}) + 1), ', ', &
+        x(mod(i + 1, {len(VALUES)}) + 1), ', ', x(mod(i + 2, {len(VALUES)}) + 1), ', ', &
+        x(mod(i + 3, {len(VALUES)}) + 1), ', ', x(mod(i + 4, {len(VALUES)}) + 1), ', ', &
+        x(mod(i + 5, {len(VALUES)}) + 1)
+  end do
+  do i = 1, {len(VALUES)}
+     write(*,fmt="(ES18.10E3)") x(i)
+  end do
+end program
+"""
+    f90 = tmp_path / "fmt.f90"
+    f90.write_text(src)
+    exe = tmp_path / "fmt"
+    subprocess.run([FLANG, str(f90), "-o", str(exe)], check=True, capture_output=True)
+    # shortest round-trip decimals, read back exactly by list-directed input
+    stdin = " ".join(repr(v) for v in VALUES) + "\n" + " ".join(repr(t) for t in TIMES) + "\n"
+    got = subprocess.run([str(exe)], input=stdin, check=True, capture_output=True,
+                         text=True).stdout.splitlines()
+    want = [output.volume_header()]
+    n = len(VALUES)
+    for i in range(1, len(TIMES) + 1):
+        want.append(output.volume_line(TIMES[i - 1], [VALUES[(i + k) % n] for k in range(6)]))
+    want += [output.fortran_es(v, 18, 10, 3) for v in VALUES]
+    assert got == want
+
+
+def test_write_volume_appends_after_the_header(tmp_path):
+    p = str(tmp_path / "Volume.txt")
+    output.write_volume(p, 0.0, [1.0, 0.0, 1000.0, 0.0, 0.0, 0.0], create=True)
+    output.write_volume(p, 1.0, [1.0, 0.0, 1000.0, 0.0, 0.0, 0.0])
+    lines = open(p).read().splitlines()
+    assert lines[0] == output.volume_header() and len(lines) == 3
+    # the reference's own reader (tests/testlib.jl:122-133) takes awk field 2
+    # of line 2 and the last line, with the comma stripped
+    assert float(lines[1].split()[1].replace(",", "")) == 1.0
+    output.write_volume(p, 2.0, [2.0] * 6, create=True)
+    assert len(open(p).read().splitlines()) == 2
