@@ -186,6 +186,18 @@ int kst_get_maxima(kst_handle* h, double* const maxima[11]);
  * the exact sums and may differ from each other in the last bits. */
 int kst_volume(kst_handle* h, double out[6]);
 
+/* The per-cell columns of OutputSolutionData_txt (src/Output.f90:754-929)
+ * for the active tiles, computed on the device: on return *n_tiles holds the
+ * active-tile count n, tiles[0..n) the tile ids (0-based, the file writes
+ * id + 1) in active-list order, and fields[(q * 14 + col) * nx_per_tile *
+ * ny_per_tile + j * nx_per_tile + i] the columns 6-19 of the file -- flow
+ * depth, flow speed, x and y velocity, density, solids fraction, x and y
+ * flux, Hnpsi, w, base elevation, elevation change, x and y slope -- of cell
+ * (i, j) of the q-th listed tile.  fields holds max_tiles * 14 * cells per
+ * tile doubles and tiles max_tiles entries; KST_ERR_CONFIG if the active set
+ * is larger. */
+int kst_get_output(kst_handle* h, int32_t max_tiles, double* fields, int32_t* tiles, int32_t* n_tiles);
+
 /* Copy the last error message (NUL-terminated) into buf. */
 int kst_last_error(const kst_handle* h, char* buf, int buflen);
 

@@ -120,4 +120,78 @@ __global__ __launch_bounds__(VOL_NT) void volume_final_kernel(const double* part
         }
 }
 
+// ---------------------------------------------------------------------------
+// OutputSolutionData_txt (src/Output.f90:754-929): per active cell, in the
+// order of the file's columns 6-19, flow depth Hn, flow speed, u, v, rho,
+// psi, rhoHnu, rhoHnv, Hnpsi, w, b0, bt, db/dx, db/dy, as tileContainer holds
+// them.  Hn, psi and rho are its desingularised variables
+// (ComputeDesingularisedVariables, src/HydraulicRHS.f90:762-878, evaluated as
+// the reference orders it), which the final implicit pass leaves consistent
+// with w and Hnpsi; u and v are the ones the step's last RHS derived (Dev::UVo),
+// since that pass changes the momentum after them.  The speed is
+// sqrt(FlowSquaredSpeedSlopeAligned) of those where Hn exceeds the height
+// threshold, else 0 (src/Output.f90:868-873).  out[list position][column][cell].
+// ---------------------------------------------------------------------------
+
+constexpr int OUT_COLS = 14;
+
+// tileContainer's velocities after kst_set_state: desingularised from the
+// state (the reference's set-up ends with ComputeDesingularisedVariables on
+// the active tiles).
+__global__ __launch_bounds__(VOL_NT) void uv_init_kernel(const Dev* dA, const Params* pA) {
+    const auto& d = const_arg(dA);
+    const auto& p = const_arg(pA);
+    const auto* c = ctrl_view(d);
+    const int na = c->n_active;
+    const double* Sn = (d.mixed && c->cur) ? d.S0b : d.S0;
+    const long long FS = d.FS;
+    const int NC = p.NC;
+    const double eps = p.hthr;
+    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+        const long long base = (long long)d.active[it] * NC;
+        for (int i = threadIdx.x; i < NC; i += VOL_NT) {
+            const long long g = base + i;
+            const double w = Sn[g], ru = Sn[FS + g], rv = Sn[2 * FS + g], hp = Sn[3 * FS + g];
+            double hn = compute_hn(w, d.b0c[g], d.TA.btc[g], d.TA.cgam[g]);
+            if (hn < 0.0) hn = 0.0;
+            const double hpc = hp < 0.0 ? 0.0 : hp;
+            const double psi = dmin(2.0 * hn * hpc / (hn * hn + dmax(hn * hn, eps * eps)), p.max_pack);
+            const double rho = density(generic(p), psi);
+            d.UVo[g] = 2.0 * hn * ru / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+            d.UVo[FS + g] = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+        }
+    }
+}
+
+__global__ __launch_bounds__(VOL_NT) void output_kernel(const Dev* dA, const Params* pA, double* out) {
+    const auto& d = const_arg(dA);
+    const auto& p = const_arg(pA);
+    const auto* c = ctrl_view(d);
+    const int na = c->n_active;
+    const double* Sn = (d.mixed && c->cur) ? d.S0b : d.S0;
+    const long long FS = d.FS;
+    const int NC = p.NC;
+    const double eps = p.hthr;
+    for (int it = blockIdx.x; it < na; it += gridDim.x) {
+        const long long base = (long long)d.active[it] * NC;
+        double* o = out + (size_t)it * OUT_COLS * NC;
+        for (int i = threadIdx.x; i < NC; i += VOL_NT) {
+            const long long g = base + i;
+            const double w = Sn[g], ru = Sn[FS + g], rv = Sn[2 * FS + g], hp = Sn[3 * FS + g];
+            const double b0 = d.b0c[g], bt = d.TA.btc[g], bx = d.TA.cbx[g], by = d.TA.cby[g];
+            const double gam = d.TA.cgam[g];
+            double hn = compute_hn(w, b0, bt, gam);
+            if (hn < 0.0) hn = 0.0;
+            const double hpc = hp < 0.0 ? 0.0 : hp;
+            const double psi = dmin(2.0 * hn * hpc / (hn * hn + dmax(hn * hn, eps * eps)), p.max_pack);
+            const double rho = density(generic(p), psi);
+            const double uu = d.UVo[g], vv = d.UVo[FS + g];
+            const double spd = hn > p.hthr ? sqrt(speed2(p.geometric, uu, vv, bx, by)) : 0.0;
+            const double col[OUT_COLS] = {hn, spd, uu, vv, rho, psi, ru, rv, hp, w, b0, bt, bx, by};
+#pragma unroll
+            for (int q = 0; q < OUT_COLS; ++q) o[(size_t)q * NC + i] = col[q];
+        }
+    }
+}
+
 }  // namespace kst

@@ -154,6 +154,12 @@ struct Dev {
     // that refresh: bit 62 set = vertex tile*NV + v, else cell tile*NC + c.
     long long* dirty;
     int dirty_cap;
+    // The velocities the RHS of the step's last stage derived, [2][tile][NC]:
+    // tileContainer's u, v after CopySolutionData (src/TimeStepper.f90:811-835)
+    // copies intermed3's u whole, i.e. before the final implicit drag update
+    // of the momentum.  Written by stage 4 of the operator that ends the step;
+    // tiles activated since hold 0, SetDefaultTileData's value.
+    double* UVo;
 };
 
 // Kernel arguments (Dev, Params) that live in device memory.  Passed by value, every field
@@ -1221,6 +1227,7 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         }
         double* Out = MIX ? (s.cur ? d.S0 : d.S0b) : d.Sout;
         Out[oidx] = ow; Out[FS + oidx] = un; Out[2 * FS + oidx] = vn; Out[3 * FS + oidx] = oh;
+        if (d.last) { d.UVo[oidx] = ou; d.UVo[FS + oidx] = ov; }
         // activation flags from this state's Hn (CheckIfNearBoundaries,
         // src/TimeStepper.f90:946-1150): bit 0 N, 1 S, 2 E, 3 W
         if (d.last && ohn > p.hthr) {
@@ -1944,6 +1951,8 @@ struct kst_handle {
     std::vector<std::pair<int, hipGraphExec_t>> graphs;
     int fixed_cap = 0;                   // kst_set_grid_cap (0: from the active set)
     double* vol_part = nullptr;          // volume partials (VOL_BLOCKS_MAX + 1 groups)
+    double* out_buf = nullptr;           // kst_get_output staging, out_cap doubles
+    size_t out_cap = 0;
     std::vector<hipEvent_t> ev;          // 8 per iteration of the current batch
     double prof_ms[KW_N] = {};
     long long prof_n[KW_N] = {};
@@ -2344,6 +2353,7 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     d.mixed = cfg->precision == KST_PREC_MIXED ? 1 : 0;
     rc |= dalloc(h, &d.S0, 4 * FS);
     rc |= dalloc(h, &h->vol_part, (size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2);
+    rc |= dalloc(h, &d.UVo, 2 * FS);
     if (d.mixed) {
         // S2, S3 (4 fields) and R (5 fields) hold floats
         rc |= dalloc(h, &d.S0b, 4 * FS);
@@ -2595,6 +2605,11 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
         hipLaunchKernelGGL(init_ghosts_kernel, dim3(gh.size()), dim3(256), 0, h->stream, h->d, h->p);
     if (!act.empty())
         hipLaunchKernelGGL(init_flags_kernel, dim3(act.size()), dim3(256), 0, h->stream, h->d, h->p);
+    // tileContainer's velocities after the state is set: desingularised from it
+    HIPCHK(h, hipMemsetAsync(h->d.UVo, 0, 2 * (size_t)h->ntiles * h->NC * sizeof(double), h->stream));
+    if (!act.empty())
+        hipLaunchKernelGGL(uv_init_kernel, dim3(std::min((int)act.size(), VOL_BLOCKS_MAX)), dim3(VOL_NT), 0,
+                           h->stream, h->d_args, h->d_par);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->t = t;
@@ -2729,6 +2744,40 @@ int kst_volume(kst_handle* h, double out[6]) {
     out[3] = bed * rhob;
     out[4] = solids * g.rhos;
     out[5] = bed * g.rhos * (1.0 - g.bed_porosity);
+    return KST_OK;
+}
+
+int kst_get_output(kst_handle* h, int32_t max_tiles, double* fields, int32_t* tiles, int32_t* n_tiles) {
+    if (!h || !fields || !tiles || !n_tiles) return KST_ERR_CONFIG;
+    if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    const int na = c.n_active;
+    if (na > max_tiles)
+        return set_err(h, KST_ERR_CONFIG, "kst_get_output: max_tiles is smaller than the active set (" +
+                                              std::to_string(na) + " tiles)");
+    *n_tiles = na;
+    if (na == 0) return KST_OK;
+    const size_t need = (size_t)na * OUT_COLS * h->NC;
+    if (need > h->out_cap) {
+        if (h->out_buf) {
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            HIPCHK(h, hipFree(h->out_buf));
+            h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)h->out_buf), h->allocs.end());
+            h->out_buf = nullptr;
+            h->out_cap = 0;
+        }
+        rc = dalloc(h, &h->out_buf, need);
+        if (rc) return rc;
+        h->out_cap = need;
+    }
+    const int nb = std::min(na, VOL_BLOCKS_MAX);
+    hipLaunchKernelGGL(output_kernel, dim3(nb), dim3(VOL_NT), 0, h->stream, h->d_args, h->d_par, h->out_buf);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(fields, h->out_buf, need * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(tiles, h->d.active, (size_t)na * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return KST_OK;
 }
 

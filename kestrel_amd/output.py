@@ -4,11 +4,23 @@ CalculateVolume (src/Output.f90:617-735) appends one line per output time to
 Volume.txt: the time as F12.2, then six ES24.15 columns separated by ', ',
 under a header of right-justified names (src/Output.f90:664-673, 720-727).
 The six values come from the device (`Stepper.volume`, i.e. kst_volume).
+
+OutputSolutionData_txt (src/Output.f90:754-929) writes, for the active tiles
+in list order and their cells row by row, the tile number (I8) and 19
+ES18.10E3 columns, with an empty record after each row of a tile; and a
+`_topo` file of the tiles' vertex b0 and bt.  The per-cell columns come from
+the device (`Stepper.output_fields`, i.e. kst_get_output); the coordinates
+are GridToPhysical's (src/Grid.f90:339-353); latitude and longitude are 0
+(no georeferencing).
 """
 from __future__ import annotations
 
 import math
 import os
+
+import numpy as np
+
+from . import scenario
 
 VOLUME_COLUMNS = ("time", "volume", "total_bed_volume", "total_mass", "bed_mass",
                   "total_solids_mass", "bed_solids_mass")
@@ -67,3 +79,67 @@ def write_volume(path: str, t: float, cols, create: bool = False) -> None:
         if new:
             f.write(volume_header() + "\n")
         f.write(volume_line(t, cols) + "\n")
+
+
+SOLUTION_COLUMNS = ("tile", "x", "y", "latitude", "longitude", "flow_depth", "flow_speed",
+                    "x_velocity", "y_velocity", "density", "solids_fraction", "x_flux", "y_flux",
+                    "Hnpsi", "w", "base_elevation", "elevation_change", "x_slope", "y_slope")
+TOPO_COLUMNS = ("tile", "B0_vertex", "Bt_vertex")
+
+
+def solution_header() -> str:
+    """Header of the 2D solution file (format (a8,a2,18(a18,a2),a18))."""
+    return SOLUTION_COLUMNS[0].rjust(8) + "".join(", " + c.rjust(18) for c in SOLUTION_COLUMNS[1:])
+
+
+def topo_header() -> str:
+    """Header of the _topo file (format (a8, a2, a18, a2, a18))."""
+    return TOPO_COLUMNS[0].rjust(8) + "".join(", " + c.rjust(18) for c in TOPO_COLUMNS[1:])
+
+
+def _row(tile_id: int, vals) -> str:
+    return f"{tile_id:8d}" + "".join(", " + fortran_es(float(v), 18, 10, 3) for v in vals)
+
+
+def solution_lines(cfg, tiles, fields):
+    """The lines of the 2D solution file for tiles (0-based ids, list order)
+    and fields[q, 14, ny_per_tile, nx_per_tile] (columns 6-19)."""
+    xc, yc = scenario.cell_centres(cfg)
+    nx, ny, ntx = cfg.nx_per_tile, cfg.ny_per_tile, cfg.nx_tiles
+    out = [solution_header()]
+    for q, k in enumerate(tiles):
+        k = int(k)
+        gi, gj = k % ntx, k // ntx
+        for j in range(ny):
+            y = yc[gj * ny + j]
+            for i in range(nx):
+                out.append(_row(k + 1, [xc[gi * nx + i], y, 0.0, 0.0, *fields[q, :, j, i]]))
+            out.append("")
+    return out
+
+
+def topo_lines(cfg, tiles, b0v: np.ndarray, tile_bt: np.ndarray):
+    """The lines of the _topo file: each listed tile's vertices, row by row,
+    with its b0 (from the global vertex b0) and its own copy of bt
+    (tile_bt[k, ny+1, nx+1])."""
+    nx, ny, ntx = cfg.nx_per_tile, cfg.ny_per_tile, cfg.nx_tiles
+    out = [topo_header()]
+    for k in tiles:
+        k = int(k)
+        gi, gj = k % ntx, k // ntx
+        for j in range(ny + 1):
+            for i in range(nx + 1):
+                out.append(_row(k + 1, [b0v[gj * ny + j, gi * nx + i], tile_bt[k, j, i]]))
+            out.append("")
+    return out
+
+
+def write_solution_txt(path: str, cfg, tiles, fields, b0v: np.ndarray | None = None,
+                       tile_bt: np.ndarray | None = None) -> None:
+    """Write the solution file at path and, given b0v and tile_bt, its
+    path + '_topo' companion (both replaced if present)."""
+    with open(path, "w") as f:
+        f.write("\n".join(solution_lines(cfg, tiles, fields)) + "\n")
+    if b0v is not None and tile_bt is not None:
+        with open(path + "_topo", "w") as f:
+            f.write("\n".join(topo_lines(cfg, tiles, b0v, tile_bt)) + "\n")

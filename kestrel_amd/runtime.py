@@ -34,7 +34,7 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read", "kst_get_tile_bt",
            "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
-           "kst_get_cell_topography", "kst_volume", "kst_set_grid_cap"]
+           "kst_get_cell_topography", "kst_get_output", "kst_volume", "kst_set_grid_cap"]
 
 
 class KstStats(ctypes.Structure):
@@ -80,6 +80,9 @@ def lib():
         L.kst_set_grid_cap.restype = ctypes.c_int
         L.kst_volume.argtypes = [ctypes.c_void_p, _dp]
         L.kst_volume.restype = ctypes.c_int
+        L.kst_get_output.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32)]
+        L.kst_get_output.restype = ctypes.c_int
         L.kst_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.kst_profile_read.restype = ctypes.c_int
         L.kst_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
@@ -192,6 +195,19 @@ class Stepper:
         out = np.zeros(6)
         self._check(self.L.kst_volume(self.h, _ptr(out)))
         return out
+
+    def output_fields(self, max_tiles: int | None = None):
+        """(tile ids, fields[n, 14, ny_per_tile, nx_per_tile]): the per-cell
+        columns 6-19 of the solution text file for the active tiles, in
+        active-list order, computed on the device."""
+        c = self.cfg
+        cap = c.ntiles if max_tiles is None else int(max_tiles)
+        f = np.zeros((cap, 14, c.ny_per_tile, c.nx_per_tile))
+        t = np.zeros(cap, dtype=np.int32)
+        n = ctypes.c_int32(0)
+        self._check(self.L.kst_get_output(self.h, cap, _ptr(f), t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          ctypes.byref(n)))
+        return t[:n.value].copy(), f[:n.value].copy()
 
     def set_grid_cap(self, tiles: int):
         """Fix the launch grids' active-tile cap (0: follow the active set)."""

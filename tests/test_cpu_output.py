@@ -151,3 +151,68 @@ def test_write_volume_appends_after_the_header(tmp_path):
     assert float(lines[1].split()[1].replace(",", "")) == 1.0
     output.write_volume(p, 2.0, [2.0] * 6, create=True)
     assert len(open(p).read().splitlines()) == 2
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not present")
+def test_solution_format_matches_the_fortran_runtime(tmp_path):
+    """OutputSolutionData_txt's 2D header and row formats and the _topo
+    formats (src/Output.f90:812-838, 876-893, 908-918) written by flang's
+    runtime, against the writer's lines for the same values."""
+    rng = np.random.default_rng(7)
+    rows = []
+    for r in range(6):
+        v = rng.standard_normal(19) * 10.0 ** rng.integers(-120, 120, 19)
+        v[r] = 0.0
+        v[(r + 3) % 19] = 5e-324 if r % 2 else -1e-310
+        rows.append((int(rng.integers(1, 99999999)), v))
+    names = [f"'{c.rjust(18)}'" for c in output.SOLUTION_COLUMNS[1:]]
+    hdr = ", ', ', ".join(names)
+    src = f"""program sol
+  implicit none
+  real(8) :: v(19), b(2)
+  integer :: k, r
+  write (*, fmt="(a8,a2,18(a18,a2),a18)", advance='yes') '    tile', ', ', &
+     {hdr}
+  do r = 1, {len(rows)}
+     read (*, *) k, v
+     write (*, fmt="(i8, a2, 18(ES18.10E3, a2), ES18.10E3)", advance='yes') k, ', ', &
+        v(1), ', ', v(2), ', ', v(3), ', ', v(4), ', ', v(5), ', ', v(6), ', ', v(7), ', ', &
+        v(8), ', ', v(9), ', ', v(10), ', ', v(11), ', ', v(12), ', ', v(13), ', ', v(14), ', ', &
+        v(15), ', ', v(16), ', ', v(17), ', ', v(18), ', ', v(19)
+     write (*, *)
+  end do
+  write (*, fmt="(a8, a2, a18, a2, a18)", advance='yes') '    tile', ', ', '         B0_vertex', ', ', &
+     '         Bt_vertex'
+  read (*, *) k, b
+  write (*, fmt="(i8, a2, ES18.10E3, a2, ES18.10E3)", advance='yes') k, ', ', b(1), ', ', b(2)
+end program
+"""
+    f90 = tmp_path / "sol.f90"
+    f90.write_text(src)
+    exe = tmp_path / "sol"
+    subprocess.run([FLANG, str(f90), "-o", str(exe)], check=True, capture_output=True)
+    stdin = "".join(f"{k} " + " ".join(repr(float(x)) for x in v) + "\n" for k, v in rows)
+    stdin += "17 -2.5e-3 1.25e101\n"
+    got = subprocess.run([str(exe)], input=stdin, check=True, capture_output=True,
+                         text=True).stdout.splitlines()
+    want = [output.solution_header()]
+    for k, v in rows:
+        want += [output._row(k, v), ""]
+    want += [output.topo_header(), output._row(17, [-2.5e-3, 1.25e101])]
+    assert got == want
+
+
+def test_solution_lines_layout():
+    """Tiles in list order, rows j outer and cells i inner, an empty record
+    after each row, tile number id + 1, GridToPhysical coordinates."""
+    cfg = C.default_config(3, 3, 4, 4, dx=2.0)
+    C.derive(cfg)
+    tiles = np.array([4, 1])
+    fields = np.arange(2 * 14 * 16, dtype=float).reshape(2, 14, 4, 4)
+    lines = output.solution_lines(cfg, tiles, fields)
+    assert len(lines) == 1 + 2 * 4 * 5
+    first = [float(x) for x in lines[1].split(",")]
+    xc, yc = S.cell_centres(cfg)
+    assert first[0] == 5 and first[1] == pytest.approx(xc[4]) and first[2] == pytest.approx(yc[4])
+    assert first[5:] == pytest.approx(list(fields[0, :, 0, 0]))
+    assert lines[5] == "" and lines[21].startswith("       2,")

@@ -142,3 +142,69 @@ def test_conservation_at_full_size(gpu, tmp_path):
             exact = math.fsum(t[q]) * a
             assert ulps(v1[col], exact, exact) <= 3, (wl.name, col, v1[col], exact)
         g.close()
+
+
+# kst_get_output column -> oracle fields13 index (W RHU RHV HNPSI HN UU VV PSI RHO B0 BT DBDX DBDY)
+OUT_TO_F13 = {0: 4, 2: 5, 3: 6, 4: 8, 5: 7, 6: 1, 7: 2, 8: 3, 9: 0, 10: 9, 11: 10, 12: 11, 13: 12}
+
+
+@pytest.mark.parametrize("case", ["set_state", "plain", "morpho", "tile_size"])
+def test_output_fields_against_oracle(gpu, case, tmp_path):
+    """OutputSolutionData_txt's columns from the device against tileContainer's
+    u in the oracle, bit for bit -- velocities included, which the reference
+    writes as its last RHS derived them, before the final implicit drag
+    update -- and the files written from both the same text."""
+    if case == "morpho":
+        wl, steps = workloads.morpho_cap(128, 0, tile=16, radius=10.0, erosion="simple",
+                                         deposition="simple", transition="step", damping="rat3"), 40
+    elif case == "tile_size":
+        wl, steps = workloads.plain_cap(120, 0, tile=8, cap_x=0.0, cap_radius=10.0), 60
+    else:
+        wl, steps = workloads.plain_cap(128, 0, tile=16, cap_x=0.0, cap_radius=10.0), \
+            (0 if case == "set_state" else 50)
+    g = stepper_for(wl)
+    o = oracle_for(wl)
+    if steps:
+        g.step_n(steps)
+        o.step_n(steps)
+    tiles, F = g.output_fields()
+    act, _ = o.tile_lists()
+    assert list(tiles) == act
+    f13 = o.fields13()
+    cfg = wl.cfg
+    nx, ny, ntx = cfg.nx_per_tile, cfg.ny_per_tile, cfg.nx_tiles
+    G = np.zeros_like(F)
+    for q, k in enumerate(act):
+        gi, gj = k % ntx, k // ntx
+        sl = (slice(gj * ny, (gj + 1) * ny), slice(gi * nx, (gi + 1) * nx))
+        for col, idx in OUT_TO_F13.items():
+            G[q, col] = f13[idx][sl]
+        # FlowSquaredSpeedSlopeAligned (src/Closures.f90:178-205) of the stored u, v
+        u, v, bx, by, hn = G[q, 2], G[q, 3], G[q, 12], G[q, 13], G[q, 0]
+        m = u * u * (1.0 + bx * bx)
+        m = m + v * v * (1.0 + by * by) + 2.0 * bx * by * u * v
+        with np.errstate(invalid="ignore"):
+            G[q, 1] = np.where(hn > cfg.height_threshold, np.sqrt(m), 0.0)
+    for col in range(14):
+        assert np.array_equal(F[:, col], G[:, col]), (case, col)
+    if case in ("plain", "tile_size"):
+        # the velocities are not the final momentum's: the drag update came after
+        hn, ru, rho = F[:, 0], F[:, 6], F[:, 4]
+        eps = cfg.height_threshold
+        fresh = 2.0 * hn * ru / (hn * hn + np.maximum(hn * hn, eps * eps)) / rho
+        assert np.any(fresh != F[:, 2])
+    pd, po = str(tmp_path / "dev.txt"), str(tmp_path / "ora.txt")
+    output.write_solution_txt(pd, cfg, tiles, F, wl.b0v, g.tile_bt())
+    output.write_solution_txt(po, cfg, act, G, wl.b0v, o.tile_bt())
+    assert open(pd).read() == open(po).read()
+    assert open(pd + "_topo").read() == open(po + "_topo").read()
+    g.close()
+
+
+def test_output_rejects_a_short_buffer(gpu):
+    wl = workloads.plain_cap(128, 0, tile=16, cap_x=0.0, cap_radius=10.0)
+    g = stepper_for(wl)
+    tiles, _ = g.output_fields()
+    with pytest.raises(Exception):
+        g.output_fields(max_tiles=len(tiles) - 1)
+    g.close()
