@@ -12,6 +12,10 @@ ES18.10E3 columns, with an empty record after each row of a tile; and a
 the device (`Stepper.output_fields`, i.e. kst_get_output); the coordinates
 are GridToPhysical's (src/Grid.f90:339-353); latitude and longitude are 0
 (no georeferencing).
+
+OutputMaximums_txt (src/Output.f90:1270-1359) writes the running maxima the
+same way, with an empty record after each row and after each tile; they come
+from the device (`Stepper.maxima`, i.e. kst_get_maxima).
 """
 from __future__ import annotations
 
@@ -143,3 +147,40 @@ def write_solution_txt(path: str, cfg, tiles, fields, b0v: np.ndarray | None = N
     if b0v is not None and tile_bt is not None:
         with open(path + "_topo", "w") as f:
             f.write("\n".join(topo_lines(cfg, tiles, b0v, tile_bt)) + "\n")
+
+
+MAXIMA_COLUMNS = ("tile", "x_distance", "y_distance", "latitude", "longitude", "max_depth",
+                  "t_max_depth", "max_speed", "t_max_speed", "max_erosion", "t_max_erosion",
+                  "max_deposit", "t_max_deposit", "max_solids_frac", "t_max_solids_frac",
+                  "inundation_time")
+
+
+def maxima_header() -> str:
+    """Header of the maxima file (format (a8,a2,14(a18,a2), a18))."""
+    return MAXIMA_COLUMNS[0].rjust(8) + "".join(", " + c.rjust(18) for c in MAXIMA_COLUMNS[1:])
+
+
+def maxima_lines(cfg, tiles, maxima):
+    """The lines of the maxima file for tiles (0-based ids, list order) and
+    the 11 global NY x NX arrays of kst_get_maxima (Hnmax, its time, umax, its
+    time, emax, its time, dmax, its time, psimax, its time, tfirst)."""
+    xc, yc = scenario.cell_centres(cfg)
+    nx, ny, ntx = cfg.nx_per_tile, cfg.ny_per_tile, cfg.nx_tiles
+    out = [maxima_header()]
+    for k in tiles:
+        k = int(k)
+        gi, gj = k % ntx, k // ntx
+        for j in range(ny):
+            J = gj * ny + j
+            for i in range(nx):
+                I = gi * nx + i
+                out.append(_row(k + 1, [xc[I], yc[J], 0.0, 0.0, *(m[J, I] for m in maxima)]))
+            out.append("")
+        out.append("")
+    return out
+
+
+def write_maxima_txt(path: str, cfg, tiles, maxima) -> None:
+    """Write the maxima file at path (replaced if present)."""
+    with open(path, "w") as f:
+        f.write("\n".join(maxima_lines(cfg, tiles, maxima)) + "\n")

@@ -216,3 +216,48 @@ def test_solution_lines_layout():
     assert first[0] == 5 and first[1] == pytest.approx(xc[4]) and first[2] == pytest.approx(yc[4])
     assert first[5:] == pytest.approx(list(fields[0, :, 0, 0]))
     assert lines[5] == "" and lines[21].startswith("       2,")
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not present")
+def test_maxima_format_matches_the_fortran_runtime(tmp_path):
+    """OutputMaximums_txt's header, row format and empty records
+    (src/Output.f90:1299-1352) written by flang's runtime for a 1-tile grid
+    of 2 x 2 cells, against maxima_lines."""
+    cfg = C.default_config(3, 3, 2, 2, dx=1.0)
+    C.derive(cfg)
+    rng = np.random.default_rng(3)
+    maxima = [rng.standard_normal((cfg.NY, cfg.NX)) * 10.0 ** rng.integers(-50, 50) for _ in range(11)]
+    maxima[10][0, 0] = -1.0
+    xc, yc = S.cell_centres(cfg)
+    names = ", ', ', ".join(f"'{c.rjust(18)}'" for c in output.MAXIMA_COLUMNS[1:])
+    src = f"""program mx
+  implicit none
+  real(8) :: v(15)
+  integer :: i, j
+  write (*, fmt="(a8,a2,14(a18,a2), a18)", advance='yes') '    tile', ', ', &
+     {names}
+  do j = 1, 2
+     do i = 1, 2
+        read (*, *) v
+        write (*, fmt="(i8,a2,14(ES18.10E3, a2), ES18.10E3)", advance='yes') 5, ', ', &
+           v(1), ', ', v(2), ', ', v(3), ', ', v(4), ', ', v(5), ', ', v(6), ', ', v(7), ', ', &
+           v(8), ', ', v(9), ', ', v(10), ', ', v(11), ', ', v(12), ', ', v(13), ', ', v(14), ', ', v(15)
+     end do
+     write (*, *)
+  end do
+  write (*, *)
+end program
+"""
+    f90 = tmp_path / "mx.f90"
+    f90.write_text(src)
+    exe = tmp_path / "mx"
+    subprocess.run([FLANG, str(f90), "-o", str(exe)], check=True, capture_output=True)
+    stdin = ""
+    for j in range(2):
+        for i in range(2):
+            J, I = 1 * 2 + j, 1 * 2 + i
+            vals = [xc[I], yc[J], 0.0, 0.0] + [m[J, I] for m in maxima]
+            stdin += " ".join(repr(float(v)) for v in vals) + "\n"
+    got = subprocess.run([str(exe)], input=stdin, check=True, capture_output=True,
+                         text=True).stdout.splitlines()
+    assert got == output.maxima_lines(cfg, [4], maxima)

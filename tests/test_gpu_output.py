@@ -208,3 +208,26 @@ def test_output_rejects_a_short_buffer(gpu):
     with pytest.raises(Exception):
         g.output_fields(max_tiles=len(tiles) - 1)
     g.close()
+
+
+@pytest.mark.parametrize("case", ["plain", "morpho"])
+def test_maxima_file_matches_oracle(gpu, tmp_path, case):
+    """OutputMaximums_txt from the device maxima and the oracle's: the same text."""
+    if case == "morpho":
+        wl = workloads.morpho_cap(128, 0, tile=16, radius=10.0, erosion="simple",
+                                  deposition="simple", transition="step", damping="rat3")
+    else:
+        wl = workloads.plain_cap(128, 0, tile=16, cap_x=0.0, cap_radius=10.0)
+    g = stepper_for(wl)
+    o = oracle_for(wl)
+    g.step_n(40)
+    o.step_n(40)
+    tiles, _ = g.output_fields()
+    act, _ = o.tile_lists()
+    pd, po = str(tmp_path / "dev_max.txt"), str(tmp_path / "ora_max.txt")
+    output.write_maxima_txt(pd, wl.cfg, tiles, g.maxima())
+    output.write_maxima_txt(po, wl.cfg, act, o.maxima())
+    text = open(pd).read()
+    assert text == open(po).read()
+    assert len(text.splitlines()) == 1 + len(act) * (wl.cfg.ny_per_tile * (wl.cfg.nx_per_tile + 1) + 1)
+    g.close()
