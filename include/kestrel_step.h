@@ -178,6 +178,12 @@ int kst_get_cell_topography(kst_handle* h, double* b0c, double* btc, double* dbd
  * psimax, psimax_t, tfirst. */
 int kst_get_maxima(kst_handle* h, double* const maxima[11]);
 
+/* Running maxima to resume from, in kst_get_maxima's order and layout (NULL
+ * entries leave that array as it is): what LoadInitialCondition_txt reads
+ * from the maxima file on a restart (src/Restart.f90:290-326).  Call after
+ * kst_set_state, which starts the maxima afresh. */
+int kst_set_maxima(kst_handle* h, const double* const maxima[11]);
+
 /* CalculateVolume (src/Output.f90:617-735), computed on the device from the
  * resident state: out[0..5] = the Volume.txt columns 2-7 -- flow volume,
  * total bed volume, total mass, bed mass, total solids mass, bed solids

@@ -2792,6 +2792,18 @@ int kst_get_maxima(kst_handle* h, double* const mx[11]) {
     return KST_OK;
 }
 
+int kst_set_maxima(kst_handle* h, const double* const mx[11]) {
+    if (!h || !mx) return KST_ERR_CONFIG;
+    const size_t FS = (size_t)h->ntiles * h->NC;
+    std::vector<double> M(11 * FS);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(M.data(), h->d.maxima, M.size() * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 11; ++q)
+        if (mx[q]) to_tiles(h, mx[q], M.data() + q * FS);
+    HIPCHK(h, hipMemcpy(h->d.maxima, M.data(), M.size() * 8, hipMemcpyHostToDevice));
+    return KST_OK;
+}
+
 int kst_get_tile_bt(kst_handle* h, double* bt_tiles) {
     if (!h || !bt_tiles) return KST_ERR_CONFIG;
     const size_t n = (size_t)(h->nx + 1) * (h->ny + 1) * h->ntiles;

@@ -13,7 +13,7 @@ module kestrel_step_binding
    public :: kst_config, kst_flux_source, kst_stats
    public :: kst_config_defaults, kst_config_derive, kst_create, kst_set_topography
    public :: kst_set_state, kst_integrate_to, kst_step_n, kst_get_state, kst_get_maxima
-   public :: kst_last_error, kst_destroy, kst_set_grid_cap, kst_volume, kst_get_output
+   public :: kst_last_error, kst_destroy, kst_set_grid_cap, kst_volume, kst_get_output, kst_set_maxima
    public :: KST_TILE_OFF, KST_TILE_ACTIVE, KST_TILE_GHOST
 
    integer(c_int8_t), parameter :: KST_TILE_OFF = 0, KST_TILE_ACTIVE = 1, KST_TILE_GHOST = 2
@@ -141,6 +141,12 @@ module kestrel_step_binding
          real(c_double), intent(out) :: fields(*)
          integer(c_int32_t), intent(out) :: tiles(*)
          integer(c_int32_t), intent(out) :: n_tiles
+         integer(c_int) :: rc
+      end function
+      function kst_set_maxima(h, maxima) bind(C, name="kst_set_maxima") result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: h
+         type(c_ptr), intent(in) :: maxima(11)
          integer(c_int) :: rc
       end function
       function kst_volume(h, out) bind(C, name="kst_volume") result(rc)

@@ -34,7 +34,7 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read", "kst_get_tile_bt",
            "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
-           "kst_get_cell_topography", "kst_get_output", "kst_volume", "kst_set_grid_cap"]
+           "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap"]
 
 
 class KstStats(ctypes.Structure):
@@ -83,6 +83,8 @@ def lib():
         L.kst_get_output.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_int32)]
         L.kst_get_output.restype = ctypes.c_int
+        L.kst_set_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
+        L.kst_set_maxima.restype = ctypes.c_int
         L.kst_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.kst_profile_read.restype = ctypes.c_int
         L.kst_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
@@ -222,6 +224,14 @@ class Stepper:
         n = (ctypes.c_int64 * PROFILE_KERNELS)()
         self._check(self.L.kst_profile_read(self.h, ms, n))
         return {name: (ms[k], n[k]) for k, name in enumerate(PROFILE_NAMES) if n[k]}
+
+    def set_maxima(self, maxima) -> None:
+        """Resume the running maxima from 11 NY x NX arrays (None: leave)."""
+        arrs = [None if m is None else np.ascontiguousarray(m, dtype=np.float64) for m in maxima]
+        if len(arrs) != 11:
+            raise ValueError("maxima: 11 arrays")
+        ptrs = (_dp * 11)(*[_ptr(a) for a in arrs])
+        self._check(self.L.kst_set_maxima(self.h, ptrs))
 
     def maxima(self) -> list:
         c = self.cfg

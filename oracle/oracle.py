@@ -146,13 +146,14 @@ class Oracle:
     def finish_sources(self):
         self._check(self.L.ko_finish_sources(self.h))
 
-    def set_state(self, st: "scenario.InitialState", t=0.0):
+    def set_state(self, st: "scenario.InitialState", t=0.0, btv=None):
         ts = np.ascontiguousarray(st.tile_state, dtype=np.uint8)
+        bt = None if btv is None else np.ascontiguousarray(btv, dtype=np.float64)
+        hm = None if st.hnmax0 is None else np.ascontiguousarray(st.hnmax0)
         self._check(self.L.ko_set_state(
             self.h, _ptr(np.ascontiguousarray(st.w)), _ptr(np.ascontiguousarray(st.rhoHnu)),
-            _ptr(np.ascontiguousarray(st.rhoHnv)), _ptr(np.ascontiguousarray(st.Hnpsi)), None,
-            ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), float(t),
-            _ptr(np.ascontiguousarray(st.hnmax0))))
+            _ptr(np.ascontiguousarray(st.rhoHnv)), _ptr(np.ascontiguousarray(st.Hnpsi)), _ptr(bt),
+            ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), float(t), _ptr(hm)))
 
     def integrate_to(self, tend):
         st = KstStats()
