@@ -14,6 +14,15 @@
  * arrays, a tileContainer plus four intermed containers refreshed by deep
  * copies, a sorted active-tile list and ghost tiles holding frozen data.
  *
+ * Pinning: the hydraulic and morphodynamic RHS, topography and
+ * redistribution equal the reference's own compiled Fortran (oracle/_ref)
+ * bit for bit on the committed vectors; the stepper, activation, maxima and
+ * ko_volume are pinned by conservation (the reference's 1e-10 check), exact
+ * sums, closed forms, symmetry, tile-layout independence and planted
+ * mistakes.  Parity unpinned: the last-ulp behaviour of tanh, log, pow and
+ * cos (glibc here, the ROCm device library on the GPU) in the closures that
+ * call them; decks using them are compared to 1e-9 relative.
+ *
  * Field indices (src/main.f90:76-103), 0-based here.
  */
 #include "kestrel_oracle.h"
