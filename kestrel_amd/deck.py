@@ -255,6 +255,115 @@ def _solver(d: Deck, c: C.KstConfig):
     return tend, tstart
 
 
+# The rest of the TopogFunc family (src/TopogFuncs.f90:95-390, 509-601), each
+# evaluated per vertex with the C library's transcendentals (Python's math
+# module), in the reference's operation order.  name -> (parameter count, f)
+# with f(params, x, y) -> b0; most depend on x only.
+def _x2slopes(p, x, y):                       # src/TopogFuncs.f90:101-135
+    alpha, beta, R = p[0], p[1], p[2]
+    xc0 = (math.sqrt(1.0 + alpha * alpha) - math.sqrt(1.0 + beta * beta)) * R / (alpha - beta)
+    zc0 = (alpha * math.sqrt(1.0 + beta * beta) - beta * math.sqrt(1.0 + alpha * alpha)) * R / (alpha - beta)
+    x1 = xc0 - alpha * R / math.sqrt(1.0 + alpha * alpha)
+    x2 = xc0 - beta * R / math.sqrt(1.0 + beta * beta)
+    if x < x1:
+        return -(alpha * x)
+    if x > x2:
+        return -(beta * x)
+    return zc0 - math.sqrt(R * R - (x - xc0) * (x - xc0))
+
+
+def _flume_like(theta0, theta1, xwall, ylo, yhi, wallH, sigma, x, y):
+    """usgs / flume (src/TopogFuncs.f90:145-242): a catenary between two
+    slopes along x, and side walls of tanh profile for x < xwall."""
+    pi = math.pi
+    alpha = 8.5 / (math.asinh(-math.tan(4.0 * pi / 180.0)) - math.asinh(-math.tan(theta0 * pi / 180.0)))
+    xc0 = -(alpha * math.asinh(-math.tan(theta0 * pi / 180.0)))
+    zc0 = -(alpha * math.cosh((-xc0) / alpha))
+    x1 = xc0 + alpha * math.asinh(-math.tan(theta1 * pi / 180.0))
+    if x < 0.0:
+        b = -(math.tan(theta0 * pi / 180.0) * x)
+    elif x > x1:
+        b = zc0 + alpha * math.cosh((x1 - xc0) / alpha) - math.tan(theta1 * pi / 180.0) * (x - x1)
+    else:
+        b = zc0 + alpha * math.cosh((x - xc0) / alpha)
+    if x < xwall:
+        b = b + 0.5 * wallH * (math.tanh(sigma * (y - ylo)) - math.tanh(sigma * (y - yhi))
+                               + math.tanh(sigma * (y + yhi)) - math.tanh(sigma * (y + ylo)))
+    return b
+
+
+def _usgs(p, x, y):                           # src/TopogFuncs.f90:145-183
+    return _flume_like(31.0, 2.4, 8.5, 1.0, 2.0, p[0], p[1], x, y)
+
+
+def _flume(p, x, y):                          # src/TopogFuncs.f90:197-242
+    return _flume_like(p[0], p[1], p[2], 0.5 * p[3], 1.5 * p[3], p[4], p[5], x, y)
+
+
+def _channel_powerlaw(p, x, y):               # src/TopogFuncs.f90:255-276
+    slope, W, alpha = p[0], p[1], p[2]
+    return slope * x + math.cos(math.atan(slope)) * math.pow(abs(y) / W, alpha)
+
+
+def _channel_trapezium(p, x, y):              # src/TopogFuncs.f90:288-308
+    slope, W, Sb = p[0], p[1], p[2]
+    return slope * x + math.cos(math.atan(slope)) * max(0.0, Sb * (abs(y) - 0.5 * W))
+
+
+def _xbislope(p, x, y):                       # src/TopogFuncs.f90:315-337
+    a1, a2, lam = math.tan(p[0] * math.pi / 180.0), math.tan(p[1] * math.pi / 180.0), p[2]
+    return -(0.5 * (a1 + a2) * x) + 0.5 * (a1 - a2) * lam * math.log(math.cosh(x / lam))
+
+
+def _xtrislope(p, x, y):                      # src/TopogFuncs.f90:346-388
+    pi = math.pi
+    s1, s2, s3 = (math.tan(p[k] * pi / 180.0) for k in range(3))
+    lam, x1, x2 = p[3], p[4], p[5]
+    c2 = (x1 - 0.5 * lam) * 0.5 * (s1 - s2)
+    c3 = (x1 + 0.5 * lam) * 0.5 * (s1 - s2) + c2
+    c4 = (x2 - 0.5 * lam) * 0.5 * (s2 - s3) + c3
+    c5 = (x2 + 0.5 * lam) * 0.5 * (s2 - s3) + c4
+    if x < x1 - 0.5 * lam:
+        return s1 * x
+    if x < x1 + 0.5 * lam:
+        A = 0.5 * (s2 - s1) * lam / pi
+        return A * math.sin((x - x1) * pi / lam - 0.5 * pi) + 0.5 * (s1 + s2) * x + c2
+    if x < x2 - 0.5 * lam:
+        return s2 * x + c3
+    if x < x2 + 0.5 * lam:
+        A = 0.5 * (s3 - s2) * lam / pi
+        return A * math.sin((x - x2) * pi / lam - 0.5 * pi) + 0.5 * (s2 + s3) * x + c4
+    return s3 * x + c5
+
+
+def _xhump(p, x, y):                          # src/TopogFuncs.f90:513-533
+    A, L = p[0], p[1]
+    return 0.5 * A * (1.0 + math.cos(math.pi * x / L)) if (-L < x < L) else 0.0
+
+
+def _xtanh(p, x, y):                          # src/TopogFuncs.f90:540-557
+    return p[1] * (1.0 + math.tanh((x - p[0]) / p[2]))
+
+
+def _xyparab(p, x, y):                        # src/TopogFuncs.f90:583-601
+    return p[0] * x * x + p[1] * y * y
+
+
+_TOPOG_FUNCS = {"x2slopes": (3, _x2slopes), "usgs": (2, _usgs), "flume": (6, _flume),
+                "channel power law": (3, _channel_powerlaw),
+                "channel trapezium": (3, _channel_trapezium), "xbislope": (3, _xbislope),
+                "xtrislope": (6, _xtrislope), "xhump": (2, _xhump), "xtanh": (3, _xtanh),
+                "xyparab": (2, _xyparab)}
+
+
+def topog_function(name: str, params, xv, yv) -> np.ndarray:
+    """A TopogFunc of the table above at vertex coordinates xv, yv:
+    b0[len(yv), len(xv)]."""
+    f = _TOPOG_FUNCS[name][1]
+    p = [float(v) for v in params]
+    return np.array([[f(p, float(x), float(y)) for x in xv] for y in yv], dtype=np.float64)
+
+
 def _topography(d: Deck, c: C.KstConfig) -> np.ndarray:
     """Function topographies at the vertices (GetHeights with TopogFunc,
     src/dem.f90:360-393; src/TopogFuncs.f90:395-577)."""
@@ -268,11 +377,13 @@ def _topography(d: Deck, c: C.KstConfig) -> np.ndarray:
     X, Y = np.meshgrid(xv, yv)
     Lx, Ly = c.x_size, c.y_size
     need = {"flat": 0, "xslope": 1, "yslope": 1, "xyslope": 2, "xsinslope": 1, "xysinslope": 1,
-            "xparab": 1}
+            "xparab": 1, **{k: v[0] for k, v in _TOPOG_FUNCS.items()}}
     if name not in need:
         raise DeckUnsupported(f"Topog function = {name}")
     if len(prm) < need[name]:
         raise DeckError(f"{need[name]} 'Topog params' needed for '{name}'; received {len(prm)}")
+    if name in _TOPOG_FUNCS:
+        return topog_function(name, prm, xv, yv)
     if name == "flat":
         b = np.zeros_like(X)
     elif name == "xslope":

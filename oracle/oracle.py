@@ -320,8 +320,26 @@ def ref_lib():
         L.kref_redistribute.restype = None
         L.kref_redistribute.argtypes = [ctypes.POINTER(KstConfig), u8, _dp, _dp, _dp, _dp, _dp,
                                         ctypes.c_int32, i32p, i32p, i32p, _dp, _dp, _dp, i32p]
+        L.kref_topog.restype = None
+        L.kref_topog.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp]
         _REF = L
     return _REF
+
+
+REF_TOPOG = ["x2slopes", "usgs", "flume", "channel power law", "channel trapezium", "xbislope",
+             "xtrislope", "xhump", "xtanh", "xyparab"]
+
+
+def ref_topog(name: str, params, xv, yv):
+    """The reference's TopogFunc `name` at vertex coordinates xv, yv:
+    b0[len(yv), len(xv)] (row-major, y outer)."""
+    L = ref_lib()
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(xv, dtype=np.float64)
+    y = np.ascontiguousarray(yv, dtype=np.float64)
+    out = np.zeros((len(y), len(x)))     # Fortran b0(nx, ny) in column order = [ny][nx] row-major
+    L.kref_topog(REF_TOPOG.index(name) + 1, _ptr(prm), len(prm), _ptr(x), len(x), _ptr(y), len(y), _ptr(out))
+    return out
 
 
 def ref_available() -> bool:

@@ -21,6 +21,7 @@ module kestrel_ref_driver
    use limiters_module
    use closures_module
    use kestrel_step_binding, only: kst_config, kst_flux_source
+   use topog_funcs_module
    implicit none
    private
    public :: kref_rhs, kref_morpho_rhs, kref_redistribute
@@ -512,5 +513,32 @@ contains
       refine = merge(1, 0, ref)
       call gather_container(c, tstate, grid%intermed3, fout, btout)
    end subroutine kref_redistribute
+
+   ! One of the reference's TopogFunc subroutines (src/TopogFuncs.f90) on the
+   ! vertex coordinates x(nxv), y(nyv) with its 'Topog params': b0(nxv, nyv).
+   ! which: 1 x2slopes, 2 usgs, 3 flume, 4 channel_powerlaw, 5 channel_trapezium,
+   ! 6 xbislope, 7 xtrislope, 8 xhump, 9 xtanh, 10 xyparab.
+   subroutine kref_topog(which, params, nparams, x, nxv, y, nyv, b0) bind(C, name="kref_topog")
+      integer(c_int), value :: which, nparams, nxv, nyv
+      real(c_double), intent(in) :: params(nparams), x(nxv), y(nyv)
+      real(c_double), intent(out) :: b0(nxv, nyv)
+      ! the module's RunSet (a local one would need the utm module's procedures)
+      if (allocated(rp%TopogFuncParams)) deallocate (rp%TopogFuncParams)
+      allocate (rp%TopogFuncParams(nparams))
+      rp%TopogFuncParams = params
+      select case (which)
+      case (1); call x2slopes(rp, x, y, b0)
+      case (2); call usgs(rp, x, y, b0)
+      case (3); call flume(rp, x, y, b0)
+      case (4); call channel_powerlaw(rp, x, y, b0)
+      case (5); call channel_trapezium(rp, x, y, b0)
+      case (6); call xbislope(rp, x, y, b0)
+      case (7); call xtrislope(rp, x, y, b0)
+      case (8); call xhump(rp, x, y, b0)
+      case (9); call xtanh(rp, x, y, b0)
+      case (10); call xyparab(rp, x, y, b0)
+      case default; b0 = 0.0_wp
+      end select
+   end subroutine kref_topog
 
 end module kestrel_ref_driver
