@@ -320,6 +320,8 @@ def ref_lib():
         L.kref_redistribute.restype = None
         L.kref_redistribute.argtypes = [ctypes.POINTER(KstConfig), u8, _dp, _dp, _dp, _dp, _dp,
                                         ctypes.c_int32, i32p, i32p, i32p, _dp, _dp, _dp, i32p]
+        L.kref_bicubic.restype = None
+        L.kref_bicubic.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
         L.kref_topog.restype = None
         L.kref_topog.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp]
         _REF = L
@@ -440,3 +442,15 @@ def volume_from_state(cfg: KstConfig, b0v: np.ndarray, state: dict) -> dict:
     a = cfg.dx * cfg.dy
     return {"volume": vol * cfg.dx * cfg.dy, "mass": mass * cfg.dx * cfg.dy,
             "bed": bed * cfg.dx * cfg.dy, "solids": solids * cfg.dx * cfg.dy, "cell_area": a}
+
+
+def ref_bicubic(z, x, y):
+    """The reference's Bicubic of z[nx, ny] (Fortran z(x, y), 1-based data
+    coordinates) at the points x, y."""
+    L = ref_lib()
+    zf = np.asfortranarray(z, dtype=np.float64)
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    ys = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(len(xs))
+    L.kref_bicubic(zf.ctypes.data_as(_dp), zf.shape[0], zf.shape[1], _ptr(xs), _ptr(ys), len(xs), _ptr(out))
+    return out

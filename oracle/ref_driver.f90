@@ -22,6 +22,7 @@ module kestrel_ref_driver
    use closures_module
    use kestrel_step_binding, only: kst_config, kst_flux_source
    use topog_funcs_module
+   use interpolate_2d_module, only: Bicubic
    implicit none
    private
    public :: kref_rhs, kref_morpho_rhs, kref_redistribute
@@ -540,5 +541,17 @@ contains
       case default; b0 = 0.0_wp
       end select
    end subroutine kref_topog
+
+   ! Bicubic (src/Interp2d.f90:214-282) of real data z(nzx, nzy) at the
+   ! npts points (x, y), 1-based data coordinates.
+   subroutine kref_bicubic(z, nzx, nzy, x, y, npts, out) bind(C, name="kref_bicubic")
+      integer(c_int), value :: nzx, nzy, npts
+      real(c_double), intent(in) :: z(nzx, nzy), x(npts), y(npts)
+      real(c_double), intent(out) :: out(npts)
+      integer :: k
+      do k = 1, npts
+         out(k) = Bicubic(x(k), y(k), z)
+      end do
+   end subroutine kref_bicubic
 
 end module kestrel_ref_driver

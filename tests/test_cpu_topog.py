@@ -1,14 +1,15 @@
-"""The TopogFunc family (src/TopogFuncs.f90) of the deck reader against the
-reference's own compiled subroutines: the committed vectors
+"""The TopogFunc family (src/TopogFuncs.f90) of the deck reader and the
+bicubic resampling of elevation data (src/Interp2d.f90) against the
+reference's own compiled routines: the committed vectors
 (tests/golden/make_topog_vectors.py) and, where oracle/_ref is built, fresh
-evaluations on another grid.  Bit for bit."""
+evaluations.  Bit for bit; the bicubic also against closed forms."""
 import os
 
 import numpy as np
 import pytest
 
-from kestrel_amd import deck
-from oracle.oracle import REF_TOPOG, ref_available, ref_topog
+from kestrel_amd import deck, dem
+from oracle.oracle import REF_TOPOG, ref_available, ref_bicubic, ref_topog
 
 GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "topog_funcs.npz"))
 
@@ -40,3 +41,38 @@ def test_deck_parameter_count_is_checked():
         from kestrel_amd import config as C
         c = C.default_config(4, 4, 8, 8)
         deck._topography(d, c)
+
+
+def test_bicubic_against_golden_vectors():
+    got = dem.bicubic(GOLD["bicubic_z"], GOLD["bicubic_x"], GOLD["bicubic_y"])
+    assert np.array_equal(got, GOLD["bicubic_out"])
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built")
+def test_bicubic_against_the_reference():
+    rng = np.random.default_rng(5)
+    z = rng.uniform(-50.0, 500.0, (40, 30))
+    x = np.concatenate([rng.uniform(2, 39, 2000), np.arange(2, 39).astype(float)])
+    y = np.concatenate([rng.uniform(2, 29, 2000), np.full(37, 7.0)])
+    assert np.array_equal(dem.bicubic(z, x, y), ref_bicubic(z, x, y))
+
+
+def test_bicubic_closed_forms():
+    """Exact at the data nodes; a bilinear-plus-quadratic surface (which the
+    central differences and the cubic Hermite patch reproduce) to rounding;
+    the coefficient matrix is integer and inverts the corner map."""
+    i = np.arange(1, 31, dtype=float)[:, None]
+    j = np.arange(1, 21, dtype=float)[None, :]
+    z = 3.0 + 2.0 * i - j + 0.5 * i * j + 0.25 * i * i - 0.125 * j * j
+    x = np.array([5.0, 7.25, 12.5, 20.75, 28.0])
+    y = np.array([4.0, 9.5, 3.25, 17.0, 12.75])
+    exact = 3.0 + 2.0 * x - y + 0.5 * x * y + 0.25 * x * x - 0.125 * y * y
+    assert np.allclose(dem.bicubic(z, x, y), exact, rtol=0, atol=1e-12)
+    assert np.array_equal(dem.bicubic(z, np.array([10.0]), np.array([6.0])), z[9:10, 5])
+    assert np.array_equal(dem.M, np.rint(dem.M)) and abs(np.linalg.det(dem.M)) > 0
+
+
+def test_bicubic_rejects_points_without_margin():
+    z = np.zeros((10, 10))
+    with pytest.raises(ValueError):
+        dem.bicubic(z, np.array([1.5]), np.array([5.0]))
