@@ -218,6 +218,16 @@ void kst_destroy(kst_handle* h);
  * 0 restores the default.  Results do not depend on the cap. */
 int kst_set_grid_cap(kst_handle* h, int tiles);
 
+/* Redistribution chunks (no counterpart in the reference, whose
+ * RedistributeGrid walks its sorted list one cell at a time,
+ * src/Redistribute.f90:203-244).  The device walks up to `chunk` sorted
+ * candidates at once (default and maximum 1024), chunk after chunk, and sorts
+ * up to `lds_sort` candidates (padded to a power of two; default and maximum
+ * 2048) in LDS, more in global memory.  Results do not depend on either; the
+ * tests lower them to run the multi-chunk walk and the global-memory sort on
+ * small candidate sets.  0 restores a default. */
+int kst_set_redistribution_chunk(kst_handle* h, int chunk, int lds_sort);
+
 /* Diagnostics (no counterpart in the reference, which has no timers;
  * src/Messages.f90:275-283 prints only t and dt).  While profiling is on, the
  * library launches one iteration at a time and brackets every kernel launch

@@ -150,6 +150,47 @@ def hydraulic_only(c: KstConfig) -> KstConfig:
     return c
 
 
+# Drag and erosion parameters: the reference gives each a default only when
+# the chosen closure is one that it names for it (src/Parameters.f90:494-596);
+# the others keep the value of a fresh RunSet, zero.  Closures may still read
+# them (FlowShieldsNumber reads ChezyCo, src/Closures.f90:221; the switch
+# functions read the Voellmy switch rate and value, src/Closures.f90:795-915).
+_DRAG_DEFAULTS = {
+    "chezy_co": (("chezy", "voellmy"), 0.01),
+    "coulomb_co": (("coulomb", "voellmy"), 0.1),
+    "manning_co": (("manning",), 0.03),
+    "pouliquen_min": (("pouliquen", "variable", "edwards2019"), 0.1),
+    "pouliquen_max": (("pouliquen", "variable", "edwards2019"), 0.4),
+    "pouliquen_beta": (("pouliquen", "variable", "edwards2019"), 0.136),
+    "pouliquen_intermediate": (("edwards2019",), 0.2),
+    "edwards_betastar": (("edwards2019",), 0.136),
+    "edwards_kappa": (("edwards2019",), 1.0),
+    "edwards_gamma": (("edwards2019",), 0.0),
+    "voellmy_switch_rate": (("variable",), 3.0),
+    "voellmy_switch_value": (("variable",), 0.2),
+}
+_EROSION_DEFAULTS = {
+    "ero_rate": (("fluid", "mixed"), 0.001),
+    "ero_rate_granular": (("granular", "mixed"), 4.0),
+    "ero_depth": (("simple", "fluid", "granular", "mixed"), 1.0),
+    "ero_critical_height": (("simple", "fluid", "granular", "mixed"), 0.01),
+}
+
+
+def reference_defaults(c: KstConfig, given=()) -> KstConfig:
+    """Drag and erosion parameters as the reference leaves them after reading
+    a deck that sets the fields in `given` (src/Parameters.f90:494-596): the
+    chosen closures' defaults, zero for the rest."""
+    drag = {v: k for k, v in DRAGS.items()}[c.drag]
+    ero = {v: k for k, v in EROSIONS.items() if k not in ("none",)}[c.erosion]
+    for table, choice in ((_DRAG_DEFAULTS, drag), (_EROSION_DEFAULTS, ero)):
+        for name, (users, default) in table.items():
+            if name in given:
+                continue
+            setattr(c, name, default if choice in users else 0.0)
+    return c
+
+
 def derive(c: KstConfig, settling_speed: float | None = None) -> KstConfig:
     """Derived parameters, in the reference's arithmetic (src/Parameters.f90:629-648)."""
     c.gred = (c.rhos / c.rhow - 1.0) * c.g

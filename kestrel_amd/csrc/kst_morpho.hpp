@@ -466,93 +466,8 @@ __device__ __forceinline__ int vertex_correction(const Params& p, int i, int j, 
     return 1;
 }
 
-// ExcessDeposition + RedistributeCell (src/Redistribute.f90:158-198,249-475),
-// 2D, for candidate cell (k, i, j), executed by one wavefront on intermed0 =
-// (SH, TA) and intermed3 = (SM[2], TB).  Every load is issued up front:
-// lanes 0..8 read the 3x3 cell around (i, j) they will update (cells do not
-// depend on one another; a corrected vertex is taken from registers), every
-// lane reads the candidate cell and its four vertices and evaluates the
-// excess and the vertex correction identically, lane 0 writes the vertex
-// copies of every tile that holds them.  In ghost tiles the reference writes
-// intermed3 too: vertex copies go to TB.vbt, cell w and Hnpsi to S3 (the
-// stage state the last RHS of each hydraulic operator reads); a ghost cell's
-// topography is recomputed where it is used and not stored, since only
-// intermed3 would hold it.  Returns 0 if no correction was needed, 1 if one
-// was made, -1 if the cell borders no depositional vertex (nothing written).
-// *ghost counts writes into ghost tiles.
-template <class DevT>
-__device__ int redistribute_cell(const DevT& d, const Params& p, int k, int i, int j, int lane,
-                                 unsigned long long* ghost, unsigned long long* nd) {
-    const long long FS = d.FS;
-    const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
-    const size_t NV = (size_t)(nx + 1) * (ny + 1);
-    double* bt3 = d.TB.vbt;
-    const double* bt0 = d.TA.vbt;
-    // ---- loads: the lane's 3x3 cell
-    int ct = k, ci = i, cj = j;
-    if (lane < 9) map_cell(p, k, i + lane / 3 - 1, j + lane % 3 - 1, ct, ci, cj);
-    const size_t g = (size_t)ct * p.NC + cj * nx + ci;
-    const double* vb = d.vb0 + (size_t)ct * NV;
-    const double* vt = bt3 + (size_t)ct * NV;
-    double c00 = 0, c10 = 0, c01 = 0, c11 = 0, e00 = 0, e10 = 0, e01 = 0, e11 = 0;
-    double cbt0 = 0, cgo = 1, cw0 = 0, ch0 = 0, cb0 = 0;
-    bool cact = false;
-    if (lane < 9) {
-        c00 = vt[cj * NV1 + ci]; c10 = vt[cj * NV1 + ci + 1];
-        c01 = vt[(cj + 1) * NV1 + ci]; c11 = vt[(cj + 1) * NV1 + ci + 1];
-        e00 = vb[cj * NV1 + ci]; e10 = vb[cj * NV1 + ci + 1];
-        e01 = vb[(cj + 1) * NV1 + ci]; e11 = vb[(cj + 1) * NV1 + ci + 1];
-        cbt0 = d.TA.btc[g]; cgo = d.TA.cgam[g]; cw0 = d.SH[g]; ch0 = d.SH[3 * FS + g]; cb0 = d.b0c[g];
-        cact = d.tstate[ct] == KST_TILE_ACTIVE;
-    }
-    // ---- loads: the candidate cell and its vertices (all lanes)
-    const size_t gc = (size_t)k * p.NC + j * nx + i;
-    const int vi[4] = {i, i + 1, i, i + 1}, vj[4] = {j, j, j + 1, j + 1};
-    double x3[4], x0[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const size_t v = (size_t)k * NV + vj[q] * NV1 + vi[q];
-        x3[q] = bt3[v];
-        x0[q] = bt0[v];
-    }
-    const double gamold = d.TA.cgam[gc], btc0 = d.TA.btc[gc], btc3 = d.TB.btc[gc];
-    const double w0 = d.SH[gc], h0 = d.SH[3 * FS + gc], b0c = d.b0c[gc], w3 = d.SM[2][gc];
-    double nv[4];
-    int di[4], dj[4], N;
-    const int res = vertex_correction(p, i, j, x3, x0, gamold, btc0, btc3, w0, h0, b0c, w3, nv, di, dj, N);
-    if (res <= 0) return res;
-    if (lane == 0)
-        for (int q = 0; q < N; ++q) put_vertex(d, p, k, di[q], dj[q], nv[q], ghost, nd);
-    if (lane >= 9) return 1;
-    // ---- the lane's cell: vertex copies of tile ct after the correction
-    const int kx0 = (k % p.ntx) * nx, ky0 = (k / p.ntx) * ny;
-    const int cx0 = (ct % p.ntx) * nx, cy0 = (ct / p.ntx) * ny;
-    auto fix = [&](double val, int a, int b) -> double {
-        const int X = cx0 + a, Y = cy0 + b;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q < N && kx0 + di[q] == X && ky0 + dj[q] == Y) val = nv[q];
-        return val;
-    };
-    const CellRe r = recompute_cell(p, fix(c00, ci, cj), fix(c10, ci + 1, cj), fix(c01, ci, cj + 1),
-                                    fix(c11, ci + 1, cj + 1), e00, e10, e01, e11, cbt0, cgo, cw0,
-                                    ch0, cb0, cact);
-    if (cact) {
-        d.TB.btc[g] = r.btc; d.TB.cbx[g] = r.dbdx; d.TB.cby[g] = r.dbdy; d.TB.cgam[g] = r.gn;
-        d.SM[2][g] = r.w;
-        d.SM[2][FS + g] = r.hp;
-    } else {
-        d.S3[g] = r.w;
-        d.S3[3 * FS + g] = r.hp;
-        note_dirty(d, (long long)g, nd);
-        ++*ghost;
-    }
-    return 1;
-}
-
-constexpr int RD_THREADS = 1024;
-constexpr int RD_WAVES = RD_THREADS / 64;
-constexpr int RD_LDS = 2048;          // candidates sorted and levelled in LDS
+constexpr int RD_THREADS = 1024;      // candidates walked at once (one per thread)
+constexpr int RD_LDS = 2048;          // candidates sorted in LDS (more: in global memory)
 
 // Neighbour k (0..7) of a cell at offset (kDI[k], kDJ[k]).
 __constant__ const int kDI[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
@@ -563,9 +478,12 @@ __constant__ const int kDJ[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 constexpr int kNbM[4][3] = {{0, 1, 3}, {1, 2, 4}, {3, 5, 6}, {4, 6, 7}};
 constexpr int kNbV[4][3] = {{3, 2, 1}, {3, 2, 0}, {3, 1, 0}, {2, 1, 0}};
 
+// The walk over candidates base .. base+n-1 (n <= RD_THREADS) of the list in
+// cand_key/cand_ex, which need not be sorted: they are ranked here.  Returns
+// true if it stopped at a candidate that borders no depositional vertex.
 template <class DevT>
-__device__ void redistribute_small(const DevT& d, const Params& p, int n, double* lx, long long* lk,
-                                   int* lev, int* cellid, unsigned long long* nd) {
+__device__ bool redistribute_small(const DevT& d, const Params& p, int base, int n, double* lx,
+                                   long long* lk, int* lev, int* cellid, unsigned long long* nd) {
     __shared__ double slot[RD_THREADS][4];     // vertex bt of each candidate's cell
     __shared__ unsigned char rec[RD_THREADS];  // its bt and w were recomputed by a correction
     __shared__ unsigned char res[RD_THREADS];  // 0 none, 1 corrected, 2 no depositional vertex
@@ -579,7 +497,7 @@ __device__ void redistribute_small(const DevT& d, const Params& p, int n, double
     const int NX = p.ntx * nx, NY = p.nty * ny;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     if (tid == 0) { s_stop = n; s_done = 0; s_ghost = 0; }
-    if (tid < n) { lk[tid] = d.cand_key[tid]; lx[tid] = d.cand_ex[tid]; lev[tid] = 0; }
+    if (tid < n) { lk[tid] = d.cand_key[base + tid]; lx[tid] = d.cand_ex[base + tid]; lev[tid] = 0; }
     __syncthreads();
     // ---- rank of each candidate in (excess, scan order): P threads per
     // candidate count over interleaved parts of the list
@@ -770,6 +688,7 @@ __device__ void redistribute_small(const DevT& d, const Params& p, int n, double
         c->redist_ghost += (long long)s_ghost;
         if (stop < n) c->mref |= 2;
     }
+    return stop < n;
 }
 
 // AddToRedistList + RedistributeGrid (src/Redistribute.f90:69-244): the
@@ -781,31 +700,30 @@ __device__ void redistribute_small(const DevT& d, const Params& p, int n, double
 // state in LDS.  A correction's decision reads only the cell's four vertex bt
 // and its bt and w, which are the stored stage values until a correction of
 // a cell within one cell (a shared vertex) recomputes them from those
-// vertices.  So thread q owns sorted candidate q: each candidate gets a level
-// one above the highest level of the earlier candidates within one cell, the
-// levels are decided in order from the candidates' vertex values held in
-// LDS (a correction updates the copies held by the later candidates that
-// share the vertex), and every decision sees the state the sequential walk
-// gives it.  The walk stops at the first candidate that borders no
-// depositional vertex (the attempt is then refined); only the corrections
-// before it are written: the vertex copies level by level, then the 3x3
-// cells around each correction from the final vertices, which is the value
-// the walk leaves in them (a later correction that moves one of a cell's
-// vertices recomputes that cell).
+// vertices.  So thread q owns sorted candidate q: the candidates are decided
+// in rounds (a candidate once every earlier candidate within one cell is)
+// from the vertex values held in LDS (a correction updates the copies held
+// by the later candidates that share the vertex), and every decision sees
+// the state the sequential walk gives it.  The walk stops at the first
+// candidate that borders no depositional vertex (the attempt is then
+// refined); only the corrections before it are written: the vertex copies,
+// then the 3x3 cells around each correction from the final vertices, which
+// is the value the walk leaves in them (a later correction that moves one of
+// a cell's vertices recomputes that cell).
 //
-// Beyond RD_THREADS candidates the kernel sorts in LDS or in global memory
-// and corrects level by level with one wavefront per candidate, two
-// corrections interacting within two cells (the 3x3 update reads the 4x4
-// vertices); there every level is processed even after a failing candidate.
+// Larger sets are sorted first (bitonic, in LDS up to RD_LDS candidates, else
+// in place in global memory) and walked in consecutive chunks of the sorted
+// list, each chunk as above on the state the chunks before it wrote, until a
+// chunk stops: the same walk, in the same order, as the reference's.
 __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
     const auto& d = const_arg(dp);
     const Params& p = *pp;
-    __shared__ int sh_n, sh_skip, sh_fail, sh_maxlev, sh_cnt, sh_changed;
-    __shared__ unsigned long long sh_done, sh_ghost, sh_ndirty;
+    __shared__ int sh_n, sh_skip;
+    __shared__ unsigned long long sh_ndirty;
     __shared__ double lx[RD_LDS];
     __shared__ long long lk[RD_LDS];
-    __shared__ int lev[RD_LDS];
-    __shared__ int list[RD_LDS];
+    __shared__ int lev[RD_THREADS];
+    __shared__ int list[RD_THREADS];
     Ctrl* c = d.ctrl;
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -818,27 +736,24 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
         if (!skip) count_work(c, KW_REDIST);
         sh_skip = skip;
         sh_n = c->ncand;
-        sh_fail = 0;
-        sh_done = 0;
-        sh_ghost = 0;
         sh_ndirty = c->ndirty;
-        sh_maxlev = 0;
     }
     __syncthreads();
     if (sh_skip) return;
     const int n = min(sh_n, d.cand_cap);
-    if (n <= RD_THREADS) {
-        redistribute_small(d, p, n, lx, lk, lev, list, &sh_ndirty);
+    const int chunk = d.rd_chunk;
+    if (n <= chunk) {
+        redistribute_small(d, p, 0, n, lx, lk, lev, list, &sh_ndirty);
         __syncthreads();
         if (tid == 0) c->ndirty = (long long)sh_ndirty;
         return;
     }
+    // ---- sort by (excess, scan order); the padding sorts last
     int m2 = 1;
     while (m2 < n) m2 <<= 1;
     long long* K = d.cand_key;
     double* X = d.cand_ex;
-    const bool in_lds = m2 <= RD_LDS;
-    // ---- sort by (excess, scan order)
+    const bool in_lds = m2 <= d.rd_lds_sort;
     if (in_lds) {
         for (int q = tid; q < m2; q += RD_THREADS) {
             lk[q] = q < n ? K[q] : 0x7fffffffffffffffLL;
@@ -870,77 +785,21 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
             __syncthreads();
         }
     }
-    // ---- candidate cell -> sorted position
-    const int nx = p.nx, ny = p.ny, NX = p.ntx * nx, NY = p.nty * ny;
-    auto cell_of = [&](int q) -> int {
-        const long long key = SK[q];
-        return d.active[(int)(key / p.NC)] * p.NC + (int)(key % p.NC);
-    };
-    int* LEV = in_lds ? lev : d.cand_done;
-    for (int q = tid; q < n; q += RD_THREADS) {
-        d.cand_pos[cell_of(q)] = q;
-        LEV[q] = 1;
-    }
-    __syncthreads();
-    // ---- levels: relax level(q) = 1 + max level of its earlier neighbours
-    for (;;) {
-        if (tid == 0) sh_changed = 0;
-        __syncthreads();
+    if (in_lds) {
         for (int q = tid; q < n; q += RD_THREADS) {
-            const int g = cell_of(q);
-            const int t = g / p.NC, cl = g % p.NC;
-            const int I = (t % p.ntx) * nx + cl % nx, J = (t / p.ntx) * ny + cl / nx;
-            int l = 1;
-            for (int dJ = -2; dJ <= 2; ++dJ)
-                for (int dI = -2; dI <= 2; ++dI) {
-                    const int I2 = I + dI, J2 = J + dJ;
-                    if ((dI == 0 && dJ == 0) || I2 < 0 || J2 < 0 || I2 >= NX || J2 >= NY) continue;
-                    const int g2 = ((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx;
-                    const int r = d.cand_pos[g2];
-                    if (r >= 0 && r < q) l = max(l, LEV[r] + 1);
-                }
-            if (l != LEV[q]) {
-                LEV[q] = l;
-                sh_changed = 1;
-            }
+            K[q] = lk[q];
+            X[q] = lx[q];
         }
-        __syncthreads();
-        if (!sh_changed) break;
     }
-    for (int q = tid; q < n; q += RD_THREADS) atomicMax(&sh_maxlev, LEV[q]);
     __syncthreads();
-    // ---- corrections, level by level
-    const int wave = tid / 64, lane = tid % 64;
-    unsigned long long ghost = 0, done = 0;
-    int* L = in_lds ? list : d.cand_done + d.cand_cap;
-    for (int level = 1; level <= sh_maxlev; ++level) {
-        if (tid == 0) sh_cnt = 0;
+    // ---- the walk, chunk by chunk (a chunk reads what the earlier ones wrote:
+    // one workgroup, so the barrier orders its global stores and loads)
+    for (int b = 0; b < n; b += chunk) {
+        const bool stopped = redistribute_small(d, p, b, min(chunk, n - b), lx, lk, lev, list, &sh_ndirty);
         __syncthreads();
-        for (int q = tid; q < n; q += RD_THREADS)
-            if (LEV[q] == level) L[atomicAdd(&sh_cnt, 1)] = q;
-        __syncthreads();
-        const int cnt = sh_cnt;
-        for (int r = wave; r < cnt; r += RD_WAVES) {
-            const int g = cell_of(L[r]);
-            const int k = g / p.NC, cl = g % p.NC;
-            const int res = redistribute_cell(d, p, k, cl % nx, cl / nx, lane, &ghost, &sh_ndirty);
-            if (lane == 0) {
-                if (res < 0) sh_fail = 1;
-                else if (res > 0) ++done;
-            }
-        }
-        __syncthreads();
+        if (stopped) break;
     }
-    for (int q = tid; q < n; q += RD_THREADS) d.cand_pos[cell_of(q)] = -1;
-    if (ghost) atomicAdd(&sh_ghost, ghost);
-    if (done) atomicAdd(&sh_done, done);
-    __syncthreads();
-    if (tid == 0) {
-        c->redist_cells += (long long)sh_done;
-        c->redist_ghost += (long long)sh_ghost;
-        c->ndirty = (long long)sh_ndirty;
-        if (sh_fail) c->mref |= 2;
-    }
+    if (tid == 0) c->ndirty = (long long)sh_ndirty;
 }
 
 // ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),

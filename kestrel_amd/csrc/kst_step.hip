@@ -144,8 +144,9 @@ struct Dev {
     long long* cand_key;              // redistribution candidates: scan-order key
     double* cand_ex;                  // and excess deposition
     int* cand_pos;                    // cell -> sorted candidate position (-1: none) [tile][NC]
-    int* cand_done;                   // levels and level lists of large candidate sets (2 x cap)
     int cand_cap;
+    int rd_chunk;                     // candidates walked at once (<= RD_THREADS)
+    int rd_lds_sort;                  // largest padded candidate count sorted in LDS (<= RD_LDS)
     int morpho;
     // Redistribution may write ghost tiles' copies of shared vertices and
     // ghost cells' w, Hnpsi into intermed3 (src/Redistribute.f90:362-470); the
@@ -2379,9 +2380,11 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
         size_t cap = 1;
         while (cap < FS) cap <<= 1;
         rc |= dalloc(h, &d.cand_key, cap); rc |= dalloc(h, &d.cand_ex, cap);
-        rc |= dalloc(h, &d.cand_pos, FS); rc |= dalloc(h, &d.cand_done, 2 * cap);
+        rc |= dalloc(h, &d.cand_pos, FS);
         if (!rc) rc |= hipMemsetAsync(d.cand_pos, 0xFF, FS * sizeof(int), h->stream) != hipSuccess;
         d.cand_cap = (int)cap;
+        d.rd_chunk = RD_THREADS;
+        d.rd_lds_sort = RD_LDS;
         d.dirty_cap = 1 << 16;
         rc |= dalloc(h, &d.dirty, (size_t)d.dirty_cap);
     }
@@ -2836,6 +2839,22 @@ int kst_last_error(const kst_handle* h, char* buf, int buflen) {
 int kst_set_grid_cap(kst_handle* h, int tiles) {
     if (!h || tiles < 0) return KST_ERR_CONFIG;
     h->fixed_cap = tiles;
+    return KST_OK;
+}
+
+int kst_set_redistribution_chunk(kst_handle* h, int chunk, int lds_sort) {
+    if (!h) return KST_ERR_CONFIG;
+    if (chunk < 0 || chunk > RD_THREADS || lds_sort < 0 || lds_sort > RD_LDS)
+        return set_err(h, KST_ERR_CONFIG, "kst_set_redistribution_chunk: chunk must lie in 1.." +
+                                              std::to_string(RD_THREADS) + " and lds_sort in 1.." +
+                                              std::to_string(RD_LDS) + " (0: default)");
+    if (!h->d.morpho) return KST_OK;
+    const int c = chunk ? chunk : RD_THREADS, l = lds_sort ? lds_sort : RD_LDS;
+    h->d.rd_chunk = h->d2.rd_chunk = c;
+    h->d.rd_lds_sort = h->d2.rd_lds_sort = l;
+    const Dev both[2] = {h->d, h->d2};
+    HIPCHK(h, hipMemcpyAsync(h->d_args, both, sizeof both, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return KST_OK;
 }
 

@@ -220,6 +220,7 @@ def _params(d: Deck, c: C.KstConfig):
     if "erosion" not in P:
         c.erosion = C.EROSIONS["mixed"]
         c.morphodynamics = 1
+    C.reference_defaults(c, {reals[k] for k in P if k in reals})
     return settling
 
 
@@ -444,7 +445,11 @@ def _sources(d: Deck) -> list:
     return out
 
 
-def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64) -> Run:
+def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64,
+          b0v: np.ndarray | None = None) -> Run:
+    """The deck's configuration, topography, initial state and sources.
+    b0v, if given, replaces the deck's topography by vertex elevations
+    ((NX+1) x (NY+1)) sampled elsewhere, as a DEM would."""
     if d.cubes:
         raise DeckUnsupported("Cube initial conditions")
     ntx, nty, nx, ny, xt, yt = _domain(d)
@@ -455,7 +460,12 @@ def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64) -> Run:
     tend, tstart = _solver(d, c)
     c.precision = precision
     C.derive(c, settling_speed=settling)
-    b0v = _topography(d, c)
+    if b0v is None:
+        b0v = _topography(d, c)
+    else:
+        b0v = np.ascontiguousarray(b0v, dtype=np.float64)
+        if b0v.shape != (c.NY + 1, c.NX + 1):
+            raise DeckError(f"topography of shape {b0v.shape}; the grid has {(c.NY + 1, c.NX + 1)} vertices")
     caps = _caps(d, c)
     srcs = _sources(d)
     if not caps and not srcs:
@@ -467,7 +477,7 @@ def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64) -> Run:
     return Run(Workload(name, c, b0v, init, 0, caps, srcs), tend, nout, tstart, d)
 
 
-def load(path: str, precision: int = C.PREC_FP64) -> Run:
+def load(path: str, precision: int = C.PREC_FP64, b0v: np.ndarray | None = None) -> Run:
     import os
     with open(path) as f:
-        return build(parse(f.read()), os.path.splitext(os.path.basename(path))[0], precision)
+        return build(parse(f.read()), os.path.splitext(os.path.basename(path))[0], precision, b0v)

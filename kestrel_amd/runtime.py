@@ -34,7 +34,8 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_last_error", "kst_destroy", "kst_profile", "kst_profile_read", "kst_get_tile_bt",
            "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
-           "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap"]
+           "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap",
+           "kst_set_redistribution_chunk"]
 
 
 class KstStats(ctypes.Structure):
@@ -78,6 +79,8 @@ def lib():
         L.kst_profile.restype = ctypes.c_int
         L.kst_set_grid_cap.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.kst_set_grid_cap.restype = ctypes.c_int
+        L.kst_set_redistribution_chunk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.kst_set_redistribution_chunk.restype = ctypes.c_int
         L.kst_volume.argtypes = [ctypes.c_void_p, _dp]
         L.kst_volume.restype = ctypes.c_int
         L.kst_get_output.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int32),
@@ -214,6 +217,11 @@ class Stepper:
     def set_grid_cap(self, tiles: int):
         """Fix the launch grids' active-tile cap (0: follow the active set)."""
         self._check(self.L.kst_set_grid_cap(self.h, int(tiles)))
+
+    def set_redistribution_chunk(self, chunk: int, lds_sort: int = 0):
+        """Walk at most `chunk` redistribution candidates at once and sort at
+        most `lds_sort` in LDS (0: defaults); results do not depend on them."""
+        self._check(self.L.kst_set_redistribution_chunk(self.h, int(chunk), int(lds_sort)))
 
     def profile(self, enable: bool = True):
         self._check(self.L.kst_profile(self.h, int(enable)))

@@ -48,6 +48,7 @@ def plain_cap(n: int = 256, steps: int = 1000, tile: int = 32, dx: float = 1.0,
     cfg.chezy_co = 0.01
     cfg.cfl = 0.25
     cfg.max_dt = 0.05
+    C.reference_defaults(cfg, {"chezy_co"})
     C.derive(cfg)
     b0v = S.topo_xslope(cfg, math.tan(math.radians(10.0)))
     L = n * dx
@@ -123,6 +124,8 @@ def morpho_hills(n: int = 1024, steps: int = 2000, tile: int = 32, dx: float = 5
     cfg.deposition = C.DEPOSITIONS["spearman manning"]
     cfg.morphodynamics = 1
     cfg.eddy_viscosity = 1e-2
+    # as the deck gives them: Chezy Co and the Voellmy switch explicitly
+    C.reference_defaults(cfg, {"chezy_co", "voellmy_switch_rate", "voellmy_switch_value"})
     C.derive(cfg)
     hills = S.Hills(seed, n=n_hills)
     b0v = S.topo_gaussian_hills(cfg, hills)

@@ -92,6 +92,8 @@ struct ko_sim {
     uint8_t* probe_ts;        /* tile states at the probe */
     int64_t redist_cells;     /* RedistributeCell calls that succeeded */
     int64_t redist_ghost;     /* their writes into ghost tiles (vertex copies, 3x3 cells) */
+    double* steplog;          /* (t, dt) of each accepted step while logging (ko_step_log) */
+    int64_t steplog_n, steplog_cap;
     double probe_dt, probe_dt_pending;
 };
 
@@ -2126,6 +2128,11 @@ static int one_step(ko_sim* s, double tend, kst_stats* st) {
     st->steps++;
     st->cell_updates += count_cells(s);
     st->last_dt = dth;
+    if (s->steplog && s->steplog_n < s->steplog_cap) {
+        s->steplog[2 * s->steplog_n] = s->t;
+        s->steplog[2 * s->steplog_n + 1] = dth;
+        s->steplog_n++;
+    }
     if (st->steps == 1 || dth < st->min_dt) st->min_dt = dth;
     if (dth > st->max_dt) st->max_dt = dth;
     return 0;
@@ -2683,6 +2690,25 @@ int ko_redistribute(ko_sim* s, const double* f0, const double* bt0, const double
 /* Per-tile vertex bt views of every allocated tile (ntiles x (ny+1) x
  * (nx+1)), then the cell topography of those tiles and the face topography
  * of the active ones. */
+/* Record (t, dt) of the next `cap` accepted steps into buf (cap = 0 stops);
+ * returns the number recorded since the last call that set a buffer. */
+int64_t ko_step_log(ko_sim* s, double* buf, int64_t cap) {
+    const int64_t n = s->steplog_n;
+    s->steplog = cap > 0 ? buf : NULL;
+    s->steplog_cap = cap;
+    s->steplog_n = 0;
+    return n;
+}
+
+/* One limiter evaluation, lim(a, b) of src/Limiters.f90:83-184, for the
+ * closed-form tests. */
+double ko_limiter(int kind, double a, double b) {
+    kst_config c;
+    memset(&c, 0, sizeof c);
+    c.limiter = kind;
+    return limiter(&c, a, b);
+}
+
 int ko_set_tile_bt(ko_sim* s, const double* btt) {
     for (int k = 0; k < s->ntiles; ++k) {
         tdata* T = &s->cont[C_TC][k];

@@ -110,6 +110,14 @@ int ko_redistribute(ko_sim* s, const double* f0, const double* bt0, const double
                     const int32_t* j, const double* excess, double* fout, double* btout,
                     int* failed);
 
+/* Record (t, dt) pairs of the next `cap` accepted steps into buf (the
+ * caller keeps buf alive; cap = 0 stops); returns how many the previous
+ * buffer received. */
+int64_t ko_step_log(ko_sim* s, double* buf, int64_t cap);
+
+/* lim(a, b) of limiter kind KST_LIMITER_* (src/Limiters.f90:83-184). */
+double ko_limiter(int kind, double a, double b);
+
 #ifdef __cplusplus
 }
 #endif

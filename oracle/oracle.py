@@ -77,6 +77,10 @@ def lib():
                      "ko_probe_candidates"]:
             getattr(L, name).restype = ctypes.c_int
         L.ko_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
+        L.ko_step_log.restype = ctypes.c_int64
+        L.ko_step_log.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
+        L.ko_limiter.restype = ctypes.c_double
+        L.ko_limiter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.ko_morpho_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.ko_set_tile_bt.restype = ctypes.c_int
         L.ko_set_tile_bt.argtypes = [ctypes.c_void_p, _dp]
@@ -287,6 +291,15 @@ class Oracle:
         res["tile_state"] = ts
         res["dt"] = self.L.ko_probe_dt(self.h)
         return res
+
+    def step_log(self, cap: int = 0):
+        """Start recording (t, dt) of up to cap accepted steps (cap = 0 stops);
+        returns the pairs the previous recording received."""
+        prev = getattr(self, "_steplog", None)
+        buf = np.zeros((max(cap, 1), 2)) if cap > 0 else None
+        n = self.L.ko_step_log(self.h, _ptr(buf) if buf is not None else None, int(cap))
+        self._steplog = buf
+        return prev[:n].copy() if prev is not None else np.zeros((0, 2))
 
     def tile_lists(self):
         n = self.cfg.ntiles

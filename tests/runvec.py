@@ -1,0 +1,133 @@
+"""Whole-run vectors of the reference program (tests/golden/run_vectors.json).
+
+tests/golden/make_run_vectors.py runs the reference's own Run (built from its
+sources by oracle/Makefile refrun) on each case below and records, after
+every output interval, the simulation time, the accepted-step and revised-dt
+counts, the active and ghost tiles and sha256 digests of the tileContainer
+state of the active tiles.  The tests compute the same digests from the
+oracle (CPU) and from the device (GPU) after kst_integrate_to to the same
+output times: equal digests are equal bits.
+
+Digests cover, per active tile in active-list order, float64 arrays with
+-0.0 folded into +0.0:
+  cons     w, rhoHnu, rhoHnv, Hnpsi              [4, ny, nx]   (u(1:4))
+  derived  Hn, u, v, psi, rho                    [5, ny, nx]   (u(5:9))
+  topo     b0, bt, dbdx, dbdy at cell centres    [4, ny, nx]   (u(10:13))
+  btv      the tile's own vertex bt copy         [ny+1, nx+1]
+  maxima   Hnmax, its time, ..., tfirst          [11, ny, nx]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from kestrel_amd import deck
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+FIXTURE = os.path.join(GOLDEN, "run_vectors.json")
+REF_DECKS = os.path.join(GOLDEN, "decks")
+OWN_DECKS = os.path.join(GOLDEN, "workloads")
+
+# name -> (deck file, vertex-topography source or None)
+CASES = {
+    "cap_conc_2d": (os.path.join(REF_DECKS, "Input_cap_conc_2d.txt"), None),
+    "cap_dilute_2d": (os.path.join(REF_DECKS, "Input_cap_dilute_2d.txt"), None),
+    "cap_morpho_2d": (os.path.join(REF_DECKS, "Input_cap_morpho_2d.txt"), None),
+    "flux_edwards2019_2d": (os.path.join(REF_DECKS, "Input_flux_edwards2019_2d.txt"), None),
+    "flux_hydro_2d": (os.path.join(REF_DECKS, "Input_flux_hydro_2d.txt"), None),
+    "flux_morpho_2d": (os.path.join(REF_DECKS, "Input_flux_morpho_2d.txt"), None),
+    "flux_single_pt": (os.path.join(REF_DECKS, "Input_flux_single_pt.txt"), None),
+    "tile_indep_dynamic_100m": (os.path.join(REF_DECKS, "Input_tile_indep_dynamic_100m.txt"), None),
+    "tile_indep_dynamic_20m": (os.path.join(REF_DECKS, "Input_tile_indep_dynamic_20m.txt"), None),
+    "tile_indep_static_100m": (os.path.join(REF_DECKS, "Input_tile_indep_static_100m.txt"), None),
+    "tile_indep_static_50m": (os.path.join(REF_DECKS, "Input_tile_indep_static_50m.txt"), None),
+    "plain_cap_256": (os.path.join(OWN_DECKS, "plain_cap_256.txt"), None),
+    "morpho_hills_1024": (os.path.join(OWN_DECKS, "morpho_hills_1024.txt"), "morpho_hills_1024"),
+}
+
+# cases whose CPU replay is long (tens of seconds): run by the slow tests only
+LONG = {"morpho_hills_1024", "tile_indep_static_50m", "tile_indep_static_100m",
+        "tile_indep_dynamic_100m", "cap_morpho_2d"}
+
+
+def topography(source: str | None):
+    if source is None:
+        return None
+    from kestrel_amd import workloads
+    if source == "morpho_hills_1024":
+        return workloads.morpho_hills_1024().b0v
+    raise KeyError(source)
+
+
+def load_case(name: str, precision: int = 0):
+    path, src = CASES[name]
+    return deck.load(path, precision=precision, b0v=topography(src))
+
+
+def fixtures() -> dict:
+    with open(FIXTURE) as f:
+        return json.load(f)
+
+
+def _digest(arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        a = np.ascontiguousarray(a, dtype=np.float64) + 0.0
+        h.update(a.tobytes())
+    return h.hexdigest()[:32]
+
+
+def digests_from_tiles(u13: np.ndarray, btv: np.ndarray, maxima: np.ndarray) -> dict:
+    """u13[q, 13, ny, nx], btv[q, ny+1, nx+1], maxima[q, 11, ny, nx] of the
+    active tiles in active-list order."""
+    return {"cons": _digest(u13[:, 0:4]), "derived": _digest(u13[:, 4:9]),
+            "topo": _digest(u13[:, 9:13]), "btv": _digest(btv), "maxima": _digest(maxima)}
+
+
+def tiles_of(cfg, active, glob: np.ndarray) -> np.ndarray:
+    """Per-tile slices [q, ..., ny, nx] of global arrays [..., NY, NX]."""
+    nx, ny, ntx = cfg.nx_per_tile, cfg.ny_per_tile, cfg.nx_tiles
+    out = []
+    for k in active:
+        gi, gj = int(k) % ntx, int(k) // ntx
+        out.append(glob[..., gj * ny:(gj + 1) * ny, gi * nx:(gi + 1) * nx])
+    return np.array(out)
+
+
+def oracle_record(o, cfg) -> dict:
+    """Digests and lists of the oracle's tileContainer."""
+    act, gho = o.tile_lists()
+    u13 = tiles_of(cfg, act, o.fields13())
+    btv = o.tile_bt()[np.array(act, dtype=np.int64)] if act else np.zeros((0,))
+    mx = tiles_of(cfg, act, np.array(o.maxima()))
+    rec = digests_from_tiles(u13, btv, mx)
+    rec.update(active=[int(k) for k in act], ghost=sorted(int(k) for k in gho))
+    return rec
+
+
+# solution-file columns (kst_get_output) of u(1:13): w rhoHnu rhoHnv Hnpsi Hn u v psi rho b0 bt dbdx dbdy
+_COLS13 = (9, 6, 7, 8, 0, 2, 3, 5, 4, 10, 11, 12, 13)
+
+
+def device_record(g, cfg) -> dict:
+    """Digests and lists of the device state (kst_get_output's columns are
+    tileContainer's fields of the active tiles in active-list order)."""
+    tiles, f = g.output_fields()
+    u13 = f[:, list(_COLS13)]
+    btv = g.tile_bt()[tiles.astype(np.int64)]
+    mx = tiles_of(cfg, tiles, np.array(g.maxima()))
+    rec = digests_from_tiles(u13, btv, mx)
+    st = g.state()["tile_state"]
+    rec.update(active=[int(k) for k in tiles], ghost=sorted(int(k) for k in np.flatnonzero(st == 2)))
+    return rec
+
+
+def intervals(run):
+    """Output times tstart + k DeltaT, k = 1..Nout (src/OutputSettings.f90:286,
+    src/TimeStepper.f90:101)."""
+    dT = (run.tend - run.tstart) / float(run.nout)
+    return [run.tstart + k * dT for k in range(1, run.nout + 1)]

@@ -193,12 +193,59 @@ def test_tile_independence():
 
 
 def test_limiters_closed_forms():
-    """The limiters of the oracle are the textbook forms (src/Limiters.f90)."""
-    from oracle.oracle import Oracle
-    # probe through a 1-cell slope: LimiterNone is the central difference,
-    # MinMod vanishes at an extremum.
-    for lim in ("minmod1", "minmod2", "vanalbada", "weno", "none"):
-        assert lim in C.LIMITERS
+    """The oracle's limiters (src/Limiters.f90:83-184) against their textbook
+    definitions, evaluated independently here over slopes of both signs,
+    zeros and a wide range of magnitudes: MinMod (Roe), generalised MinMod
+    with theta = 1.3, van Albada, the WENO-weighted average and the central
+    difference; plus the properties a mistake would break (symmetry,
+    consistency lim(a, a) = a, zero at a sign change for the MinMods, the
+    TVD bounds, the weighted average lying between its arguments)."""
+    from oracle.oracle import lib
+    L = lib()
+    rng = np.random.default_rng(7)
+    vals = np.concatenate([rng.standard_normal(400) * 10.0 ** rng.integers(-6, 4, 400),
+                           [0.0, -0.0, 1.0, -1.0, 2.5, -2.5, 1e-300]])
+    A, B = np.meshgrid(vals[::7], vals[::5])
+    A, B = A.ravel(), B.ravel()
+    lim = {name: np.array([L.ko_limiter(C.LIMITERS[name], a, b) for a, b in zip(A, B)])
+           for name in ("minmod1", "minmod2", "vanalbada", "weno", "none")}
+    same = A * B > 0
+    sgn = np.sign(A)
+    # MinMod: the smaller magnitude with the common sign, else 0
+    mm1 = np.where(same, sgn * np.minimum(np.abs(A), np.abs(B)), 0.0)
+    assert np.array_equal(lim["minmod1"], mm1)
+    # generalised MinMod, theta = 1.3: min(theta|a|, theta|b|, |a+b|/2), common sign
+    th = 1.3
+    mm2 = np.where(same, sgn * np.minimum(np.minimum(np.abs(th * A), np.abs(th * B)),
+                                           np.abs(0.5 * (A + B))), 0.0)
+    assert np.array_equal(lim["minmod2"], mm2)
+    assert np.all(np.abs(lim["minmod2"]) <= th * np.minimum(np.abs(A), np.abs(B)))
+    # van Albada: ab(a + b) / (a^2 + b^2), 0 where both vanish
+    den = A * A + B * B
+    with np.errstate(invalid="ignore", divide="ignore"):
+        va = np.where(den == 0.0, 0.0, (A * A * B + A * B * B) / den)
+    assert np.allclose(lim["vanalbada"], va, rtol=1e-14, atol=0.0)
+    # WENO weights 1/(s^2 + eps)^2: an average that lies between a and b
+    ea, eb = A * A + 1e-6, B * B + 1e-6
+    wa, wb = 1.0 / (ea * ea), 1.0 / (eb * eb)
+    assert np.allclose(lim["weno"], (wa * A + wb * B) / (wa + wb), rtol=1e-14, atol=0.0)
+    lo, hi = np.minimum(A, B), np.maximum(A, B)
+    eps = 1e-15 * np.maximum(np.abs(lo), np.abs(hi))
+    assert np.all((lim["weno"] >= lo - eps) & (lim["weno"] <= hi + eps))
+    # no limiter: the central difference
+    assert np.array_equal(lim["none"], 0.5 * (A + B))
+    # symmetry and consistency
+    for name, f in lim.items():
+        g = np.array([L.ko_limiter(C.LIMITERS[name], b, a) for a, b in zip(A, B)])
+        if name in ("minmod1", "minmod2", "none"):
+            assert np.array_equal(f, g), name
+        else:   # the two orders round differently: a few ulps of the slopes
+            assert np.all(np.abs(f - g) <= 4e-16 * np.maximum(np.abs(A), np.abs(B))), name
+        for a in vals[:50]:
+            assert L.ko_limiter(C.LIMITERS[name], a, a) == pytest.approx(a, rel=1e-15, abs=0.0), name
+    # the MinMods vanish at a sign change
+    for name in ("minmod1", "minmod2"):
+        assert np.all(lim[name][A * B <= 0] == 0.0), name
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,123 @@
+"""The oracle against the reference program's own Run (tests/runvec.py,
+tests/golden/run_vectors.json): the initial state its set-up paints
+(InitialiseGrid, LoadSourceConditions: src/SetSources.f90:243-375) and, after
+every output interval of kst_integrate_to's restatement (IntegrateTo,
+src/TimeStepper.f90:116-277, with activation, ghost tiles, the hydraulic and
+morphodynamic steppers, redistribution and the maxima), the same time, the
+same accepted steps and revised-dt count, the same active list and ghost set
+and the same bits in every field of tileContainer's active tiles.
+
+Long cases replay their first output interval here; the GPU tests replay
+every interval of every case."""
+import os
+
+import pytest
+
+from kestrel_amd import config as C
+from tests import runvec
+
+pytestmark = pytest.mark.skipif(not os.path.exists(runvec.FIXTURE), reason="run vectors not generated")
+
+FIX = runvec.fixtures() if os.path.exists(runvec.FIXTURE) else {}
+
+
+def _oracle(run):
+    from oracle.oracle import Oracle
+    o = Oracle(run.workload.cfg)
+    o.set_topography(run.workload.b0v)
+    o.set_state(run.workload.init, t=run.tstart)
+    return o
+
+
+def _check(rec, ref, label, initial=False):
+    assert rec["active"] == ref["active"], f"{label}: active list"
+    assert rec["ghost"] == ref["ghost"], f"{label}: ghost tiles"
+    # before the first step the reference's derived fields hold the painted
+    # velocities and solids fraction, which the first RHS recomputes; the
+    # conserved fields, the topography and the bed are the painted state
+    keys = ("cons", "topo", "btv") if initial else ("cons", "derived", "topo", "btv", "maxima")
+    for k in keys:
+        assert rec[k] == ref[k], f"{label}: {k} differs from the reference"
+
+
+@pytest.mark.parametrize("name", sorted(FIX))
+def test_oracle_equals_reference_run(name):
+    fx = FIX[name]
+    run = runvec.load_case(name)
+    o = _oracle(run)
+    _check(runvec.oracle_record(o, run.workload.cfg), fx["intervals"][0], f"{name} t=0", initial=True)
+    times = runvec.intervals(run)
+    n = 1 if name in runvec.LONG else len(times)
+    for k in range(1, n + 1):
+        ref = fx["intervals"][k]
+        st = o.integrate_to(times[k - 1])
+        label = f"{name} interval {k}"
+        assert st["t"] == ref["t"], f"{label}: t {st['t']!r} vs {ref['t']!r}"
+        assert st["steps"] == ref["steps"], f"{label}: steps {st['steps']} vs {ref['steps']}"
+        assert st["rejected"] == ref["revised"], f"{label}: rejected {st['rejected']} vs {ref['revised']}"
+        _check(runvec.oracle_record(o, run.workload.cfg), ref, label)
+    o.close()
+
+
+def test_run_vectors_cover_the_workloads():
+    """The measured workloads' decks are the bench's configurations exactly."""
+    import numpy as np
+    from kestrel_amd import workloads
+    for name, wl in (("plain_cap_256", workloads.plain_cap(256, 1000)),
+                     ("morpho_hills_1024", workloads.morpho_hills_1024())):
+        run = runvec.load_case(name)
+        assert run.workload.cfg.as_dict() == wl.cfg.as_dict(), name
+        assert np.array_equal(run.workload.b0v, wl.b0v), name
+        for f in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "tile_state"):
+            assert np.array_equal(getattr(run.workload.init, f), getattr(wl.init, f)), (name, f)
+        assert name in FIX
+    c1 = FIX["plain_cap_256"]["intervals"]
+    assert sum(r["steps"] for r in c1[1:]) >= 1000
+
+
+# the reference's RunSet scalars (ref_run -params) -> this library's configuration
+_REAL = {"deltaX": "dx", "deltaY": "dy", "xSize": "x_size", "ySize": "y_size", "g": "g",
+         "rhow": "rhow", "rhos": "rhos", "gred": "gred", "ChezyCo": "chezy_co",
+         "ManningCo": "manning_co", "CoulombCo": "coulomb_co", "PouliquenMinSlope": "pouliquen_min",
+         "PouliquenMaxSlope": "pouliquen_max", "PouliquenIntermediateSlope": "pouliquen_intermediate",
+         "PouliquenBeta": "pouliquen_beta", "Edwards2019betastar": "edwards_betastar",
+         "Edwards2019kappa": "edwards_kappa", "Edwards2019Gamma": "edwards_gamma",
+         "VoellmySwitchRate": "voellmy_switch_rate", "VoellmySwitchValue": "voellmy_switch_value",
+         "EroRate": "ero_rate", "EroRateGranular": "ero_rate_granular",
+         "CriticalShields": "critical_shields", "EroDepth": "ero_depth",
+         "EroCriticalHeight": "ero_critical_height", "BedPorosity": "bed_porosity",
+         "maxPack": "max_pack", "SolidDiameter": "solid_diameter", "Rep": "rep", "ws0": "ws0",
+         "nsettling": "nsettling", "EddyViscosity": "eddy_viscosity",
+         "heightThreshold": "height_threshold", "cfl": "cfl",
+         "diffusiveTimeScale": "diffusive_time_scale", "maxdt": "max_dt"}
+_CHOICE = {"limiter": ("limiter", C.LIMITERS, {"van albada": "vanalbada"}),
+           "DragChoice": ("drag", C.DRAGS, {}),
+           "ErosionChoice": ("erosion", C.EROSIONS, {}),
+           "DepositionChoice": ("deposition", C.DEPOSITIONS, {}),
+           "ErosionTransition": ("ero_transition", C.TRANSITIONS, {}),
+           "MorphoDamp": ("morpho_damping", C.DAMPINGS, {}),
+           "fswitch": ("fswitch", C.SWITCHES, {"0.5": "equal", "off": "zero", "0": "zero",
+                                                "1": "one"})}
+
+
+@pytest.mark.parametrize("name", sorted(FIX))
+def test_deck_reader_equals_reference_params(name):
+    """Every RunSet scalar the step reads, as the reference's own input
+    reader leaves it for the deck (src/Input.f90, src/Parameters.f90:82-652,
+    src/SolverSettings.f90, src/DomainSettings.f90), equals the deck
+    reader's configuration bit for bit, including the parameters the
+    reference leaves at zero for closures other than the chosen ones."""
+    prm = FIX[name]["params"]
+    run = runvec.load_case(name)
+    c = run.workload.cfg
+    assert prm["tiles"] == [c.nx_tiles, c.ny_tiles, c.nx_per_tile, c.ny_per_tile]
+    for ref_name, ours in _REAL.items():
+        assert getattr(c, ours) == prm[ref_name], f"{name}: {ref_name} {prm[ref_name]!r} vs {ours} {getattr(c, ours)!r}"
+    for ref_name, (ours, table, alias) in _CHOICE.items():
+        v = str(prm[ref_name]).strip().lower()
+        assert getattr(c, ours) == table[alias.get(v, v)], f"{name}: {ref_name} = {prm[ref_name]}"
+    assert bool(c.geometric_factors) == prm["geometric_factors"]
+    assert bool(c.morphodynamics) == prm["MorphodynamicsOn"]
+    assert c.tile_buffer == prm["TileBuffer"]
+    assert prm["bcs"] == "halt" and c.bc == C.BC_HALT
+    assert (run.tstart, run.tend) == (prm["tstart"], prm["tend"])

@@ -22,8 +22,8 @@ def load(name):
 @pytest.mark.parametrize("name,steps,tol", [
     ("Input_flux_hydro_2d.txt", 60, 0.0),             # Chezy, morphodynamics off: bit for bit
     ("Input_flux_single_pt.txt", 60, 0.0),
-    ("Input_cap_morpho_2d.txt", 40, 1e-9),             # Variable drag (tanh switch), Mixed erosion
-    ("Input_tile_indep_dynamic_20m.txt", 40, 1e-9),    # tiles of 10 cells, eddy viscosity
+    ("Input_cap_morpho_2d.txt", 40, 1e-12),            # Variable drag (tanh switch), Mixed erosion
+    ("Input_tile_indep_dynamic_20m.txt", 40, 1e-12),   # tiles of 10 cells, eddy viscosity
 ])
 def test_reference_deck_against_oracle(gpu, name, steps, tol):
     run = load(name)

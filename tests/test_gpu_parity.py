@@ -69,8 +69,9 @@ def test_plain_cap_maxima(gpu):
     mo, mg = o.maxima(), g.maxima()
     st = o.state()
     m = active_mask(wl.cfg, st["tile_state"])
-    for k in (0, 1, 2, 3, 10):
-        assert rel_inf(mg[k], mo[k], m) <= TOL, f"maxima array {k}"
+    for k in range(11):
+        assert np.array_equal(mg[k][m], mo[k][m]), f"maxima array {k}"
+    assert mo[0][m].max() > 0.0 and (mo[10][m] > 0.0).any()
 
 
 def test_integrate_to_matches(gpu):
@@ -97,8 +98,9 @@ def test_closure_decks_parity(gpu, case):
     assert sg["steps"] == so["steps"]
     gs, os_ = g.state(), o.state()
     m = active_mask(wl.cfg, os_["tile_state"])
-    # tanh/pow/cos of the device library may differ from glibc by ulps
-    tol = 1e-12 if case not in ("edwards2019", "manning", "variable") else 1e-9
+    # tanh/pow/cos of the device library may differ from glibc by ulps; the
+    # fields stay within the 1e-12 parity bar (measured <= 2.2e-13)
+    tol = 1e-12
     for f in FIELDS:
         r = rel_inf(gs[f], os_[f], m)
         assert r <= tol, f"{case}: {f} rel err {r:.3e}"
