@@ -21,8 +21,14 @@ events on the library's stream around every launch, launches that did work
 only); the roofline of the dominant kernel (the most total time); the plain
 FP64 hydraulic path at 2048x2048 with its own roofline; the wall time of one
 kst_volume call (CalculateVolume on the device) on that grid; and the CPU
-baseline (the oracle, a C restatement, 1 thread, on the first steps of the
-same workload).  --volume-only runs the kst_volume leg alone.
+baselines: the reference program itself (its Fortran sources built with
+release optimisation, serial, oracle/_ref/time) on a bounded sample of the
+same deck -- the first 8 s of simulated time of the morphodynamic run, two
+steps of the 2048 x 2048 plain run -- beside the oracle (the C restatement,
+1 thread) and, for all host cores, one reference process per core on the
+256 x 256 cap release.  Launch shapes come from the library's autotuner
+(timed trials per active-set bucket, made in the warm-up).
+--volume-only runs the kst_volume leg alone.
 """
 from __future__ import annotations
 
@@ -79,7 +85,7 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def cpu_baseline(wl, budget_s: float = 20.0) -> dict:
+def cpu_port(wl, budget_s: float = 20.0) -> dict:
     """The oracle (C restatement, 1 thread) on a bounded sample of the
     workload: its first steps from the same initial state, until budget_s."""
     from oracle.oracle import Oracle
@@ -102,6 +108,65 @@ def cpu_baseline(wl, budget_s: float = 20.0) -> dict:
                       f"{el:.1f} s) on {cpu_model()}, 1 thread"}
 
 
+DECKS = os.path.join(ROOT, "tests", "golden", "workloads")
+
+
+def _sample_deck(name: str, tend: float, work: str) -> str:
+    """The workload's deck (tests/golden/workloads) ending at tend, one output."""
+    import re
+    with open(os.path.join(DECKS, name + ".txt")) as f:
+        text = f.read()
+    text = re.sub(r"(?m)^T end = .*$", f"T end = {tend!r}", text)
+    text = re.sub(r"(?m)^N out = .*$", "N out = 1", text)
+    path = os.path.join(work, name + "_sample.txt")
+    with open(path, "w") as f:
+        f.write(text)
+    return path
+
+
+def cpu_reference(wl, tend: float, copies: int = 1) -> dict | None:
+    """The reference program (oracle/_ref/time/ref_run: its own Fortran
+    sources built with release optimisation, serial) integrating the
+    workload's deck to tend from its initial state, `copies` independent
+    processes at once (one per core).  The cell-updates of the sample are
+    those the device counts for the same integration (the same steps and
+    active tiles).  Wall time includes the program's set-up and its two
+    solution files.  None where the binary was not built."""
+    import tempfile
+    from kestrel_amd.runtime import Stepper
+    from kestrel_amd import scenario as S
+    from oracle import refrun
+    if not refrun.available(refrun.TIME_BIN):
+        return None
+    g = Stepper(wl.cfg)
+    g.set_topography(wl.b0v)
+    g.set_state(wl.init)
+    st = g.integrate_to(tend)
+    g.close()
+    with tempfile.TemporaryDirectory() as work:
+        path = _sample_deck(wl.name, tend, work)
+        b0 = None
+        if wl.name.startswith("morpho_hills"):
+            xv, yv = S.vertex_coords(wl.cfg)
+            b0 = os.path.join(work, "b0.bin")
+            refrun.write_b0(b0, wl.b0v, float(xv[0]), float(yv[0]), wl.cfg.dx, wl.cfg.dy)
+        procs = []
+        t0 = time.perf_counter()
+        for k in range(copies):
+            cmd = [refrun.TIME_BIN, path, os.path.join(work, f"out{k}"), "none"] + ([b0] if b0 else [])
+            procs.append(subprocess.Popen(cmd, cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+        rcs = [p.wait() for p in procs]
+        el = time.perf_counter() - t0
+    if any(rcs):
+        return {"error": f"reference exited {rcs}"}
+    cells = st["cell_updates"] * copies
+    return {"value": cells / el, "unit": "cell-updates/s", "cores": copies, "kind": "reference",
+            "sample": f"{wl.name} deck to t = {tend} s ({st['steps']} steps, {st['cell_updates']} "
+                      f"cell-updates{' per process' if copies > 1 else ''}), "
+                      f"{'%d concurrent processes' % copies if copies > 1 else '1 process'} of the "
+                      f"reference built from its sources (-O2), {el:.1f} s wall on {cpu_model()}"}
+
+
 def run(wl, steps: int, warmup: int):
     """Warm-up on its own handle, then K graph-launched steps from the initial
     state (value), then the same K steps again on a fresh handle with every
@@ -111,6 +176,7 @@ def run(wl, steps: int, warmup: int):
 
     def fresh():
         g = Stepper(wl.cfg)
+        g.autotune(True)       # choices cached per configuration: the warm-up pass makes them
         g.set_topography(wl.b0v)
         g.set_state(wl.init)
         return g
@@ -125,6 +191,7 @@ def run(wl, steps: int, warmup: int):
     st = g.step_n(steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    st["autotune"] = g.autotune_choices()
     g.close()
     g = fresh()
     g.profile(True)
@@ -323,7 +390,7 @@ def main():
                    "active_cells_per_step": cells_per_step, "final_active_tiles": st["active_tiles"],
                    "simulated_time_s": st["t"], "precision": "fp64 plain",
                    "rhs_evals": st["rhs_evals"], "rejected": st["rejected"],
-                   "redistributed_cells": st["redist_cells"]},
+                   "redistributed_cells": st["redist_cells"], "autotune": st["autotune"]},
         "roofline": roofline(wl.name, rows, cells_per_step),
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
         "step_roofline_frac": value * STEP_WORDS[mode] * 8 / (HBM_PEAK_GBS * 1e9),
@@ -337,7 +404,10 @@ def main():
                              "value": pst["cell_updates"] / pel, "unit": "cell-updates/s",
                              "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pw.name, prows, pcells),
                              "step_roofline_frac": pst["cell_updates"] / pel * STEP_WORDS["hydraulic"] * 8
-                             / (HBM_PEAK_GBS * 1e9)}
+                             / (HBM_PEAK_GBS * 1e9), "autotune": pst["autotune"]}
+        if not args.no_cpu:
+            # two steps of the same deck in the reference (about 10 s of CPU work)
+            res["plain_fp64"]["cpu_baseline"] = cpu_reference(pw, 0.0225)
     if not args.no_plain:
         res["volume"] = volume_leg()
     if not args.no_batch:
@@ -345,7 +415,19 @@ def main():
     if not args.no_serve:
         res["serving"] = serving_leg(args.serve_requests)
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
+        ref = None
+        if wl.name == "morpho_hills_1024":
+            ref = cpu_reference(wl, 8.0)
+        elif wl.name in ("plain_cap_256", "plain_cap_2048"):
+            ref = cpu_reference(wl, 2.8125 if wl.name == "plain_cap_256" else 0.0225)
+        port = cpu_port(wl, args.cpu_budget)
+        res["cpu_baseline"] = ref if ref and "error" not in ref else port
+        res["cpu_port"] = port
+        # all host cores: independent reference processes of the 256 x 256 cap
+        # release (the reference is serial), one per core up to 16
+        from kestrel_amd import workloads as W
+        ncore = min(16, os.cpu_count() or 1)
+        res["cpu_all_cores"] = cpu_reference(W.plain_cap(256, 250), 2.8125, copies=ncore)
     print(json.dumps(res), flush=True)
 
 

@@ -218,6 +218,27 @@ void kst_destroy(kst_handle* h);
  * 0 restores the default.  Results do not depend on the cap. */
 int kst_set_grid_cap(kst_handle* h, int tiles);
 
+/* Autotuner (no counterpart in the reference, which loops over its tiles on
+ * one CPU thread).  With enable != 0 the library times, before the first
+ * launch on each new bucket of active-set sizes (active tiles rounded up to a
+ * power of two), the first RK stage kernel in each candidate launch shape --
+ * 16 x 16 sub-blocks on the default grid, on a grid of the blocks the device
+ * holds at once (blocks then take further sub-blocks in turn) or twice that,
+ * and 16 x 4 sub-blocks (FP64) -- and launches that bucket in the fastest.
+ * Choices are cached for the process per grid layout, operator and bucket,
+ * so later handles of the same configuration reuse them without trials.
+ * The trials leave the state and the control block as they were: results
+ * do not depend on the choices.  enable = 0 restores the fixed rule (16 x 4
+ * below 128 sub-blocks of 16 x 16, the default grid) and forgets the
+ * choices. */
+int kst_autotune(kst_handle* h, int enable);
+/* The choices made so far, at most max: per bucket, whether 16 x 4
+ * sub-blocks were chosen, the grid limit of the 16 x 16 shape (0: default
+ * grid) and the stage-1 launch time in microseconds of the four candidates
+ * in the order above (0: not tried).  Returns the number written. */
+int kst_autotune_choices(const kst_handle* h, int max, int32_t* bucket, int32_t* small,
+                         int32_t* grid, double* us);
+
 /* Redistribution chunks (no counterpart in the reference, whose
  * RedistributeGrid walks its sorted list one cell at a time,
  * src/Redistribute.f90:203-244).  The device walks up to `chunk` sorted

@@ -742,62 +742,59 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
     if (sh_skip) return;
     const int n = min(sh_n, d.cand_cap);
     const int chunk = d.rd_chunk;
-    if (n <= chunk) {
-        redistribute_small(d, p, 0, n, lx, lk, lev, list, &sh_ndirty);
-        __syncthreads();
-        if (tid == 0) c->ndirty = (long long)sh_ndirty;
-        return;
-    }
-    // ---- sort by (excess, scan order); the padding sorts last
-    int m2 = 1;
-    while (m2 < n) m2 <<= 1;
-    long long* K = d.cand_key;
-    double* X = d.cand_ex;
-    const bool in_lds = m2 <= d.rd_lds_sort;
-    if (in_lds) {
-        for (int q = tid; q < m2; q += RD_THREADS) {
-            lk[q] = q < n ? K[q] : 0x7fffffffffffffffLL;
-            lx[q] = q < n ? X[q] : kHuge;
-        }
-    } else {
-        for (int q = n + tid; q < m2; q += RD_THREADS) {
-            K[q] = 0x7fffffffffffffffLL;
-            X[q] = kHuge;
-        }
-    }
-    __syncthreads();
-    long long* SK = in_lds ? lk : K;
-    double* SX = in_lds ? lx : X;
-    for (int kk = 2; kk <= m2; kk <<= 1) {
-        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+    if (n > chunk) {
+        // ---- sort by (excess, scan order); the padding sorts last
+        int m2 = 1;
+        while (m2 < n) m2 <<= 1;
+        long long* K = d.cand_key;
+        double* X = d.cand_ex;
+        const bool in_lds = m2 <= d.rd_lds_sort;
+        if (in_lds) {
             for (int q = tid; q < m2; q += RD_THREADS) {
-                const int l = q ^ jj;
-                if (l <= q) continue;
-                const double xq = SX[q], xl = SX[l];
-                const long long kq = SK[q], kl = SK[l];
-                const bool l_less = (xl < xq) || (xl == xq && kl < kq);
-                const bool up = (q & kk) == 0;
-                if (l_less == up) {
-                    SX[q] = xl; SX[l] = xq;
-                    SK[q] = kl; SK[l] = kq;
-                }
+                lk[q] = q < n ? K[q] : 0x7fffffffffffffffLL;
+                lx[q] = q < n ? X[q] : kHuge;
             }
-            __syncthreads();
+        } else {
+            for (int q = n + tid; q < m2; q += RD_THREADS) {
+                K[q] = 0x7fffffffffffffffLL;
+                X[q] = kHuge;
+            }
         }
-    }
-    if (in_lds) {
-        for (int q = tid; q < n; q += RD_THREADS) {
-            K[q] = lk[q];
-            X[q] = lx[q];
+        __syncthreads();
+        long long* SK = in_lds ? lk : K;
+        double* SX = in_lds ? lx : X;
+        for (int kk = 2; kk <= m2; kk <<= 1) {
+            for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+                for (int q = tid; q < m2; q += RD_THREADS) {
+                    const int l = q ^ jj;
+                    if (l <= q) continue;
+                    const double xq = SX[q], xl = SX[l];
+                    const long long kq = SK[q], kl = SK[l];
+                    const bool l_less = (xl < xq) || (xl == xq && kl < kq);
+                    const bool up = (q & kk) == 0;
+                    if (l_less == up) {
+                        SX[q] = xl; SX[l] = xq;
+                        SK[q] = kl; SK[l] = kq;
+                    }
+                }
+                __syncthreads();
+            }
         }
+        if (in_lds) {
+            for (int q = tid; q < n; q += RD_THREADS) {
+                K[q] = lk[q];
+                X[q] = lx[q];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // ---- the walk, chunk by chunk (a chunk reads what the earlier ones wrote:
-    // one workgroup, so the barrier orders its global stores and loads)
-    for (int b = 0; b < n; b += chunk) {
+    // one workgroup, so the barrier orders its global stores and loads); one
+    // call site, so that the walk is inlined with its arrays in registers
+    for (int b = 0;; b += chunk) {
         const bool stopped = redistribute_small(d, p, b, min(chunk, n - b), lx, lk, lev, list, &sh_ndirty);
         __syncthreads();
-        if (stopped) break;
+        if (stopped || b + chunk >= n) break;
     }
     if (tid == 0) c->ndirty = (long long)sh_ndirty;
 }

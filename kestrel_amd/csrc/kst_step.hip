@@ -22,6 +22,8 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/kestrel_step.h"
@@ -1949,8 +1951,16 @@ struct kst_handle {
     // profiling: per-iteration event pairs around the four stage kernels
     bool profiling = false;
     // kGraphIters whole iterations, per (grid cap, sub-block shape)
-    std::vector<std::pair<int, hipGraphExec_t>> graphs;
+    std::vector<std::pair<long long, hipGraphExec_t>> graphs;
     int fixed_cap = 0;                   // kst_set_grid_cap (0: from the active set)
+    int resident = 0;                    // 16 x 16 stage-kernel blocks resident on the device at once
+    bool autotune = false;               // kst_autotune: choices per active-set bucket
+    struct Tune {
+        int small;                       // 16 x 4 sub-blocks
+        int grid;                        // launch-grid limit of the 16 x 16 shape (0: the cap's grid)
+        double us[4];                    // stage-1 launch time of each candidate (0: not tried)
+    };
+    std::map<int, Tune> tuned;           // bucket (active tiles, rounded up to a power of two)
     double* vol_part = nullptr;          // volume partials (VOL_BLOCKS_MAX + 1 groups)
     double* out_buf = nullptr;           // kst_get_output staging, out_cap doubles
     size_t out_cap = 0;
@@ -2124,7 +2134,7 @@ const StageSet& stage_set(bool visc, int limiter, bool mixed, bool small) {
 
 // Launch grids cover `cap` active tiles (the kernels take further items in
 // turn when the active set has outgrown it).
-int launch_iteration(kst_handle* h, bool profiled, bool small, int cap) {
+int launch_iteration(kst_handle* h, bool profiled, bool small, int cap, int grid_limit = 0) {
     const Params& p = h->p;
     const Dev& d = h->d;
     // while profiling, the dispatch of each launch carries the event pair of
@@ -2144,7 +2154,9 @@ int launch_iteration(kst_handle* h, bool profiled, bool small, int cap) {
     const bool visc = p.nu != 0.0;
     const bool sm = small && !d.mixed;
     const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
-    const int nblk_stage = sm ? 8 * ((cap * spt_s + 7) / 8) : nblk;
+    // a grid limit (a multiple of 8) keeps that many blocks, each taking
+    // further items in turn
+    const int nblk_stage = sm ? 8 * ((cap * spt_s + 7) / 8) : (grid_limit > 0 ? std::min(nblk, grid_limit) : nblk);
     const Dev* const A = h->d_args;
     const Params* const P = h->d_par;
     LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), A, P);
@@ -2191,14 +2203,14 @@ int work_cap(const kst_handle* h, int n) {
 // One hipGraph holding kGraphIters whole iterations (activation, four stage
 // kernels, finalize), per grid cap and sub-block shape; kernel arguments
 // never change for a handle.
-int ensure_graph(kst_handle* h, bool small, int cap, hipGraphExec_t* out) {
-    const int key = 2 * cap + (small ? 1 : 0);
+int ensure_graph(kst_handle* h, bool small, int cap, int grid_limit, hipGraphExec_t* out) {
+    const long long key = ((long long)cap << 32) | ((long long)grid_limit << 1) | (small ? 1 : 0);
     for (const auto& kv : h->graphs)
         if (kv.first == key) { *out = kv.second; return 0; }
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int q = 0; q < kGraphIters; ++q) {
-        int rc = launch_iteration(h, false, small, cap);
+        int rc = launch_iteration(h, false, small, cap, grid_limit);
         if (rc) {
             hipGraph_t dummy = nullptr;
             (void)hipStreamEndCapture(h->stream, &dummy);
@@ -2213,6 +2225,80 @@ int ensure_graph(kst_handle* h, bool small, int cap, hipGraphExec_t* out) {
     HIPCHK(h, e);
     h->graphs.emplace_back(key, gx);
     *out = gx;
+    return 0;
+}
+
+
+// Autotuner (kst_autotune).  For the active set of the coming launches it
+// times the first RK stage kernel -- a launch that reads u^n and writes only
+// the RHS buffer, the CFL minimum and the edge flags, all of which the
+// iteration's own stage 1 rewrites from the same u^n -- in each candidate
+// launch shape: 16 x 16 sub-blocks on the cap's grid, the same capped at the
+// resident blocks of the device (each block taking further items in turn),
+// at twice that, and (FP64) 16 x 4 sub-blocks.  The control block is saved
+// before the trials and restored after them, so the run is unchanged.  The
+// fastest shape is kept per bucket of active-set sizes (powers of two).
+int tune_bucket(int n) {
+    int b = 1;
+    while (b < n) b <<= 1;
+    return b;
+}
+
+// Choices are cached for the process, per grid layout, operator and bucket,
+// so that a fresh handle of the same configuration launches tuned at once.
+std::mutex g_tune_mutex;
+std::map<std::string, kst_handle::Tune> g_tune_cache;
+
+std::string tune_key(const kst_handle* h, int bucket) {
+    const Params& p = h->p;
+    char k[160];
+    std::snprintf(k, sizeof k, "%d %d %d %d %d %d %d %d %d", p.nx, p.ny, p.ntx, p.nty, p.nu != 0.0,
+                  p.limiter, h->d.mixed, h->d.morpho, bucket);
+    return k;
+}
+
+int autotune_bucket(kst_handle* h, const Ctrl& c0, int cap, kst_handle::Tune* out) {
+    constexpr int kTrials = 6;
+    Ctrl c = c0;
+    c.status = ST_NEW;          // stage 1 of the first operator does its work
+    int rc = write_ctrl(h, &c);
+    if (rc) return rc;
+    const Params& p = h->p;
+    const bool visc = p.nu != 0.0, mixed = h->d.mixed != 0;
+    const int spt = ((p.nx + B - 1) / B) * ((p.ny + B - 1) / B);
+    const int nblk = 8 * ((cap * spt + 7) / 8);
+    struct Cand { int small, grid; } cand[4] = {{0, 0}, {0, h->resident}, {0, 2 * h->resident}, {1, 0}};
+    const int ncand = mixed ? 3 : 4;
+    kst_handle::Tune t{};
+    t.small = 0; t.grid = 0;
+    double best = 1e300;
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    for (int q = 0; q < ncand; ++q) {
+        const Cand& k = cand[q];
+        if (!k.small && k.grid > 0 && k.grid >= nblk) continue;   // no different from the cap's grid
+        const StageSet& set = stage_set(visc, p.limiter, mixed, k.small != 0);
+        const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
+        const int grid = k.small ? 8 * ((cap * spt_s + 7) / 8) : (k.grid > 0 ? std::min(nblk, k.grid) : nblk);
+        hipLaunchKernelGGL(set.fn[0], dim3(grid), dim3(STAGE_NT), 0, h->stream, h->d_args, h->d_par);
+        HIPCHK(h, hipEventRecord(e0, h->stream));
+        for (int r = 0; r < kTrials; ++r)
+            hipLaunchKernelGGL(set.fn[0], dim3(grid), dim3(STAGE_NT), 0, h->stream, h->d_args, h->d_par);
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / kTrials;
+        t.us[q] = us;
+        if (us < best) { best = us; t.small = k.small; t.grid = k.small ? 0 : k.grid; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(h, hipGetLastError());
+    rc = write_ctrl(h, &c0);
+    if (rc) return rc;
+    *out = t;
     return 0;
 }
 
@@ -2238,14 +2324,38 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
         // the sub-block shape and grid cap for this batch, from the last
         // known active set
         const int spt = ((h->p.nx + B - 1) / B) * ((h->p.ny + B - 1) / B);
-        const bool small = (long long)c.n_active * spt < SMALL_WORK;
+        bool small = (long long)c.n_active * spt < SMALL_WORK;
         const int cap = work_cap(h, c.n_active);
+        int grid_limit = 0;
+        if (h->autotune) {
+            const int bucket = tune_bucket(c.n_active);
+            auto it = h->tuned.find(bucket);
+            if (it == h->tuned.end()) {
+                kst_handle::Tune tu{};
+                const std::string key = tune_key(h, bucket);
+                bool cached = false;
+                {
+                    std::lock_guard<std::mutex> lk(g_tune_mutex);
+                    auto ct = g_tune_cache.find(key);
+                    if (ct != g_tune_cache.end()) { tu = ct->second; cached = true; }
+                }
+                if (!cached) {
+                    rc = autotune_bucket(h, c, cap, &tu);
+                    if (rc) return rc;
+                    std::lock_guard<std::mutex> lk(g_tune_mutex);
+                    g_tune_cache[key] = tu;
+                }
+                it = h->tuned.emplace(bucket, tu).first;
+            }
+            small = it->second.small != 0;
+            grid_limit = it->second.grid;
+        }
         if (h->profiling) {
-            rc = launch_iteration(h, true, small, cap);
+            rc = launch_iteration(h, true, small, cap, grid_limit);
             if (rc) return rc;
         } else {
             hipGraphExec_t gx = nullptr;
-            rc = ensure_graph(h, small, cap, &gx);
+            rc = ensure_graph(h, small, cap, grid_limit, &gx);
             if (rc) return rc;
             for (int q = 0; q < batch; q += kGraphIters) HIPCHK(h, hipGraphLaunch(gx, h->stream));
         }
@@ -2345,6 +2455,18 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
         delete h;
         return KST_ERR_HIP;
+    }
+    {
+        // blocks of the 16 x 16 stage kernel the device holds at once (the
+        // autotuner's resident grid)
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void*>(stage_kernel<2, false, KST_LIMITER_MINMOD2, false, B>),
+                STAGE_NT, 0) != hipSuccess) {
+            cus = 0; per = 0;
+        }
+        h->resident = 8 * ((std::max(cus * per, 8) + 7) / 8);
     }
     Dev& d = h->d;
     const size_t FS = (size_t)h->ntiles * h->NC;
@@ -2840,6 +2962,29 @@ int kst_set_grid_cap(kst_handle* h, int tiles) {
     if (!h || tiles < 0) return KST_ERR_CONFIG;
     h->fixed_cap = tiles;
     return KST_OK;
+}
+
+int kst_autotune(kst_handle* h, int enable) {
+    if (!h) return KST_ERR_CONFIG;
+    h->autotune = enable != 0;
+    h->tuned.clear();
+    return KST_OK;
+}
+
+int kst_autotune_choices(const kst_handle* h, int max, int32_t* bucket, int32_t* small,
+                         int32_t* grid, double* us) {
+    if (!h || max < 0) return KST_ERR_CONFIG;
+    int n = 0;
+    for (const auto& kv : h->tuned) {
+        if (n >= max) break;
+        if (bucket) bucket[n] = kv.first;
+        if (small) small[n] = kv.second.small;
+        if (grid) grid[n] = kv.second.grid;
+        if (us)
+            for (int q = 0; q < 4; ++q) us[4 * n + q] = kv.second.us[q];
+        ++n;
+    }
+    return n;
 }
 
 int kst_set_redistribution_chunk(kst_handle* h, int chunk, int lds_sort) {

@@ -2,10 +2,13 @@
 
     python -m kestrel_amd.profsum trace RESULTS_DB [--bench BENCH_LOG]
     python -m kestrel_amd.profsum pmc FETCH_DB WRITE_DB [--bench BENCH_LOG]
+    python -m kestrel_amd.profsum counters DB [--bench BENCH_LOG] [--kernels a,b]
 
 `trace` tabulates a --kernel-trace run: dispatches, total and mean duration
 per kernel.  `pmc` tabulates the FETCH_SIZE and WRITE_SIZE counters (KiB per
-dispatch) of two --pmc runs.  With --bench, the JSON line that bench.py
+dispatch) of two --pmc runs.  `counters` tabulates every counter of one --pmc
+run (mean per productive dispatch) and, where SQ_INSTS_VALU and SQ_WAVES are
+present, the VALU instructions per wave.  With --bench, the JSON line that bench.py
 printed in the profiled run names how many launches of each kernel did work
 in one pass; bench.py runs the same steps twice (timed, then profiled with HIP
 events), so the 2 x N longest dispatches (largest FETCH_SIZE for `pmc`) are
@@ -118,6 +121,30 @@ def pmc_table(fetch_db: str, write_db: str, bench: dict | None = None,
     return "\n".join(out)
 
 
+def counter_table(db: str, bench: dict | None = None, kernels=None) -> str:
+    con = sqlite3.connect(db)
+    names = [r[0] for r in con.execute("select distinct counter_name from counters_collection")]
+    data = {c: _counter(db, c) for c in names}
+    prod = productive_counts(bench) if bench else {}
+    key = "SQ_WAVES" if "SQ_WAVES" in data else names[0]
+    ks = sorted(data[key], key=lambda k: -sum(data[key][k].values()))
+    if kernels:
+        ks = [k for k in ks if k in kernels]
+    out = ["| kernel | dispatches | " + " | ".join(names) + " | VALU insts / wave |",
+           "|---|---|" + "---|" * len(names) + "---|"]
+    for k in ks:
+        # productive dispatches: the n with the most waves (or the first counter)
+        ids = sorted(data[key][k], key=lambda i: data[key][k][i])
+        n = prod.get(k)
+        sel = ids[-n:] if n and n <= len(ids) else ids
+        means = {c: statistics.mean(data[c][k].get(i, 0.0) for i in sel) for c in names}
+        vpw = (means["SQ_INSTS_VALU"] / means["SQ_WAVES"]
+               if means.get("SQ_WAVES") and "SQ_INSTS_VALU" in means else None)
+        out.append(f"| `{k}` | {len(sel)} | " + " | ".join(f"{means[c]:.4g}" for c in names)
+                   + f" | {f'{vpw:.1f}' if vpw is not None else '-'} |")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -128,10 +155,16 @@ def main(argv=None):
     m.add_argument("fetch_db")
     m.add_argument("write_db")
     m.add_argument("--bench")
+    c = sub.add_parser("counters")
+    c.add_argument("db")
+    c.add_argument("--bench")
+    c.add_argument("--kernels")
     a = ap.parse_args(argv)
     bench = bench_line(a.bench) if a.bench else None
     if a.cmd == "trace":
         print(trace_table(a.db, bench))
+    elif a.cmd == "counters":
+        print(counter_table(a.db, bench, a.kernels.split(",") if a.kernels else None))
     else:
         print(pmc_table(a.fetch_db, a.write_db, bench))
 

@@ -35,7 +35,7 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
            "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap",
-           "kst_set_redistribution_chunk"]
+           "kst_set_redistribution_chunk", "kst_autotune", "kst_autotune_choices"]
 
 
 class KstStats(ctypes.Structure):
@@ -81,6 +81,11 @@ def lib():
         L.kst_set_grid_cap.restype = ctypes.c_int
         L.kst_set_redistribution_chunk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.kst_set_redistribution_chunk.restype = ctypes.c_int
+        L.kst_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.kst_autotune.restype = ctypes.c_int
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.kst_autotune_choices.argtypes = [ctypes.c_void_p, ctypes.c_int, i32p, i32p, i32p, _dp]
+        L.kst_autotune_choices.restype = ctypes.c_int
         L.kst_volume.argtypes = [ctypes.c_void_p, _dp]
         L.kst_volume.restype = ctypes.c_int
         L.kst_get_output.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int32),
@@ -222,6 +227,25 @@ class Stepper:
         """Walk at most `chunk` redistribution candidates at once and sort at
         most `lds_sort` in LDS (0: defaults); results do not depend on them."""
         self._check(self.L.kst_set_redistribution_chunk(self.h, int(chunk), int(lds_sort)))
+
+    def autotune(self, enable: bool = True):
+        """Choose the launch shape per active-set bucket from timed trials."""
+        self._check(self.L.kst_autotune(self.h, int(enable)))
+
+    def autotune_choices(self) -> list:
+        n = 64
+        b = np.zeros(n, np.int32); sm = np.zeros(n, np.int32); gr = np.zeros(n, np.int32)
+        us = np.zeros((n, 4))
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        k = self.L.kst_autotune_choices(self.h, n, b.ctypes.data_as(i32p), sm.ctypes.data_as(i32p),
+                                        gr.ctypes.data_as(i32p), _ptr(us))
+        if k < 0:
+            raise KestrelError(k, "kst_autotune_choices")
+        names = ("16x16 default grid", "16x16 resident grid", "16x16 twice resident", "16x4")
+        return [{"active_tiles_bucket": int(b[q]),
+                 "shape": "16x4" if sm[q] else ("16x16" if gr[q] == 0 else f"16x16 grid {int(gr[q])}"),
+                 "stage1_us": {names[c]: round(float(us[q, c]), 2) for c in range(4) if us[q, c] > 0}}
+                for q in range(k)]
 
     def profile(self, enable: bool = True):
         self._check(self.L.kst_profile(self.h, int(enable)))

@@ -71,6 +71,15 @@ def test_run_vectors_cover_the_workloads():
         for f in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "tile_state"):
             assert np.array_equal(getattr(run.workload.init, f), getattr(wl.init, f)), (name, f)
         assert name in FIX
+    # the plain path's full-chip workload (its deck is the CPU baseline's sample)
+    import os as _os
+    wl = workloads.plain_cap_2048(200)
+    run = runvec.deck.load(_os.path.join(runvec.OWN_DECKS, "plain_cap_2048.txt"))
+    assert run.workload.cfg.as_dict() == wl.cfg.as_dict()
+    assert np.array_equal(run.workload.b0v, wl.b0v)
+    for f in ("w", "rhoHnu", "rhoHnv", "Hnpsi", "tile_state"):
+        assert np.array_equal(getattr(run.workload.init, f), getattr(wl.init, f)), f
+    assert run.tend == 2.25
     c1 = FIX["plain_cap_256"]["intervals"]
     assert sum(r["steps"] for r in c1[1:]) >= 1000
 

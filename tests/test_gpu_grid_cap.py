@@ -52,3 +52,30 @@ def test_grid_cap_rejects_negative(gpu):
     with pytest.raises(Exception):
         g.set_grid_cap(-1)
     g.close()
+
+
+@pytest.mark.parametrize("case", ["plain_large", "plain_small", "morpho"])
+def test_autotune_same_bits(gpu, case):
+    """The autotuner's timed trials (stage-1 launches between saved and
+    restored control blocks) and whichever launch shape it picks per
+    active-set bucket leave the results bit for bit as the fixed rule's."""
+    if case == "plain_large":
+        # ~280 active tiles of 32 x 32: the default grid exceeds the resident one
+        wl, steps = workloads.plain_cap(1024, 0, tile=32, cap_x=0.0, cap_radius=300.0), 20
+    elif case == "plain_small":
+        wl, steps = workloads.plain_cap(256, 0, tile=16, cap_radius=20.0), 40
+    else:
+        wl, steps = workloads.morpho_cap(128, 0, tile=16, radius=12.0), 30
+    st0, s0 = _run(wl, steps, None)
+    g = stepper_for(wl)
+    g.autotune(True)
+    st1 = g.step_n(steps)
+    s1 = g.state()
+    ch = g.autotune_choices()
+    g.close()
+    assert ch and all(c["stage1_us"] for c in ch), ch
+    print(f"{case}: {ch}")
+    for k in STATS:
+        assert st0[k] == st1[k], f"{case}: {k} {st0[k]} vs {st1[k]}"
+    for f in FIELDS:
+        assert np.array_equal(s0[f], s1[f]), f"{case}: {f} differs with the autotuner on"
