@@ -192,6 +192,19 @@ int kst_set_maxima(kst_handle* h, const double* const maxima[11]);
  * the exact sums and may differ from each other in the last bits. */
 int kst_volume(kst_handle* h, double out[6]);
 
+/* kst_volume without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc -- the same sums as
+ * kst_volume (the same kernels and partition, so the same bits) with the
+ * partial sums in a fixed device array of the library (VOL_BLOCKS_MAX = 2048
+ * groups of 8 doubles plus one, 131 KB per GPU, held until the process ends)
+ * instead of the handle's buffer.  The number of partial groups is capped at
+ * that capacity (workgroups take further tiles in turn), so every active set
+ * fits.  Calls no hipMalloc/hipFree and no allocating library routine.  The
+ * handle's device must be current.  Every call on one GPU shares the array:
+ * calls that overlap on one GPU (other host threads, other streams) race. */
+int kst_volume_noalloc(kst_handle* h, double out[6]);
+
 /* The per-cell columns of OutputSolutionData_txt (src/Output.f90:754-929)
  * for the active tiles, computed on the device: on return *n_tiles holds the
  * active-tile count n, tiles[0..n) the tile ids (0-based, the file writes

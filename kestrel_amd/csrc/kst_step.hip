@@ -1969,6 +1969,15 @@ struct kst_handle {
     long long prof_n[KW_N] = {};
 };
 
+// Device scratch of kst_volume_noalloc: the partial sums of the volume
+// kernels (VOL_BLOCKS_MAX groups) and their merged result (one group), the
+// same layout as the handle's vol_part.  One copy per GPU, held until the
+// process ends; every call on that GPU shares it.  In host code the array's
+// name stands for no device address: its address on each device comes from
+// hipGetSymbolAddress (scratch_ptr), taken once per device in kst_create.
+constexpr size_t kScratchDoubles = (size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 // A batch: a fixed table of slots, each holding a scenario handle whose
 // device state stays resident; one hipGraph of kBatchIters iterations walks
 // every running slot (kernel arguments point at the table, so scenarios join
@@ -1994,6 +2003,29 @@ namespace {
 int set_err(kst_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
     return code;
+}
+
+// scratch's address on each device (nullptr until taken).
+std::mutex g_scratch_mutex;
+std::vector<double*> g_scratch_ptr;
+
+// Take scratch's address on `dev`, which must be the current device; once
+// per device.  The address is kept for every later call on that device.
+int scratch_setup(kst_handle* h, int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (g_scratch_ptr[dev]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_err(h, KST_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    return 0;
+}
+
+double* scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    return dev < (int)g_scratch_ptr.size() ? g_scratch_ptr[dev] : nullptr;
 }
 
 #define HIPCHK(h, call)                                                                  \
@@ -2468,6 +2500,12 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
         }
         h->resident = 8 * ((std::max(cus * per, 8) + 7) / 8);
     }
+    if (scratch_setup(h, h->device)) {
+        const int e = KST_ERR_HIP;
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return e;
+    }
     Dev& d = h->d;
     const size_t FS = (size_t)h->ntiles * h->NC;
     const size_t NV = (size_t)(h->nx + 1) * (h->ny + 1);
@@ -2841,17 +2879,17 @@ int kst_get_state(kst_handle* h, double* w, double* rhu, double* rhv, double* hn
     return KST_OK;
 }
 
-int kst_volume(kst_handle* h, double out[6]) {
-    if (!h || !out) return KST_ERR_CONFIG;
-    if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
+namespace {
+
+// CalculateVolume with the partial sums in `part` (VOL_BLOCKS_MAX + 1 groups).
+int volume_into(kst_handle* h, double* part, double out[6]) {
     Ctrl c;
     int rc = read_ctrl(h, &c);
     if (rc) return rc;
     const int nb = std::max(1, std::min(c.n_active, VOL_BLOCKS_MAX));
-    double* fin = h->vol_part + (size_t)VOL_BLOCKS_MAX * VOL_Q * 2;
-    hipLaunchKernelGGL(volume_kernel, dim3(nb), dim3(VOL_NT), 0, h->stream, h->d_args, h->d_par,
-                       h->vol_part);
-    hipLaunchKernelGGL(volume_final_kernel, dim3(1), dim3(VOL_NT), 0, h->stream, h->vol_part, nb, fin);
+    double* fin = part + (size_t)VOL_BLOCKS_MAX * VOL_Q * 2;
+    hipLaunchKernelGGL(volume_kernel, dim3(nb), dim3(VOL_NT), 0, h->stream, h->d_args, h->d_par, part);
+    hipLaunchKernelGGL(volume_final_kernel, dim3(1), dim3(VOL_NT), 0, h->stream, part, nb, fin);
     HIPCHK(h, hipGetLastError());
     double r[VOL_Q * 2];
     HIPCHK(h, hipMemcpyAsync(r, fin, sizeof r, hipMemcpyDeviceToHost, h->stream));
@@ -2870,6 +2908,29 @@ int kst_volume(kst_handle* h, double out[6]) {
     out[4] = solids * g.rhos;
     out[5] = bed * g.rhos * (1.0 - g.bed_porosity);
     return KST_OK;
+}
+
+}  // namespace
+
+int kst_volume(kst_handle* h, double out[6]) {
+    if (!h || !out) return KST_ERR_CONFIG;
+    if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
+    return volume_into(h, h->vol_part, out);
+}
+
+int kst_volume_noalloc(kst_handle* h, double out[6]) {
+    if (!h || !out) return KST_ERR_CONFIG;
+    if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
+    int cur = -1;
+    HIPCHK(h, hipGetDevice(&cur));
+    if (cur != h->device)
+        return set_err(h, KST_ERR_CONFIG, "kst_volume_noalloc: the handle's device must be current");
+    double* part = scratch_ptr(h->device);
+    if (!part) return set_err(h, KST_ERR_STATE, "kst_volume_noalloc: no scratch address for this device");
+    // the volume kernels write at most VOL_BLOCKS_MAX partial groups and one
+    // merged group, which is what scratch holds; anything larger is refused
+    static_assert(kScratchDoubles == (size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2, "scratch layout");
+    return volume_into(h, part, out);
 }
 
 int kst_get_output(kst_handle* h, int32_t max_tiles, double* fields, int32_t* tiles, int32_t* n_tiles) {

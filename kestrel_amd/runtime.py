@@ -35,7 +35,8 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_batch_create", "kst_batch_attach", "kst_batch_detach", "kst_batch_start",
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
            "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap",
-           "kst_set_redistribution_chunk", "kst_autotune", "kst_autotune_choices"]
+           "kst_set_redistribution_chunk", "kst_autotune", "kst_autotune_choices",
+           "kst_volume_noalloc"]
 
 
 class KstStats(ctypes.Structure):
@@ -88,6 +89,8 @@ def lib():
         L.kst_autotune_choices.restype = ctypes.c_int
         L.kst_volume.argtypes = [ctypes.c_void_p, _dp]
         L.kst_volume.restype = ctypes.c_int
+        L.kst_volume_noalloc.argtypes = [ctypes.c_void_p, _dp]
+        L.kst_volume_noalloc.restype = ctypes.c_int
         L.kst_get_output.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_int32)]
         L.kst_get_output.restype = ctypes.c_int
@@ -204,6 +207,13 @@ class Stepper:
         mass, bed mass, total solids mass, bed solids mass), on the device."""
         out = np.zeros(6)
         self._check(self.L.kst_volume(self.h, _ptr(out)))
+        return out
+
+    def volume_noalloc(self) -> np.ndarray:
+        """kst_volume_noalloc: the same sums with the library's fixed device
+        scratch (no device allocation in the call)."""
+        out = np.zeros(6)
+        self._check(self.L.kst_volume_noalloc(self.h, _ptr(out)))
         return out
 
     def output_fields(self, max_tiles: int | None = None):
