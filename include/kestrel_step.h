@@ -38,7 +38,20 @@ extern "C" {
 /* ---- enumerations: the runtime-selected closures (src/Closures.f90:67-151,
  *      src/Limiters.f90:68-75, bound from the input deck in
  *      src/Parameters.f90:175-420 and src/SolverSettings.f90:102-126) ---- */
-enum { KST_BC_HALT = 0 };                                   /* "halt" (src/UpdateTiles.f90:61-65) */
+/* Boundary conditions (RunSet%bcs, src/DomainSettings.f90:123-137):
+ * halt      a tile on the domain edge may not become active: fatal
+ *           (src/UpdateTiles.f90:61-65);
+ * periodic  tiles on opposite domain edges are neighbours
+ *           (SetPeriodicBoundaryConds, src/UpdateTiles.f90:521-566), edge
+ *           tiles may be active, ghosts have no corner tiles (:405-409);
+ * dirichlet edge tiles are never activated (:66-68) and ghost tiles on the
+ *           domain edge hold the boundary data bc_hn, bc_u, bc_v, bc_psi
+ *           (SetDomainBoundaryData, :590-643);
+ * sponge    edge tiles are never activated, edge ghosts hold the default
+ *           dry data, and ApplySpongeLayer (src/TimeStepper.f90:879-920)
+ *           damps the explicit RHS of active tiles without a neighbour --
+ *           which no active tile is, edge tiles never being active. */
+enum { KST_BC_HALT = 0, KST_BC_PERIODIC = 1, KST_BC_DIRICHLET = 2, KST_BC_SPONGE = 3 };
 enum { KST_LIMITER_MINMOD1 = 0, KST_LIMITER_MINMOD2 = 1, KST_LIMITER_VANALBADA = 2,
        KST_LIMITER_WENO = 3, KST_LIMITER_NONE = 4 };
 enum { KST_DRAG_CHEZY = 0, KST_DRAG_COULOMB = 1, KST_DRAG_VOELLMY = 2, KST_DRAG_POULIQUEN = 3,
@@ -104,6 +117,9 @@ typedef struct kst_config {
     /* derived (src/Parameters.f90:629-648) */
     double gred, rep, ws0, nsettling, critical_shields, diffusive_time_scale;
     const kst_flux_source* sources;
+    /* Dirichlet boundary data (bcsHnval, bcsuval, bcsvval, bcspsival) and
+     * the sponge layer's strength (SpongeStrength, default 0.2) */
+    double bc_hn, bc_u, bc_v, bc_psi, sponge_strength;
 } kst_config;
 
 /* Statistics of one kst_integrate_to call. */

@@ -12,7 +12,8 @@ import ctypes
 import math
 from typing import Iterable
 
-BC_HALT = 0
+BC_HALT, BC_PERIODIC, BC_DIRICHLET, BC_SPONGE = 0, 1, 2, 3
+BCS = {"halt": BC_HALT, "periodic": BC_PERIODIC, "dirichlet": BC_DIRICHLET, "sponge": BC_SPONGE}
 
 LIMITERS = {"minmod1": 0, "minmod2": 1, "minmod": 1, "vanalbada": 2, "weno": 3, "none": 4}
 DRAGS = {"chezy": 0, "coulomb": 1, "voellmy": 2, "pouliquen": 3, "edwards2019": 4,
@@ -63,7 +64,8 @@ class KstConfig(ctypes.Structure):
                 + [(n, ctypes.c_double) for n in _DBL_GRID]
                 + [(n, ctypes.c_int32) for n in _INT_OPTS]
                 + [(n, ctypes.c_double) for n in _DBL_PARAMS]
-                + [("sources", ctypes.POINTER(KstFluxSource))])
+                + [("sources", ctypes.POINTER(KstFluxSource))]
+                + [(n, ctypes.c_double) for n in ("bc_hn", "bc_u", "bc_v", "bc_psi", "sponge_strength")])
 
     # Keep the flux-source arrays alive as long as the struct.
     _keep: list
@@ -138,6 +140,7 @@ def default_config(nx_tiles: int, ny_tiles: int, nx_per_tile: int = 32, ny_per_t
     c.cfl = 0.25
     c.max_dt = 1.7976931348623157e308
     c.height_threshold = 1e-6
+    c.sponge_strength = 0.2
     c._keep = []
     derive(c)
     return c

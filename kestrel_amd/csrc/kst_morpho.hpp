@@ -142,9 +142,11 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
         int tc, ci, cj;
         map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
         const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
-        // ghost tiles keep the zero derived fields SetDefaultTileData gave them
+        // ghost tiles keep the derived fields they were given: zero
+        // (SetDefaultTileData), or the Dirichlet boundary depth
         double h = 0.0;
         if (d.tstate[tc] == KST_TILE_ACTIVE) h = compute_hn(Win[gi], d.b0c[gi], Tin.btc[gi], Tin.cgam[gi]);
+        else if (p.bc == KST_BC_DIRICHLET && dirichlet_ghost(p, tc)) h = p.bc_hn;
         hn[q] = h < 0.0 ? 0.0 : h;
     }
     __syncthreads();
@@ -319,8 +321,9 @@ __device__ __forceinline__ void note_dirty(const DevT& d, long long e, unsigned 
 
 // The 3x3 update of RedistributeCell (src/Redistribute.f90:402-470) for one
 // cell: topography from its four vertex bt (t) and b0 (e), then w and Hnpsi
-// from intermed0's Hn, Hnpsi and gamma (cbt0, cgo, cw0, ch0, cb0; Hn is zero
-// in ghost tiles, whose derived fields the reference never computes).
+// from intermed0's Hn, Hnpsi and gamma (cbt0, cgo, cw0, ch0, cb0; in ghost
+// tiles, whose derived fields the reference never computes, Hn is hghost:
+// zero, or the Dirichlet boundary depth).
 struct CellRe {
     double btc, dbdx, dbdy, gn, w, hp;
 };
@@ -328,7 +331,8 @@ struct CellRe {
 __device__ __forceinline__ CellRe recompute_cell(const Params& p, double t00, double t10, double t01,
                                                  double t11, double e00, double e10, double e01,
                                                  double e11, double cbt0, double cgo, double cw0,
-                                                 double ch0, double cb0, bool cact) {
+                                                 double ch0, double cb0, bool cact,
+                                                 double hghost = 0.0) {
     Kahan kt;
     kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
     Kahan kx;
@@ -343,7 +347,7 @@ __device__ __forceinline__ CellRe recompute_cell(const Params& p, double t00, do
     r.dbdy = 0.5 * p.dyr * ky.s;
     r.gn = gamma_of(p.geometric, r.dbdx, r.dbdy);
     const double dbb = r.btc - cbt0;
-    double ho = 0.0;
+    double ho = hghost;
     if (cact) {
         ho = compute_hn(cw0, cb0, cbt0, cgo);
         if (ho < 0.0) ho = 0.0;
@@ -363,7 +367,7 @@ template <class DevT>
 __device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k, int di, int dj,
                                            double val, unsigned long long* ghost,
                                            unsigned long long* nd, bool write = true) {
-    const int nx = p.nx, ny = p.ny, NV1 = nx + 1, ntx = p.ntx;
+    const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     double* bt3 = d.TB.vbt;
     auto put = [&](int tile, int ii, int jj) {
@@ -375,17 +379,17 @@ __device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k
         }
     };
     if (write) bt3[(size_t)k * NV + dj * NV1 + di] = val;
-    if (di == 0) put(k - 1, nx, dj);
-    else if (di == nx) put(k + 1, 0, dj);
+    if (di == 0) put(nb_tile(p, k, -1, 0), nx, dj);
+    else if (di == nx) put(nb_tile(p, k, 1, 0), 0, dj);
     if (dj == 0) {
-        put(k - ntx, di, ny);
-        if (di == 0) put(k - ntx - 1, nx, ny);
-        if (di == nx) put(k - ntx + 1, 0, ny);
+        put(nb_tile(p, k, 0, -1), di, ny);
+        if (di == 0) put(nb_tile(p, k, -1, -1), nx, ny);
+        if (di == nx) put(nb_tile(p, k, 1, -1), 0, ny);
     }
     if (dj == ny) {
-        put(k + ntx, di, 0);
-        if (di == 0) put(k + ntx - 1, nx, 0);
-        if (di == nx) put(k + ntx + 1, 0, 0);
+        put(nb_tile(p, k, 0, 1), di, 0);
+        if (di == 0) put(nb_tile(p, k, -1, 1), nx, 0);
+        if (di == nx) put(nb_tile(p, k, 1, 1), 0, 0);
     }
 }
 
@@ -544,7 +548,11 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
     if (tid < n) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int I2 = I + kDI[q], J2 = J + kDJ[q];
+            int I2 = I + kDI[q], J2 = J + kDJ[q];
+            if (p.periodic) {   // the neighbours across the domain edge
+                I2 += I2 < 0 ? NX : (I2 >= NX ? -NX : 0);
+                J2 += J2 < 0 ? NY : (J2 >= NY ? -NY : 0);
+            }
             int r = -1;
             if (I2 >= 0 && J2 >= 0 && I2 < NX && J2 < NY)
                 r = d.cand_pos[((J2 / ny) * p.ntx + I2 / nx) * p.NC + (J2 % ny) * nx + I2 % nx];
@@ -667,7 +675,8 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
         const CellRe r = recompute_cell(p, vt[cj * NV1 + ci], vt[cj * NV1 + ci + 1], vt[(cj + 1) * NV1 + ci],
                                         vt[(cj + 1) * NV1 + ci + 1], vb[cj * NV1 + ci], vb[cj * NV1 + ci + 1],
                                         vb[(cj + 1) * NV1 + ci], vb[(cj + 1) * NV1 + ci + 1], d.TA.btc[gcl],
-                                        d.TA.cgam[gcl], d.SH[gcl], d.SH[3 * FS + gcl], d.b0c[gcl], cact);
+                                        d.TA.cgam[gcl], d.SH[gcl], d.SH[3 * FS + gcl], d.b0c[gcl], cact,
+                                        (!cact && p.bc == KST_BC_DIRICHLET && dirichlet_ghost(p, ct)) ? p.bc_hn : 0.0);
         if (cact) {
             d.TB.btc[gcl] = r.btc; d.TB.cbx[gcl] = r.dbdx; d.TB.cby[gcl] = r.dbdy; d.TB.cgam[gcl] = r.gn;
             d.SM[2][gcl] = r.w;

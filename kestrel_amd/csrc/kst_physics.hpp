@@ -33,6 +33,12 @@ struct Params {
     // PouliquenStaticSlope (pmin + t1) / (1 - pmin t1), t1 = tan(1 degree),
     // of GranularErosion (src/Closures.f90:626-627), computed on the host
     double gran_stat;
+    // boundary conditions (KST_BC_*); with Dirichlet ones the flow state of
+    // the ghost tiles on the domain edge (SetDomainBoundaryData,
+    // src/UpdateTiles.f90:595-643): depth, velocities, solids fraction and
+    // their density
+    int bc, periodic;
+    double bc_hn, bc_u, bc_v, bc_psi, bc_rho;
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
@@ -105,6 +111,34 @@ __device__ __forceinline__ double speed2(int geometric, double u, double v, doub
 
 __device__ __forceinline__ double density(const Params& p, double psi) {
     return p.rhow + (p.rhos - p.rhow) * psi;
+}
+
+// Tile di, dj (each -1..1) tiles from t.  Without periodic boundaries plain
+// arithmetic: the tiles an active tile reads lie inside the grid, since edge
+// tiles are never activated then (AddTile, src/UpdateTiles.f90:56-78).  With
+// them the neighbours wrap around the grid (SetPeriodicBoundaryConds,
+// src/UpdateTiles.f90:521-566; a diagonal tile is the wrapped neighbour's
+// wrapped neighbour, as the reference reaches it through North/South of the
+// East/West tile).
+__device__ __forceinline__ int nb_tile(const Params& p, int t, int di, int dj) {
+    if (!p.periodic) return t + di + dj * p.ntx;
+    int ti = t % p.ntx + di, tj = t / p.ntx + dj;
+    ti += ti < 0 ? p.ntx : (ti >= p.ntx ? -p.ntx : 0);
+    tj += tj < 0 ? p.nty : (tj >= p.nty ? -p.nty : 0);
+    return ti + tj * p.ntx;
+}
+
+// A tile in the first or last column or row of the grid (the domain-edge
+// test of AddTile and UpdateGhostTileBCs, src/UpdateTiles.f90:56-78,571-585).
+__device__ __forceinline__ bool edge_tile(const Params& p, int t) {
+    const int ti = t % p.ntx, tj = t / p.ntx;
+    return ti == 0 || ti == p.ntx - 1 || (p.nty > 1 && (tj == 0 || tj == p.nty - 1));
+}
+
+// A ghost tile holding the Dirichlet boundary data, whose derived fields are
+// the boundary values (no other ghost has any: SetDefaultTileData zeroes them).
+__device__ __forceinline__ bool dirichlet_ghost(const Params& p, int t) {
+    return p.bc == KST_BC_DIRICHLET && edge_tile(p, t);
 }
 
 // fswitch family (src/Closures.f90:797-915).

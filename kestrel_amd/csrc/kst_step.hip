@@ -343,11 +343,13 @@ __device__ __forceinline__ int rg(int rx, int ry) { return (ry + 2) * RW + (rx +
 // both outside) to (tile, ci, cj).
 __device__ __forceinline__ void map_cell(const Params& p, int t, int i, int j, int& tile, int& ci,
                                          int& cj) {
-    tile = t; ci = i; cj = j;
-    if (i < 0) { tile -= 1; ci += p.nx; }
-    else if (i >= p.nx) { tile += 1; ci -= p.nx; }
-    if (j < 0) { tile -= p.ntx; cj += p.ny; }
-    else if (j >= p.ny) { tile += p.ntx; cj -= p.ny; }
+    ci = i; cj = j;
+    int di = 0, dj = 0;
+    if (i < 0) { di = -1; ci += p.nx; }
+    else if (i >= p.nx) { di = 1; ci -= p.nx; }
+    if (j < 0) { dj = -1; cj += p.ny; }
+    else if (j >= p.ny) { dj = 1; cj -= p.ny; }
+    tile = nb_tile(p, t, di, dj);
 }
 
 // The face topography a face's flux reads: db/d(along), gamma, b0 and bt of
@@ -700,12 +702,12 @@ __device__ __forceinline__ double ghost_bed(const DevT& d, const Params& p, int 
 template <bool YDIR, class DevT>
 __device__ __forceinline__ double face_bed(const DevT& d, const Params& p, int t, int K, int r) {
     if (!YDIR) {
-        if (K < 0) return d.T.xib[(t - 1) * p.NFX + r * (p.nx + 1) + (p.nx - 1)];
-        if (K > p.nx) return d.T.xib[(t + 1) * p.NFX + r * (p.nx + 1) + 1];
+        if (K < 0) return d.T.xib[nb_tile(p, t, -1, 0) * p.NFX + r * (p.nx + 1) + (p.nx - 1)];
+        if (K > p.nx) return d.T.xib[nb_tile(p, t, 1, 0) * p.NFX + r * (p.nx + 1) + 1];
         return d.T.xib[t * p.NFX + r * (p.nx + 1) + K];
     } else {
-        if (K < 0) return d.T.yib[(t - p.ntx) * p.NFY + (p.ny - 1) * p.nx + r];
-        if (K > p.ny) return d.T.yib[(t + p.ntx) * p.NFY + 1 * p.nx + r];
+        if (K < 0) return d.T.yib[nb_tile(p, t, 0, -1) * p.NFY + (p.ny - 1) * p.nx + r];
+        if (K > p.ny) return d.T.yib[nb_tile(p, t, 0, 1) * p.NFY + 1 * p.nx + r];
         return d.T.yib[t * p.NFY + K * p.nx + r];
     }
 }
@@ -938,6 +940,8 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             rho = density(p, psi);
             uu = 2.0 * hn * ru / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
             vv = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+        } else if (p.bc == KST_BC_DIRICHLET && dirichlet_ghost(p, tc)) {
+            hn = p.bc_hn; psi = p.bc_psi; rho = p.bc_rho; uu = p.bc_u; vv = p.bc_v;
         }
         const int r = rg(rx, ry);
         s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho;
@@ -999,13 +1003,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         if (r < bya) {
             const int row = (s.by0 + r) * (p.nx + 1);
             if (s.bx0 == 0) {
-                const int tn = t - 1;
+                const int tn = nb_tile(p, t, -1, 0);
                 s.xside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed(d, p, tn, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
                                     : d.T.xib[tn * p.NFX + row + p.nx];
             }
             if (s.bx0 + bxa == p.nx) {
-                const int tn = t + 1;
+                const int tn = nb_tile(p, t, 1, 0);
                 s.xside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed(d, p, tn, 0, s.by0 + r, 0, s.by0 + r + 1)
                                     : d.T.xib[tn * p.NFX + row];
@@ -1014,13 +1018,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         if (r < bxa) {
             const int col = s.bx0 + r;
             if (s.by0 == 0) {
-                const int tn = t - p.ntx;
+                const int tn = nb_tile(p, t, 0, -1);
                 s.yside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed(d, p, tn, col, p.ny, col + 1, p.ny)
                                     : d.T.yib[tn * p.NFY + p.ny * p.nx + col];
             }
             if (s.by0 + bya == p.ny) {
-                const int tn = t + p.ntx;
+                const int tn = nb_tile(p, t, 0, 1);
                 s.yside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed(d, p, tn, col, 0, col + 1, 0)
                                     : d.T.yib[tn * p.NFY + col];
@@ -1293,6 +1297,19 @@ __device__ void tile_cell_topo(const DevT& d, const Params& p, const TopoT& T, i
     }
 }
 
+// The W, E, S, N neighbours of tile t; without periodic boundaries a tile on
+// the domain edge is its own neighbour beyond the edge.
+__device__ __forceinline__ void side_tiles(const Params& p, int t, int& tW, int& tE, int& tS, int& tN) {
+    if (p.periodic) {
+        tW = nb_tile(p, t, -1, 0); tE = nb_tile(p, t, 1, 0);
+        tS = nb_tile(p, t, 0, -1); tN = nb_tile(p, t, 0, 1);
+        return;
+    }
+    const int ti = t % p.ntx, tj = t / p.ntx;
+    tW = ti > 0 ? t - 1 : t; tE = ti < p.ntx - 1 ? t + 1 : t;
+    tS = tj > 0 ? t - p.ntx : t; tN = tj < p.nty - 1 ? t + p.ntx : t;
+}
+
 // part/nparts: the share of the tile's faces one block of several computes.
 // With STATIC the faces' CFL numerators (tile_static_terms) are computed in
 // the same pass; they read the cell gamma of the tiles on both sides.
@@ -1301,10 +1318,10 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
                                int nparts = 1) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
     const int nx = p.nx, ny = p.ny, NV1 = nx + 1;
-    const int ti = t % p.ntx, tj = t / p.ntx;
-    // neighbour tiles (clamped at the domain edge: their values are never read)
-    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
-    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
+    // neighbour tiles (without periodic boundaries clamped at the domain
+    // edge, where their values are never read)
+    int tW, tE, tS, tN;
+    side_tiles(p, t, tW, tE, tS, tN);
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     const double* b = d.vb0 + (size_t)t * NV;
     const double* bt = T.vbt + (size_t)t * NV;
@@ -1424,9 +1441,8 @@ __device__ void tile_static_terms(const DevT& d, const Params& p, const TopoT& T
                                   int nparts = 1) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * nparts;
     const int nx = p.nx, ny = p.ny;
-    const int ti = t % p.ntx, tj = t / p.ntx;
-    const int tW = ti > 0 ? t - 1 : t, tE = ti < p.ntx - 1 ? t + 1 : t;
-    const int tS = tj > 0 ? t - p.ntx : t, tN = tj < p.nty - 1 ? t + p.ntx : t;
+    int tW, tE, tS, tN;
+    side_tiles(p, t, tW, tE, tS, tN);
     tile_prefactors(d, p, T, t, part, nparts);
     for (int f = lane0; f < (nx + 1) * ny; f += stride) {
         const int i = f % (nx + 1), j = f / (nx + 1);
@@ -1460,28 +1476,28 @@ __device__ void ghost_slopes(const DevT& d, const Params& p, int g) {
     const double* w = (d.mixed && d.ctrl->cur) ? d.S0b : d.S0;
     const size_t base = (size_t)g * NC;
     auto on = [&](int k) { return k >= 0 && k < p.ntiles && d.tstate[k] == KST_TILE_ACTIVE; };
-    int nb = g - 1;
+    int nb = nb_tile(p, g, -1, 0);
     if (on(nb))
         for (int j = threadIdx.x; j < ny; j += blockDim.x) {
             const double c = w[base + j * nx], e = w[base + j * nx + 1];
             const double wv = w[(size_t)nb * NC + j * nx + nx - 1];
             d.gsx[base + j * nx] = p.dxr * limiter(p.limiter, e - c, c - wv);
         }
-    nb = g + 1;
+    nb = nb_tile(p, g, 1, 0);
     if (on(nb))
         for (int j = threadIdx.x; j < ny; j += blockDim.x) {
             const double c = w[base + j * nx + nx - 1], we = w[base + j * nx + nx - 2];
             const double ev = w[(size_t)nb * NC + j * nx];
             d.gsx[base + j * nx + nx - 1] = p.dxr * limiter(p.limiter, ev - c, c - we);
         }
-    nb = g - p.ntx;
+    nb = nb_tile(p, g, 0, -1);
     if (on(nb))
         for (int i = threadIdx.x; i < nx; i += blockDim.x) {
             const double c = w[base + i], n = w[base + nx + i];
             const double sv = w[(size_t)nb * NC + (ny - 1) * nx + i];
             d.gsy[base + i] = p.dyr * limiter(p.limiter, n - c, c - sv);
         }
-    nb = g + p.ntx;
+    nb = nb_tile(p, g, 0, 1);
     if (on(nb))
         for (int i = threadIdx.x; i < nx; i += blockDim.x) {
             const double c = w[base + (ny - 1) * nx + i], sv = w[base + (ny - 2) * nx + i];
@@ -1490,22 +1506,47 @@ __device__ void ghost_slopes(const DevT& d, const Params& p, int g) {
         }
 }
 
-// Dry data of a freshly activated or ghost tile (ActivateTile sets w = b0,
-// SetDefaultTileData zeroes the rest; src/UpdateTiles.f90:342-384,647-664).
+// The state of a freshly activated tile (ActivateTile sets w = b0 over
+// AllocateU's or the ghost data's zero momenta; src/UpdateTiles.f90:342-384)
+// or of a new ghost tile (UpdateGhostTileBCs, src/UpdateTiles.f90:571-585):
+// on the domain edge with Dirichlet boundaries the boundary data
+// (SetDomainBoundaryData, :595-643), with periodic ones none (AllocateU's
+// zeros, w = 0 included), else the dry data of SetDefaultTileData (:647-664).
+struct CellData {
+    double w, ru, rv, hp;
+};
+
+template <class DevT>
+__device__ __forceinline__ CellData new_tile_cell(const DevT& d, const Params& p, int k, size_t gi,
+                                                 bool active) {
+    const double b = d.b0c[gi];
+    if (!active && p.bc != KST_BC_HALT && edge_tile(p, k)) {
+        if (p.bc == KST_BC_PERIODIC) return {0.0, 0.0, 0.0, 0.0};
+        if (p.bc == KST_BC_DIRICHLET) {
+            const double hn = p.bc_hn;
+            const double hp = hn / d.T.cgam[gi];
+            double w = d.T.btc[gi] + hp;
+            w = w + b;
+            return {w, p.bc_rho * hn * p.bc_u, p.bc_rho * hn * p.bc_v, hn * p.bc_psi};
+        }
+    }
+    return {b, 0.0, 0.0, 0.0};
+}
+
 template <class DevT>
 __device__ void dry_tile(const DevT& d, const Params& p, int k, bool active) {
     const size_t base = (size_t)k * p.NC;
     const long long FS = d.FS;
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
-        const double b = d.b0c[base + c];
+        const CellData v = new_tile_cell(d, p, k, base + c, active);
         if (d.mixed) {
-            // both FP64 states hold the dry data; increments and RHS are zero
+            // both FP64 states hold the new data; increments and RHS are zero
             double* S[2] = {d.S0, d.S0b};
             for (int q = 0; q < 2; ++q) {
-                S[q][base + c] = b;
-                S[q][FS + base + c] = 0.0;
-                S[q][2 * FS + base + c] = 0.0;
-                S[q][3 * FS + base + c] = 0.0;
+                S[q][base + c] = v.w;
+                S[q][FS + base + c] = v.ru;
+                S[q][2 * FS + base + c] = v.rv;
+                S[q][3 * FS + base + c] = v.hp;
             }
             float* S2f = (float*)d.S2;
             float* S3f = (float*)d.S3;
@@ -1516,15 +1557,15 @@ __device__ void dry_tile(const DevT& d, const Params& p, int k, bool active) {
         }
         double* S[4] = {d.S0, d.S2, d.S3, d.SH};
         for (int q = 0; q < (d.morpho ? 4 : 3); ++q) {
-            S[q][base + c] = b;
-            S[q][FS + base + c] = 0.0;
-            S[q][2 * FS + base + c] = 0.0;
-            S[q][3 * FS + base + c] = 0.0;
+            S[q][base + c] = v.w;
+            S[q][FS + base + c] = v.ru;
+            S[q][2 * FS + base + c] = v.rv;
+            S[q][3 * FS + base + c] = v.hp;
         }
         for (int q = 0; q < 5; ++q) d.R[q * FS + base + c] = 0.0;
         if (d.morpho) {
             for (int q = 0; q < 5; ++q) d.R2[q * FS + base + c] = 0.0;
-            for (int q = 0; q < 3; ++q) { d.SM[q][base + c] = b; d.SM[q][FS + base + c] = 0.0; }
+            for (int q = 0; q < 3; ++q) { d.SM[q][base + c] = v.w; d.SM[q][FS + base + c] = v.hp; }
             d.UV[base + c] = 0.0; d.UV[FS + base + c] = 0.0;
         }
         if (active) {
@@ -1574,6 +1615,34 @@ __device__ void refresh_ghost_overlay(const DevT& d, const Params& p) {
     __syncthreads();
 }
 
+// Neighbour r of tile k in the order of the activation passes: N, S, E, W
+// (NearNorthBoundary .. NearWestBoundary, src/TimeStepper.f90:946-1150).
+__device__ __forceinline__ int side_tile(const Params& p, int k, int r) {
+    const int di = r == 2 ? 1 : (r == 3 ? -1 : 0), dj = r == 0 ? 1 : (r == 1 ? -1 : 0);
+    return nb_tile(p, k, di, dj);
+}
+
+// What AddTile (src/UpdateTiles.f90:56-78) does with neighbour n of an
+// active tile whose edge the flow nears: nothing if n is no tile (id -1 in
+// the reference's 0-based terms) or already active; outside the domain or on
+// its edge (but with periodic boundaries) a halt with 'halt' boundaries and
+// nothing otherwise; else activation.
+enum { ADD_NONE = 0, ADD_TILE = 1, ADD_HALT = 2 };
+__device__ __forceinline__ int add_action(const Params& p, const uint8_t* ts, int n) {
+    if (n <= -1 || (n < p.ntiles && ts[n] == KST_TILE_ACTIVE)) return ADD_NONE;
+    if (n >= p.ntiles || (!p.periodic && edge_tile(p, n)))
+        return p.bc == KST_BC_HALT ? ADD_HALT : ADD_NONE;
+    return ADD_TILE;
+}
+
+// The ghost candidates of a new tile (AddGhostTiles, src/UpdateTiles.f90:389-481):
+// W, E, N, S and, but with periodic boundaries, the four diagonal tiles.
+__device__ __forceinline__ int ghost_count(const Params& p) { return p.periodic ? 4 : 8; }
+__device__ __forceinline__ int ghost_candidate(const Params& p, int k, int r) {
+    constexpr int DI[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, DJ[8] = {0, 0, 1, -1, -1, -1, 1, 1};
+    return nb_tile(p, k, DI[r], DJ[r]);
+}
+
 template <class DevT>
 __device__ void activate_body(const DevT& d, const Params& p) {
     __shared__ int newt[MAXNEW];
@@ -1586,16 +1655,13 @@ __device__ void activate_body(const DevT& d, const Params& p) {
     if (threadIdx.x == 0) need = 0;
     __syncthreads();
     {
-        const int dirs[4] = {p.ntx, -p.ntx, 1, -1};
         int mine = 0;
         for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) {
             const int k = d.active[q];
             const unsigned f = d.eflags[k];
             if (!f) continue;
-            for (int r = 0; r < 4; ++r) {
-                const int n = k + dirs[r];
-                if ((f & (1u << r)) && n > -1 && !(n < p.ntiles && d.tstate[n] == KST_TILE_ACTIVE)) mine = 1;
-            }
+            for (int r = 0; r < 4; ++r)
+                if ((f & (1u << r)) && add_action(p, d.tstate, side_tile(p, k, r)) != ADD_NONE) mine = 1;
         }
         if (mine) atomicOr(&need, 1);
     }
@@ -1607,19 +1673,15 @@ __device__ void activate_body(const DevT& d, const Params& p) {
     if (threadIdx.x == 0) {
         nnew = 0;
         bad = 0;
-        const int dirs[4] = {p.ntx, -p.ntx, 1, -1};  // N, S, E, W passes in order
-        for (int q = 0; q < 4 && !bad; ++q) {
+        for (int q = 0; q < 4 && !bad; ++q) {     // N, S, E, W passes in order
             const int size = c->n_active;
             for (int tt = 0; tt < size && !bad; ++tt) {
                 const int k = d.active[tt];
-                const int n = k + dirs[q];
-                if (n > -1 && !(n < p.ntiles && d.tstate[n] == KST_TILE_ACTIVE) &&
-                    (d.eflags[k] & (1u << q))) {
-                    // AddTile (src/UpdateTiles.f90:56-78): halt at the domain edge
-                    const int gi = n % p.ntx, gj = n / p.ntx;
-                    const bool edge = n >= p.ntiles || gi == 0 || gi == p.ntx - 1 ||
-                                      (p.nty > 1 && (gj == 0 || gj == p.nty - 1));
-                    if (edge) { bad = 1; break; }
+                const int n = side_tile(p, k, q);
+                if (!(d.eflags[k] & (1u << q))) continue;
+                const int act = add_action(p, d.tstate, n);
+                if (act == ADD_HALT) { bad = 1; break; }
+                if (act == ADD_TILE) {
                     // sorted insertion; later indices of this pass shift as in the reference
                     int pos = c->n_active;
                     while (pos > 0 && d.active[pos - 1] > n) { d.active[pos] = d.active[pos - 1]; --pos; }
@@ -1668,10 +1730,8 @@ __device__ void activate_body(const DevT& d, const Params& p) {
         ng_start = c->n_ghost;
         for (int q = 0; q < nnew; ++q) {
             const int k = newt[q];
-            const int nb[8] = {k - 1, k + 1, k + p.ntx, k - p.ntx,
-                               k - p.ntx - 1, k - p.ntx + 1, k + p.ntx - 1, k + p.ntx + 1};
-            for (int r = 0; r < 8; ++r) {
-                const int g = nb[r];
+            for (int r = 0; r < ghost_count(p); ++r) {
+                const int g = ghost_candidate(p, k, r);
                 if (g >= 0 && g < p.ntiles && d.tstate[g] == KST_TILE_OFF) {
                     d.tstate[g] = KST_TILE_GHOST;
                     d.ghosts[c->n_ghost++] = g;
@@ -1686,11 +1746,9 @@ __device__ void activate_body(const DevT& d, const Params& p) {
     for (int q = 0; q < c->n_ghost; ++q) {
         const int g = d.ghosts[q];
         bool touched = q >= ng_start;
-        for (int r = 0; r < nnew && !touched; ++r) {
-            const int k = newt[r];
-            const int di = (g % p.ntx) - (k % p.ntx), dj = (g / p.ntx) - (k / p.ntx);
-            if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1) touched = true;
-        }
+        for (int r = 0; r < nnew && !touched; ++r)
+            for (int u = 0; u < ghost_count(p) && !touched; ++u)
+                if (ghost_candidate(p, newt[r], u) == g) touched = true;
         if (touched) ghost_slopes(d, p, g);
     }
 }
@@ -2102,6 +2160,10 @@ void fill_params(kst_handle* h) {
     const double pi = 4.0 * std::atan(1.0);
     const double t1 = std::tan(pi / 180.0);
     p.gran_stat = (p.pmin + t1) / (1.0 - p.pmin * t1);
+    p.bc = c.bc;
+    p.periodic = c.bc == KST_BC_PERIODIC;
+    p.bc_hn = c.bc_hn; p.bc_u = c.bc_u; p.bc_v = c.bc_v; p.bc_psi = c.bc_psi;
+    p.bc_rho = c.rhow + (c.rhos - c.rhow) * c.bc_psi;   // Density (src/Closures.f90)
 }
 
 int read_ctrl(kst_handle* h, Ctrl* c) {
@@ -2449,6 +2511,7 @@ void kst_config_defaults(kst_config* c) {
     c->ero_critical_height = 0.01; c->voellmy_switch_rate = 3.0; c->voellmy_switch_value = 0.2;
     c->max_pack = 0.65; c->bed_porosity = 0.35; c->solid_diameter = 1e-3; c->eddy_viscosity = 0.0;
     c->cfl = 0.25; c->max_dt = DBL_MAX; c->height_threshold = 1e-6;
+    c->sponge_strength = 0.2;
     kst_config_derive(c);
 }
 
@@ -2468,9 +2531,12 @@ void kst_config_derive(kst_config* c) {
 int kst_create(const kst_config* cfg, kst_handle** out) {
     if (!cfg || !out) return KST_ERR_CONFIG;
     *out = nullptr;
-    if (cfg->nx_per_tile < 4 || cfg->ny_per_tile < 4 || cfg->nx_tiles < 3 || cfg->ny_tiles < 3)
+    if (cfg->bc < KST_BC_HALT || cfg->bc > KST_BC_SPONGE) return KST_ERR_CONFIG;
+    // without periodic boundaries only interior tiles are ever active
+    const int min_tiles = cfg->bc == KST_BC_PERIODIC ? 1 : 3;
+    if (cfg->nx_per_tile < 4 || cfg->ny_per_tile < 4 || cfg->nx_tiles < min_tiles ||
+        cfg->ny_tiles < min_tiles)
         return KST_ERR_CONFIG;
-    if (cfg->bc != KST_BC_HALT) return KST_ERR_CONFIG;
     if (cfg->precision != KST_PREC_FP64 && cfg->precision != KST_PREC_MIXED) return KST_ERR_CONFIG;
     // mixed precision covers the hydraulic operator only
     if (cfg->precision == KST_PREC_MIXED && cfg->morphodynamics) return KST_ERR_CONFIG;
@@ -2645,22 +2711,41 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     if (!h || !w || !rhu || !rhv || !hnpsi || !ts) return KST_ERR_CONFIG;
     if (!h->have_topo) return set_err(h, KST_ERR_STATE, "kst_set_topography must come first");
     const size_t FS = (size_t)h->ntiles * h->NC;
-    // tile lists: active sorted by id; ghosts = non-active 8-neighbours of active tiles
+    // tile lists: active sorted by id; ghosts = the non-active neighbours of
+    // active tiles that AddGhostTiles takes (src/UpdateTiles.f90:389-481):
+    // W, E, N, S and, but with periodic boundaries, the diagonal tiles
+    const bool periodic = h->cfg.bc == KST_BC_PERIODIC;
+    auto edge = [&](int k) {
+        const int gi = k % h->ntx, gj = k / h->ntx;
+        return gi == 0 || gi == h->ntx - 1 || (h->nty > 1 && (gj == 0 || gj == h->nty - 1));
+    };
+    auto nb = [&](int k, int di, int dj) {
+        int gi = k % h->ntx + di, gj = k / h->ntx + dj;
+        if (periodic) {
+            gi = (gi + h->ntx) % h->ntx;
+            gj = (gj + h->nty) % h->nty;
+        }
+        return gi + gj * h->ntx;
+    };
     std::vector<uint8_t> st(h->ntiles, KST_TILE_OFF);
     std::vector<int> act, gh;
     for (int k = 0; k < h->ntiles; ++k)
         if (ts[k] == KST_TILE_ACTIVE) {
-            const int gi = k % h->ntx, gj = k / h->ntx;
-            if (gi == 0 || gi == h->ntx - 1 || gj == 0 || gj == h->nty - 1)
-                return set_err(h, KST_ERR_HALT, "Error: tried to add a tile outside the domain.");
+            // AddTile (src/UpdateTiles.f90:56-78): an edge tile halts the run
+            // with 'halt' boundaries and is left off with Dirichlet or sponge ones
+            if (!periodic && edge(k)) {
+                if (h->cfg.bc == KST_BC_HALT)
+                    return set_err(h, KST_ERR_HALT, "Error: tried to add a tile outside the domain.");
+                continue;
+            }
             st[k] = KST_TILE_ACTIVE;
             act.push_back(k);
         }
-    for (int k : act) {
-        const int gi = k % h->ntx, gj = k / h->ntx;
-        for (int dj = -1; dj <= 1; ++dj)
-            for (int di = -1; di <= 1; ++di) {
-                const int n = (gi + di) + (gj + dj) * h->ntx;
+    {
+        const int DI[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, DJ[8] = {0, 0, 1, -1, -1, -1, 1, 1};
+        for (int k : act)
+            for (int r = 0; r < (periodic ? 4 : 8); ++r) {
+                const int n = nb(k, DI[r], DJ[r]);
                 if (st[n] == KST_TILE_OFF) { st[n] = KST_TILE_GHOST; gh.push_back(n); }
             }
     }
@@ -2686,20 +2771,45 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
         if (rc) return rc;
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
-    // fields: active tiles take the given state, ghosts take dry data
-    std::vector<double> b0c(FS);
+    // fields: active tiles take the given state, ghosts the data of
+    // UpdateGhostTileBCs (src/UpdateTiles.f90:571-585): on the domain edge
+    // the Dirichlet boundary data, or none with periodic boundaries
+    // (AllocateU's zeros), else SetDefaultTileData's dry data
+    std::vector<double> b0c(FS), btc, cgam;
     HIPCHK(h, hipMemcpy(b0c.data(), h->d.b0c, FS * 8, hipMemcpyDeviceToHost));
+    if (h->cfg.bc == KST_BC_DIRICHLET) {
+        btc.resize(FS);
+        cgam.resize(FS);
+        HIPCHK(h, hipMemcpy(btc.data(), h->d.T.btc, FS * 8, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(cgam.data(), h->d.T.cgam, FS * 8, hipMemcpyDeviceToHost));
+    }
     std::vector<double> S(4 * FS, 0.0);
     to_tiles(h, w, S.data());
     to_tiles(h, rhu, S.data() + FS);
     to_tiles(h, rhv, S.data() + 2 * FS);
     to_tiles(h, hnpsi, S.data() + 3 * FS);
+    const kst_config& cf = h->cfg;
+    const double bc_rho = cf.rhow + (cf.rhos - cf.rhow) * cf.bc_psi;
     for (int k = 0; k < h->ntiles; ++k) {
         if (st[k] == KST_TILE_ACTIVE) continue;
+        const bool on_edge = st[k] == KST_TILE_GHOST && edge(k);
         for (int c = 0; c < h->NC; ++c) {
             const size_t gi = (size_t)k * h->NC + c;
-            S[gi] = b0c[gi];
-            S[FS + gi] = 0.0; S[2 * FS + gi] = 0.0; S[3 * FS + gi] = 0.0;
+            if (on_edge && cf.bc == KST_BC_PERIODIC) {
+                S[gi] = 0.0;
+                S[FS + gi] = 0.0; S[2 * FS + gi] = 0.0; S[3 * FS + gi] = 0.0;
+            } else if (on_edge && cf.bc == KST_BC_DIRICHLET) {
+                const double hn = cf.bc_hn;
+                double wv = btc[gi] + hn / cgam[gi];
+                wv = wv + b0c[gi];
+                S[gi] = wv;
+                S[FS + gi] = bc_rho * hn * cf.bc_u;
+                S[2 * FS + gi] = bc_rho * hn * cf.bc_v;
+                S[3 * FS + gi] = hn * cf.bc_psi;
+            } else {
+                S[gi] = b0c[gi];
+                S[FS + gi] = 0.0; S[2 * FS + gi] = 0.0; S[3 * FS + gi] = 0.0;
+            }
         }
     }
     HIPCHK(h, hipMemcpy(h->d.S0, S.data(), S.size() * 8, hipMemcpyHostToDevice));
@@ -2848,7 +2958,16 @@ int kst_get_state(kst_handle* h, double* w, double* rhu, double* rhv, double* hn
         HIPCHK(h, hipMemcpy(bt.data(), h->d.TA.btc, FS * 8, hipMemcpyDeviceToHost));
         HIPCHK(h, hipMemcpy(g.data(), h->d.TA.cgam, FS * 8, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < FS; ++i) {
-            if (st[i / h->NC] == KST_TILE_OFF) continue;
+            const int k = (int)(i / h->NC);
+            if (st[k] == KST_TILE_OFF) continue;
+            if (st[k] == KST_TILE_GHOST) {
+                // a ghost's Hn is the one it was given: zero, or the
+                // Dirichlet boundary depth on the domain edge
+                const int gi = k % h->ntx, gj = k / h->ntx;
+                const bool edge = gi == 0 || gi == h->ntx - 1 || (h->nty > 1 && (gj == 0 || gj == h->nty - 1));
+                H[i] = (h->cfg.bc == KST_BC_DIRICHLET && edge) ? h->cfg.bc_hn : 0.0;
+                continue;
+            }
             double x = -bt[i];
             x = x + (S[i] - b0c[i]);
             x = x * g[i];
@@ -2881,12 +3000,17 @@ int kst_get_state(kst_handle* h, double* w, double* rhu, double* rhv, double* hn
 
 namespace {
 
-// CalculateVolume with the partial sums in `part` (VOL_BLOCKS_MAX + 1 groups).
-int volume_into(kst_handle* h, double* part, double out[6]) {
+// CalculateVolume with the partial sums in `part`, which holds `cap` doubles
+// (VOL_BLOCKS_MAX + 1 groups of VOL_Q pairs suffice for any active set).
+int volume_into(kst_handle* h, double* part, size_t cap, double out[6]) {
     Ctrl c;
     int rc = read_ctrl(h, &c);
     if (rc) return rc;
     const int nb = std::max(1, std::min(c.n_active, VOL_BLOCKS_MAX));
+    if ((size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2 > cap)
+        return set_err(h, KST_ERR_CONFIG, "volume: the partial-sum buffer holds " + std::to_string(cap) +
+                                              " doubles; " + std::to_string((VOL_BLOCKS_MAX + 1) * VOL_Q * 2) +
+                                              " are needed");
     double* fin = part + (size_t)VOL_BLOCKS_MAX * VOL_Q * 2;
     hipLaunchKernelGGL(volume_kernel, dim3(nb), dim3(VOL_NT), 0, h->stream, h->d_args, h->d_par, part);
     hipLaunchKernelGGL(volume_final_kernel, dim3(1), dim3(VOL_NT), 0, h->stream, part, nb, fin);
@@ -2915,7 +3039,7 @@ int volume_into(kst_handle* h, double* part, double out[6]) {
 int kst_volume(kst_handle* h, double out[6]) {
     if (!h || !out) return KST_ERR_CONFIG;
     if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
-    return volume_into(h, h->vol_part, out);
+    return volume_into(h, h->vol_part, (size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2, out);
 }
 
 int kst_volume_noalloc(kst_handle* h, double out[6]) {
@@ -2928,9 +3052,9 @@ int kst_volume_noalloc(kst_handle* h, double out[6]) {
     double* part = scratch_ptr(h->device);
     if (!part) return set_err(h, KST_ERR_STATE, "kst_volume_noalloc: no scratch address for this device");
     // the volume kernels write at most VOL_BLOCKS_MAX partial groups and one
-    // merged group, which is what scratch holds; anything larger is refused
-    static_assert(kScratchDoubles == (size_t)(VOL_BLOCKS_MAX + 1) * VOL_Q * 2, "scratch layout");
-    return volume_into(h, part, out);
+    // merged group, which is what scratch holds; volume_into refuses a
+    // buffer smaller than that
+    return volume_into(h, part, kScratchDoubles, out);
 }
 
 int kst_get_output(kst_handle* h, int32_t max_tiles, double* fields, int32_t* tiles, int32_t* n_tiles) {

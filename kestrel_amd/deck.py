@@ -12,8 +12,7 @@ the Function topographies this path supports (src/TopogSettings.f90:213-342,
 src/TopogFuncs.f90:395-577); Cap_Set (src/InitConds.f90:254-511) and Source_Set
 (46-208).  Labels of the Domain, Parameters, Solver, Output and Topog blocks
 compare in lower case; Source and Cap keys compare as written, as there.
-Settings the device path does not run (periodic, dirichlet or sponge
-boundaries, Cube initial conditions, DEM topographies, restarts) raise
+Settings the device path does not run (DEM topographies, restarts) raise
 DeckUnsupported naming the setting.
 """
 from __future__ import annotations
@@ -147,12 +146,14 @@ def _domain(d: Deck):
     else:
         yt = _real(dom["ytilesize"])
         xt = yt * float(nx) / float(ny)
-    bcs = dom.get("boundary conditions", "halt").lower()
-    if bcs != "halt":
-        raise DeckUnsupported(f"Boundary Conditions = {bcs}: the device path runs 'halt' only")
+    bcs = dom.get("boundary conditions", "halt").strip().lower()
+    if bcs == "reflect":
+        raise DeckError("This boundary condition 'reflect' is not supported yet")
+    if bcs not in C.BCS:
+        raise DeckError(f"'Boundary Conditions' value {bcs} is not recognized")
     if nty * ny == 1:
         raise DeckUnsupported("one-dimensional domains (nYtiles * nYpertile = 1)")
-    return ntx, nty, nx, ny, xt, yt
+    return ntx, nty, nx, ny, xt, yt, bcs
 
 
 _SWITCH = {"tanh": "tanh", "rat3": "rat3", "cos": "cos", "linear": "linear", "equal": "equal",
@@ -247,7 +248,9 @@ def _solver(d: Deck, c: C.KstConfig):
         elif k == "restart":
             if lv == "on":
                 raise DeckUnsupported("Restart = on")
-        elif k in ("sponge strength", "initial condition"):
+        elif k == "sponge strength":
+            c.sponge_strength = _real(v)
+        elif k == "initial condition":
             pass
         else:
             d.warnings.append(f"Solver: label '{k}' not recognized")
@@ -402,6 +405,16 @@ def _topography(d: Deck, c: C.KstConfig) -> np.ndarray:
     return np.ascontiguousarray(b, dtype=np.float64)
 
 
+def _inside(c: C.KstConfig, what: str, k: int, x: float, y: float, hx: float, hy: float):
+    """The domain checks of Source_Set, Cap_Set and Cube_Set
+    (src/InitConds.f90:192-195, 410-413, 643-646): the object's extent
+    [x - hx, x + hx] x [y - hy, y + hy] lies in the domain (touching its edge
+    is allowed)."""
+    if (x - hx < -0.5 * c.x_size or x + hx > 0.5 * c.x_size or
+            y - hy < -0.5 * c.y_size or y + hy > 0.5 * c.y_size):
+        raise DeckError(f"In input file, {what} {k + 1} outside of the domain")
+
+
 def _caps(d: Deck, c: C.KstConfig) -> list:
     """Cap_Set (src/InitConds.f90:254-425)."""
     out = []
@@ -421,13 +434,14 @@ def _caps(d: Deck, c: C.KstConfig) -> list:
             r = math.sqrt(vol / math.pi / h) if shape == "flat" else math.sqrt(2.0 * vol / math.pi / h)
         if r is None or h is None:
             raise DeckError(f"Cap {k + 1}: two of capRadius, capHeight, capVolume are needed")
+        _inside(c, "cap", k, _real(cp["capX"]), _real(cp["capY"]), r, r)
         out.append(S.Cap(_real(cp["capX"]), _real(cp["capY"]), r, h,
                          psi=_real(cp.get("capConc", "0")), u=_real(cp.get("capU", "0")),
                          v=_real(cp.get("capV", "0")), shape=shape))
     return out
 
 
-def _sources(d: Deck) -> list:
+def _sources(d: Deck, c: C.KstConfig) -> list:
     """Source_Set (src/InitConds.f90:46-208)."""
     out = []
     for k, s in enumerate(d.sources):
@@ -440,8 +454,40 @@ def _sources(d: Deck) -> list:
         t = _set(s["sourceTime"]) if "sourceTime" in s else [0.0]
         q = _set(s["sourceFlux"]) if "sourceFlux" in s else [0.0] * len(t)
         p = _set(s["sourceConc"]) if "sourceConc" in s else [0.0] * len(t)
+        rad = _real(s["sourceRadius"])
+        _inside(c, "Source", k, _real(s["sourceX"]), _real(s["sourceY"]), rad, rad)
         out.append(C.FluxSource(_real(s["sourceX"]), _real(s["sourceY"]), _real(s["sourceRadius"]),
                                 t, q, p))
+    return out
+
+
+def _bc_data(d: Deck, c: C.KstConfig, bcs: str):
+    """Boundary conditions and the Dirichlet data (src/DomainSettings.f90:123-215):
+    the values default to zero for 'dirichlet'; the reference keeps them (and
+    warns) for the other kinds, where nothing reads them."""
+    c.bc = C.BCS[bcs]
+    for key, name in (("boundary hn", "bc_hn"), ("boundary u", "bc_u"), ("boundary v", "bc_v"),
+                      ("boundary psi", "bc_psi")):
+        setattr(c, name, _real(d.domain[key]) if key in d.domain else 0.0)
+
+
+def _cubes(d: Deck, c: C.KstConfig) -> list:
+    """Cube_Set (src/InitConds.f90:523-660): cuboid initial conditions."""
+    out = []
+    for k, cb in enumerate(d.cubes):
+        if "cubeLat" in cb or "cubeLon" in cb:
+            raise DeckUnsupported("cubes placed by latitude/longitude")
+        for key in ("cubeX", "cubeY", "cubeHeight", "cubeLength", "cubeWidth"):
+            if key not in cb:
+                raise DeckError(f"Cube {k + 1}: {key} is not given")
+        shape = cb.get("cubeShape", "flat").strip().lower()
+        if shape not in ("flat", "level"):
+            raise DeckError(f"Cube {k + 1}: cubeShape {shape} is not recognized")
+        _inside(c, "cube", k, _real(cb["cubeX"]), _real(cb["cubeY"]), 0.5 * _real(cb["cubeLength"]),
+                0.5 * _real(cb["cubeWidth"]))
+        out.append(S.Cube(_real(cb["cubeX"]), _real(cb["cubeY"]), _real(cb["cubeLength"]),
+                          _real(cb["cubeWidth"]), _real(cb["cubeHeight"]),
+                          psi=_real(cb.get("cubeConc", "0")), shape=shape))
     return out
 
 
@@ -450,27 +496,33 @@ def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64,
     """The deck's configuration, topography, initial state and sources.
     b0v, if given, replaces the deck's topography by vertex elevations
     ((NX+1) x (NY+1)) sampled elsewhere, as a DEM would."""
-    if d.cubes:
-        raise DeckUnsupported("Cube initial conditions")
-    ntx, nty, nx, ny, xt, yt = _domain(d)
+    ntx, nty, nx, ny, xt, yt, bcs = _domain(d)
     c = C.default_config(ntx, nty, nx, ny, dx=xt / float(nx), dy=yt / float(ny))
     c.x_size = ntx * xt
     c.y_size = nty * yt
+    _bc_data(d, c, bcs)
     settling = _params(d, c)
     tend, tstart = _solver(d, c)
     c.precision = precision
     C.derive(c, settling_speed=settling)
     if b0v is None:
         b0v = _topography(d, c)
+        if c.bc == C.BC_PERIODIC:
+            # EqualiseTopographicBoundaryData (src/MorphodynamicRHS.f90:588-687)
+            # across the periodic edges: a tile's east and north edge vertices
+            # take its (wrapped) east and north neighbours' west and south ones
+            b0v[:, -1] = b0v[:, 0]
+            b0v[-1, :] = b0v[0, :]
     else:
         b0v = np.ascontiguousarray(b0v, dtype=np.float64)
         if b0v.shape != (c.NY + 1, c.NX + 1):
             raise DeckError(f"topography of shape {b0v.shape}; the grid has {(c.NY + 1, c.NX + 1)} vertices")
     caps = _caps(d, c)
-    srcs = _sources(d)
-    if not caps and not srcs:
-        raise DeckError("An 'Initial Conditions' block (Cap or Source) is needed")
-    init = S.initial_state(c, b0v, caps, srcs)
+    cubes = _cubes(d, c)
+    srcs = _sources(d, c)
+    if not caps and not cubes and not srcs:
+        raise DeckError("An 'Initial Conditions' block (Cap, Cube or Source) is needed")
+    init = S.initial_state(c, b0v, caps, srcs, cubes=cubes)
     if srcs:
         C.set_sources(c, srcs)
     nout = int(d.output.get("n out", "1"))

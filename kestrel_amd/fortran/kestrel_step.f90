@@ -40,6 +40,7 @@ module kestrel_step_binding
       real(c_double) :: cfl, max_dt, height_threshold
       real(c_double) :: gred, rep, ws0, nsettling, critical_shields, diffusive_time_scale
       type(c_ptr) :: sources
+      real(c_double) :: bc_hn, bc_u, bc_v, bc_psi, sponge_strength
    end type kst_config
 
    type, bind(C) :: kst_stats

@@ -16,7 +16,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .config import KstConfig, FluxSource, TILE_ACTIVE, TILE_GHOST, TILE_OFF
+from .config import (BC_DIRICHLET, BC_HALT, BC_PERIODIC, BC_SPONGE, FluxSource, KstConfig,
+                     TILE_ACTIVE, TILE_GHOST, TILE_OFF)
 
 
 def cell_centres(cfg: KstConfig) -> tuple[np.ndarray, np.ndarray]:
@@ -139,6 +140,21 @@ class Cap:
 
 
 @dataclass
+class Cube:
+    """A cuboid initial condition (Cube_Set, src/InitConds.f90:523-660): cells
+    whose centres lie within length/2 and width/2 of (x, y) get a flat layer
+    of depth `height` or, with shape 'level', a free surface at `height`; its
+    velocities are not painted (src/SetSources.f90:305-355)."""
+    x: float
+    y: float
+    length: float
+    width: float
+    height: float
+    psi: float = 0.0
+    shape: str = "flat"    # 'flat' | 'level'
+
+
+@dataclass
 class InitialState:
     w: np.ndarray
     rhoHnu: np.ndarray
@@ -150,17 +166,24 @@ class InitialState:
 
 def _ghost_ring(cfg: KstConfig, active: np.ndarray) -> np.ndarray:
     """Tile states: active tiles plus their 8 neighbours as ghosts
-    (AddGhostTiles, src/UpdateTiles.f90:389-481)."""
+    (AddGhostTiles, src/UpdateTiles.f90:389-481); with periodic boundaries the
+    neighbours wrap around the domain and there are no corner ghosts."""
     a = active.reshape(cfg.ny_tiles, cfg.nx_tiles)
     g = np.zeros_like(a)
+    periodic = cfg.bc == BC_PERIODIC
     for dj in (-1, 0, 1):
         for di in (-1, 0, 1):
-            sh = np.zeros_like(a)
-            ys = slice(max(dj, 0), cfg.ny_tiles + min(dj, 0))
-            yd = slice(max(-dj, 0), cfg.ny_tiles + min(-dj, 0))
-            xs = slice(max(di, 0), cfg.nx_tiles + min(di, 0))
-            xd = slice(max(-di, 0), cfg.nx_tiles + min(-di, 0))
-            sh[yd, xd] = a[ys, xs]
+            if periodic and di != 0 and dj != 0:
+                continue
+            if periodic:
+                sh = np.roll(np.roll(a, dj, axis=0), di, axis=1)
+            else:
+                sh = np.zeros_like(a)
+                ys = slice(max(dj, 0), cfg.ny_tiles + min(dj, 0))
+                yd = slice(max(-dj, 0), cfg.ny_tiles + min(-dj, 0))
+                xs = slice(max(di, 0), cfg.nx_tiles + min(di, 0))
+                xd = slice(max(-di, 0), cfg.nx_tiles + min(-di, 0))
+                sh[yd, xd] = a[ys, xs]
             g |= sh
     st = np.full(a.shape, TILE_OFF, dtype=np.uint8)
     st[g.astype(bool)] = TILE_GHOST
@@ -173,19 +196,28 @@ def _tile_of_cells(cfg: KstConfig) -> np.ndarray:
     return (J // cfg.ny_per_tile) * cfg.nx_tiles + (I // cfg.nx_per_tile)
 
 
+def _edge_tiles(cfg: KstConfig) -> np.ndarray:
+    """OnDomainEdge (src/Grid.f90:322-335) per tile."""
+    edge = np.zeros((cfg.ny_tiles, cfg.nx_tiles), dtype=bool)
+    edge[:, 0] = edge[:, -1] = True
+    if cfg.ny_tiles > 1:
+        edge[0, :] = edge[-1, :] = True
+    return edge.reshape(-1)
+
+
 def _check_halt(cfg: KstConfig, active: np.ndarray):
-    a = active.reshape(cfg.ny_tiles, cfg.nx_tiles)
-    edge = np.zeros_like(a, dtype=bool)
-    edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
-    if np.any(a.astype(bool) & edge):
+    if cfg.bc == BC_HALT and np.any(active.astype(bool) & _edge_tiles(cfg)):
         raise ValueError("Error: tried to add a tile outside the domain "
                          "(halt boundary; src/UpdateTiles.f90:61-65)")
 
 
 def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
-                  sources: list[FluxSource] = ()) -> InitialState:
-    """Paint caps and count flux-source cells as LoadSourceConditions does
-    (src/SetSources.f90:228-303 caps, 357-375 flux sources)."""
+                  sources: list[FluxSource] = (), cubes: list[Cube] = ()) -> InitialState:
+    """Paint caps and cubes and count flux-source cells as LoadSourceConditions
+    does (src/SetSources.f90:228-303 caps, 305-355 cubes, 357-375 flux
+    sources).  Only a tile that AddTile activates is painted: with dirichlet
+    or sponge boundaries a tile on the domain edge is silently left inactive
+    (src/UpdateTiles.f90:61-69)."""
     NX, NY = cfg.NX, cfg.NY
     x, y = cell_centres(cfg)
     X, Y = np.meshgrid(x, y)
@@ -198,10 +230,16 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
     rhv = np.zeros((NY, NX))
     hnpsi = np.zeros((NY, NX))
     hn = np.zeros((NY, NX))
+    # tiles AddTile may activate (the others stay off and are not painted)
+    allowed = np.ones(cfg.ntiles, dtype=bool)
+    if cfg.bc in (BC_DIRICHLET, BC_SPONGE):
+        allowed &= ~_edge_tiles(cfg)
+    cell_ok = allowed[tiles]
     for cap in caps:
         R2 = (X - cap.x) * (X - cap.x) + (Y - cap.y) * (Y - cap.y)
-        inside = R2 <= cap.radius * cap.radius
-        active[np.unique(tiles[inside])] = 1
+        inside = (R2 <= cap.radius * cap.radius)
+        active[np.unique(tiles[inside & cell_ok])] = 1
+        inside = inside & cell_ok
         rho = cfg.rhow + (cfg.rhos - cfg.rhow) * cap.psi
         if cap.shape == "flat":
             inc_w = cap.height / gam
@@ -233,11 +271,29 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
         rhu = np.where(inside, rhu + inc_rhu, rhu)
         rhv = np.where(inside, rhv + inc_rhv, rhv)
         hnpsi = np.where(inside, hnpsi + inc_psi, hnpsi)
+    for cube in cubes:
+        inside = ((np.abs(X - cube.x) <= 0.5 * cube.length) & (np.abs(Y - cube.y) <= 0.5 * cube.width))
+        active[np.unique(tiles[inside & cell_ok])] = 1
+        inside = inside & cell_ok
+        if cube.shape == "level":
+            hp = cube.height - b0c
+            Hn = hp * gam
+            inside = inside & (Hn > 0.0)
+            inc_w, inc_h, inc_psi = hp, Hn, Hn * cube.psi
+        else:
+            inc_w = cube.height / gam
+            inc_h = np.full_like(gam, cube.height)
+            inc_psi = np.full_like(gam, cube.psi * cube.height)
+        w = np.where(inside, w + inc_w, w)
+        hn = np.where(inside, hn + inc_h, hn)
+        rhu = np.where(inside, rhu + 0.0, rhu)
+        rhv = np.where(inside, rhv + 0.0, rhv)
+        hnpsi = np.where(inside, hnpsi + inc_psi, hnpsi)
     for src in sources:
         R2 = (X - src.x) * (X - src.x) + (Y - src.y) * (Y - src.y)
         inside = R2 <= src.radius * src.radius
         src.num_cells = int(np.count_nonzero(inside))
-        active[np.unique(tiles[inside])] = 1
+        active[np.unique(tiles[inside & cell_ok])] = 1
     _check_halt(cfg, active)
     state = _ghost_ring(cfg, active)
     return InitialState(np.ascontiguousarray(w), rhu, rhv, hnpsi, state, hn)

@@ -34,6 +34,24 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef KO_PERTURB_LIBM
+/* Conditioning probe (liboracle_pert.so): the transcendental functions the
+ * closures evaluate per cell return their result moved one ulp up.  A run
+ * compared with the plain oracle's measures how far last-bit differences
+ * between two libm implementations (the host's and the device's) carry each
+ * field; where that spread exceeds the parity bar, the device is held to it.
+ * The constants pi = 4 atan(1) and tan(pi/180) are left exact. */
+static double ko_up(double x) { return nextafter(x, INFINITY); }
+static double ko_tanh(double x) { return ko_up(tanh(x)); }
+static double ko_cos(double x) { return ko_up(cos(x)); }
+static double ko_pow(double x, double y) { return ko_up(pow(x, y)); }
+static double ko_log(double x) { return ko_up(log(x)); }
+#define tanh ko_tanh
+#define cos ko_cos
+#define pow ko_pow
+#define log ko_log
+#endif
+
 enum { W = 0, RHU = 1, RHV = 2, HNPSI = 3, HN = 4, UU = 5, VV = 6, PSI = 7, RHO = 8,
        B0 = 9, BT = 10, DBDX = 11, DBDY = 12, NF = 13 };
 
@@ -1220,12 +1238,36 @@ static int in_list(const int* l, int n, int k) {
     return 0;
 }
 
-/* SetTileBoundaries (src/UpdateTiles.f90:484-512), non-periodic. */
+/* SetTileBoundaries (src/UpdateTiles.f90:484-512) and, with periodic
+ * boundaries, SetPeriodicBoundaryConds (:521-566): a tile in the first or
+ * last column gets the opposite column's tile as its west/east neighbour and
+ * its diagonal tiles from there; a tile in the first or last row gets the
+ * opposite row's as its south/north neighbour (its diagonals stay the
+ * plain-arithmetic ones, which periodic ghosts never use). */
 static void set_tile_boundaries(ko_sim* s, int k) {
     tmeta* m = &s->meta[k];
-    int ntx = s->ntx;
+    int ntx = s->ntx, nty = s->nty;
     m->nb[0] = k - 1; m->nb[1] = k + 1; m->nb[2] = k + ntx; m->nb[3] = k - ntx;
     m->cr[0] = (k - ntx) - 1; m->cr[1] = (k - ntx) + 1; m->cr[2] = (k + ntx) - 1; m->cr[3] = (k + ntx) + 1;
+    if (s->cfg.bc != KST_BC_PERIODIC) return;
+    int gi, gj;
+    tile_coords(s, k, &gi, &gj);
+#define TID(i, j) ((i) + (j) * ntx)
+    if (gi == 0) {
+        m->nb[0] = TID(ntx - 1, gj);
+        if (gj > 0 && gj < nty - 1) { m->cr[0] = TID(ntx - 1, gj - 1); m->cr[2] = TID(ntx - 1, gj + 1); }
+        else if (gj == 0) { m->cr[0] = TID(ntx - 1, nty - 1); m->cr[2] = TID(ntx - 1, gj + 1); }
+        else if (gj == nty - 1) { m->cr[0] = TID(ntx - 1, gj - 1); m->cr[2] = TID(ntx - 1, 0); }
+    }
+    if (gi == ntx - 1) {
+        m->nb[1] = TID(0, gj);
+        if (gj > 0 && gj < nty - 1) { m->cr[1] = TID(0, gj - 1); m->cr[3] = TID(0, gj + 1); }
+        else if (gj == 0) { m->cr[1] = TID(0, nty - 1); m->cr[3] = TID(0, gj + 1); }
+        else if (gj == nty - 1) { m->cr[1] = TID(0, gj - 1); m->cr[3] = TID(0, 0); }
+    }
+    if (gj == 0) m->nb[3] = TID(gi, nty - 1);
+    if (gj == nty - 1) m->nb[2] = TID(gi, 0);
+#undef TID
 }
 
 /* AllocateTopographicData (src/UpdateTiles.f90:271-337): coordinates via
@@ -1305,6 +1347,30 @@ static void reconstruct_w_at_edges(ko_sim* s, int k) {
     }
 }
 
+/* SetDomainBoundaryData (src/UpdateTiles.f90:590-643) for a ghost tile on
+ * the domain edge with Dirichlet boundaries: w from the boundary depth over
+ * the cell's own bed, the momenta, solids and derived fields constant. */
+static void set_dirichlet_tile_data(ko_sim* s, int k) {
+    const kst_config* c = &s->cfg;
+    tdata* T = &s->cont[C_TC][k];
+    const double hn = c->bc_hn, uv = c->bc_u, vv = c->bc_v, ps = c->bc_psi;
+    const double rho = c->rhow + (c->rhos - c->rhow) * ps;   /* Density */
+    for (int i = 0; i < s->NC; ++i) {
+        const double hp = hn / gamma_of(c, T->u[DBDX * s->NC + i], T->u[DBDY * s->NC + i]);
+        double w = T->u[BT * s->NC + i] + hp;
+        w = w + T->u[B0 * s->NC + i];
+        T->u[W * s->NC + i] = w;
+        T->u[RHU * s->NC + i] = rho * hn * uv;
+        T->u[RHV * s->NC + i] = rho * hn * vv;
+        T->u[HNPSI * s->NC + i] = hn * ps;
+        T->u[HN * s->NC + i] = hn;
+        T->u[UU * s->NC + i] = uv;
+        T->u[VV * s->NC + i] = vv;
+        T->u[PSI * s->NC + i] = ps;
+        T->u[RHO * s->NC + i] = rho;
+    }
+}
+
 /* SetDefaultTileData (src/UpdateTiles.f90:647-664). */
 static void set_default_tile_data(ko_sim* s, int k) {
     tdata* T = &s->cont[C_TC][k];
@@ -1316,10 +1382,16 @@ static void set_default_tile_data(ko_sim* s, int k) {
     }
 }
 
-/* AddGhostTiles (src/UpdateTiles.f90:389-481), non-periodic. */
+/* AddGhostTiles (src/UpdateTiles.f90:389-481): the four neighbours and, but
+ * with periodic boundaries, the four diagonal tiles.  A new ghost gets
+ * UpdateGhostTileBCs's data (:571-585): on the domain edge
+ * SetDomainBoundaryData's -- none with periodic boundaries (AllocateU's zeros
+ * and rho = rhow stay), the boundary data with Dirichlet ones, else the
+ * default dry data -- and off the edge the default dry data. */
 static int add_ghost_tiles(ko_sim* s, int k) {
     int newg[8], ng = 0;
-    for (int q = 0; q < 8; ++q) {
+    const int nq = s->cfg.bc == KST_BC_PERIODIC ? 4 : 8;
+    for (int q = 0; q < nq; ++q) {
         int tile = (q < 4) ? s->meta[k].nb[q] : s->meta[k].cr[q - 4];
         if (!in_bounds(s, tile)) return fail(s, KST_ERR_HALT, "ghost tile outside the domain");
         int already = in_list(s->ghosts, s->nghost, tile);
@@ -1338,7 +1410,9 @@ static int add_ghost_tiles(ko_sim* s, int k) {
         set_tile_boundaries(s, tile);
         get_heights(s, tile, &s->cont[C_TC][tile]);
         cell_topo(s, &s->cont[C_TC][tile]);
-        set_default_tile_data(s, tile); /* halt: SetDomainBoundaryData -> SetDefaultTileData */
+        if (!on_domain_edge(s, tile)) set_default_tile_data(s, tile);
+        else if (s->cfg.bc == KST_BC_DIRICHLET) set_dirichlet_tile_data(s, tile);
+        else if (s->cfg.bc != KST_BC_PERIODIC) set_default_tile_data(s, tile);
         reconstruct_w_at_edges(s, tile);
         s->meta[tile].ghost = 1;
     }
@@ -1378,10 +1452,15 @@ static int activate_tile(ko_sim* s, int k) {
     return 0;
 }
 
-/* AddTile (src/UpdateTiles.f90:56-78), halt boundary. */
+/* AddTile (src/UpdateTiles.f90:56-78): a tile outside the domain, or on its
+ * edge but with periodic boundaries, is fatal with 'halt' and silently not
+ * added otherwise. */
 static int add_tile(ko_sim* s, int k) {
-    if (!in_bounds(s, k) || on_domain_edge(s, k))
-        return fail(s, KST_ERR_HALT, "Error: tried to add a tile outside the domain (tile %d)", k);
+    if (!in_bounds(s, k) || (on_domain_edge(s, k) && s->cfg.bc != KST_BC_PERIODIC)) {
+        if (s->cfg.bc == KST_BC_HALT)
+            return fail(s, KST_ERR_HALT, "Error: tried to add a tile outside the domain (tile %d)", k);
+        return 0;
+    }
     add_to_active(s, k);
     /* AllocateTile (src/UpdateTiles.f90:120-219) */
     s->meta[k].on = 1;
@@ -1544,7 +1623,23 @@ static void update_maxima(ko_sim* s, int k, double t) {
     }
 }
 
-/* HydraulicTimeStepper (src/TimeStepper.f90:333-527), no sponge layer. */
+/* ApplySpongeLayer (src/TimeStepper.f90:879-920), hydraulic operator: the
+ * explicit RHS of an active tile that lacks a neighbour on some side (tile id
+ * 0 in the reference's 1-based ids, -1 here: the first tile's west, the
+ * last of the first row's south) is damped by SpongeStrength times the
+ * stage state.  Tiles on the domain edge are never activated with sponge
+ * boundaries (AddTile), so no active tile qualifies; the check is kept. */
+static void apply_sponge(ko_sim* s, int k, tdata* I) {
+    if (s->cfg.bc != KST_BC_SPONGE) return;
+    const tmeta* m = &s->meta[k];
+    if (!(m->nb[2] == -1 || m->nb[3] == -1 || m->nb[1] == -1 || m->nb[0] == -1)) return;
+    const int NC = s->NC;
+    for (int v = 0; v < 4; ++v)
+        for (int i = 0; i < NC; ++i)
+            I[k].dte[v * NC + i] = I[k].dte[v * NC + i] - s->cfg.sponge_strength * I[k].u[v * NC + i];
+}
+
+/* HydraulicTimeStepper (src/TimeStepper.f90:333-527). */
 static double hydraulic_stepper(ko_sim* s, double* thisdt, int* refine, int64_t* nrhs) {
     const kst_config* c = &s->cfg;
     tdata *I0 = s->cont[C_I0], *I1 = s->cont[C_I1], *I2 = s->cont[C_I2], *I3 = s->cont[C_I3];
@@ -1555,6 +1650,7 @@ static double hydraulic_stepper(ko_sim* s, double* thisdt, int* refine, int64_t*
     *refine = 0;
     for (int q = 0; q < s->nactive; ++q) {
         int k = s->active[q];
+        apply_sponge(s, k, I0);
         for (int v = 0; v < 4; ++v) {
             double *u1 = &I1[k].u[v * NC], *u0 = &I0[k].u[v * NC];
             double *e = &I0[k].dte[v * NC], *im = &I0[k].dti[v * NC];
@@ -1569,6 +1665,7 @@ static double hydraulic_stepper(ko_sim* s, double* thisdt, int* refine, int64_t*
     if (dt1 < dt) { *thisdt = 0.9 * dt1; *refine = 1; return nextT; }
     for (int q = 0; q < s->nactive; ++q) {
         int k = s->active[q];
+        apply_sponge(s, k, I1);
         for (int v = 0; v < 4; ++v) {
             double *u2 = &I2[k].u[v * NC], *u0 = &I0[k].u[v * NC], *u1 = &I1[k].u[v * NC];
             double *e = &I1[k].dte[v * NC], *im = &I1[k].dti[v * NC];
@@ -1598,6 +1695,7 @@ static double hydraulic_stepper(ko_sim* s, double* thisdt, int* refine, int64_t*
     const double third = 1.0 / 3.0, twothird = 2.0 / 3.0;
     for (int q = 0; q < s->nactive; ++q) {
         int k = s->active[q];
+        apply_sponge(s, k, I2);
         for (int v = 0; v < 4; ++v) {
             double *u3 = &I3[k].u[v * NC], *u0 = &I0[k].u[v * NC], *u2 = &I2[k].u[v * NC];
             double *e = &I2[k].dte[v * NC], *im = &I2[k].dti[v * NC];
@@ -1986,7 +2084,10 @@ static void morpho_mass_update(ko_sim* s, tdata* I0, tdata* Ik, int k) {
     desingularise(s, Ik, k, 0);
 }
 
-/* MorphodynamicTimeStepper (src/TimeStepper.f90:532-781), no sponge layer. */
+/* MorphodynamicTimeStepper (src/TimeStepper.f90:532-781).  Its sponge-layer
+ * calls (ApplySpongeLayer with morpho_flag, :572,618,663) damp the bt rate of
+ * active tiles lacking a neighbour, which no active tile does (see
+ * apply_sponge): they are left out. */
 static void morpho_stepper(ko_sim* s, double* thisdt, int* refine) {
     const kst_config* c = &s->cfg;
     tdata *I0 = s->cont[C_I0], *I1 = s->cont[C_I1], *I2 = s->cont[C_I2], *I3 = s->cont[C_I3];
@@ -2213,7 +2314,7 @@ ko_sim* ko_create(const kst_config* cfg) {
         s->src[i].psi = s->src_arrays[3 * i + 2];
     }
     s->cfg.sources = s->src;
-    if (cfg->bc != KST_BC_HALT) fail(s, KST_ERR_CONFIG, "only the halt boundary condition is supported");
+    if (cfg->bc < KST_BC_HALT || cfg->bc > KST_BC_SPONGE) fail(s, KST_ERR_CONFIG, "unknown boundary condition");
     return s;
 }
 

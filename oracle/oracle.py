@@ -15,6 +15,7 @@ from kestrel_amd import scenario
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_LIB_PERT = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -37,71 +38,81 @@ class KstStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
-def lib():
-    global _LIB
+def lib(perturbed: bool = False):
+    """liboracle.so, or with perturbed=True liboracle_pert.so, whose per-cell
+    transcendental functions return one ulp up (the conditioning probe)."""
+    global _LIB, _LIB_PERT
+    if perturbed:
+        if _LIB_PERT is None:
+            _LIB_PERT = _load("liboracle_pert.so")
+        return _LIB_PERT
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
-        L = ctypes.CDLL(path)
-        L.ko_create.restype = ctypes.c_void_p
-        L.ko_create.argtypes = [ctypes.POINTER(KstConfig)]
-        L.ko_destroy.argtypes = [ctypes.c_void_p]
-        L.ko_error.restype = ctypes.c_char_p
-        L.ko_error.argtypes = [ctypes.c_void_p]
-        for name in ["ko_set_topography", "ko_add_caps", "ko_add_flux_sources", "ko_finish_sources",
-                     "ko_set_state", "ko_integrate_to", "ko_step_n", "ko_get_state",
-                     "ko_get_maxima", "ko_get_fields13", "ko_get_ghost_slopes", "ko_rhs",
-                     "ko_tile_lists"]:
-            getattr(L, name).restype = ctypes.c_int
-        L.ko_set_topography.argtypes = [ctypes.c_void_p, _dp]
-        L.ko_add_caps.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
-        L.ko_add_flux_sources.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
-        L.ko_finish_sources.argtypes = [ctypes.c_void_p]
-        L.ko_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
-                                   ctypes.POINTER(ctypes.c_uint8), ctypes.c_double, _dp]
-        L.ko_integrate_to.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(KstStats)]
-        L.ko_step_n.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(KstStats)]
-        L.ko_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
-                                   ctypes.POINTER(ctypes.c_uint8), _dp]
-        L.ko_get_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
-        L.ko_get_fields13.argtypes = [ctypes.c_void_p, _dp]
-        L.ko_get_ghost_slopes.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.ko_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
-        L.ko_tile_lists.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
-        L.ko_volume.restype = ctypes.c_int
-        L.ko_volume.argtypes = [ctypes.c_void_p, _dp]
-        L.ko_volume_terms.restype = ctypes.c_int
-        L.ko_volume_terms.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
-        for name in ["ko_get_tile_bt", "ko_morpho_rhs", "ko_probe_count", "ko_probe_get",
-                     "ko_probe_candidates"]:
-            getattr(L, name).restype = ctypes.c_int
-        L.ko_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
-        L.ko_step_log.restype = ctypes.c_int64
-        L.ko_step_log.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
-        L.ko_limiter.restype = ctypes.c_double
-        L.ko_limiter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
-        L.ko_morpho_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.ko_set_tile_bt.restype = ctypes.c_int
-        L.ko_set_tile_bt.argtypes = [ctypes.c_void_p, _dp]
-        L.ko_set_fields13.restype = ctypes.c_int
-        L.ko_set_fields13.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.ko_redistribute.restype = ctypes.c_int
-        L.ko_redistribute.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_int,
-                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
-                                      ctypes.POINTER(ctypes.c_int32), _dp, _dp, _dp,
-                                      ctypes.POINTER(ctypes.c_int)]
-        L.ko_probe_enable.restype = None
-        L.ko_probe_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.ko_probe_count.argtypes = [ctypes.c_void_p]
-        L.ko_probe_dt.restype = ctypes.c_double
-        L.ko_probe_dt.argtypes = [ctypes.c_void_p]
-        L.ko_probe_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
-        i32p = ctypes.POINTER(ctypes.c_int32)
-        L.ko_probe_candidates.argtypes = [ctypes.c_void_p, i32p, i32p, i32p, _dp,
-                                          ctypes.POINTER(ctypes.c_uint8)]
-        _LIB = L
+        _LIB = _load("liboracle.so")
     return _LIB
+
+
+def _load(name: str):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle/{name} is not built (make -C oracle)")
+    L = ctypes.CDLL(path)
+    L.ko_create.restype = ctypes.c_void_p
+    L.ko_create.argtypes = [ctypes.POINTER(KstConfig)]
+    L.ko_destroy.argtypes = [ctypes.c_void_p]
+    L.ko_error.restype = ctypes.c_char_p
+    L.ko_error.argtypes = [ctypes.c_void_p]
+    for name in ["ko_set_topography", "ko_add_caps", "ko_add_flux_sources", "ko_finish_sources",
+                 "ko_set_state", "ko_integrate_to", "ko_step_n", "ko_get_state",
+                 "ko_get_maxima", "ko_get_fields13", "ko_get_ghost_slopes", "ko_rhs",
+                 "ko_tile_lists"]:
+        getattr(L, name).restype = ctypes.c_int
+    L.ko_set_topography.argtypes = [ctypes.c_void_p, _dp]
+    L.ko_add_caps.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
+    L.ko_add_flux_sources.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+    L.ko_finish_sources.argtypes = [ctypes.c_void_p]
+    L.ko_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
+                               ctypes.POINTER(ctypes.c_uint8), ctypes.c_double, _dp]
+    L.ko_integrate_to.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(KstStats)]
+    L.ko_step_n.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(KstStats)]
+    L.ko_get_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp,
+                               ctypes.POINTER(ctypes.c_uint8), _dp]
+    L.ko_get_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
+    L.ko_get_fields13.argtypes = [ctypes.c_void_p, _dp]
+    L.ko_get_ghost_slopes.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ko_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+    L.ko_tile_lists.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
+    L.ko_volume.restype = ctypes.c_int
+    L.ko_volume.argtypes = [ctypes.c_void_p, _dp]
+    L.ko_volume_terms.restype = ctypes.c_int
+    L.ko_volume_terms.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
+    for name in ["ko_get_tile_bt", "ko_morpho_rhs", "ko_probe_count", "ko_probe_get",
+                 "ko_probe_candidates"]:
+        getattr(L, name).restype = ctypes.c_int
+    L.ko_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
+    L.ko_step_log.restype = ctypes.c_int64
+    L.ko_step_log.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
+    L.ko_limiter.restype = ctypes.c_double
+    L.ko_limiter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.ko_morpho_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ko_set_tile_bt.restype = ctypes.c_int
+    L.ko_set_tile_bt.argtypes = [ctypes.c_void_p, _dp]
+    L.ko_set_fields13.restype = ctypes.c_int
+    L.ko_set_fields13.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ko_redistribute.restype = ctypes.c_int
+    L.ko_redistribute.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                  ctypes.POINTER(ctypes.c_int32), _dp, _dp, _dp,
+                                  ctypes.POINTER(ctypes.c_int)]
+    L.ko_probe_enable.restype = None
+    L.ko_probe_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.ko_probe_count.argtypes = [ctypes.c_void_p]
+    L.ko_probe_dt.restype = ctypes.c_double
+    L.ko_probe_dt.argtypes = [ctypes.c_void_p]
+    L.ko_probe_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.ko_probe_candidates.argtypes = [ctypes.c_void_p, i32p, i32p, i32p, _dp,
+                                      ctypes.POINTER(ctypes.c_uint8)]
+    return L
 
 
 SHAPES = {"flat": 0, "para": 1, "level": 2}
@@ -110,9 +121,9 @@ SHAPES = {"flat": 0, "para": 1, "level": 2}
 class Oracle:
     """One simulation in the oracle."""
 
-    def __init__(self, cfg: KstConfig):
+    def __init__(self, cfg: KstConfig, perturbed: bool = False):
         self.cfg = cfg
-        self.L = lib()
+        self.L = lib(perturbed)
         self.h = self.L.ko_create(ctypes.byref(cfg))
         self._check(0)
 

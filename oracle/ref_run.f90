@@ -156,6 +156,11 @@ contains
       call pr('deltaX', RunParams%deltaX); call pr('deltaY', RunParams%deltaY)
       call pr('xSize', RunParams%xSize); call pr('ySize', RunParams%ySize)
       call ps('bcs', RunParams%bcs%s)
+      if (RunParams%bcs%s == 'dirichlet') then
+         call pr('bcsHnval', RunParams%bcsHnval); call pr('bcsuval', RunParams%bcsuval)
+         call pr('bcsvval', RunParams%bcsvval); call pr('bcspsival', RunParams%bcspsival)
+      end if
+      call pr('SpongeStrength', RunParams%SpongeStrength)
       call ps('limiter', RunParams%limiter%s)
       call ps('DragChoice', RunParams%DragChoice%s)
       call ps('ErosionChoice', RunParams%ErosionChoice%s)

@@ -1,7 +1,9 @@
 """Write tests/golden/run_vectors.json from the reference program's own Run.
+(--params re-reads only the RunSet scalars of the cases, keeping their runs.)
 
     make -C oracle refrun
-    python tests/golden/make_run_vectors.py [case ...]
+    python tests/golden/make_run_vectors.py [-j N] [case ...]
+    python tests/golden/make_run_vectors.py --params [case ...]
 
 Runs oracle/_ref/run/ref_run (every Fortran file of the reference compiled
 where it lies, with oracle/ref_run.f90 as its main program) on each case of
@@ -69,8 +71,20 @@ def main(names, jobs: int = 1):
                 json.dump(out, f, indent=1, sort_keys=True)
 
 
+def refresh_params(names):
+    out = runvec.fixtures()
+    for name in names or list(out):
+        with tempfile.TemporaryDirectory() as work:
+            out[name]["params"] = refrun.params(runvec.CASES[name][0], os.path.join(work, "params"))
+    with open(runvec.FIXTURE, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
+    if args[:1] == ["--params"]:
+        refresh_params(args[1:])
+        sys.exit(0)
     jobs = 1
     if args[:1] == ["-j"]:
         jobs, args = int(args[1]), args[2:]

@@ -41,11 +41,18 @@ CASES = {
     "flux_hydro_2d": (os.path.join(REF_DECKS, "Input_flux_hydro_2d.txt"), None),
     "flux_morpho_2d": (os.path.join(REF_DECKS, "Input_flux_morpho_2d.txt"), None),
     "flux_single_pt": (os.path.join(REF_DECKS, "Input_flux_single_pt.txt"), None),
+    "lake_at_rest_hydro_2d": (os.path.join(REF_DECKS, "Input_lake_at_rest_hydro_2d.txt"), None),
+    "lake_at_rest_morpho_2d": (os.path.join(REF_DECKS, "Input_lake_at_rest_morpho_2d.txt"), None),
+    "flat_depositional_2d": (os.path.join(REF_DECKS, "Input_flat_depositional_2d.txt"), None),
     "tile_indep_dynamic_100m": (os.path.join(REF_DECKS, "Input_tile_indep_dynamic_100m.txt"), None),
     "tile_indep_dynamic_20m": (os.path.join(REF_DECKS, "Input_tile_indep_dynamic_20m.txt"), None),
     "tile_indep_static_100m": (os.path.join(REF_DECKS, "Input_tile_indep_static_100m.txt"), None),
     "tile_indep_static_50m": (os.path.join(REF_DECKS, "Input_tile_indep_static_50m.txt"), None),
     "plain_cap_256": (os.path.join(OWN_DECKS, "plain_cap_256.txt"), None),
+    "periodic_cap_2d": (os.path.join(OWN_DECKS, "periodic_cap_2d.txt"), None),
+    "periodic_morpho_2d": (os.path.join(OWN_DECKS, "periodic_morpho_2d.txt"), None),
+    "dirichlet_cap_2d": (os.path.join(OWN_DECKS, "dirichlet_cap_2d.txt"), None),
+    "sponge_cap_2d": (os.path.join(OWN_DECKS, "sponge_cap_2d.txt"), None),
     "morpho_hills_1024": (os.path.join(OWN_DECKS, "morpho_hills_1024.txt"), "morpho_hills_1024"),
 }
 

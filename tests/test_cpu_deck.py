@@ -21,10 +21,8 @@ EXPECT = {
     "Input_cap_conc.txt": "one-dimensional", "Input_cap_dilute.txt": "one-dimensional",
     "Input_cap_morpho.txt": "one-dimensional", "Input_flux_edwards2019.txt": "one-dimensional",
     "Input_flux_hydro.txt": "one-dimensional", "Input_flux_morpho.txt": "one-dimensional",
-    "Input_flat_depositional.txt": "periodic", "Input_lake_at_rest_hydro.txt": "periodic",
-    "Input_lake_at_rest_morpho.txt": "periodic",
-    "Input_flat_depositional_2d.txt": "Cube", "Input_lake_at_rest_hydro_2d.txt": "Cube",
-    "Input_lake_at_rest_morpho_2d.txt": "Cube",
+    "Input_flat_depositional.txt": "one-dimensional", "Input_lake_at_rest_hydro.txt": "one-dimensional",
+    "Input_lake_at_rest_morpho.txt": "one-dimensional",
 }
 
 
@@ -134,6 +132,37 @@ def test_erosion_off_and_labels_case_insensitive():
     assert c.morphodynamics == 0 and c.erosion == C.EROSIONS["off"]
     assert c.drag == C.DRAGS["pouliquen"] and c.eddy_viscosity == 1e-2
     assert c.diffusive_time_scale == 1.0 / 1e-2
+
+
+def test_cube_and_periodic_deck_values():
+    """Cube_Set (src/InitConds.f90:523-660) and the periodic boundaries of
+    the reference's 2D lake-at-rest deck: a level cube over the whole domain
+    paints w = 1.5 everywhere (the level surface), and the topography's
+    last vertex row and column repeat the first (EqualiseTopographicBoundaryData
+    across the periodic edges)."""
+    run = deck.load(os.path.join(DECKS, "Input_lake_at_rest_hydro_2d.txt"))
+    c = run.workload.cfg
+    assert c.bc == C.BC_PERIODIC and (c.nx_tiles, c.ny_tiles) == (1, 1)
+    assert np.all(run.workload.init.w == 1.5)
+    assert np.array_equal(run.workload.b0v[:, -1], run.workload.b0v[:, 0])
+    assert np.array_equal(run.workload.b0v[-1, :], run.workload.b0v[0, :])
+    assert np.all(run.workload.init.tile_state == 1)
+
+
+def test_objects_outside_the_domain_are_refused():
+    """The domain checks of Cap_Set, Source_Set and Cube_Set
+    (src/InitConds.f90:192-195, 410-413, 643-646); touching the edge is allowed."""
+    base = open(os.path.join(DECKS, "Input_lake_at_rest_hydro_2d.txt")).read()
+    with pytest.raises(deck.DeckError, match="cube 1 outside"):
+        deck.build(deck.parse(base.replace("cubeX = 0.0", "cubeX = 0.5")))
+    cap = base.replace("Cube:", "Cap:").replace("cubeX = 0.0", "capX = 10").replace(
+        "cubeY = 0.0", "capY = 0").replace("cubeHeight", "capHeight").replace(
+        "cubeLength = 40", "capRadius = 10").replace("cubeWidth = 40\n", "").replace(
+        "cubeShape = level", "capShape = flat").replace("cubeU", "capU").replace(
+        "cubeV", "capV").replace("cubeConc", "capConc")
+    deck.build(deck.parse(cap))
+    with pytest.raises(deck.DeckError, match="cap 1 outside"):
+        deck.build(deck.parse(cap.replace("capX = 10", "capX = 10.5")))
 
 
 def test_malformed_decks_are_refused():

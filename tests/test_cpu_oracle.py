@@ -318,3 +318,30 @@ def test_product_has_no_oracle_dependency():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h", ".f90")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.replace("oracle's", ""), os.path.join(dirpath, f)
+
+
+def test_conditioning_probe_moves_only_transcendental_paths():
+    """oracle/liboracle_pert.so (per-cell tanh, cos, pow, log one ulp up)
+    gives the plain oracle's bits on a deck that evaluates none of them
+    (Chezy drag, no morphodynamics) and differs, slightly, on one that does
+    (Variable drag with erosion and deposition)."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    from tests import runvec
+
+    def run(name, perturbed, tend):
+        r = runvec.load_case(name)
+        o = Oracle(r.workload.cfg, perturbed=perturbed)
+        o.set_topography(r.workload.b0v)
+        o.set_state(r.workload.init, t=r.tstart)
+        o.integrate_to(tend)
+        f = o.fields13()
+        o.close()
+        return f
+
+    a, b = run("lake_at_rest_hydro_2d", False, 0.5), run("lake_at_rest_hydro_2d", True, 0.5)
+    assert np.array_equal(a, b)
+    a, b = run("flat_depositional_2d", False, 20.0), run("flat_depositional_2d", True, 20.0)
+    d = np.abs(a[3] - b[3]).max() / np.abs(a[3]).max() if np.abs(a[3]).max() > 0 else 0.0
+    assert not np.array_equal(a, b)
+    assert np.abs(a[0] - b[0]).max() < 1e-12 and d < 1e-6

@@ -128,5 +128,10 @@ def test_deck_reader_equals_reference_params(name):
     assert bool(c.geometric_factors) == prm["geometric_factors"]
     assert bool(c.morphodynamics) == prm["MorphodynamicsOn"]
     assert c.tile_buffer == prm["TileBuffer"]
-    assert prm["bcs"] == "halt" and c.bc == C.BC_HALT
+    assert c.bc == C.BCS[str(prm["bcs"]).strip().lower()], f"{name}: bcs = {prm['bcs']}"
+    if c.bc == C.BC_DIRICHLET:
+        for ref_name, ours in (("bcsHnval", "bc_hn"), ("bcsuval", "bc_u"), ("bcsvval", "bc_v"),
+                               ("bcspsival", "bc_psi")):
+            assert getattr(c, ours) == prm[ref_name], f"{name}: {ref_name}"
+    assert c.sponge_strength == prm["SpongeStrength"]
     assert (run.tstart, run.tend) == (prm["tstart"], prm["tend"])

@@ -139,3 +139,33 @@ def test_batch_rejects_mixed_settings(gpu):
     with pytest.raises(KestrelError):
         b.attach(stepper_for(members(C.PREC_MIXED, 1)[0]))
     b.close()
+
+
+def test_full_ensemble_batch_equals_alone(gpu):
+    """The whole batched ensemble at its stated size: 64 scenarios, sides
+    log-uniform 128-1024, mixed precision, in one batch, 100 steps each;
+    every member's counters and fields are the same bits as its run alone."""
+    n, steps = 64, 100
+    wls = workloads.ensemble(n, steps, precision=C.PREC_MIXED)
+    assert len(wls) == 64 and max(w.cfg.NX for w in wls) >= 512
+    alone = []
+    for w in wls:
+        g = stepper_for(w)
+        st = g.step_n(steps)
+        alone.append((st, g.state()))
+        g.close()
+    b = Batch(n)
+    gs = [stepper_for(w) for w in wls]
+    slots = [b.attach(g) for g in gs]
+    for s in slots:
+        b.start(s, nsteps=steps)
+    status = b.run()
+    for k, (s, g) in enumerate(zip(slots, gs)):
+        assert status[s] == SLOT_DONE, wls[k].name
+        st = b.stats(s)
+        for key in STATS:
+            assert st[key] == alone[k][0][key], f"{wls[k].name}: {key} {st[key]} vs {alone[k][0][key]}"
+        same(g.state(), alone[k][1], wls[k].name)
+    b.close()
+    for g in gs:
+        g.close()

@@ -48,3 +48,30 @@ def test_serving_admission_respects_budget(gpu):
     assert out["completed"] == len(reqs)
     assert out["peak_device_bytes"] <= 2 * need
     assert out["peak_resident"] <= 2
+
+
+def test_serving_full_trace_sample_equals_alone(gpu):
+    """The serving trace at its stated size: Poisson arrivals at 4 per second,
+    sides log-uniform 64-2048, T ~ U[60, 600] s, closures sampled as in the
+    ensemble, a 200 GB budget, mixed precision; every request completes and
+    a seeded sample of six results (the largest grid among them) are the
+    same bits as their runs alone."""
+    reqs = serving.poisson_trace(32, 4.0)
+    assert max(r.workload.cfg.NX for r in reqs) >= 1024
+    srv = serving.Server(256, 200e9, keep_state=True)
+    out = srv.run(reqs)
+    srv.close()
+    assert out["completed"] == len(reqs), [r.status for r in srv.done]
+    assert out["latency_p99_s"] >= out["latency_p50_s"] > 0.0
+    rng = np.random.default_rng(7)
+    done = sorted(srv.done, key=lambda r: r.workload.cfg.NX)
+    sample = [done[-1]] + [done[k] for k in rng.choice(len(done) - 1, 5, replace=False)]
+    for r in sample:
+        g = stepper_for(r.workload)
+        st = g.integrate_to(r.tend)
+        a = g.state()
+        g.close()
+        assert st["steps"] == r.stats["steps"] and st["cell_updates"] == r.stats["cell_updates"]
+        assert a["t"] == r.state["t"] == r.tend
+        for f in FIELDS:
+            assert np.array_equal(a[f], r.state[f]), f"{r.workload.name}: {f}"
