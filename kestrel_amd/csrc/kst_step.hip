@@ -769,6 +769,20 @@ __device__ __forceinline__ int cap_item() {
     return (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
 }
 
+// Item of thread tid when na items of one kind and nb of another (each at
+// most 128) share a workgroup of four waves: the first kind on wave 0 and,
+// beyond 64, on wave 3; the second on waves 1 and 2.  No wave mixes the two
+// kinds (a mixed wave runs both code paths one after the other, so its
+// neighbours wait at the next barrier for twice the work).  Returns the item
+// index within its kind, or -1; *second tells the kind.
+__device__ __forceinline__ int split_item(int tid, int na, int nb, bool* second) {
+    if (tid < 64) { *second = false; return tid < na ? tid : -1; }
+    if (tid < 192) { *second = true; const int g = tid - 64; return g < nb ? g : -1; }
+    *second = false;
+    const int f = tid - 128;   // 64 + (tid - 192)
+    return f < na ? f : -1;
+}
+
 // stage_body does one work item (one sub-block of an active tile): witem is
 // its index in the scenario's active-list order, or -1 to take the pass-th
 // item of the block in a one-scenario launch.  Returns whether the block has
@@ -862,14 +876,15 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     constexpr bool PRE = !MIX && Smem::NFX + Smem::NFY <= STAGE_NT;
     const bool pre = PRE && bxa == B && bya == BY;
     constexpr int NXB = BY * (B + 3), NYB = B * (BY + 3);   // x-face and y-face beds
-    static_assert(!PRE || NXB + NYB <= STAGE_NT, "one bed per thread");
+    static_assert(!PRE || (NXB <= 128 && NYB <= 128 && Smem::NFX <= 128 && Smem::NFY <= 128),
+                  "split_item: one bed and one face per thread, each kind within two waves");
     double bedv = 0.0;
     if (pre) {
-        if (tid < NXB) {
-            bedv = face_bed<false>(d, p, t, s.bx0 + tid % (B + 3) - 1, s.by0 + tid / (B + 3));
-        } else if (tid < NXB + NYB) {
-            const int q = tid - NXB;
-            bedv = face_bed<true>(d, p, t, s.by0 + q % (BY + 3) - 1, s.bx0 + q / (BY + 3));
+        bool yb;
+        const int q = split_item(tid, NXB, NYB, &yb);
+        if (q >= 0) {
+            if (!yb) bedv = face_bed<false>(d, p, t, s.bx0 + q % (B + 3) - 1, s.by0 + q / (B + 3));
+            else bedv = face_bed<true>(d, p, t, s.by0 + q % (BY + 3) - 1, s.bx0 + q / (BY + 3));
         }
     }
     for (int round = 0; round < (one_round ? 1 : 2); ++round) {
@@ -961,11 +976,11 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     double pbx = 0, pby = 0, pgxu = 0, pgxv = 0, pgyv = 0, pw0 = 0, pu0 = 0, pv0 = 0, ph0 = 0;
     const bool prec = pre || STAGE == 1 || STAGE == 4;
     if (pre) {
-        if (tid < Smem::NFX) {
-            pft = load_face_topo<false, STAGE != 4>(d, p, s, tid % (B + 1), tid / (B + 1));
-        } else if (tid < Smem::NFX + Smem::NFY) {
-            const int g = tid - Smem::NFX;
-            pft = load_face_topo<true, STAGE != 4>(d, p, s, g / B, g % B);
+        bool yf;
+        const int f = split_item(tid, Smem::NFX, Smem::NFY, &yf);
+        if (f >= 0) {
+            if (!yf) pft = load_face_topo<false, STAGE != 4>(d, p, s, f % (B + 1), f / (B + 1));
+            else pft = load_face_topo<true, STAGE != 4>(d, p, s, f / B, f % B);
         }
         if (own && STAGE == 3) { pw0 = d.Uw[oidx]; pu0 = d.Uu[oidx]; pv0 = d.Uv[oidx]; ph0 = d.Uh[oidx]; }
     }
@@ -985,8 +1000,12 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     }
     // face beds: rows of x-faces -1..bxa+1 and columns of y-faces -1..bya+1
     if (pre) {
-        if (tid < NXB) s.xbed[tid / (B + 3)][tid % (B + 3)] = bedv;
-        else if (tid < NXB + NYB) s.ybed[(tid - NXB) / (BY + 3)][(tid - NXB) % (BY + 3)] = bedv;
+        bool yb;
+        const int q = split_item(tid, NXB, NYB, &yb);
+        if (q >= 0) {
+            if (!yb) s.xbed[q / (B + 3)][q % (B + 3)] = bedv;
+            else s.ybed[q / (BY + 3)][q % (BY + 3)] = bedv;
+        }
     } else {
         for (int q = tid; q < B * (B + 3); q += STAGE_NT) {
             const int r = q / (B + 3), k = q % (B + 3) - 1;
@@ -1049,11 +1068,11 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm);
         }
     } else if (pre) {
-        if (tid < Smem::NFX) {
-            face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, tid % (B + 1), tid / (B + 1), cflm, &pft);
-        } else if (tid < Smem::NFX + Smem::NFY) {
-            const int g = tid - Smem::NFX;
-            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, g / B, g % B, cflm, &pft);
+        bool yf;
+        const int f = split_item(tid, Smem::NFX, Smem::NFY, &yf);
+        if (f >= 0) {
+            if (!yf) face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (B + 1), f / (B + 1), cflm, &pft);
+            else face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / B, f % B, cflm, &pft);
         }
     } else if (bxa == B && bya == BY) {
         // x- and y-faces as one list over the block
