@@ -106,7 +106,9 @@ __device__ __forceinline__ double excess_deposition(const DevT& d, const Params&
 //   cell topography (ComputeCellCentredTopographicData, :308-360)
 //   w, Hnpsi from the change of bt (src/TimeStepper.f90:587-610)
 // Stage 3 then runs the checks of src/TimeStepper.f90:713-752.
-template <int S, int BY, class DevT>
+// ONED: a 1D grid (one row of cells; its vertex rates and cell topography
+// take the reference's 1D forms, its upper vertex row keeps a zero rate).
+template <int S, int BY, bool ONED, class DevT>
 __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int pass) {
     constexpr int NT = morpho_nt<BY>();
     constexpr int NH = RW * (BY + 4), NE = MR1 * (BY + 2), NVB = MV * (BY + 1);
@@ -139,6 +141,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     for (int q = tid; q < NH; q += NT) {
         const int rx = q % RW - 2, ry = q / RW - 2;
         if (rx > bxa + 1 || ry > bya + 1) continue;
+        if (ONED && ry != 0) continue;   // one row of cells
         int tc, ci, cj;
         map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
         const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
@@ -155,6 +158,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     for (int q = tid; q < NE; q += NT) {
         const int rx = q % MR1 - 1, ry = q / MR1 - 1;
         if (rx > bxa || ry > bya) continue;
+        if (ONED && ry != 0) continue;
         int tc, ci, cj;
         map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
         const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
@@ -176,14 +180,15 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
             if (S == 1) {
                 const double rho = density(p, psi);
                 u = 2.0 * h * d.S3[FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
-                v = 2.0 * h * d.S3[2 * FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
+                v = ONED ? 0.0 : 2.0 * h * d.S3[2 * FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
                 if (own) { d.UV[gi] = u; d.UV[FS + gi] = v; }
             } else {
                 u = d.UV[gi];
                 v = d.UV[FS + gi];
             }
+            // the S and N neighbours only on a 2D grid (src/MorphodynamicRHS.f90:113-135)
             if (!(h < eps || hn[rg(rx - 1, ry)] < eps || hn[rg(rx + 1, ry)] < eps ||
-                  hn[rg(rx, ry - 1)] < eps || hn[rg(rx, ry + 1)] < eps)) {
+                  (!ONED && (hn[rg(rx, ry - 1)] < eps || hn[rg(rx, ry + 1)] < eps)))) {
                 CellState cs{h, u, v, psi, Tin.btc[gi], bxc, byc};
                 e = erosion_minus_deposition(p, cs);
             }
@@ -200,6 +205,29 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
         const int a = q % MV, b = q / MV;
         if (a > bxa || b > bya) continue;
         const int vi = bx0 + a, vj = by0 + b;
+        double rate;
+        if constexpr (ONED) {
+            // BtSourceTerm on a 1D grid (src/MorphodynamicRHS.f90:272-302):
+            // cells vi-1 and vi of the row; at a tile edge without an active
+            // neighbour the cell's own half; the upper vertex row keeps a zero rate
+            rate = 0.0;
+            if (b == 0) {
+                const int cl = MR1 + a, cr = MR1 + a + 1;   // cells vi-1, vi (row 0)
+                bool both = true;
+                if (vi == 0) both = d.tstate[nb_tile(p, t, -1, 0)] == KST_TILE_ACTIVE;
+                if (vi == p.nx) both = d.tstate[nb_tile(p, t, 1, 0)] == KST_TILE_ACTIVE;
+                if (both) {
+                    const double bx = 0.5 * (sbx[cl] + sbx[cr]);
+                    const double gam = gamma_of(p.geometric, bx, 0.0);
+                    rate = -0.5 * gam * (emd[cl] + emd[cr]) / psib;
+                } else {
+                    const int cc = (vi == 0) ? cr : cl;   // the tile's own edge cell
+                    const double bx = 0.5 * sbx[cc];
+                    const double gam = gamma_of(p.geometric, bx, 0.0);
+                    rate = -0.5 * gam * emd[cc] / psib;
+                }
+            }
+        } else {
         // cells (vi-1, vj-1), (vi-1, vj), (vi, vj-1), (vi, vj)
         const int i00 = b * MR1 + a, i01 = (b + 1) * MR1 + a;
         const int i10 = b * MR1 + a + 1, i11 = (b + 1) * MR1 + a + 1;
@@ -212,7 +240,8 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
         if (vi == 0 && vj >= 1 && vj <= p.ny - 1) ke.add(emd[i10] + emd[i11]);   // :210
         else { ke.add(emd[i10]); ke.add(emd[i11]); }
         const double gam = gamma_of(p.geometric, 0.25 * kx.s, 0.25 * ky.s);
-        const double rate = -0.25 * gam / psib * ke.s;
+        rate = -0.25 * gam / psib * ke.s;
+        }
         const size_t vidx = (size_t)t * NV + vj * NV1 + vi;
         const double bt0 = d.TA.vbt[vidx];
         double bn;
@@ -233,17 +262,27 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
 #define VB(a, b) vb[(b) * MV + (a)]
 #define V0(ii, jj) b0[(jj) * NV1 + (ii)]
     const double t00 = VB(tx, ty), t10 = VB(tx + 1, ty), t01 = VB(tx, ty + 1), t11 = VB(tx + 1, ty + 1);
-    Kahan kt;
-    kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
-    const double btc = 0.25 * kt.s;
-    Kahan kx;
-    kx.add(V0(i + 1, j)); kx.add(t10); kx.add(-V0(i, j)); kx.add(-t00);
-    kx.add(V0(i + 1, j + 1)); kx.add(t11); kx.add(-V0(i, j + 1)); kx.add(-t01);
-    const double dbdx = 0.5 * p.dxr * kx.s;
-    Kahan ky;
-    ky.add(V0(i, j + 1)); ky.add(t01); ky.add(-V0(i, j)); ky.add(-t00);
-    ky.add(V0(i + 1, j + 1)); ky.add(t11); ky.add(-V0(i + 1, j)); ky.add(-t10);
-    const double dbdy = 0.5 * p.dyr * ky.s;
+    double btc, dbdx, dbdy;
+    if constexpr (ONED) {
+        // src/MorphodynamicRHS.f90:348-357
+        btc = 0.5 * (t00 + t10);
+        Kahan kx;
+        kx.add(V0(i + 1, j)); kx.add(t10); kx.add(-V0(i, j)); kx.add(-t00);
+        dbdx = p.dxr * kx.s;
+        dbdy = 0.0;
+    } else {
+        Kahan kt;
+        kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
+        btc = 0.25 * kt.s;
+        Kahan kx;
+        kx.add(V0(i + 1, j)); kx.add(t10); kx.add(-V0(i, j)); kx.add(-t00);
+        kx.add(V0(i + 1, j + 1)); kx.add(t11); kx.add(-V0(i, j + 1)); kx.add(-t01);
+        dbdx = 0.5 * p.dxr * kx.s;
+        Kahan ky;
+        ky.add(V0(i, j + 1)); ky.add(t01); ky.add(-V0(i, j)); ky.add(-t00);
+        ky.add(V0(i + 1, j + 1)); ky.add(t11); ky.add(-V0(i + 1, j)); ky.add(-t10);
+        dbdy = 0.5 * p.dyr * ky.s;
+    }
 #undef VB
 #undef V0
     const double gn = gamma_of(p.geometric, dbdx, dbdy);
@@ -294,7 +333,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
 
 // The grid is sized for the active set the host last saw; a block whose
 // share of the list holds more items takes them in turn.
-template <int S, int BY = B>
+template <int S, int BY = B, bool ONED = false>
 __global__ __launch_bounds__(morpho_nt<BY>()) void morpho_kernel(const Dev* __restrict__ dp,
                                                                const Params* __restrict__ pp) {
     for (int pass = 0;; ++pass) {
@@ -303,7 +342,7 @@ __global__ __launch_bounds__(morpho_nt<BY>()) void morpho_kernel(const Dev* __re
         const Dev* d = dp;
         const Params* p = pp;
         asm("" : "+s"(d), "+s"(p) : "s"(pass));
-        if (!morpho_pass<S, BY>(const_arg(d), *p, pass)) break;
+        if (!morpho_pass<S, BY, ONED>(const_arg(d), *p, pass)) break;
         __syncthreads();
     }
 }
@@ -333,6 +372,29 @@ __device__ __forceinline__ CellRe recompute_cell(const Params& p, double t00, do
                                                  double e11, double cbt0, double cgo, double cw0,
                                                  double ch0, double cb0, bool cact,
                                                  double hghost = 0.0) {
+    if (p.oned) {
+        // a 1D grid: the lower vertex row and the 1D order of the w update
+        // (src/MorphodynamicRHS.f90:408-414, src/Redistribute.f90:444-470)
+        CellRe r;
+        r.btc = 0.5 * (t00 + t10);
+        Kahan kx;
+        kx.add(e10); kx.add(t10); kx.add(-e00); kx.add(-t00);
+        r.dbdx = p.dxr * kx.s;
+        r.dbdy = 0.0;
+        r.gn = gamma_of(p.geometric, r.dbdx, 0.0);
+        const double dbb = r.btc - cbt0;
+        double ho = hghost;
+        if (cact) {
+            ho = compute_hn(cw0, cb0, cbt0, cgo);
+            if (ho < 0.0) ho = 0.0;
+        }
+        double w = -dbb / r.gn / r.gn;
+        w = w + ho * cgo / r.gn / r.gn;
+        w = w + r.btc;
+        r.w = w + cb0;
+        r.hp = ch0 * cgo / r.gn - (1.0 - p.porosity) * dbb / r.gn;
+        return r;
+    }
     Kahan kt;
     kt.add(t00); kt.add(t10); kt.add(t01); kt.add(t11);
     Kahan kx;
@@ -381,6 +443,7 @@ __device__ __forceinline__ void put_vertex(const DevT& d, const Params& p, int k
     if (write) bt3[(size_t)k * NV + dj * NV1 + di] = val;
     if (di == 0) put(nb_tile(p, k, -1, 0), nx, dj);
     else if (di == nx) put(nb_tile(p, k, 1, 0), 0, dj);
+    if (p.oned) return;   // no row copies on a 1D grid (src/Redistribute.f90:369)
     if (dj == 0) {
         put(nb_tile(p, k, 0, -1), di, ny);
         if (di == 0) put(nb_tile(p, k, -1, -1), nx, ny);
@@ -549,9 +612,9 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int I2 = I + kDI[q], J2 = J + kDJ[q];
-            if (p.periodic) {   // the neighbours across the domain edge
+            if (p.periodic) {   // the neighbours across the domain edge (along x only on a 1D grid)
                 I2 += I2 < 0 ? NX : (I2 >= NX ? -NX : 0);
-                J2 += J2 < 0 ? NY : (J2 >= NY ? -NY : 0);
+                if (!p.oned) J2 += J2 < 0 ? NY : (J2 >= NY ? -NY : 0);
             }
             int r = -1;
             if (I2 >= 0 && J2 >= 0 && I2 < NX && J2 < NY)
@@ -664,6 +727,7 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
     for (int u = tid; u < 9 * stop; u += RD_THREADS) {
         const int q = u / 9, lane = u % 9;
         if (res[q] != 1) continue;
+        if (p.oned && lane % 3 != 1) continue;   // the row's three cells on a 1D grid
         const int g = cellid[q];
         const int kq = g / p.NC, iq = (g % p.NC) % nx, jq = (g % p.NC) / nx;
         int ct, ci, cj;

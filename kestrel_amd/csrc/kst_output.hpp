@@ -158,7 +158,8 @@ __global__ __launch_bounds__(VOL_NT) void uv_init_kernel(const Dev* dA, const Pa
             const double psi = dmin(2.0 * hn * hpc / (hn * hn + dmax(hn * hn, eps * eps)), p.max_pack);
             const double rho = density(generic(p), psi);
             d.UVo[g] = 2.0 * hn * ru / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-            d.UVo[FS + g] = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+            // a 1D grid never derives v (src/HydraulicRHS.f90:870-874)
+            d.UVo[FS + g] = p.oned ? 0.0 : 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
         }
     }
 }

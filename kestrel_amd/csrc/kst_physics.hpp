@@ -39,6 +39,8 @@ struct Params {
     // their density
     int bc, periodic;
     double bc_hn, bc_u, bc_v, bc_psi, bc_rho;
+    // a 1D grid (isOneD: nYtiles * nYpertile = 1, src/DomainSettings.f90:216-220)
+    int oned;
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }

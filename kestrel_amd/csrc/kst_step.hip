@@ -279,8 +279,13 @@ __device__ void flux_sources(const DevT& d, const Params& p, double t, double gr
                 }
             }
         }
-        Qf = Qf / (double)d.src_cells[J] / p.dx / p.dy;
-        psiQf = psiQf / (double)d.src_cells[J] / p.dx / p.dy;
+        if (p.oned) {   // per unit length on a 1D grid (src/Equations.f90:582-585)
+            Qf = Qf / (double)d.src_cells[J] / p.dx;
+            psiQf = psiQf / (double)d.src_cells[J] / p.dx;
+        } else {
+            Qf = Qf / (double)d.src_cells[J] / p.dx / p.dy;
+            psiQf = psiQf / (double)d.src_cells[J] / p.dx / p.dy;
+        }
         qsum = qsum + Qf;
         psum = psum + psiQf;
     }
@@ -312,16 +317,20 @@ __device__ __forceinline__ T lim(T a, T b) {
 // Working set of one sub-block.  With MIX the per-cell Hnpsi, u, v and the
 // fluxes other than the hydrostatic one (q = 4, in fx4/fy4) are float.
 // BY: the sub-block's height (B or B/2); the region is RW x (BY + 4) cells.
-template <int NQ, bool MIX = false, int BY = B>
+// ONED: a 1D grid (one row of cells), which also keeps the conserved rhoHnv
+// for its x-face reconstruction.
+template <int NQ, bool MIX = false, int BY = B, bool ONED = false>
 struct SmemT {
     using FT = typename std::conditional<MIX, float, double>::type;
     static constexpr int by = BY;
+    static constexpr bool oned = ONED;
     static constexpr int NR = RW * (BY + 4);       // region cells
     static constexpr int NFX = (B + 1) * BY;       // x-faces
     static constexpr int NFY = B * (BY + 1);       // y-faces
     double w[NR];
     FT h[NR], u[NR], v[NR];
     double rho[NR];
+    double rv[ONED ? NR : 1];  // 1D: the conserved rhoHnv
     double wst[NR];          // w behind the ghost tiles' stored face values (their tileContainer w)
     double xbed[BY][B + 3];  // CorrectSlopes bed of x-faces -1..B+1 of each row (relative)
     double ybed[B][BY + 3];  // CorrectSlopes bed of y-faces -1..BY+1 of each column
@@ -458,7 +467,19 @@ __device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& 
     const double HnR = compute_hn(wFR, fb0, fbt, fg);
     const double unL = YDIR ? vFL : uFL, unR = YDIR ? vFR : uFR;
     const double ruL = rFL * HnL * uFL, ruR = rFR * HnR * uFR;
-    const double rvL = rFL * HnL * vFL, rvR = rFR * HnR * vFR;
+    double rvL, rvR;
+    if constexpr (Smem::oned) {
+        // a 1D grid keeps the x-face reconstruction of the conserved rhoHnv
+        // (rho Hn v is rebuilt in 2D only, src/HydraulicRHS.f90:536-551)
+        const double qLL = s.rv[iLL], qL = s.rv[iL], qR = s.rv[iR], qRR = s.rv[iRR];
+        const double sLq = actL ? dxr * lim<LIM>(qR - qL, qL - qLL) : 0.0;
+        const double sRq = actR ? dxr * lim<LIM>(qRR - qR, qR - qL) : 0.0;
+        rvL = qL + sLq * 0.5 * dx;
+        rvR = qR - sRq * 0.5 * dx;
+    } else {
+        rvL = rFL * HnL * vFL;
+        rvR = rFR * HnR * vFR;
+    }
 
     // --- wave speeds and CFL (src/HydraulicRHS.f90:949-1006)
     const double rP = wave_root(p, HnR, fbp, fg);
@@ -684,11 +705,13 @@ __device__ void reduce_min_to(Smem& s, double v, unsigned long long* slot) {
 // InterpolateB (src/HydraulicRHS.f90:741-753) of tile tn's copy of vertices
 // (i0, j0) and (i1, j1) in the bed the morphodynamic operator left (TB), which
 // is intermed3's: a ghost tile's copy there may carry redistribution writes.
-template <class DevT>
+// On a 1D grid the bed of vertex (i0, j0) alone, b0 + bt (src/HydraulicRHS.f90:593-595).
+template <bool ONED = false, class DevT>
 __device__ __forceinline__ double ghost_bed(const DevT& d, const Params& p, int tn, int i0, int j0,
                                            int i1, int j1) {
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
     const size_t a = (size_t)tn * NV + j0 * (p.nx + 1) + i0, b = (size_t)tn * NV + j1 * (p.nx + 1) + i1;
+    if constexpr (ONED) return d.vb0[a] + d.TB.vbt[a];
     double ib = d.TB.vbt[a] + d.TB.vbt[b];
     ib = ib + d.vb0[a];
     ib = ib + d.vb0[b];
@@ -898,8 +921,8 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             const int strip = hq >> 5, k = hq & 31, dd = k >> 4, pos = k & 15;
             if (strip == 0) { rx = -2 + dd; ry = pos; valid = pos < bya; }
             else if (strip == 1) { rx = bxa + dd; ry = pos; valid = pos < bya; }
-            else if (strip == 2) { rx = pos; ry = -2 + dd; valid = pos < bxa; }
-            else { rx = pos; ry = bya + dd; valid = pos < bxa; }
+            else if (strip == 2) { rx = pos; ry = -2 + dd; valid = pos < bxa && !Smem::oned; }
+            else { rx = pos; ry = bya + dd; valid = pos < bxa && !Smem::oned; }   // no rows beyond a 1D grid's
         }
         if (!valid) continue;
         int tc, ci, cj;
@@ -954,11 +977,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             psi = dmin(2.0 * hn * hpc / den, p.max_pack);
             rho = density(p, psi);
             uu = 2.0 * hn * ru / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-            vv = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+            // a 1D grid never derives v (src/HydraulicRHS.f90:870-874): it stays 0
+            if constexpr (!Smem::oned) vv = 2.0 * hn * rv / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
         } else if (p.bc == KST_BC_DIRICHLET && dirichlet_ghost(p, tc)) {
             hn = p.bc_hn; psi = p.bc_psi; rho = p.bc_rho; uu = p.bc_u; vv = p.bc_v;
         }
         const int r = rg(rx, ry);
+        if constexpr (Smem::oned) s.rv[r] = rv;
         s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho;
         s.wst[r] = act ? w : Un[gi];
         s.gidx[r] = gi;
@@ -1010,7 +1035,8 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         for (int q = tid; q < B * (B + 3); q += STAGE_NT) {
             const int r = q / (B + 3), k = q % (B + 3) - 1;
             if (r < bya && k <= bxa + 1) s.xbed[r][k + 1] = face_bed<false>(d, p, t, s.bx0 + k, s.by0 + r);
-            if (r < bxa && k <= bya + 1) s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
+            if (!Smem::oned && r < bxa && k <= bya + 1)
+                s.ybed[r][k + 1] = face_bed<true>(d, p, t, s.by0 + k, s.bx0 + r);
         }
     }
     // the neighbour tiles' own beds at the shared edge faces; in the last
@@ -1024,17 +1050,17 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             if (s.bx0 == 0) {
                 const int tn = nb_tile(p, t, -1, 0);
                 s.xside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
-                                    ? ghost_bed(d, p, tn, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
+                                    ? ghost_bed<Smem::oned>(d, p, tn, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
                                     : d.T.xib[tn * p.NFX + row + p.nx];
             }
             if (s.bx0 + bxa == p.nx) {
                 const int tn = nb_tile(p, t, 1, 0);
                 s.xside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
-                                    ? ghost_bed(d, p, tn, 0, s.by0 + r, 0, s.by0 + r + 1)
+                                    ? ghost_bed<Smem::oned>(d, p, tn, 0, s.by0 + r, 0, s.by0 + r + 1)
                                     : d.T.xib[tn * p.NFX + row];
             }
         }
-        if (r < bxa) {
+        if (!Smem::oned && r < bxa) {
             const int col = s.bx0 + r;
             if (s.by0 == 0) {
                 const int tn = nb_tile(p, t, 0, -1);
@@ -1087,8 +1113,9 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     } else {
         for (int f = tid; f < (bxa + 1) * bya; f += STAGE_NT)
             face_work<false, VISC, LIM, STAGE != 4, MIX>(d, p, s, f % (bxa + 1), f / (bxa + 1), cflm);
-        for (int f = tid; f < (bya + 1) * bxa; f += STAGE_NT)
-            face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / bxa, f % bxa, cflm);
+        if constexpr (!Smem::oned)   // no y faces on a 1D grid (src/HydraulicRHS.f90:1075)
+            for (int f = tid; f < (bya + 1) * bxa; f += STAGE_NT)
+                face_work<true, VISC, LIM, STAGE != 4, MIX>(d, p, s, f / bxa, f % bxa, cflm);
     }
     if (STAGE != 4) {
         reduce_min_to(s, cflm, &d.ctrl->slot[(d.second ? 3 : 0) + STAGE - 1].bits);
@@ -1106,11 +1133,25 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     // prefactors (1+by^2)/gam, -(bx by/gam), (1+bx^2)/gam are per-bed topography
     const double gxu = prec ? pgxu : d.T.pxu[oidx], gxv = prec ? pgxv : d.T.pxv[oidx], gyu = gxv;
     const double gyv = prec ? pgyv : d.T.pyv[oidx];
-    // ConstructHydraulicRHS (src/HydraulicRHS.f90:1228-1271)
-    const double stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
-    const double sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
-    double stu, stv;
-    {
+    // ConstructHydraulicRHS (src/HydraulicRHS.f90:1228-1271; on a 1D grid
+    // :1276-1294, x fluxes only and the hydrostatic term divided by gamma)
+    double stw, sth, stu, stv;
+    if constexpr (Smem::oned) {
+        stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam);
+        sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam;
+        Kahan k1;
+        k1.add(FX(1, 0) - FX(1, 1));
+        k1.add((FX(4, 0) - FX(4, 1)) / gam);
+        k1.add(VISC ? (FX(5, 1) - FX(5, 0)) : 0.0);
+        stu = k1.s * p.dxr;
+        Kahan k3;
+        k3.add(FX(2, 0) - FX(2, 1));
+        k3.add((FX(4, 0) - FX(4, 1)) / gam);
+        k3.add(VISC ? (FX(6, 1) - FX(6, 0)) : 0.0);
+        stv = k3.s * p.dxr;
+    } else {
+        stw = (FX(0, 0) - FX(0, 1)) * p.dxr / (gam * gam) + (FY(0, 0) - FY(0, B)) * p.dyr / (gam * gam);
+        sth = (FX(3, 0) - FX(3, 1)) * p.dxr / gam + (FY(3, 0) - FY(3, B)) * p.dyr / gam;
         Kahan k1;
         k1.add(FX(1, 0) - FX(1, 1));
         k1.add((FX(4, 0) - FX(4, 1)) * gxu);
@@ -1243,7 +1284,7 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
                 const double psi = dmin(2.0 * hn * hp0 / den, p.max_pack);
                 const double rho = density(p, psi);
                 const double uu = 2.0 * hn * u0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
-                const double vv = 2.0 * hn * v0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
+                const double vv = Smem::oned ? 0.0 : 2.0 * hn * v0 / (hn * hn + dmax(hn * hn, eps * eps)) / rho;
                 if (mxv[5] == -1) M[10 * FS + oidx] = tn;
                 if (hn > mxv[0]) { M[oidx] = hn; M[FS + oidx] = tn; }
                 const double spd = sqrt(speed2(p.geometric, uu, vv, bx, by));
@@ -1271,10 +1312,10 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
 
 // The grid is sized for the active set the host last saw (with room to
 // grow); a block whose share of the list holds more items takes them in turn.
-template <int STAGE, bool VISC, int LIM, bool MIX = false, int BY = B>
+template <int STAGE, bool VISC, int LIM, bool MIX = false, int BY = B, bool ONED = false>
 __global__ __launch_bounds__(STAGE_NT) void stage_kernel(const Dev* __restrict__ d,
                                                          const Params* __restrict__ p) {
-    __shared__ SmemT<VISC ? 7 : 5, MIX, BY> s;
+    __shared__ SmemT<VISC ? 7 : 5, MIX, BY, ONED> s;
     for (int pass = 0; stage_body<STAGE, VISC, LIM, MIX>(const_arg(d), *p, s, -1, pass); ++pass)
         __syncthreads();
 }
@@ -1294,6 +1335,20 @@ __device__ void tile_cell_topo(const DevT& d, const Params& p, const TopoT& T, i
     const size_t NV = (size_t)(nx + 1) * (p.ny + 1);
     const double* b = d.vb0 + (size_t)t * NV;
     const double* bt = T.vbt + (size_t)t * NV;
+    if (p.oned) {
+        // 1D: two-vertex forms along the lower vertex row (src/MorphodynamicRHS.f90:348-357)
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const double b0c = 0.5 * (b[i] + b[i + 1]);
+            const double btc = zero_btc ? 0.0 : 0.5 * (bt[i] + bt[i + 1]);
+            Kahan kx;
+            kx.add(b[i + 1]); kx.add(bt[i + 1]); kx.add(-b[i]); kx.add(-bt[i]);
+            const double dbdx = p.dxr * kx.s;
+            const size_t gi = (size_t)t * p.NC + i;
+            d.b0c[gi] = b0c; T.btc[gi] = btc; T.cbx[gi] = dbdx; T.cby[gi] = 0.0;
+            T.cgam[gi] = gamma_of(p.geometric, dbdx, 0.0);
+        }
+        return;
+    }
     for (int c = threadIdx.x; c < p.NC; c += blockDim.x) {
         const int i = c % nx, j = c / nx;
         Kahan k;
@@ -1348,6 +1403,41 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
     const double* bE = d.vb0 + (size_t)tE * NV; const double* btE = T.vbt + (size_t)tE * NV;
     const double* bS = d.vb0 + (size_t)tS * NV; const double* btS = T.vbt + (size_t)tS * NV;
     const double* bN = d.vb0 + (size_t)tN * NV; const double* btN = T.vbt + (size_t)tN * NV;
+    if (p.oned) {
+        // 1D (src/MorphodynamicRHS.f90:545-577, src/dem.f90:387-390,
+        // src/HydraulicRHS.f90:593-595): x faces of the lower vertex row, the
+        // face's own vertex b0 and bt, a two-vertex central dbdx, no
+        // perpendicular slope, the CorrectSlopes bed b0 + bt
+        for (int i = lane0; i <= nx; i += stride) {
+            double gl = 0.0, gr_ = 0.0;
+            if (STATIC) {
+                gl = (i == 0) ? T.cgam[tW * p.NC + nx - 1] : T.cgam[t * p.NC + i - 1];
+                gr_ = (i == nx) ? T.cgam[tE * p.NC] : T.cgam[t * p.NC + i];
+            }
+            Kahan kx;
+            if (i >= 1 && i <= nx - 1) {
+                kx.add(V(b, i + 1, 0)); kx.add(V(bt, i + 1, 0)); kx.add(-V(b, i - 1, 0)); kx.add(-V(bt, i - 1, 0));
+            } else if (i == 0) {
+                kx.add(V(b, 1, 0)); kx.add(V(bt, 1, 0)); kx.add(-V(bW, nx - 1, 0)); kx.add(-V(btW, nx - 1, 0));
+            } else {
+                kx.add(V(bE, 1, 0)); kx.add(V(btE, 1, 0)); kx.add(-V(b, nx - 1, 0)); kx.add(-V(bt, nx - 1, 0));
+            }
+            const double dbdx = 0.5 * p.dxr * kx.s;
+            const size_t fi = (size_t)t * p.NFX + i;
+            const double fg = gamma_of(p.geometric, dbdx, 0.0);
+            d.xb0[fi] = V(b, i, 0);
+            T.xbt[fi] = V(bt, i, 0);
+            T.xby[fi] = 0.0;
+            T.xgam[fi] = fg;
+            T.xib[fi] = V(b, i, 0) + V(bt, i, 0);
+            if (STATIC) {
+                const double a = fmin(gl / fg, 1.0), c = fmin(gr_ / fg, 1.0);
+                T.xnp[fi] = a * a * p.dx;
+                T.xnm[fi] = c * c * p.dx;
+            }
+        }
+        return;
+    }
     // x-faces: i in [0, nx], j in [0, ny)
     for (int f = lane0; f < (nx + 1) * ny; f += stride) {
         const int i = f % (nx + 1), j = f / (nx + 1);
@@ -1472,7 +1562,7 @@ __device__ void tile_static_terms(const DevT& d, const Params& p, const TopoT& T
         T.xnp[t * p.NFX + f] = a * a * p.dx;
         T.xnm[t * p.NFX + f] = b * b * p.dx;
     }
-    for (int f = lane0; f < nx * (ny + 1); f += stride) {
+    for (int f = lane0; f < (p.oned ? 0 : nx * (ny + 1)); f += stride) {
         const int i = f % nx, j = f / nx;
         const double fg = T.ygam[t * p.NFY + f];
         const double gl = (j == 0) ? T.cgam[tS * p.NC + (ny - 1) * nx + i] : T.cgam[t * p.NC + (j - 1) * nx + i];
@@ -1655,8 +1745,9 @@ __device__ __forceinline__ int add_action(const Params& p, const uint8_t* ts, in
 }
 
 // The ghost candidates of a new tile (AddGhostTiles, src/UpdateTiles.f90:389-481):
-// W, E, N, S and, but with periodic boundaries, the four diagonal tiles.
-__device__ __forceinline__ int ghost_count(const Params& p) { return p.periodic ? 4 : 8; }
+// W, E, N, S and, but with periodic boundaries, the four diagonal tiles; W
+// and E on a 1D grid.
+__device__ __forceinline__ int ghost_count(const Params& p) { return p.oned ? 2 : (p.periodic ? 4 : 8); }
 __device__ __forceinline__ int ghost_candidate(const Params& p, int k, int r) {
     constexpr int DI[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, DJ[8] = {0, 0, 1, -1, -1, -1, 1, 1};
     return nb_tile(p, k, DI[r], DJ[r]);
@@ -1679,7 +1770,7 @@ __device__ void activate_body(const DevT& d, const Params& p) {
             const int k = d.active[q];
             const unsigned f = d.eflags[k];
             if (!f) continue;
-            for (int r = 0; r < 4; ++r)
+            for (int r = p.oned ? 2 : 0; r < 4; ++r)   // E and W only on a 1D grid
                 if ((f & (1u << r)) && add_action(p, d.tstate, side_tile(p, k, r)) != ADD_NONE) mine = 1;
         }
         if (mine) atomicOr(&need, 1);
@@ -1692,7 +1783,9 @@ __device__ void activate_body(const DevT& d, const Params& p) {
     if (threadIdx.x == 0) {
         nnew = 0;
         bad = 0;
-        for (int q = 0; q < 4 && !bad; ++q) {     // N, S, E, W passes in order
+        // N, S, E, W passes in order; E and W only on a 1D grid
+        // (CheckIfNearBoundaries, src/TimeStepper.f90:935-940)
+        for (int q = p.oned ? 2 : 0; q < 4 && !bad; ++q) {
             const int size = c->n_active;
             for (int tt = 0; tt < size && !bad; ++tt) {
                 const int k = d.active[tt];
@@ -2183,6 +2276,7 @@ void fill_params(kst_handle* h) {
     p.periodic = c.bc == KST_BC_PERIODIC;
     p.bc_hn = c.bc_hn; p.bc_u = c.bc_u; p.bc_v = c.bc_v; p.bc_psi = c.bc_psi;
     p.bc_rho = c.rhow + (c.rhos - c.rhow) * c.bc_psi;   // Density (src/Closures.f90)
+    p.oned = c.ny_tiles * c.ny_per_tile == 1;
 }
 
 int read_ctrl(kst_handle* h, Ctrl* c) {
@@ -2216,19 +2310,21 @@ void from_tiles(const kst_handle* h, const double* t, double* g) {
 typedef void (*StageFn)(const Dev*, const Params*);
 struct StageSet { StageFn fn[4]; };
 
-template <bool VISC, int LIM, bool MIX, int BY>
+template <bool VISC, int LIM, bool MIX, int BY, bool ONED>
 StageSet make_set() {
-    return StageSet{{stage_kernel<1, VISC, LIM, MIX, BY>, stage_kernel<2, VISC, LIM, MIX, BY>,
-                     stage_kernel<3, VISC, LIM, MIX, BY>, stage_kernel<4, VISC, LIM, MIX, BY>}};
+    return StageSet{{stage_kernel<1, VISC, LIM, MIX, BY, ONED>, stage_kernel<2, VISC, LIM, MIX, BY, ONED>,
+                     stage_kernel<3, VISC, LIM, MIX, BY, ONED>, stage_kernel<4, VISC, LIM, MIX, BY, ONED>}};
 }
 
-template <bool MIX, int BY>
+template <bool MIX, int BY, bool ONED = false>
 const StageSet& stage_set_of(bool visc, int limiter) {
     static const StageSet sets[2][5] = {
-        {make_set<false, 0, MIX, BY>(), make_set<false, 1, MIX, BY>(), make_set<false, 2, MIX, BY>(),
-         make_set<false, 3, MIX, BY>(), make_set<false, 4, MIX, BY>()},
-        {make_set<true, 0, MIX, BY>(), make_set<true, 1, MIX, BY>(), make_set<true, 2, MIX, BY>(),
-         make_set<true, 3, MIX, BY>(), make_set<true, 4, MIX, BY>()}};
+        {make_set<false, 0, MIX, BY, ONED>(), make_set<false, 1, MIX, BY, ONED>(),
+         make_set<false, 2, MIX, BY, ONED>(), make_set<false, 3, MIX, BY, ONED>(),
+         make_set<false, 4, MIX, BY, ONED>()},
+        {make_set<true, 0, MIX, BY, ONED>(), make_set<true, 1, MIX, BY, ONED>(),
+         make_set<true, 2, MIX, BY, ONED>(), make_set<true, 3, MIX, BY, ONED>(),
+         make_set<true, 4, MIX, BY, ONED>()}};
     return sets[visc ? 1 : 0][(limiter >= 0 && limiter < 5) ? limiter : 1];
 }
 
@@ -2240,7 +2336,10 @@ const StageSet& stage_set_of(bool visc, int limiter) {
 constexpr int BY_SMALL = B / 4;
 constexpr int SMALL_WORK = 128;       // sub-blocks of 16 x 16 below which the small shape is used
 
-const StageSet& stage_set(bool visc, int limiter, bool mixed, bool small) {
+// A 1D grid always takes the 16 x 4 shape (16 cells of its one row per
+// workgroup) and FP64.
+const StageSet& stage_set(bool visc, int limiter, bool mixed, bool small, bool oned) {
+    if (oned) return stage_set_of<false, BY_SMALL, true>(visc, limiter);
     if (mixed) return stage_set_of<true, B>(visc, limiter);
     return small ? stage_set_of<false, BY_SMALL>(visc, limiter) : stage_set_of<false, B>(visc, limiter);
 }
@@ -2265,7 +2364,7 @@ int launch_iteration(kst_handle* h, bool profiled, bool small, int cap, int grid
     // multiples of 8 blocks, as the XCD-chunked mapping assumes
     const int nblk = 8 * ((cap * spt + 7) / 8);
     const bool visc = p.nu != 0.0;
-    const bool sm = small && !d.mixed;
+    const bool sm = (small && !d.mixed) || p.oned;
     const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
     // a grid limit (a multiple of 8) keeps that many blocks, each taking
     // further items in turn
@@ -2273,12 +2372,17 @@ int launch_iteration(kst_handle* h, bool profiled, bool small, int cap, int grid
     const Dev* const A = h->d_args;
     const Params* const P = h->d_par;
     LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), A, P);
-    const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0, sm);
+    const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0, sm, p.oned != 0);
     for (int q = 0; q < 4; ++q)
         LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A, P);
     if (d.morpho) {
         // M(2dt): three SSP stages, redistribution, face topography of the new bed
-        if (sm) {
+        if (p.oned) {
+            constexpr int NT = morpho_nt<BY_SMALL>();
+            LAUNCH(KW_MORPHO + 0, (morpho_kernel<1, BY_SMALL, true>), dim3(nblk_stage), dim3(NT), A, P);
+            LAUNCH(KW_MORPHO + 1, (morpho_kernel<2, BY_SMALL, true>), dim3(nblk_stage), dim3(NT), A, P);
+            LAUNCH(KW_MORPHO + 2, (morpho_kernel<3, BY_SMALL, true>), dim3(nblk_stage), dim3(NT), A, P);
+        } else if (sm) {
             constexpr int NT = morpho_nt<BY_SMALL>();
             LAUNCH(KW_MORPHO + 0, (morpho_kernel<1, BY_SMALL>), dim3(nblk_stage), dim3(NT), A, P);
             LAUNCH(KW_MORPHO + 1, (morpho_kernel<2, BY_SMALL>), dim3(nblk_stage), dim3(NT), A, P);
@@ -2391,7 +2495,7 @@ int autotune_bucket(kst_handle* h, const Ctrl& c0, int cap, kst_handle::Tune* ou
     for (int q = 0; q < ncand; ++q) {
         const Cand& k = cand[q];
         if (!k.small && k.grid > 0 && k.grid >= nblk) continue;   // no different from the cap's grid
-        const StageSet& set = stage_set(visc, p.limiter, mixed, k.small != 0);
+        const StageSet& set = stage_set(visc, p.limiter, mixed, k.small != 0, false);
         const int spt_s = ((p.nx + B - 1) / B) * ((p.ny + BY_SMALL - 1) / BY_SMALL);
         const int grid = k.small ? 8 * ((cap * spt_s + 7) / 8) : (k.grid > 0 ? std::min(nblk, k.grid) : nblk);
         hipLaunchKernelGGL(set.fn[0], dim3(grid), dim3(STAGE_NT), 0, h->stream, h->d_args, h->d_par);
@@ -2440,7 +2544,7 @@ int run_until_done(kst_handle* h, kst_stats* st, long long max_iters) {
         bool small = (long long)c.n_active * spt < SMALL_WORK;
         const int cap = work_cap(h, c.n_active);
         int grid_limit = 0;
-        if (h->autotune) {
+        if (h->autotune && !h->p.oned) {   // a 1D grid has one launch shape
             const int bucket = tune_bucket(c.n_active);
             auto it = h->tuned.find(bucket);
             if (it == h->tuned.end()) {
@@ -2551,11 +2655,14 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
     if (!cfg || !out) return KST_ERR_CONFIG;
     *out = nullptr;
     if (cfg->bc < KST_BC_HALT || cfg->bc > KST_BC_SPONGE) return KST_ERR_CONFIG;
-    // without periodic boundaries only interior tiles are ever active
+    // without periodic boundaries only interior tiles are ever active; a 1D
+    // grid is one row of tiles of one row of cells
     const int min_tiles = cfg->bc == KST_BC_PERIODIC ? 1 : 3;
-    if (cfg->nx_per_tile < 4 || cfg->ny_per_tile < 4 || cfg->nx_tiles < min_tiles ||
-        cfg->ny_tiles < min_tiles)
-        return KST_ERR_CONFIG;
+    const bool oned = cfg->ny_tiles * cfg->ny_per_tile == 1;
+    if (cfg->nx_per_tile < 4 || cfg->nx_tiles < min_tiles) return KST_ERR_CONFIG;
+    if (!oned && (cfg->ny_per_tile < 4 || cfg->ny_tiles < min_tiles)) return KST_ERR_CONFIG;
+    // the 1D path runs the FP64 kernels
+    if (oned && cfg->precision != KST_PREC_FP64) return KST_ERR_CONFIG;
     if (cfg->precision != KST_PREC_FP64 && cfg->precision != KST_PREC_MIXED) return KST_ERR_CONFIG;
     // mixed precision covers the hydraulic operator only
     if (cfg->precision == KST_PREC_MIXED && cfg->morphodynamics) return KST_ERR_CONFIG;
@@ -2732,7 +2839,8 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     const size_t FS = (size_t)h->ntiles * h->NC;
     // tile lists: active sorted by id; ghosts = the non-active neighbours of
     // active tiles that AddGhostTiles takes (src/UpdateTiles.f90:389-481):
-    // W, E, N, S and, but with periodic boundaries, the diagonal tiles
+    // W, E, N, S and, but with periodic boundaries, the diagonal tiles (W and
+    // E on a 1D grid)
     const bool periodic = h->cfg.bc == KST_BC_PERIODIC;
     auto edge = [&](int k) {
         const int gi = k % h->ntx, gj = k / h->ntx;
@@ -2763,7 +2871,7 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
     {
         const int DI[8] = {-1, 1, 0, 0, -1, 1, -1, 1}, DJ[8] = {0, 0, 1, -1, -1, -1, 1, 1};
         for (int k : act)
-            for (int r = 0; r < (periodic ? 4 : 8); ++r) {
+            for (int r = 0; r < (h->p.oned ? 2 : (periodic ? 4 : 8)); ++r) {
                 const int n = nb(k, DI[r], DJ[r]);
                 if (st[n] == KST_TILE_OFF) { st[n] = KST_TILE_GHOST; gh.push_back(n); }
             }
@@ -2877,7 +2985,9 @@ int kst_set_state(kst_handle* h, const double* w, const double* rhu, const doubl
                 const int ti = I / h->nx, tj = J / h->ny, i = I % h->nx, j = J % h->ny;
                 const double x = -0.5 * h->cfg.x_size + h->cfg.dx * ((ti + 1 - 1.0) * h->nx + (i + 1 - 0.5));
                 const double y = -0.5 * h->cfg.y_size + h->cfg.dy * ((tj + 1 - 1.0) * h->ny + (j + 1 - 0.5));
-                if ((x - sx) * (x - sx) + (y - sy) * (y - sy) <= sr * sr) hs[ti + tj * h->ntx] = 1;
+                // along x only on a 1D grid (src/SetSources.f90:219-231)
+                const double r2 = h->p.oned ? (x - sx) * (x - sx) : (x - sx) * (x - sx) + (y - sy) * (y - sy);
+                if (r2 <= sr * sr) hs[ti + tj * h->ntx] = 1;
             }
     }
     HIPCHK(h, hipMemcpy(h->d.has_src, hs.data(), hs.size(), hipMemcpyHostToDevice));
@@ -3336,6 +3446,7 @@ int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot) {
     if (!b || !h || !slot) return KST_ERR_CONFIG;
     if (!h->have_state) return batch_err(b, KST_ERR_STATE, "kst_set_state must come before kst_batch_attach");
     if (h->d.morpho) return batch_err(b, KST_ERR_CONFIG, "batches run the hydraulic operator only");
+    if (h->p.oned) return batch_err(b, KST_ERR_CONFIG, "batches run 2D grids only");
     if (h->p.limiter != KST_LIMITER_MINMOD2)
         return batch_err(b, KST_ERR_CONFIG, "batches use the MinMod2 limiter");
     const int visc = h->p.nu != 0.0 ? 1 : 0, mixed = h->d.mixed ? 1 : 0;

@@ -13,7 +13,8 @@ src/TopogFuncs.f90:395-577); Cap_Set (src/InitConds.f90:254-511) and Source_Set
 (46-208).  Labels of the Domain, Parameters, Solver, Output and Topog blocks
 compare in lower case; Source and Cap keys compare as written, as there.
 Settings the device path does not run (DEM topographies, restarts) raise
-DeckUnsupported naming the setting.
+DeckUnsupported naming the setting.  One-dimensional domains
+(nYtiles * nYpertile = 1) take the reference's 1D branches throughout.
 """
 from __future__ import annotations
 
@@ -151,8 +152,6 @@ def _domain(d: Deck):
         raise DeckError("This boundary condition 'reflect' is not supported yet")
     if bcs not in C.BCS:
         raise DeckError(f"'Boundary Conditions' value {bcs} is not recognized")
-    if nty * ny == 1:
-        raise DeckUnsupported("one-dimensional domains (nYtiles * nYpertile = 1)")
     return ntx, nty, nx, ny, xt, yt, bcs
 
 
@@ -409,7 +408,9 @@ def _inside(c: C.KstConfig, what: str, k: int, x: float, y: float, hx: float, hy
     """The domain checks of Source_Set, Cap_Set and Cube_Set
     (src/InitConds.f90:192-195, 410-413, 643-646): the object's extent
     [x - hx, x + hx] x [y - hy, y + hy] lies in the domain (touching its edge
-    is allowed)."""
+    is allowed).  A 1D grid has no such check."""
+    if S.is_1d(c):
+        return
     if (x - hx < -0.5 * c.x_size or x + hx > 0.5 * c.x_size or
             y - hy < -0.5 * c.y_size or y + hy > 0.5 * c.y_size):
         raise DeckError(f"In input file, {what} {k + 1} outside of the domain")
@@ -421,7 +422,7 @@ def _caps(d: Deck, c: C.KstConfig) -> list:
     for k, cp in enumerate(d.caps):
         if "capLat" in cp or "capLon" in cp:
             raise DeckUnsupported("caps placed by latitude/longitude")
-        if "capX" not in cp or "capY" not in cp:
+        if "capX" not in cp or ("capY" not in cp and not S.is_1d(c)):
             raise DeckError(f"Cap {k + 1}: no position (capX, capY)")
         shape = {"parabolic": "para", "para": "para", "flat": "flat",
                  "level": "level"}.get(cp.get("capShape", "para").strip().lower(), "para")
@@ -434,8 +435,8 @@ def _caps(d: Deck, c: C.KstConfig) -> list:
             r = math.sqrt(vol / math.pi / h) if shape == "flat" else math.sqrt(2.0 * vol / math.pi / h)
         if r is None or h is None:
             raise DeckError(f"Cap {k + 1}: two of capRadius, capHeight, capVolume are needed")
-        _inside(c, "cap", k, _real(cp["capX"]), _real(cp["capY"]), r, r)
-        out.append(S.Cap(_real(cp["capX"]), _real(cp["capY"]), r, h,
+        _inside(c, "cap", k, _real(cp["capX"]), _real(cp.get("capY", "0")), r, r)
+        out.append(S.Cap(_real(cp["capX"]), _real(cp.get("capY", "0")), r, h,
                          psi=_real(cp.get("capConc", "0")), u=_real(cp.get("capU", "0")),
                          v=_real(cp.get("capV", "0")), shape=shape))
     return out
@@ -510,9 +511,13 @@ def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64,
         if c.bc == C.BC_PERIODIC:
             # EqualiseTopographicBoundaryData (src/MorphodynamicRHS.f90:588-687)
             # across the periodic edges: a tile's east and north edge vertices
-            # take its (wrapped) east and north neighbours' west and south ones
-            b0v[:, -1] = b0v[:, 0]
-            b0v[-1, :] = b0v[0, :]
+            # take its (wrapped) east and north neighbours' west and south ones;
+            # on a 1D grid the east vertex of the lower row only
+            if S.is_1d(c):
+                b0v[0, -1] = b0v[0, 0]
+            else:
+                b0v[:, -1] = b0v[:, 0]
+                b0v[-1, :] = b0v[0, :]
     else:
         b0v = np.ascontiguousarray(b0v, dtype=np.float64)
         if b0v.shape != (c.NY + 1, c.NX + 1):

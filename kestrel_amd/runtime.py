@@ -166,6 +166,11 @@ class Stepper:
         bt = None if btv is None else np.ascontiguousarray(btv, dtype=np.float64)
         self._check(self.L.kst_set_state(self.h, *[_ptr(a) for a in arrs], _ptr(bt),
                                          ts.ctypes.data_as(_u8p), float(t), _ptr(hm)))
+        if getattr(init, "psimax0", None) is not None:
+            # a 1D cap also paints the running maximum of the solids fraction
+            mx = self.maxima()
+            mx[8] = np.ascontiguousarray(init.psimax0, dtype=np.float64)
+            self.set_maxima(mx)
 
     def integrate_to(self, tend: float) -> dict:
         st = KstStats()

@@ -96,11 +96,23 @@ def _kahan(*xs):
     return s
 
 
+def is_1d(cfg: KstConfig) -> bool:
+    """isOneD (src/DomainSettings.f90:216-220): one row of cells."""
+    return cfg.ny_tiles * cfg.ny_per_tile == 1
+
+
 def cell_topography(cfg: KstConfig, b0v: np.ndarray, btv: np.ndarray | None = None):
     """Cell-centred b0, bt, db/dx, db/dy
-    (ComputeCellCentredTopographicData, src/MorphodynamicRHS.f90:308-360)."""
+    (ComputeCellCentredTopographicData, src/MorphodynamicRHS.f90:308-360; on a
+    1D grid two-vertex forms along the lower vertex row, db/dy zero)."""
     if btv is None:
         btv = np.zeros_like(b0v)
+    if is_1d(cfg):
+        b, t = b0v[0], btv[0]
+        b0c = 0.5 * (b[:-1] + b[1:])
+        btc = 0.5 * (t[:-1] + t[1:])
+        dbdx = (1.0 / cfg.dx) * _kahan(b[1:], t[1:], -b[:-1], -t[:-1])
+        return b0c[None, :], btc[None, :], dbdx[None, :], np.zeros((1, cfg.NX))
     b00, b10, b01, b11 = b0v[:-1, :-1], b0v[:-1, 1:], b0v[1:, :-1], b0v[1:, 1:]
     t00, t10, t01, t11 = btv[:-1, :-1], btv[:-1, 1:], btv[1:, :-1], btv[1:, 1:]
     b0c = 0.25 * _kahan(b00, b10, b01, b11)
@@ -162,6 +174,9 @@ class InitialState:
     Hnpsi: np.ndarray
     tile_state: np.ndarray
     hnmax0: np.ndarray
+    # the running maximum of the solids fraction a 1D cap paints
+    # (src/SetSources.f90:131-160); None where nothing is painted
+    psimax0: np.ndarray | None = None
 
 
 def _ghost_ring(cfg: KstConfig, active: np.ndarray) -> np.ndarray:
@@ -174,6 +189,8 @@ def _ghost_ring(cfg: KstConfig, active: np.ndarray) -> np.ndarray:
     for dj in (-1, 0, 1):
         for di in (-1, 0, 1):
             if periodic and di != 0 and dj != 0:
+                continue
+            if is_1d(cfg) and dj != 0:   # W and E only on a 1D grid
                 continue
             if periodic:
                 sh = np.roll(np.roll(a, dj, axis=0), di, axis=1)
@@ -215,12 +232,16 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
                   sources: list[FluxSource] = (), cubes: list[Cube] = ()) -> InitialState:
     """Paint caps and cubes and count flux-source cells as LoadSourceConditions
     does (src/SetSources.f90:228-303 caps, 305-355 cubes, 357-375 flux
-    sources).  Only a tile that AddTile activates is painted: with dirichlet
-    or sponge boundaries a tile on the domain edge is silently left inactive
+    sources; on a 1D grid :95-240: distances along x only, no y velocity).
+    Only a tile that AddTile activates is painted: with dirichlet or sponge
+    boundaries a tile on the domain edge is silently left inactive
     (src/UpdateTiles.f90:61-69)."""
     NX, NY = cfg.NX, cfg.NY
+    oned = is_1d(cfg)
     x, y = cell_centres(cfg)
     X, Y = np.meshgrid(x, y)
+    if oned:
+        Y = None
     b0c, btc, bx, by = cell_topography(cfg, b0v)
     gam = gamma(cfg, bx, by)
     tiles = _tile_of_cells(cfg)
@@ -230,29 +251,36 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
     rhv = np.zeros((NY, NX))
     hnpsi = np.zeros((NY, NX))
     hn = np.zeros((NY, NX))
+    psimax = np.zeros((NY, NX))
     # tiles AddTile may activate (the others stay off and are not painted)
     allowed = np.ones(cfg.ntiles, dtype=bool)
     if cfg.bc in (BC_DIRICHLET, BC_SPONGE):
         allowed &= ~_edge_tiles(cfg)
     cell_ok = allowed[tiles]
+    def dist2(px, py):
+        if oned:
+            return (X - px) * (X - px)
+        return (X - px) * (X - px) + (Y - py) * (Y - py)
+
     for cap in caps:
-        R2 = (X - cap.x) * (X - cap.x) + (Y - cap.y) * (Y - cap.y)
+        R2 = dist2(cap.x, cap.y)
         inside = (R2 <= cap.radius * cap.radius)
         active[np.unique(tiles[inside & cell_ok])] = 1
         inside = inside & cell_ok
         rho = cfg.rhow + (cfg.rhos - cfg.rhow) * cap.psi
+        cv = 0.0 if oned else cap.v
         if cap.shape == "flat":
             inc_w = cap.height / gam
             inc_h = np.full_like(gam, cap.height)
             inc_rhu = np.full_like(gam, rho * cap.height * cap.u)
-            inc_rhv = np.full_like(gam, rho * cap.height * cap.v)
+            inc_rhv = np.full_like(gam, rho * cap.height * cv)
             inc_psi = np.full_like(gam, cap.psi * cap.height)
         elif cap.shape == "para":
             prof = 1.0 - R2 / cap.radius / cap.radius
             inc_w = cap.height * prof / gam
             inc_h = cap.height * prof
             inc_rhu = np.full_like(gam, rho * cap.height * cap.u)
-            inc_rhv = np.full_like(gam, rho * cap.height * cap.v)
+            inc_rhv = np.full_like(gam, rho * cap.height * cv)
             inc_psi = cap.psi * cap.height * prof
         elif cap.shape == "level":
             hp = cap.height - b0c
@@ -262,7 +290,7 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
             inc_w = hp
             inc_h = Hn
             inc_rhu = rho * Hn * cap.u
-            inc_rhv = rho * Hn * cap.v
+            inc_rhv = rho * Hn * cv
             inc_psi = cap.psi * Hn
         else:
             raise ValueError(f"unknown cap shape {cap.shape!r}")
@@ -271,8 +299,12 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
         rhu = np.where(inside, rhu + inc_rhu, rhu)
         rhv = np.where(inside, rhv + inc_rhv, rhv)
         hnpsi = np.where(inside, hnpsi + inc_psi, hnpsi)
+        if oned:
+            psimax = np.where(inside, psimax + cap.psi, psimax)
     for cube in cubes:
-        inside = ((np.abs(X - cube.x) <= 0.5 * cube.length) & (np.abs(Y - cube.y) <= 0.5 * cube.width))
+        inside = np.abs(X - cube.x) <= 0.5 * cube.length
+        if not oned:
+            inside = inside & (np.abs(Y - cube.y) <= 0.5 * cube.width)
         active[np.unique(tiles[inside & cell_ok])] = 1
         inside = inside & cell_ok
         if cube.shape == "level":
@@ -290,10 +322,11 @@ def initial_state(cfg: KstConfig, b0v: np.ndarray, caps: list[Cap] = (),
         rhv = np.where(inside, rhv + 0.0, rhv)
         hnpsi = np.where(inside, hnpsi + inc_psi, hnpsi)
     for src in sources:
-        R2 = (X - src.x) * (X - src.x) + (Y - src.y) * (Y - src.y)
+        R2 = dist2(src.x, src.y)
         inside = R2 <= src.radius * src.radius
         src.num_cells = int(np.count_nonzero(inside))
         active[np.unique(tiles[inside & cell_ok])] = 1
     _check_halt(cfg, active)
     state = _ghost_ring(cfg, active)
-    return InitialState(np.ascontiguousarray(w), rhu, rhv, hnpsi, state, hn)
+    return InitialState(np.ascontiguousarray(w), rhu, rhv, hnpsi, state, hn,
+                        psimax if oned and caps else None)

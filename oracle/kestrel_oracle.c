@@ -94,6 +94,7 @@ struct ko_sim {
     double** src_arrays;
     int nx, ny, ntx, nty, ntiles, NX, NY;
     int NC, NFX, NFY, NV;
+    int oned;                 /* isOneD: nYtiles * nYpertile = 1 (src/DomainSettings.f90:216-220) */
     double dxr, dyr;
     tmeta* meta;
     tdata* cont[NCONT];
@@ -190,11 +191,13 @@ static double kahan_sum(const double* x, int n) {
     return s;
 }
 
-/* FlowSquaredSpeedSlopeAligned (src/Closures.f90:178-205), 2D. */
+/* FlowSquaredSpeedSlopeAligned (src/Closures.f90:178-205); on a 1D grid the
+ * x part only. */
 static double speed2(const kst_config* c, double u, double v, double bx, double by) {
     double m;
     if (c->geometric_factors) m = u * u * (1.0 + bx * bx);
     else m = u * u;
+    if (c->ny_tiles * c->ny_per_tile == 1) return m;
     if (c->geometric_factors) m = m + v * v * (1.0 + by * by) + 2.0 * bx * by * u * v;
     else m = m + v * v;
     return m;
@@ -509,9 +512,20 @@ static double gam_cell(ko_sim* s, tdata* T, int i, int j) {
 /* Topographic data (src/MorphodynamicRHS.f90:308-578, src/dem.f90:360-393) */
 /* ------------------------------------------------------------------ */
 
-/* ComputeCellCentredTopographicData_tileID (src/MorphodynamicRHS.f90:308-360), 2D. */
+/* ComputeCellCentredTopographicData_tileID (src/MorphodynamicRHS.f90:308-360);
+ * on a 1D grid the two-vertex forms of the lower vertex row (:348-357), dbdy
+ * left as it is. */
 static void cell_topo(ko_sim* s, tdata* T) {
     int nx = s->nx, ny = s->ny;
+    if (s->oned) {
+        for (int i = 0; i < nx; ++i) {
+            UF(s, T, B0, i, 0) = 0.5 * (B0V(s, T, i, 0) + B0V(s, T, i + 1, 0));
+            UF(s, T, BT, i, 0) = 0.5 * (BTV(s, T, i, 0) + BTV(s, T, i + 1, 0));
+            double a[4] = {B0V(s, T, i + 1, 0), BTV(s, T, i + 1, 0), -B0V(s, T, i, 0), -BTV(s, T, i, 0)};
+            UF(s, T, DBDX, i, 0) = s->dxr * kahan_sum(a, 4);
+        }
+        return;
+    }
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             double a[8];
@@ -544,6 +558,13 @@ static void get_heights(ko_sim* s, int k, tdata* T) {
     for (int j = 0; j <= ny; ++j)
         for (int i = 0; i <= nx; ++i)
             B0V(s, T, i, j) = s->gb0[(size_t)(gj * ny + j) * (s->NX + 1) + gi * nx + i];
+    if (s->oned) {   /* the lower vertex row (src/dem.f90:387-390) */
+        for (int i = 0; i <= nx; ++i) {
+            PX(s, T, B0, i, 0) = B0V(s, T, i, 0);
+            MX(s, T, B0, i, 0) = B0V(s, T, i, 0);
+        }
+        return;
+    }
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i <= nx; ++i) {
             double v = 0.5 * (B0V(s, T, i, j) + B0V(s, T, i, j + 1));
@@ -558,15 +579,32 @@ static void get_heights(ko_sim* s, int k, tdata* T) {
         }
 }
 
-/* ComputeInterfacialTopographicData (src/MorphodynamicRHS.f90:419-578), 2D. */
+/* ComputeInterfacialTopographicData (src/MorphodynamicRHS.f90:419-578); on a
+ * 1D grid (:545-577) the face bt is the vertex's and dbdx a two-vertex
+ * central difference of the lower row. */
 static void face_topo(ko_sim* s, tdata* C, int k) {
     int nx = s->nx, ny = s->ny;
     tdata* T = &C[k];
     tdata* Wt = &C[s->meta[k].nb[0]];
     tdata* Et = &C[s->meta[k].nb[1]];
+    double a[8];
+    if (s->oned) {
+        for (int i = 0; i <= nx; ++i) {
+            tdata* L = (i == 0) ? Wt : T;          /* vertex i - 1 */
+            int il = (i == 0) ? nx - 1 : i - 1;
+            tdata* R = (i == nx) ? Et : T;         /* vertex i + 1 */
+            int ir = (i == nx) ? 1 : i + 1;
+            double bt = BTV(s, T, i, 0);
+            a[0] = B0V(s, R, ir, 0); a[1] = BTV(s, R, ir, 0);
+            a[2] = -B0V(s, L, il, 0); a[3] = -BTV(s, L, il, 0);
+            double dbdx = 0.5 * s->dxr * kahan_sum(a, 4);
+            PX(s, T, BT, i, 0) = bt; MX(s, T, BT, i, 0) = bt;
+            PX(s, T, DBDX, i, 0) = dbdx; MX(s, T, DBDX, i, 0) = dbdx;
+        }
+        return;
+    }
     tdata* Nt = &C[s->meta[k].nb[2]];
     tdata* St = &C[s->meta[k].nb[3]];
-    double a[8];
     for (int i = 1; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             a[0] = B0V(s, T, i + 1, j); a[1] = BTV(s, T, i + 1, j);
@@ -696,18 +734,20 @@ static void limited_derivs(ko_sim* s, tdata* C, const int* vars, int nv, int k) 
                                                      UF(s, T, d, i, j) - UF(s, T, d, i, j - 1));
             }
     int nb;
-    for (int i = 0; i < nx; ++i)
-        for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, 0, k, 'S');
-    nb = s->meta[k].nb[3];
-    if (tile_on(s, nb))
+    if (!s->oned) {   /* a 1D grid has no y slopes (they stay zero) */
         for (int i = 0; i < nx; ++i)
-            for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, ny - 1, nb, 'N');
-    for (int i = 0; i < nx; ++i)
-        for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, ny - 1, k, 'N');
-    nb = s->meta[k].nb[2];
-    if (tile_on(s, nb))
+            for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, 0, k, 'S');
+        nb = s->meta[k].nb[3];
+        if (tile_on(s, nb))
+            for (int i = 0; i < nx; ++i)
+                for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, ny - 1, nb, 'N');
         for (int i = 0; i < nx; ++i)
-            for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, 0, nb, 'S');
+            for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, ny - 1, k, 'N');
+        nb = s->meta[k].nb[2];
+        if (tile_on(s, nb))
+            for (int i = 0; i < nx; ++i)
+                for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], i, 0, nb, 'S');
+    }
     for (int j = 0; j < ny; ++j)
         for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], 0, j, k, 'W');
     nb = s->meta[k].nb[0];
@@ -722,7 +762,8 @@ static void limited_derivs(ko_sim* s, tdata* C, const int* vars, int nv, int k) 
             for (int q = 0; q < nv; ++q) lim_bdry(s, C, vars[q], 0, j, nb, 'W');
 }
 
-/* Reconstruct (src/HydraulicRHS.f90:397-552), 2D. */
+/* Reconstruct (src/HydraulicRHS.f90:397-552); on a 1D grid only the x faces,
+ * and their rhoHnv keeps its reconstruction (:536-551 are 2D only). */
 static void reconstruct(ko_sim* s, tdata* C, int desing, int k) {
     const kst_config* c = &s->cfg;
     int nx = s->nx, ny = s->ny;
@@ -731,6 +772,7 @@ static void reconstruct(ko_sim* s, tdata* C, int desing, int k) {
     tdata* Et = &C[s->meta[k].nb[1]];
     tdata* Nt = &C[s->meta[k].nb[2]];
     tdata* St = &C[s->meta[k].nb[3]];
+    const int oned = s->oned;
     const int* dims = desing ? IDESING : IFLUX;
     int nd = desing ? 5 : 4;
     double hdx = 0.5, dx = c->dx, dy = c->dy;
@@ -746,6 +788,7 @@ static void reconstruct(ko_sim* s, tdata* C, int desing, int k) {
             PX(s, T, d, nx, j) = UF(s, Et, d, 0, j) - LX(s, Et, d, 0, j) * hdx * dx;
             MX(s, T, d, nx, j) = UF(s, T, d, nx - 1, j) + LX(s, T, d, nx - 1, j) * hdx * dx;
         }
+        if (oned) continue;
         for (int i = 0; i < nx; ++i) {
             PY(s, T, d, i, 0) = UF(s, T, d, i, 0) - LY(s, T, d, i, 0) * hdx * dy;
             MY(s, T, d, i, 0) = UF(s, St, d, i, ny - 1) + LY(s, St, d, i, ny - 1) * hdx * dy;
@@ -765,6 +808,13 @@ static void reconstruct(ko_sim* s, tdata* C, int desing, int k) {
             gam = gamma_of(c, MX(s, T, DBDX, i, j), MX(s, T, DBDY, i, j));
             MX(s, T, HN, i, j) = compute_hn(MX(s, T, W, i, j), MX(s, T, B0, i, j), MX(s, T, BT, i, j), gam);
         }
+    if (oned) {
+        for (int f = 0; f < s->NFX; ++f) {
+            T->upx[RHU * s->NFX + f] = T->upx[RHO * s->NFX + f] * T->upx[HN * s->NFX + f] * T->upx[UU * s->NFX + f];
+            T->umx[RHU * s->NFX + f] = T->umx[RHO * s->NFX + f] * T->umx[HN * s->NFX + f] * T->umx[UU * s->NFX + f];
+        }
+        return;
+    }
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j <= ny; ++j) {
             double gam = gamma_of(c, PY(s, T, DBDX, i, j), PY(s, T, DBDY, i, j));
@@ -798,15 +848,23 @@ static double interp_b(double b00, double b01, double bt0, double bt1) {
     return b * 0.5;
 }
 
-/* CorrectSlopes (src/HydraulicRHS.f90:560-736), 2D. */
+/* The face bed CorrectSlopes compares with: InterpolateB of the face's two
+ * vertices, or on a 1D grid b0 + bt of its lower vertex
+ * (src/HydraulicRHS.f90:582-595). */
+static double face_bed_x(ko_sim* s, tdata* T, int i, int j) {
+    if (s->oned) return B0V(s, T, i, 0) + BTV(s, T, i, 0);
+    return interp_b(B0V(s, T, i, j), B0V(s, T, i, j + 1), BTV(s, T, i, j), BTV(s, T, i, j + 1));
+}
+
+/* CorrectSlopes (src/HydraulicRHS.f90:560-736); a 1D grid has x faces only. */
 static void correct_slopes(ko_sim* s, tdata* C, int k) {
     int nx = s->nx, ny = s->ny;
     tdata* T = &C[k];
     for (int j = 0; j < ny; ++j) {
         int pv = s->meta[k].nb[0];
         tdata* P = &C[pv];
-        double bl = interp_b(B0V(s, P, nx - 1, j), B0V(s, P, nx - 1, j + 1), BTV(s, P, nx - 1, j), BTV(s, P, nx - 1, j + 1));
-        double br = interp_b(B0V(s, P, nx, j), B0V(s, P, nx, j + 1), BTV(s, P, nx, j), BTV(s, P, nx, j + 1));
+        double bl = face_bed_x(s, P, nx - 1, j);
+        double br = face_bed_x(s, P, nx, j);
         if ((MX(s, P, W, nx, j) < br) || (PX(s, P, W, nx - 1, j) < bl)) {
             MX(s, T, W, 0, j) = UF(s, P, W, nx - 1, j) + 0.5 * (br - bl);
         } else if (MX(s, T, W, 0, j) != MX(s, P, W, nx, j) && !s->meta[pv].ghost) {
@@ -815,8 +873,8 @@ static void correct_slopes(ko_sim* s, tdata* C, int k) {
         if (MX(s, T, HNPSI, 0, j) < 0.0) MX(s, T, HNPSI, 0, j) = UF(s, P, HNPSI, nx - 1, j);
 
         for (int i = 0; i < nx; ++i) {
-            bl = interp_b(B0V(s, T, i, j), B0V(s, T, i, j + 1), BTV(s, T, i, j), BTV(s, T, i, j + 1));
-            br = interp_b(B0V(s, T, i + 1, j), B0V(s, T, i + 1, j + 1), BTV(s, T, i + 1, j), BTV(s, T, i + 1, j + 1));
+            bl = face_bed_x(s, T, i, j);
+            br = face_bed_x(s, T, i + 1, j);
             if ((MX(s, T, W, i + 1, j) < br) || (PX(s, T, W, i, j) < bl)) {
                 MX(s, T, W, i + 1, j) = UF(s, T, W, i, j) + 0.5 * (br - bl);
                 PX(s, T, W, i, j) = UF(s, T, W, i, j) + 0.5 * (bl - br);
@@ -829,8 +887,8 @@ static void correct_slopes(ko_sim* s, tdata* C, int k) {
 
         int nxt = s->meta[k].nb[1];
         tdata* N = &C[nxt];
-        bl = interp_b(B0V(s, N, 0, j), B0V(s, N, 0, j + 1), BTV(s, N, 0, j), BTV(s, N, 0, j + 1));
-        br = interp_b(B0V(s, N, 1, j), B0V(s, N, 1, j + 1), BTV(s, N, 1, j), BTV(s, N, 1, j + 1));
+        bl = face_bed_x(s, N, 0, j);
+        br = face_bed_x(s, N, 1, j);
         if ((MX(s, N, W, 1, j) < br) || (PX(s, N, W, 0, j) < bl)) {
             PX(s, T, W, nx, j) = UF(s, N, W, 0, j) + 0.5 * (bl - br);
         } else if (PX(s, T, W, nx, j) != PX(s, N, W, 0, j) && !s->meta[nxt].ghost) {
@@ -838,6 +896,7 @@ static void correct_slopes(ko_sim* s, tdata* C, int k) {
         }
         if (PX(s, T, HNPSI, nx, j) < 0.0) PX(s, T, HNPSI, nx, j) = UF(s, N, HNPSI, 0, j);
     }
+    if (s->oned) return;
     for (int i = 0; i < nx; ++i) {
         int pv = s->meta[k].nb[3];
         tdata* P = &C[pv];
@@ -897,8 +956,10 @@ static void desingularise(ko_sim* s, tdata* C, int k, int velocities) {
             UF(s, T, RHO, i, j) = rho;
             if (velocities) {
                 UF(s, T, UU, i, j) = 2.0 * hn * rhu / (hn * hn + fmax_(hn * hn, eps * eps)) / rho;
-                double rhv = UF(s, T, RHV, i, j);
-                UF(s, T, VV, i, j) = 2.0 * hn * rhv / (hn * hn + fmax_(hn * hn, eps * eps)) / rho;
+                if (!s->oned) {   /* a 1D grid keeps v (src/HydraulicRHS.f90:870-874) */
+                    double rhv = UF(s, T, RHV, i, j);
+                    UF(s, T, VV, i, j) = 2.0 * hn * rhv / (hn * hn + fmax_(hn * hn, eps * eps)) / rho;
+                }
             }
         }
 }
@@ -998,6 +1059,7 @@ static double calc_fluxes(ko_sim* s, tdata* C, int k) {
 #undef FP
 #undef FM
         }
+    if (s->oned) return unit;   /* no y fluxes on a 1D grid (src/HydraulicRHS.f90:1075) */
     for (int j = 0; j <= ny; ++j)
         for (int i = 0; i < nx; ++i) {
             int f = FYI(s, i, j);
@@ -1074,7 +1136,7 @@ static double calc_fluxes(ko_sim* s, tdata* C, int k) {
     return unit;
 }
 
-/* ExplicitSourceTerms (src/Equations.f90:456-620), 2D. */
+/* ExplicitSourceTerms (src/Equations.f90:456-620). */
 static void explicit_sources(ko_sim* s, double t, double x, double y, const double* uv, int has_src,
                              double* st) {
     const kst_config* c = &s->cfg;
@@ -1111,8 +1173,13 @@ static void explicit_sources(ko_sim* s, double t, double x, double y, const doub
                     }
                 }
             }
-            Qf = Qf / (double)fs->num_cells / c->dx / c->dy;
-            psiQf = psiQf / (double)fs->num_cells / c->dx / c->dy;
+            if (s->oned) {   /* per unit length on a 1D grid (src/Equations.f90:582-585) */
+                Qf = Qf / (double)fs->num_cells / c->dx;
+                psiQf = psiQf / (double)fs->num_cells / c->dx;
+            } else {
+                Qf = Qf / (double)fs->num_cells / c->dx / c->dy;
+                psiQf = psiQf / (double)fs->num_cells / c->dx / c->dy;
+            }
             qsum = qsum + Qf;
             psisum = psisum + psiQf;
         }
@@ -1129,12 +1196,69 @@ static void explicit_sources(ko_sim* s, double t, double x, double y, const doub
     st[RHV] = st[RHV] - c->g * uv[RHO] * hpg * uv[DBDY];
 }
 
+/* The implicit drag terms of a cell (ImplicitSourceTerms,
+ * src/Equations.f90:627-658) and its explicit source terms. */
+static void cell_sources(ko_sim* s, tdata* T, int k, int i, int j, double t, const double* stf) {
+    const kst_config* c = &s->cfg;
+    double uv[NF], ste[4];
+    cell_vec(s, T, i, j, uv);
+    explicit_sources(s, t, s->meta[k].x[i], s->meta[k].y[j], uv, s->meta[k].src, ste);
+    for (int q = 0; q < 4; ++q) T->dte[q * s->NC + CI(s, i, j)] = stf[q] + ste[q];
+    cstate cs = {uv[HN], uv[UU], uv[VV], uv[PSI], uv[BT], uv[DBDX], uv[DBDY]};
+    double fric = drag(c, &cs);
+    double sti[4] = {0, 0, 0, 0};
+    if (uv[HN] > c->height_threshold) {
+        double modu = sqrt(speed2(c, uv[UU], uv[VV], uv[DBDX], uv[DBDY]));
+        if (modu > 1.0e-8) {
+            double hr = 1.0 / uv[HN];
+            sti[RHU] = -(fric * hr / modu);
+            sti[RHV] = -(fric * hr / modu);
+        }
+    }
+    for (int q = 0; q < 4; ++q) T->dti[q * s->NC + CI(s, i, j)] = sti[q];
+}
+
+/* ConstructHydraulicRHS on a 1D grid (src/HydraulicRHS.f90:1276-1302): x
+ * fluxes only, the hydrostatic flux difference divided by gamma; the first
+ * (plain-sum) expression of the momentum terms is overwritten by the Kahan
+ * sum. */
+static void construct_rhs_1d(ko_sim* s, tdata* C, int k, double t) {
+    tdata* T = &C[k];
+    for (int i = 0; i < s->nx; ++i) {
+        double gam = gam_cell(s, T, i, 0);
+        double stf[4];
+        int fx0 = FXI(s, i, 0), fx1 = FXI(s, i + 1, 0);
+        for (int q = 0; q < 4; ++q) {
+            int d = IFLUX[q];
+            const double* hX = &T->hx[d * s->NFX];
+            const double* gX = &T->gx[d * s->NFX];
+            const double* pX = &T->px[d * s->NFX];
+            if (d == W) {
+                stf[q] = (hX[fx0] - hX[fx1]) * s->dxr / (gam * gam);
+            } else if (d == HNPSI) {
+                stf[q] = (hX[fx0] - hX[fx1]) * s->dxr / gam;
+            } else {
+                double a[3];
+                a[0] = hX[fx0] - hX[fx1];
+                a[1] = (gX[fx0] - gX[fx1]) / gam;
+                a[2] = pX[fx1] - pX[fx0];
+                stf[q] = kahan_sum(a, 3) * s->dxr;
+            }
+        }
+        cell_sources(s, T, k, i, 0, t, stf);
+    }
+}
+
 /* ConstructHydraulicRHS (src/HydraulicRHS.f90:1190-1304) with
- * ImplicitSourceTerms (src/Equations.f90:627-658), 2D. */
+ * ImplicitSourceTerms (src/Equations.f90:627-658). */
 static void construct_rhs(ko_sim* s, tdata* C, int k, double t) {
     const kst_config* c = &s->cfg;
     tdata* T = &C[k];
     int nx = s->nx, ny = s->ny;
+    if (s->oned) {
+        construct_rhs_1d(s, C, k, t);
+        return;
+    }
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
             double gam = gam_cell(s, T, i, j);
@@ -1175,22 +1299,7 @@ static void construct_rhs(ko_sim* s, tdata* C, int k, double t) {
                     stf[q] = stf[q] + kahan_sum(a, 3) * s->dyr;
                 }
             }
-            double uv[NF], ste[4];
-            cell_vec(s, T, i, j, uv);
-            explicit_sources(s, t, s->meta[k].x[i], s->meta[k].y[j], uv, s->meta[k].src, ste);
-            for (int q = 0; q < 4; ++q) T->dte[q * s->NC + CI(s, i, j)] = stf[q] + ste[q];
-            cstate cs = {uv[HN], uv[UU], uv[VV], uv[PSI], uv[BT], uv[DBDX], uv[DBDY]};
-            double fric = drag(c, &cs);
-            double sti[4] = {0, 0, 0, 0};
-            if (uv[HN] > c->height_threshold) {
-                double modu = sqrt(speed2(c, uv[UU], uv[VV], bx, by));
-                if (modu > 1.0e-8) {
-                    double hr = 1.0 / uv[HN];
-                    sti[RHU] = -(fric * hr / modu);
-                    sti[RHV] = -(fric * hr / modu);
-                }
-            }
-            for (int q = 0; q < 4; ++q) T->dti[q * s->NC + CI(s, i, j)] = sti[q];
+            cell_sources(s, T, k, i, j, t, stf);
         }
 }
 
@@ -1329,6 +1438,7 @@ static void reconstruct_w_at_edges(ko_sim* s, int k) {
             MX(s, T, W, nx, j) = UF(s, T, W, nx - 1, j) + LX(s, T, W, nx - 1, j) * 0.5 * dx;
         }
     }
+    if (s->oned) return;   /* x edges only on a 1D grid (src/UpdateTiles.f90:718) */
     nb = s->meta[k].nb[3];
     if (in_bounds(s, nb) && s->meta[nb].on) {
         for (int i = 0; i < nx; ++i) lim_bdry(s, C, W, i, 0, k, 'S');
@@ -1382,15 +1492,17 @@ static void set_default_tile_data(ko_sim* s, int k) {
     }
 }
 
-/* AddGhostTiles (src/UpdateTiles.f90:389-481): the four neighbours and, but
- * with periodic boundaries, the four diagonal tiles.  A new ghost gets
+/* AddGhostTiles (src/UpdateTiles.f90:389-481): the four neighbours (W and E
+ * on a 1D grid) and, but with periodic boundaries, the four diagonal tiles.  A new ghost gets
  * UpdateGhostTileBCs's data (:571-585): on the domain edge
  * SetDomainBoundaryData's -- none with periodic boundaries (AllocateU's zeros
  * and rho = rhow stay), the boundary data with Dirichlet ones, else the
  * default dry data -- and off the edge the default dry data. */
 static int add_ghost_tiles(ko_sim* s, int k) {
     int newg[8], ng = 0;
-    const int nq = s->cfg.bc == KST_BC_PERIODIC ? 4 : 8;
+    /* grid%dim*2 + 4*(grid%dim - 1) candidates, the corners skipped with
+     * periodic boundaries: W, E on a 1D grid */
+    const int nq = s->oned ? 2 : (s->cfg.bc == KST_BC_PERIODIC ? 4 : 8);
     for (int q = 0; q < nq; ++q) {
         int tile = (q < 4) ? s->meta[k].nb[q] : s->meta[k].cr[q - 4];
         if (!in_bounds(s, tile)) return fail(s, KST_ERR_HALT, "ghost tile outside the domain");
@@ -1483,8 +1595,8 @@ static int check_near_boundaries(ko_sim* s) {
     int nx = s->nx, ny = s->ny, tb = c->tile_buffer;
     double thr = c->height_threshold;
     int size, rc;
-    /* North */
-    size = s->nactive;
+    /* North (and South) only on a 2D grid (src/TimeStepper.f90:935-938) */
+    size = s->oned ? 0 : s->nactive;
     for (int tt = 0; tt < size; ++tt) {
         int k = s->active[tt], n = s->meta[k].nb[2];
         if (n > -1 && !(in_bounds(s, n) && s->meta[n].on)) {
@@ -1497,7 +1609,7 @@ static int check_near_boundaries(ko_sim* s) {
         }
     }
     /* South */
-    size = s->nactive;
+    size = s->oned ? 0 : s->nactive;
     for (int tt = 0; tt < size; ++tt) {
         int k = s->active[tt], n = s->meta[k].nb[3];
         if (n > -1 && !(in_bounds(s, n) && s->meta[n].on)) {
@@ -1632,7 +1744,7 @@ static void update_maxima(ko_sim* s, int k, double t) {
 static void apply_sponge(ko_sim* s, int k, tdata* I) {
     if (s->cfg.bc != KST_BC_SPONGE) return;
     const tmeta* m = &s->meta[k];
-    if (!(m->nb[2] == -1 || m->nb[3] == -1 || m->nb[1] == -1 || m->nb[0] == -1)) return;
+    if (!((!s->oned && (m->nb[2] == -1 || m->nb[3] == -1)) || m->nb[1] == -1 || m->nb[0] == -1)) return;
     const int NC = s->NC;
     for (int v = 0; v < 4; ++v)
         for (int i = 0; i < NC; ++i)
@@ -1765,10 +1877,53 @@ static double kahan3(double a, double b, double c) {
 
 /* BtSourceTerm (src/MorphodynamicRHS.f90:154-305), 2D, including the
  * three-term KahanSums of the west edge (:210) and the NE-corner dbdx (:266). */
+/* BtSourceTerm on a 1D grid (src/MorphodynamicRHS.f90:272-302): two-cell
+ * averages along the row; at a tile edge without an active neighbour the
+ * cell's own half. */
+static void bt_source_1d(ko_sim* s, tdata* C) {
+    const kst_config* c = &s->cfg;
+    int nx = s->nx;
+    double psib = 1.0 - c->bed_porosity;
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q];
+        tdata* T = &C[k];
+        tmeta* m = &s->meta[k];
+        for (int i = 1; i < nx; ++i) {
+            double bx = 0.5 * (UF(s, T, DBDX, i - 1, 0) + UF(s, T, DBDX, i, 0));
+            double gam = gamma_of(c, bx, 0.0);
+            DTBT(s, T, i, 0) = -0.5 * gam * (EMD(s, T, i - 1, 0) + EMD(s, T, i, 0)) / psib;
+        }
+        int tw = m->nb[0];
+        if (tile_on(s, tw)) {
+            double bx = 0.5 * (UF(s, &C[tw], DBDX, nx - 1, 0) + UF(s, T, DBDX, 0, 0));
+            double gam = gamma_of(c, bx, 0.0);
+            DTBT(s, T, 0, 0) = -0.5 * gam * (EMD(s, &C[tw], nx - 1, 0) + EMD(s, T, 0, 0)) / psib;
+        } else {
+            double bx = 0.5 * UF(s, T, DBDX, 0, 0);
+            double gam = gamma_of(c, bx, 0.0);
+            DTBT(s, T, 0, 0) = -0.5 * gam * EMD(s, T, 0, 0) / psib;
+        }
+        int te = m->nb[1];
+        if (tile_on(s, te)) {
+            double bx = 0.5 * (UF(s, T, DBDX, nx - 1, 0) + UF(s, &C[te], DBDX, 0, 0));
+            double gam = gamma_of(c, bx, 0.0);
+            DTBT(s, T, nx, 0) = -0.5 * gam * (EMD(s, T, nx - 1, 0) + EMD(s, &C[te], 0, 0)) / psib;
+        } else {
+            double bx = 0.5 * UF(s, T, DBDX, nx - 1, 0);
+            double gam = gamma_of(c, bx, 0.0);
+            DTBT(s, T, nx, 0) = -0.5 * gam * EMD(s, T, nx - 1, 0) / psib;
+        }
+    }
+}
+
 static void bt_source(ko_sim* s, tdata* C) {
     const kst_config* c = &s->cfg;
     int nx = s->nx, ny = s->ny;
     double psib = 1.0 - c->bed_porosity;
+    if (s->oned) {
+        bt_source_1d(s, C);
+        return;
+    }
     for (int q = 0; q < s->nactive; ++q) {
         int k = s->active[q];
         tdata* T = &C[k];
@@ -1850,8 +2005,12 @@ static void morpho_rhs(ko_sim* s, tdata* C) {
             for (int j = 0; j < ny; ++j) {
                 double hW = (i == 0) ? UF(s, &C[m->nb[0]], HN, nx - 1, j) : UF(s, T, HN, i - 1, j);
                 double hE = (i == nx - 1) ? UF(s, &C[m->nb[1]], HN, 0, j) : UF(s, T, HN, i + 1, j);
-                double hS = (j == 0) ? UF(s, &C[m->nb[3]], HN, i, ny - 1) : UF(s, T, HN, i, j - 1);
-                double hN = (j == ny - 1) ? UF(s, &C[m->nb[2]], HN, i, 0) : UF(s, T, HN, i, j + 1);
+                /* the S and N neighbours only on a 2D grid (src/MorphodynamicRHS.f90:113-135) */
+                double hS = eps, hN = eps;
+                if (!s->oned) {
+                    hS = (j == 0) ? UF(s, &C[m->nb[3]], HN, i, ny - 1) : UF(s, T, HN, i, j - 1);
+                    hN = (j == ny - 1) ? UF(s, &C[m->nb[2]], HN, i, 0) : UF(s, T, HN, i, j + 1);
+                }
                 if (UF(s, T, HN, i, j) < eps || hW < eps || hE < eps || hS < eps || hN < eps) {
                     for (int d = 0; d < 4; ++d) T->dte[d * s->NC + CI(s, i, j)] = 0.0;
                     EMD(s, T, i, j) = 0.0;
@@ -1867,8 +2026,14 @@ static void morpho_rhs(ko_sim* s, tdata* C) {
     bt_source(s, C);
 }
 
-/* ComputeCellCentredTopographicData_ij (src/MorphodynamicRHS.f90:363-416), 2D. */
+/* ComputeCellCentredTopographicData_ij (src/MorphodynamicRHS.f90:363-416). */
 static void cell_topo_ij(ko_sim* s, tdata* T, int i, int j) {
+    if (s->oned) {
+        UF(s, T, BT, i, 0) = 0.5 * (BTV(s, T, i, 0) + BTV(s, T, i + 1, 0));
+        double a[4] = {B0V(s, T, i + 1, 0), BTV(s, T, i + 1, 0), -B0V(s, T, i, 0), -BTV(s, T, i, 0)};
+        UF(s, T, DBDX, i, 0) = s->dxr * kahan_sum(a, 4);
+        return;
+    }
     double bt = 0.25 * kahan4(BTV(s, T, i, j), BTV(s, T, i + 1, j), BTV(s, T, i, j + 1), BTV(s, T, i + 1, j + 1));
     UF(s, T, BT, i, j) = bt;
     double a[8];
@@ -1962,7 +2127,8 @@ static int redistribute_cell(ko_sim* s, int k, int i, int j, double corr) {
             int nb[3], nn = 0;
             if (di[q] == 0) nb[nn++] = m->nb[0];
             else if (di[q] == nx) nb[nn++] = m->nb[1];
-            if (dj[q] == 0) {
+            if (s->oned) { /* no row copies on a 1D grid */ }
+            else if (dj[q] == 0) {
                 nb[nn++] = m->nb[3];
                 if (di[q] == 0) nb[nn++] = s->meta[m->nb[3]].nb[0];
                 else if (di[q] == nx) nb[nn++] = s->meta[m->nb[3]].nb[1];
@@ -1976,6 +2142,7 @@ static int redistribute_cell(ko_sim* s, int k, int i, int j, double corr) {
         }
         if (di[q] == 0) BTV(s, &I3[m->nb[0]], nx, dj[q]) = nv;
         else if (di[q] == nx) BTV(s, &I3[m->nb[1]], 0, dj[q]) = nv;
+        if (s->oned) continue;   /* src/Redistribute.f90:369 */
         if (dj[q] == 0) {
             BTV(s, &I3[m->nb[3]], di[q], ny) = nv;
             if (di[q] == 0) BTV(s, &I3[s->meta[m->nb[3]].nb[0]], nx, ny) = nv;
@@ -1991,6 +2158,30 @@ static int redistribute_cell(ko_sim* s, int k, int i, int j, double corr) {
     int ct, ci, cj;
     if (i == 0) { ct = s->meta[k].nb[0]; ci = nx - 1; }
     else { ct = k; ci = i - 1; }
+    if (s->oned) {
+        /* the three cells of the row, with the 1D order of the w update
+         * (src/Redistribute.f90:444-470) */
+        for (int kk = -1; kk <= 1; ++kk) {
+            tdata* A = &I3[ct];
+            tdata* Bo = &I0[ct];
+            if (!s->meta[ct].on) s->redist_ghost++;
+            cell_topo_ij(s, A, ci, 0);
+            double dbb = UF(s, A, BT, ci, 0) - UF(s, Bo, BT, ci, 0);
+            double go = gam_cell(s, Bo, ci, 0);
+            double gn = gam_cell(s, A, ci, 0);
+            double ho = UF(s, Bo, HN, ci, 0);
+            double w = -dbb / gn / gn;
+            w = w + ho * go / gn / gn;
+            w = w + UF(s, A, BT, ci, 0);
+            w = w + UF(s, A, B0, ci, 0);
+            UF(s, A, W, ci, 0) = w;
+            double hpo = UF(s, Bo, HNPSI, ci, 0);
+            UF(s, A, HNPSI, ci, 0) = hpo * go / gn - (1.0 - c->bed_porosity) * dbb / gn;
+            ci = ci + 1;
+            if (ci == nx) { ct = s->meta[ct].nb[1]; ci = 0; }
+        }
+        return 1;
+    }
     for (int kk = -1; kk <= 1; ++kk) {
         if (j == 0) { ct = s->meta[ct].nb[3]; cj = ny - 1; }
         else cj = j - 1;
@@ -2287,6 +2478,7 @@ ko_sim* ko_create(const kst_config* cfg) {
     s->nx = cfg->nx_per_tile; s->ny = cfg->ny_per_tile;
     s->ntx = cfg->nx_tiles; s->nty = cfg->ny_tiles;
     s->ntiles = s->ntx * s->nty;
+    s->oned = s->nty * s->ny == 1;
     s->NX = s->ntx * s->nx; s->NY = s->nty * s->ny;
     s->NC = s->nx * s->ny; s->NFX = (s->nx + 1) * s->ny; s->NFY = s->nx * (s->ny + 1);
     s->NV = (s->nx + 1) * (s->ny + 1);
@@ -2471,7 +2663,9 @@ int ko_set_state(ko_sim* s, const double* w, const double* rhu, const double* rh
                 for (int ii = 0; ii < nx; ++ii) {
                     double x = -0.5 * c->x_size + c->dx * (((gi + 1) - 1.0) * nx + ((ii + 1) - 0.5));
                     double y = -0.5 * c->y_size + c->dy * (((gj + 1) - 1.0) * ny + ((jj + 1) - 0.5));
-                    double R2 = (x - s->src[q].x) * (x - s->src[q].x) + (y - s->src[q].y) * (y - s->src[q].y);
+                    /* along x only on a 1D grid (src/SetSources.f90:219-231) */
+                    double R2 = (x - s->src[q].x) * (x - s->src[q].x);
+                    if (!s->oned) R2 = R2 + (y - s->src[q].y) * (y - s->src[q].y);
                     if (R2 <= s->src[q].radius * s->src[q].radius) s->meta[k].src = 1;
                 }
     }
@@ -2562,6 +2756,28 @@ int ko_get_maxima(ko_sim* s, double* const mx[11]) {
             scatter_field(s, k, srcs[a] + s->NC, mx[2 * a + 1]);
         }
         scatter_field(s, k, m->tfirst, mx[10]);
+    }
+    return 0;
+}
+
+/* The inverse of ko_get_maxima for the allocated tiles (maxima painted by
+ * the initial conditions, restarts). */
+int ko_set_maxima(ko_sim* s, const double* const mx[11]) {
+    int gi, gj, nx = s->nx, ny = s->ny;
+    for (int k = 0; k < s->ntiles; ++k) {
+        tmeta* m = &s->meta[k];
+        if (!m->hnmax) continue;
+        tile_coords(s, k, &gi, &gj);
+        double* dsts[5] = {m->hnmax, m->umax, m->emax, m->dmax, m->psimax};
+        for (int jj = 0; jj < ny; ++jj)
+            for (int ii = 0; ii < nx; ++ii) {
+                size_t g = (size_t)(gj * ny + jj) * s->NX + gi * nx + ii;
+                for (int a = 0; a < 5; ++a) {
+                    dsts[a][CI(s, ii, jj)] = mx[2 * a][g];
+                    dsts[a][s->NC + CI(s, ii, jj)] = mx[2 * a + 1][g];
+                }
+                m->tfirst[CI(s, ii, jj)] = mx[10][g];
+            }
     }
     return 0;
 }

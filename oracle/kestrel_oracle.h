@@ -56,6 +56,7 @@ int ko_get_maxima(ko_sim* s, double* const maxima[11]);
 /* Full tile-level dump for cross-checks: all 13 fields of every allocated
  * tile's u (tileContainer), cell-major global layout, NX*NY per field
  * (unallocated tiles give 0). */
+int ko_set_maxima(ko_sim* s, const double* const mx[11]);
 int ko_get_fields13(ko_sim* s, double* out13);
 /* CalculateVolume (src/Output.f90:617-735): the Volume.txt columns 2-7
  * (volume, total bed volume, total mass, bed mass, total solids mass, bed

@@ -64,9 +64,10 @@ def _load(name: str):
     for name in ["ko_set_topography", "ko_add_caps", "ko_add_flux_sources", "ko_finish_sources",
                  "ko_set_state", "ko_integrate_to", "ko_step_n", "ko_get_state",
                  "ko_get_maxima", "ko_get_fields13", "ko_get_ghost_slopes", "ko_rhs",
-                 "ko_tile_lists"]:
+                 "ko_tile_lists", "ko_set_maxima"]:
         getattr(L, name).restype = ctypes.c_int
     L.ko_set_topography.argtypes = [ctypes.c_void_p, _dp]
+    L.ko_set_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
     L.ko_add_caps.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
     L.ko_add_flux_sources.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
     L.ko_finish_sources.argtypes = [ctypes.c_void_p]
@@ -169,6 +170,15 @@ class Oracle:
             self.h, _ptr(np.ascontiguousarray(st.w)), _ptr(np.ascontiguousarray(st.rhoHnu)),
             _ptr(np.ascontiguousarray(st.rhoHnv)), _ptr(np.ascontiguousarray(st.Hnpsi)), _ptr(bt),
             ts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), float(t), _ptr(hm)))
+        if getattr(st, "psimax0", None) is not None:
+            mx = self.maxima()
+            mx[8] = np.ascontiguousarray(st.psimax0, dtype=np.float64)
+            self.set_maxima(mx)
+
+    def set_maxima(self, arrs):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+        ptrs = (_dp * 11)(*[_ptr(a) for a in arrs])
+        self._check(self.L.ko_set_maxima(self.h, ptrs))
 
     def integrate_to(self, tend):
         st = KstStats()

