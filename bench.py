@@ -72,6 +72,16 @@ PMC_TRAFFIC = {
     ("morpho_hills_1024", "stage_kernel<2>"): (1082.6, 181.9, 5795.84),
     ("morpho_hills_1024", "morpho_kernel<3>"): (498.0, 318.0, 5795.84),
 }
+# VALU wave-instructions per productive dispatch (SQ_INSTS_VALU, active cells
+# per launch of that run): profiles/stage_kernel_plain_cap_2048_counters.md.
+# Issue peak: one wave64 VALU instruction per SIMD every 4 cycles, 1024
+# SIMDs at 2.4 GHz.
+PMC_VALU = {
+    ("plain_cap_2048", "stage_kernel<2>"): (3.365e7, 1634048.0),
+    ("plain_cap_2048", "stage_kernel<3>"): (3.286e7, 1634048.0),
+    ("plain_cap_2048", "stage_kernel<4>"): (3.201e7, 1634048.0),
+}
+VALU_PEAK = 1024 * 2.4e9 / 4   # wave-instructions/s
 
 
 def cpu_model() -> str:
@@ -238,10 +248,16 @@ def roofline(wl_name: str, rows: list, cells: float) -> dict:
     if key in PMC_TRAFFIC:
         f, w, c0 = PMC_TRAFFIC[key]
         traffic = (f * PMC_FETCH_SCALE + w * PMC_WRITE_SCALE) * 1024 * cells / c0
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": top["kernel"], "algorithmic_bytes_per_launch": top["algorithmic_bytes"],
-            "mean_launch_us": top["mean_us"], "cells_per_launch": cells}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "kernel": top["kernel"], "algorithmic_bytes_per_launch": top["algorithmic_bytes"],
+           "mean_launch_us": top["mean_us"], "cells_per_launch": cells}
+    if key in PMC_VALU and s > 0:
+        n, c0 = PMC_VALU[key]
+        rate = n * cells / c0 / s
+        out["valu_issue"] = {"achieved": rate, "peak": VALU_PEAK, "unit": "wave-instructions/s",
+                             "frac": rate / VALU_PEAK}
+    return out
 
 
 def ensemble_leg(n: int = 64, steps: int = 100) -> dict:
