@@ -189,6 +189,18 @@ int kst_get_tile_bt(kst_handle* h, double* bt_tiles);
  * src/MorphodynamicRHS.f90:308-416). */
 int kst_get_cell_topography(kst_handle* h, double* b0c, double* btc, double* dbdx, double* dbdy);
 
+/* Cell-centred bt, db/dx, db/dy (NX*NY each, layout of kst_get_cell_topography)
+ * for the cells where cells[] (NX*NY) is nonzero: what LoadInitialCondition_txt
+ * reads from a solution file on a restart (src/Restart.f90:216-236) and keeps,
+ * since ActivateTile(restart=.true.) rebuilds only the face topography
+ * (src/UpdateTiles.f90:342-384).  The cell gamma, surface-gradient prefactors,
+ * the faces' CFL numerators, the activation flags and tileContainer's
+ * velocities are derived again from them.  Call after kst_set_state, which
+ * derives the cell topography from the vertex bt; the morphodynamic operator
+ * derives it from the vertices again at its first stage, as the reference's does. */
+int kst_set_cell_topography(kst_handle* h, const double* btc, const double* dbdx,
+                            const double* dbdy, const uint8_t* cells);
+
 /* Running maxima (src/TimeStepper.f90:1155-1303): 11 arrays of NX*NY doubles
  * in the order Hnmax, Hnmax_t, umax, umax_t, emax, emax_t, dmax, dmax_t,
  * psimax, psimax_t, tfirst. */

@@ -2264,6 +2264,14 @@ __global__ __launch_bounds__(256) void topo2_kernel(Dev d, Params p, Topo T) {
     tile_static_terms(d, p, T, blockIdx.x);
 }
 
+// Cell gamma of tile blockIdx.x from its cell slopes (after
+// kst_set_cell_topography replaced some of them); the form of tile_cell_topo.
+__global__ __launch_bounds__(256) void cell_gamma_kernel(Dev d, Params p, Topo T) {
+    const size_t base = (size_t)blockIdx.x * p.NC;
+    for (int c = threadIdx.x; c < p.NC; c += blockDim.x)
+        T.cgam[base + c] = gamma_of(p.geometric, T.cbx[base + c], p.oned ? 0.0 : T.cby[base + c]);
+}
+
 }  // namespace kst
 
 #include "kst_morpho.hpp"
@@ -3419,6 +3427,51 @@ int kst_get_tile_bt(kst_handle* h, double* bt_tiles) {
     const size_t n = (size_t)(h->nx + 1) * (h->ny + 1) * h->ntiles;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(bt_tiles, h->d.TA.vbt, n * 8, hipMemcpyDeviceToHost));
+    return KST_OK;
+}
+
+int kst_set_cell_topography(kst_handle* h, const double* btc, const double* dbdx, const double* dbdy,
+                            const uint8_t* cells) {
+    if (!h || !btc || !dbdx || !dbdy || !cells) return KST_ERR_CONFIG;
+    if (!h->have_state) return set_err(h, KST_ERR_STATE, "kst_set_state must come first");
+    const size_t FS = (size_t)h->ntiles * h->NC, NC = (size_t)h->NX * h->NY;
+    std::vector<double> g(NC), in(FS), cur(FS);
+    std::vector<double> mk(NC), mt(FS);
+    for (size_t q = 0; q < NC; ++q) mk[q] = cells[q] ? 1.0 : 0.0;
+    to_tiles(h, mk.data(), mt.data());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    Topo* sets[4] = {&h->d.TA, &h->d.TB, &h->d.TM[0], &h->d.TM[1]};
+    const int nsets = h->d.morpho ? 4 : 1;
+    const double* src[3] = {btc, dbdx, dbdy};
+    for (int f = 0; f < 3; ++f) {
+        double* dst = f == 0 ? h->d.TA.btc : (f == 1 ? h->d.TA.cbx : h->d.TA.cby);
+        HIPCHK(h, hipMemcpy(cur.data(), dst, FS * 8, hipMemcpyDeviceToHost));
+        to_tiles(h, src[f], in.data());
+        for (size_t q = 0; q < FS; ++q)
+            if (mt[q] != 0.0) cur[q] = in[q];
+        for (int k = 0; k < nsets; ++k) {
+            double* to = f == 0 ? sets[k]->btc : (f == 1 ? sets[k]->cbx : sets[k]->cby);
+            HIPCHK(h, hipMemcpy(to, cur.data(), FS * 8, hipMemcpyHostToDevice));
+        }
+    }
+    for (int k = 0; k < nsets; ++k)
+        hipLaunchKernelGGL(cell_gamma_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p, *sets[k]);
+    for (int k = 0; k < nsets; ++k)
+        hipLaunchKernelGGL(topo2_kernel, dim3(h->ntiles), dim3(256), 0, h->stream, h->d, h->p, *sets[k]);
+    // what kst_set_state derived from the cell topography: the activation
+    // flags of the active tiles and tileContainer's velocities
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    HIPCHK(h, hipMemset(h->d.eflags, 0, (size_t)h->ntiles * 4));
+    if (c.n_active > 0)
+        hipLaunchKernelGGL(init_flags_kernel, dim3(c.n_active), dim3(256), 0, h->stream, h->d, h->p);
+    HIPCHK(h, hipMemsetAsync(h->d.UVo, 0, 2 * FS * sizeof(double), h->stream));
+    if (c.n_active > 0)
+        hipLaunchKernelGGL(uv_init_kernel, dim3(std::min(c.n_active, VOL_BLOCKS_MAX)), dim3(VOL_NT), 0,
+                           h->stream, h->d_args, h->d_par);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return KST_OK;
 }
 

@@ -58,6 +58,10 @@ class Run:
     nout: int
     tstart: float
     deck: Deck
+    # Restart = on: resume from the run's output folder (kestrel_amd.restart.restart_run);
+    # 'Initial condition': a solution file to start from (kestrel_amd.restart.restart)
+    restart: bool = False
+    initial_condition: str | None = None
 
 
 def _linetype(s: str) -> int:
@@ -245,8 +249,7 @@ def _solver(d: Deck, c: C.KstConfig):
         elif k == "t start":
             tstart = _real(v)
         elif k == "restart":
-            if lv == "on":
-                raise DeckUnsupported("Restart = on")
+            pass
         elif k == "sponge strength":
             c.sponge_strength = _real(v)
         elif k == "initial condition":
@@ -531,7 +534,9 @@ def build(d: Deck, name: str = "deck", precision: int = C.PREC_FP64,
     if srcs:
         C.set_sources(c, srcs)
     nout = int(d.output.get("n out", "1"))
-    return Run(Workload(name, c, b0v, init, 0, caps, srcs), tend, nout, tstart, d)
+    restart = d.solver.get("restart", "off").strip().lower() == "on"
+    ic = d.solver.get("initial condition", "").strip() or None
+    return Run(Workload(name, c, b0v, init, 0, caps, srcs), tend, nout, tstart, d, restart, ic)
 
 
 def load(path: str, precision: int = C.PREC_FP64, b0v: np.ndarray | None = None) -> Run:

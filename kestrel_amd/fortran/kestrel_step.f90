@@ -124,6 +124,12 @@ module kestrel_step_binding
          type(c_ptr), value :: h, b0c, btc, dbdx, dbdy
          integer(c_int) :: rc
       end function
+      function kst_set_cell_topography(h, btc, dbdx, dbdy, cells) result(rc) &
+            bind(C, name="kst_set_cell_topography")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: h, btc, dbdx, dbdy, cells
+         integer(c_int) :: rc
+      end function
       function kst_last_error(h, buf, buflen) result(rc) bind(C, name="kst_last_error")
          import :: c_ptr, c_char, c_int
          type(c_ptr), value :: h

@@ -36,7 +36,7 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
            "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap",
            "kst_set_redistribution_chunk", "kst_autotune", "kst_autotune_choices",
-           "kst_volume_noalloc"]
+           "kst_volume_noalloc", "kst_set_cell_topography"]
 
 
 class KstStats(ctypes.Structure):
@@ -100,6 +100,8 @@ def lib():
         L.kst_profile_read.restype = ctypes.c_int
         L.kst_get_tile_bt.argtypes = [ctypes.c_void_p, _dp]
         L.kst_get_cell_topography.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.kst_set_cell_topography.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _u8p]
+        L.kst_set_cell_topography.restype = ctypes.c_int
         L.kst_destroy.argtypes = [ctypes.c_void_p]
         L.kst_destroy.restype = None
         L.kst_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
@@ -206,6 +208,14 @@ class Stepper:
         out = {k: np.zeros((c.NY, c.NX)) for k in ("b0c", "btc", "dbdx", "dbdy")}
         self._check(self.L.kst_get_cell_topography(self.h, *[_ptr(out[k]) for k in ("b0c", "btc", "dbdx", "dbdy")]))
         return out
+
+    def set_cell_topography(self, btc, dbdx, dbdy, cells) -> None:
+        """Cell bt, db/dx, db/dy (NY x NX) of the cells where `cells` is true,
+        as a restart reads them from a solution file (kst_set_cell_topography)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (btc, dbdx, dbdy)]
+        m = np.ascontiguousarray(cells, dtype=np.uint8)
+        self._check(self.L.kst_set_cell_topography(self.h, *[_ptr(a) for a in arrs],
+                                                   m.ctypes.data_as(_u8p)))
 
     def volume(self) -> np.ndarray:
         """The Volume.txt columns 2-7 (flow volume, total bed volume, total

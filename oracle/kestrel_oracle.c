@@ -2704,6 +2704,30 @@ int ko_set_state(ko_sim* s, const double* w, const double* rhu, const double* rh
     return ko_finish_sources(s);
 }
 
+/* The cell bt, db/dx, db/dy of LoadInitialCondition_txt (src/Restart.f90:216-236)
+ * for the active tiles' cells where cells[] is set: ActivateTile(restart)
+ * keeps them (src/UpdateTiles.f90:342-384); the derived variables follow
+ * (on a 1D grid db/dy is not read). */
+int ko_set_cell_topography(ko_sim* s, const double* btc, const double* dbdx, const double* dbdy,
+                           const uint8_t* cells) {
+    int nx = s->nx, ny = s->ny;
+    for (int q = 0; q < s->nactive; ++q) {
+        int k = s->active[q], gi, gj;
+        tile_coords(s, k, &gi, &gj);
+        tdata* T = &s->cont[C_TC][k];
+        for (int jj = 0; jj < ny; ++jj)
+            for (int ii = 0; ii < nx; ++ii) {
+                size_t g = (size_t)(gj * ny + jj) * s->NX + gi * nx + ii;
+                if (!cells[g]) continue;
+                UF(s, T, BT, ii, jj) = btc[g];
+                UF(s, T, DBDX, ii, jj) = dbdx[g];
+                if (!s->oned) UF(s, T, DBDY, ii, jj) = dbdy[g];
+            }
+        desingularise(s, s->cont[C_TC], k, 1);
+    }
+    return 0;
+}
+
 static void scatter_field(ko_sim* s, int k, const double* src, double* dst) {
     int gi, gj, nx = s->nx, ny = s->ny;
     tile_coords(s, k, &gi, &gj);

@@ -57,6 +57,8 @@ int ko_get_maxima(ko_sim* s, double* const maxima[11]);
  * tile's u (tileContainer), cell-major global layout, NX*NY per field
  * (unallocated tiles give 0). */
 int ko_set_maxima(ko_sim* s, const double* const mx[11]);
+int ko_set_cell_topography(ko_sim* s, const double* btc, const double* dbdx, const double* dbdy,
+                           const uint8_t* cells);
 int ko_get_fields13(ko_sim* s, double* out13);
 /* CalculateVolume (src/Output.f90:617-735): the Volume.txt columns 2-7
  * (volume, total bed volume, total mass, bed mass, total solids mass, bed

@@ -64,10 +64,11 @@ def _load(name: str):
     for name in ["ko_set_topography", "ko_add_caps", "ko_add_flux_sources", "ko_finish_sources",
                  "ko_set_state", "ko_integrate_to", "ko_step_n", "ko_get_state",
                  "ko_get_maxima", "ko_get_fields13", "ko_get_ghost_slopes", "ko_rhs",
-                 "ko_tile_lists", "ko_set_maxima"]:
+                 "ko_tile_lists", "ko_set_maxima", "ko_set_cell_topography"]:
         getattr(L, name).restype = ctypes.c_int
     L.ko_set_topography.argtypes = [ctypes.c_void_p, _dp]
     L.ko_set_maxima.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp)]
+    L.ko_set_cell_topography.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_uint8)]
     L.ko_add_caps.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 7 + [ctypes.POINTER(ctypes.c_int)]
     L.ko_add_flux_sources.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
     L.ko_finish_sources.argtypes = [ctypes.c_void_p]
@@ -179,6 +180,12 @@ class Oracle:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
         ptrs = (_dp * 11)(*[_ptr(a) for a in arrs])
         self._check(self.L.ko_set_maxima(self.h, ptrs))
+
+    def set_cell_topography(self, btc, dbdx, dbdy, cells):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (btc, dbdx, dbdy)]
+        m = np.ascontiguousarray(cells, dtype=np.uint8)
+        self._check(self.L.ko_set_cell_topography(self.h, *[_ptr(a) for a in arrs],
+                                                  m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
 
     def integrate_to(self, tend):
         st = KstStats()

@@ -21,7 +21,10 @@
 ! would, then writes <dump-prefix>_<k>.bin (unformatted stream, native
 ! little-endian) holding grid%t, grid%dt, the active and ghost lists and,
 ! per listed tile, its u(1:13,:,:), b0, bt and maxima.  Restart is set after
-! the first call only to skip Run's start-of-run output.
+! the first call only to skip Run's start-of-run output.  A deck with
+! Restart = on resumes, through LoadInitialCondition, from the output folder's
+! RunInfo.txt and last file; the intervals then run from that file's number
+! + 1 (<dump-prefix>_0.bin holds the restarted state).
 !
 ! <b0-file> (optional) replaces the deck's analytic topography by vertex
 ! elevations sampled on the whole grid: int32 nxv, nyv; float64 x0, y0, dx,
@@ -72,7 +75,7 @@ program ref_run
    type(GridType), target :: grid
    character(len=:), allocatable :: version
    character(len=4096) :: outdir, prefix, b0file, fname
-   integer :: nout, k, nargs
+   integer :: nout, k, nargs, first
    integer(int32) :: nxv, nyv
 
    version = GetVersion()
@@ -127,7 +130,8 @@ program ref_run
    if (trim(prefix) /= 'none') call dump(trim(prefix) // '_0.bin')
 
    nout = RunParams%Nout
-   do k = 1, nout
+   first = RunParams%FirstOut   ! 0, or on a restart the last file's number (src/Restart.f90:632)
+   do k = first + 1, nout
       RunParams%FirstOut = k - 1
       RunParams%Nout = k
       call Run(RunParams, grid)

@@ -129,15 +129,17 @@ def params(deck_path: str, workdir: str, binary: str = BIN) -> dict:
 
 
 def run(deck_path: str, workdir: str, b0_file: str | None = None, binary: str = BIN,
-        timeout: float = 3600.0, keep: bool = True) -> dict:
-    """Run the reference on a deck; returns the dumps (0..Nout), the
-    accepted-step count and last printed (t, dt) of every interval, the
-    number of revised-dt messages and the wall time."""
+        timeout: float = 3600.0, keep: bool = True, name: str = "dump") -> dict:
+    """Run the reference on a deck; returns the dumps (the set-up state, then
+    one per interval run: 1..Nout, or on a restart from the restart file's
+    number + 1), the accepted-step count and last printed (t, dt) of every
+    interval, the number of revised-dt messages and the wall time.  The
+    output folder is workdir/out."""
     import time
     if not os.path.exists(binary):
         raise RuntimeError(f"{binary} is not built (make -C oracle refrun)")
     os.makedirs(workdir, exist_ok=True)
-    prefix = os.path.join(workdir, "dump")
+    prefix = os.path.join(workdir, name)
     cmd = [binary, os.path.abspath(deck_path), os.path.join(workdir, "out"), prefix]
     if b0_file is not None:
         cmd.append(b0_file)
@@ -160,11 +162,12 @@ def run(deck_path: str, workdir: str, b0_file: str | None = None, binary: str = 
             steps.append(cur)
             revised.append(nrev)
             cur, nrev = 0, 0
+    import re
+    ks = sorted(int(m.group(1)) for f in os.listdir(workdir)
+                if (m := re.fullmatch(re.escape(name) + r"_(\d+)\.bin", f)))
     dumps = []
-    k = 0
-    while os.path.exists(f"{prefix}_{k}.bin"):
+    for k in ks:
         dumps.append(read_dump(f"{prefix}_{k}.bin"))
         if not keep:
             os.remove(f"{prefix}_{k}.bin")
-        k += 1
-    return dict(dumps=dumps, steps=steps, revised=revised, wall=wall, log=text)
+    return dict(dumps=dumps, steps=steps, revised=revised, wall=wall, log=text, intervals=ks[1:])

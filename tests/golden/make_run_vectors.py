@@ -12,7 +12,10 @@ interval: grid%t, grid%dt, the accepted steps and revised-dt messages of the
 interval (counted from the program's 't = ...' and 'revised nextT' lines),
 the active list, the ghost set and the digests of the active tiles' state.
 Cases with sampled topography get it as a vertex file (oracle/refrun.py
-write_b0).  Existing entries of other cases are kept.
+write_b0).  A restart case (tests/runvec.py RESTART) runs its first deck, keeps
+the files its restart reads (RunInfo.txt, the last solution file and its _topo
+companion, Maximums.txt) in tests/golden/restart/<case>/ (gzip), then runs the
+case's deck, which resumes from them.  Existing entries of other cases are kept.
 """
 from __future__ import annotations
 
@@ -39,6 +42,9 @@ def make(name: str, work: str) -> dict:
         xv, yv = S.vertex_coords(c)
         b0file = os.path.join(work, "b0.bin")
         refrun.write_b0(b0file, run.workload.b0v, float(xv[0]), float(yv[0]), c.dx, c.dy)
+    if name in runvec.RESTART:
+        refrun.run(runvec.RESTART[name], work, b0file, keep=False, name="first")
+        keep_restart_files(name, os.path.join(work, "out"))
     prm = refrun.params(path, os.path.join(work, "params"))
     R = refrun.run(path, work, b0file, keep=False)
     recs = []
@@ -52,7 +58,25 @@ def make(name: str, work: str) -> dict:
     print(f"{name}: {len(recs) - 1} intervals, steps {R['steps']}, revised {R['revised']}, "
           f"reference wall {R['wall']:.1f} s", flush=True)
     return {"deck": os.path.relpath(path, runvec.HERE), "topography": src, "params": prm,
-            "nout": run.nout, "tend": run.tend, "intervals": recs}
+            "nout": run.nout, "tend": run.tend, "first": (R["intervals"] or [1])[0], "intervals": recs}
+
+
+def keep_restart_files(name: str, out: str) -> None:
+    """The files a restart from out reads, as the reference wrote them."""
+    import gzip
+    import shutil
+    from kestrel_amd import restart
+    dest = os.path.join(runvec.RESTART_DIR, name)
+    if os.path.isdir(dest):
+        shutil.rmtree(dest)
+    os.makedirs(dest)
+    info = restart.read_run_info(os.path.join(out, "RunInfo.txt"))
+    last = f"{info.last_file:06d}.txt"
+    shutil.copy(os.path.join(out, "RunInfo.txt"), os.path.join(dest, "RunInfo.txt"))
+    for f in (last, last + "_topo", "Maximums.txt"):
+        with open(os.path.join(out, f), "rb") as a, gzip.GzipFile(os.path.join(dest, f + ".gz"), "wb",
+                                                                  compresslevel=9, mtime=0) as b:
+            b.write(a.read())
 
 
 def _one(name: str):

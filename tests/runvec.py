@@ -63,7 +63,17 @@ CASES = {
     "dirichlet_cap_2d": (os.path.join(OWN_DECKS, "dirichlet_cap_2d.txt"), None),
     "sponge_cap_2d": (os.path.join(OWN_DECKS, "sponge_cap_2d.txt"), None),
     "morpho_hills_1024": (os.path.join(OWN_DECKS, "morpho_hills_1024.txt"), "morpho_hills_1024"),
+    "restart_bowl_2d": (os.path.join(OWN_DECKS, "restart_bowl_2d.txt"), None),
 }
+
+# Restart cases (Restart = on): the reference first runs the deck named here,
+# which writes its text output in out/, then the case's own deck, which
+# resumes from RunInfo.txt and the last file there (LoadInitialCondition,
+# src/Restart.f90:55-146).  The files that restart reads, as the reference
+# wrote them, are kept in RESTART_DIR/<case>/ (gzip); replays read them back
+# through kestrel_amd.restart.
+RESTART = {"restart_bowl_2d": os.path.join(OWN_DECKS, "restart_bowl_2d_first.txt")}
+RESTART_DIR = os.path.join(GOLDEN, "restart")
 
 # Cases whose reference runs this library does not reproduce bit for bit, and
 # why.  The reference evaluates each tile's topography at that tile's own
@@ -94,6 +104,37 @@ def topography(source: str | None):
 def load_case(name: str, precision: int = 0):
     path, src = CASES[name]
     return deck.load(path, precision=precision, b0v=topography(src))
+
+
+def unpack_restart(name: str, dest: str) -> str:
+    """The reference's restart files of a restart case, unpacked into
+    dest/out (the case deck's output folder); returns that folder."""
+    import gzip
+    import shutil
+    out = os.path.join(dest, "out")
+    os.makedirs(out, exist_ok=True)
+    src = os.path.join(RESTART_DIR, name)
+    for f in sorted(os.listdir(src)):
+        if f.endswith(".gz"):
+            with gzip.open(os.path.join(src, f), "rb") as a, open(os.path.join(out, f[:-3]), "wb") as b:
+                b.write(a.read())
+        else:
+            shutil.copy(os.path.join(src, f), os.path.join(out, f))
+    return out
+
+
+def start(obj, run, name: str, tmp: str) -> int:
+    """Set obj (a Stepper or an Oracle, topography set) to the case's initial
+    state: the deck's painting at t start or, for a restart case, the
+    reference's files (unpacked under tmp).  Returns the number of the first
+    output interval to run: 1, or the restart file's number + 1."""
+    if name in RESTART:
+        from kestrel_amd import restart
+        rs, info, t = restart.restart_run(obj, run.workload.cfg, run.workload.b0v,
+                                          unpack_restart(name, tmp))
+        return info.last_file + 1
+    obj.set_state(run.workload.init, t=run.tstart)
+    return 1
 
 
 def fixtures() -> dict:
@@ -154,8 +195,8 @@ def device_record(g, cfg) -> dict:
     return rec
 
 
-def intervals(run):
-    """Output times tstart + k DeltaT, k = 1..Nout (src/OutputSettings.f90:286,
+def intervals(run, first: int = 1):
+    """Output times tstart + k DeltaT, k = first..Nout (src/OutputSettings.f90:286,
     src/TimeStepper.f90:101)."""
     dT = (run.tend - run.tstart) / float(run.nout)
-    return [run.tstart + k * dT for k in range(1, run.nout + 1)]
+    return [run.tstart + k * dT for k in range(first, run.nout + 1)]

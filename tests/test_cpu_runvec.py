@@ -21,12 +21,14 @@ pytestmark = pytest.mark.skipif(not os.path.exists(runvec.FIXTURE), reason="run 
 FIX = runvec.fixtures() if os.path.exists(runvec.FIXTURE) else {}
 
 
-def _oracle(run):
+def _oracle(run, name, tmp):
+    """The oracle at the case's start: the deck's painting, or the
+    reference's restart files; returns it and the first interval to run."""
     from oracle.oracle import Oracle
     o = Oracle(run.workload.cfg)
     o.set_topography(run.workload.b0v)
-    o.set_state(run.workload.init, t=run.tstart)
-    return o
+    first = runvec.start(o, run, name, tmp)
+    return o, first
 
 
 def _check(rec, ref, label, initial=False):
@@ -41,19 +43,20 @@ def _check(rec, ref, label, initial=False):
 
 
 @pytest.mark.parametrize("name", sorted(FIX))
-def test_oracle_equals_reference_run(name):
+def test_oracle_equals_reference_run(name, tmp_path):
     if name in runvec.KNOWN_DIFFERENT:
         pytest.skip(runvec.KNOWN_DIFFERENT[name])
     fx = FIX[name]
     run = runvec.load_case(name)
-    o = _oracle(run)
+    o, first = _oracle(run, name, str(tmp_path))
+    assert first == fx.get("first", 1)
     _check(runvec.oracle_record(o, run.workload.cfg), fx["intervals"][0], f"{name} t=0", initial=True)
-    times = runvec.intervals(run)
+    times = runvec.intervals(run, first)
     n = 1 if name in runvec.LONG else len(times)
     for k in range(1, n + 1):
         ref = fx["intervals"][k]
         st = o.integrate_to(times[k - 1])
-        label = f"{name} interval {k}"
+        label = f"{name} interval {first + k - 1}"
         assert st["t"] == ref["t"], f"{label}: t {st['t']!r} vs {ref['t']!r}"
         assert st["steps"] == ref["steps"], f"{label}: steps {st['steps']} vs {ref['steps']}"
         assert st["rejected"] == ref["revised"], f"{label}: rejected {st['rejected']} vs {ref['revised']}"

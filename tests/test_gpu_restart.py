@@ -1,7 +1,8 @@
 """Restart from text output on the device: files written from a device run
-are read back (kestrel_amd.restart) into a new handle and into the oracle
-from the same parsed arrays; both continue bit for bit.  The maxima resume
-through kst_set_maxima."""
+are read back (kestrel_amd.restart) into a new handle and into the oracle;
+both continue bit for bit.  The maxima resume through kst_set_maxima, the
+file's cell topography through kst_set_cell_topography.  (The reference's
+own restart, from files it wrote, is a run-vector case: restart_bowl_2d.)"""
 import numpy as np
 import pytest
 
@@ -32,7 +33,7 @@ def test_restart_continues_as_the_oracle(gpu, tmp_path, case):
     for q in range(11):
         assert np.array_equal(mx[q][rs.mask], rs.maxima[q][rs.mask])
     o = oracle_for(wl)
-    o.set_state(rs.init, t=st["t"], btv=rs.btv)
+    restart.restart(o, wl.cfg, wl.b0v, st["t"], p, maxima_path=pm)
     sr = r.step_n(20)
     so = o.step_n(20)
     assert sr["rejected"] == so["rejected"]

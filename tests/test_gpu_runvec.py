@@ -41,7 +41,7 @@ pytestmark = pytest.mark.gpu
 FIX = runvec.fixtures()
 
 # cases the device reproduces bit for bit (only IEEE +, -, *, /, sqrt on the path)
-BITWISE = {"cap_dilute_2d", "flux_hydro_2d", "flux_single_pt", "plain_cap_256",
+BITWISE = {"restart_bowl_2d", "cap_dilute_2d", "flux_hydro_2d", "flux_single_pt", "plain_cap_256",
            "lake_at_rest_hydro_2d", "periodic_cap_2d", "dirichlet_cap_2d", "sponge_cap_2d",
            "cap_dilute", "flux_hydro", "lake_at_rest_hydro"}
 # the parity bar of the conserved fields, Hn and the bed; the 1918 Strang steps
@@ -50,19 +50,21 @@ BITWISE = {"cap_dilute_2d", "flux_hydro_2d", "flux_single_pt", "plain_cap_256",
 FIELD_TOL = {"morpho_hills_1024": 2e-12}
 
 
-def _device(run):
+def _device(run, name, tmp):
+    """The device at the case's start (the deck's painting, or the
+    reference's restart files) and the first interval to run."""
     from kestrel_amd.runtime import Stepper
     g = Stepper(run.workload.cfg)
     g.set_topography(run.workload.b0v)
-    g.set_state(run.workload.init, t=run.tstart)
-    return g
+    first = runvec.start(g, run, name, tmp)
+    return g, first
 
 
-def _oracle(run):
+def _oracle(run, name, tmp):
     from oracle.oracle import Oracle
     o = Oracle(run.workload.cfg)
     o.set_topography(run.workload.b0v)
-    o.set_state(run.workload.init, t=run.tstart)
+    runvec.start(o, run, name, tmp)
     return o
 
 
@@ -139,16 +141,16 @@ class _Probe:
     functions one ulp up (oracle/liboracle_pert.so), replayed over the same
     intervals when a field first exceeds its bar."""
 
-    def __init__(self, run):
-        self.run, self.o, self.k = run, None, 0
+    def __init__(self, run, name, tmp):
+        self.run, self.name, self.tmp, self.o, self.k, self.first = run, name, tmp, None, 0, 1
 
     def at(self, k, tiles):
         from oracle.oracle import Oracle
         if self.o is None:
             self.o = Oracle(self.run.workload.cfg, perturbed=True)
             self.o.set_topography(self.run.workload.b0v)
-            self.o.set_state(self.run.workload.init, t=self.run.tstart)
-        times = runvec.intervals(self.run)
+            self.first = runvec.start(self.o, self.run, self.name, self.tmp)
+        times = runvec.intervals(self.run, self.first)
         while self.k < k:
             self.o.integrate_to(times[self.k])
             self.k += 1
@@ -160,19 +162,21 @@ class _Probe:
 
 
 @pytest.mark.parametrize("name", sorted(FIX))
-def test_device_equals_reference_run(gpu, name):
+def test_device_equals_reference_run(gpu, name, tmp_path):
     if name in runvec.KNOWN_DIFFERENT:
         pytest.skip(runvec.KNOWN_DIFFERENT[name])
     fx = FIX[name]
     run = runvec.load_case(name)
     cfg = run.workload.cfg
-    g = _device(run)
-    o = None if name in BITWISE else _oracle(run)
-    probe = _Probe(run)
-    print(f"{name}: {cfg.NX}x{cfg.NY}, {run.nout} intervals to t = {run.tend}", flush=True)
-    for k, tend in enumerate(runvec.intervals(run), start=1):
+    tmp = str(tmp_path)
+    g, first = _device(run, name, tmp)
+    assert first == fx.get("first", 1)
+    o = None if name in BITWISE else _oracle(run, name, tmp)
+    probe = _Probe(run, name, tmp)
+    print(f"{name}: {cfg.NX}x{cfg.NY}, intervals {first}..{run.nout} to t = {run.tend}", flush=True)
+    for k, tend in enumerate(runvec.intervals(run, first), start=1):
         ref = fx["intervals"][k]
-        label = f"{name} interval {k}"
+        label = f"{name} interval {first + k - 1}"
         sg = g.integrate_to(tend)
         assert sg["t"] == ref["t"], f"{label}: t {sg['t']!r} vs {ref['t']!r}"
         assert sg["steps"] == ref["steps"], f"{label}: steps {sg['steps']} vs {ref['steps']}"
