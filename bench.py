@@ -421,6 +421,15 @@ def main():
                              "ms_per_step": pel * 1e3 / 200, "roofline": roofline(pw.name, prows, pcells),
                              "step_roofline_frac": pst["cell_updates"] / pel * STEP_WORDS["hydraulic"] * 8
                              / (HBM_PEAK_GBS * 1e9), "autotune": pst["autotune"]}
+        # the same deck in mixed precision (FP32 RHS, stage increments and
+        # non-hydrostatic face arithmetic; FP64 u^n and free surface)
+        from kestrel_amd import config as C
+        mw = workloads.plain_cap_2048(200)
+        mw.cfg.precision = C.PREC_MIXED
+        mst, mel, _ = run(mw, 200, 10)
+        res["plain_fp64"]["mixed"] = {"value": mst["cell_updates"] / mel, "unit": "cell-updates/s",
+                                      "ms_per_step": mel * 1e3 / 200,
+                                      "vs_fp64": (mst["cell_updates"] / mel) / res["plain_fp64"]["value"]}
         if not args.no_cpu:
             # two steps of the same deck in the reference (about 10 s of CPU work)
             res["plain_fp64"]["cpu_baseline"] = cpu_reference(pw, 0.0225)
@@ -428,6 +437,10 @@ def main():
         res["volume"] = volume_leg()
     if not args.no_batch:
         res["ensemble"] = ensemble_leg()
+        if "plain_fp64" in res:
+            # the batched ensemble against the plain FP64 path on the full chip
+            res["ensemble"]["batched_mixed_vs_plain_2048"] = (res["ensemble"]["batched_mixed"]
+                                                             / res["plain_fp64"]["value"])
     if not args.no_serve:
         res["serving"] = serving_leg(args.serve_requests)
     if not args.no_cpu:

@@ -297,12 +297,19 @@ __device__ void flux_sources(const DevT& d, const Params& p, double t, double gr
 // the fused stage kernel
 // ---------------------------------------------------------------------------
 
-// Limiters at compile time (src/Limiters.f90:83-184).  MinMod1/2 use the
-// identity max(-x,-y) = -min(x,y): negation and |.| are exact, so the value
-// equals the reference's branchy form.
+// Limiters at compile time (src/Limiters.f90:83-184), or with LIM_RUNTIME the
+// limiter rt a scenario of a batch carries.  MinMod1/2 use the identity
+// max(-x,-y) = -min(x,y): negation and |.| are exact, so the value equals the
+// reference's branchy form.
+constexpr int LIM_RUNTIME = 100;
+
 template <int LIM, typename T>
-__device__ __forceinline__ T lim(T a, T b) {
-    if (LIM == KST_LIMITER_MINMOD2) {
+__device__ __forceinline__ T lim(int rt, T a, T b) {
+    if (LIM == LIM_RUNTIME) {
+        if (rt == KST_LIMITER_MINMOD2) return lim<KST_LIMITER_MINMOD2>(rt, a, b);
+        if (rt == KST_LIMITER_MINMOD1) return lim<KST_LIMITER_MINMOD1>(rt, a, b);
+        return (T)limiter(rt, (double)a, (double)b);
+    } else if (LIM == KST_LIMITER_MINMOD2) {
         const T th = T(1.3);
         const T m = fmin(th * fabs(a), fmin(th * fabs(b), T(0.5) * fabs(a + b)));
         return (a * b <= T(0)) ? T(0) : copysign(m, a);
@@ -411,20 +418,20 @@ __device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& 
     // non-active tiles keep their stored w slope and zero slopes otherwise.
     const double wLL = s.w[iLL], wL = s.w[iL], wR = s.w[iR], wRR = s.w[iRR];
     const double* gs = YDIR ? d.gsy : d.gsx;
-    const double sLw = actL ? dxr * lim<LIM>(wR - wL, wL - wLL) : gs[s.gidx[iL]];
-    const double sRw = actR ? dxr * lim<LIM>(wRR - wR, wR - wL) : gs[s.gidx[iR]];
+    const double sLw = actL ? dxr * lim<LIM>(p.limiter, wR - wL, wL - wLL) : gs[s.gidx[iL]];
+    const double sRw = actR ? dxr * lim<LIM>(p.limiter, wRR - wR, wR - wL) : gs[s.gidx[iR]];
     const double hLL = s.h[iLL], hL = s.h[iL], hR = s.h[iR], hRR = s.h[iRR];
-    const double sLh = actL ? dxr * lim<LIM>(hR - hL, hL - hLL) : 0.0;
-    const double sRh = actR ? dxr * lim<LIM>(hRR - hR, hR - hL) : 0.0;
+    const double sLh = actL ? dxr * lim<LIM>(p.limiter, hR - hL, hL - hLL) : 0.0;
+    const double sRh = actR ? dxr * lim<LIM>(p.limiter, hRR - hR, hR - hL) : 0.0;
     const double uLL = s.u[iLL], uL = s.u[iL], uR = s.u[iR], uRR = s.u[iRR];
-    const double sLu = actL ? dxr * lim<LIM>(uR - uL, uL - uLL) : 0.0;
-    const double sRu = actR ? dxr * lim<LIM>(uRR - uR, uR - uL) : 0.0;
+    const double sLu = actL ? dxr * lim<LIM>(p.limiter, uR - uL, uL - uLL) : 0.0;
+    const double sRu = actR ? dxr * lim<LIM>(p.limiter, uRR - uR, uR - uL) : 0.0;
     const double vLL = s.v[iLL], vL = s.v[iL], vR = s.v[iR], vRR = s.v[iRR];
-    const double sLv = actL ? dxr * lim<LIM>(vR - vL, vL - vLL) : 0.0;
-    const double sRv = actR ? dxr * lim<LIM>(vRR - vR, vR - vL) : 0.0;
+    const double sLv = actL ? dxr * lim<LIM>(p.limiter, vR - vL, vL - vLL) : 0.0;
+    const double sRv = actR ? dxr * lim<LIM>(p.limiter, vRR - vR, vR - vL) : 0.0;
     const double rLL = s.rho[iLL], rL = s.rho[iL], rR = s.rho[iR], rRR = s.rho[iRR];
-    const double sLr = actL ? dxr * lim<LIM>(rR - rL, rL - rLL) : 0.0;
-    const double sRr = actR ? dxr * lim<LIM>(rRR - rR, rR - rL) : 0.0;
+    const double sLr = actL ? dxr * lim<LIM>(p.limiter, rR - rL, rL - rLL) : 0.0;
+    const double sRr = actR ? dxr * lim<LIM>(p.limiter, rRR - rR, rR - rL) : 0.0;
 
     // --- MUSCL face values (Reconstruct, src/HydraulicRHS.f90:397-490)
     const double wEL = wL + sLw * 0.5 * dx;   // L's east face
@@ -477,8 +484,8 @@ __device__ __forceinline__ void face_flux(const DevT& d, const Params& p, Smem& 
         // a 1D grid keeps the x-face reconstruction of the conserved rhoHnv
         // (rho Hn v is rebuilt in 2D only, src/HydraulicRHS.f90:536-551)
         const double qLL = s.rv[iLL], qL = s.rv[iL], qR = s.rv[iR], qRR = s.rv[iRR];
-        const double sLq = actL ? dxr * lim<LIM>(qR - qL, qL - qLL) : 0.0;
-        const double sRq = actR ? dxr * lim<LIM>(qRR - qR, qR - qL) : 0.0;
+        const double sLq = actL ? dxr * lim<LIM>(p.limiter, qR - qL, qL - qLL) : 0.0;
+        const double sRq = actR ? dxr * lim<LIM>(p.limiter, qRR - qR, qR - qL) : 0.0;
         rvL = qL + sLq * 0.5 * dx;
         rvR = qR - sRq * 0.5 * dx;
     } else {
@@ -575,21 +582,21 @@ __device__ __forceinline__ void face_flux_mixed(const DevT& d, const Params& p, 
     // FP64: w slopes (non-active tiles keep their stored slope), density slopes
     const double wLL = s.w[iLL], wL = s.w[iL], wR = s.w[iR], wRR = s.w[iRR];
     const double* gs = YDIR ? d.gsy : d.gsx;
-    const double sLw = actL ? dxr * lim<LIM>(wR - wL, wL - wLL) : gs[s.gidx[iL]];
-    const double sRw = actR ? dxr * lim<LIM>(wRR - wR, wR - wL) : gs[s.gidx[iR]];
+    const double sLw = actL ? dxr * lim<LIM>(p.limiter, wR - wL, wL - wLL) : gs[s.gidx[iL]];
+    const double sRw = actR ? dxr * lim<LIM>(p.limiter, wRR - wR, wR - wL) : gs[s.gidx[iR]];
     const double rLL = s.rho[iLL], rL = s.rho[iL], rR = s.rho[iR], rRR = s.rho[iRR];
-    const double sLr = actL ? dxr * lim<LIM>(rR - rL, rL - rLL) : 0.0;
-    const double sRr = actR ? dxr * lim<LIM>(rRR - rR, rR - rL) : 0.0;
+    const double sLr = actL ? dxr * lim<LIM>(p.limiter, rR - rL, rL - rLL) : 0.0;
+    const double sRr = actR ? dxr * lim<LIM>(p.limiter, rRR - rR, rR - rL) : 0.0;
     // FP32: Hnpsi, u, v slopes
     const float hLL = s.h[iLL], hL = s.h[iL], hR = s.h[iR], hRR = s.h[iRR];
-    const float sLh = actL ? dxrf * lim<LIM>(hR - hL, hL - hLL) : 0.f;
-    const float sRh = actR ? dxrf * lim<LIM>(hRR - hR, hR - hL) : 0.f;
+    const float sLh = actL ? dxrf * lim<LIM>(p.limiter, hR - hL, hL - hLL) : 0.f;
+    const float sRh = actR ? dxrf * lim<LIM>(p.limiter, hRR - hR, hR - hL) : 0.f;
     const float uLL = s.u[iLL], uL = s.u[iL], uR = s.u[iR], uRR = s.u[iRR];
-    const float sLu = actL ? dxrf * lim<LIM>(uR - uL, uL - uLL) : 0.f;
-    const float sRu = actR ? dxrf * lim<LIM>(uRR - uR, uR - uL) : 0.f;
+    const float sLu = actL ? dxrf * lim<LIM>(p.limiter, uR - uL, uL - uLL) : 0.f;
+    const float sRu = actR ? dxrf * lim<LIM>(p.limiter, uRR - uR, uR - uL) : 0.f;
     const float vLL = s.v[iLL], vL = s.v[iL], vR = s.v[iR], vRR = s.v[iRR];
-    const float sLv = actL ? dxrf * lim<LIM>(vR - vL, vL - vLL) : 0.f;
-    const float sRv = actR ? dxrf * lim<LIM>(vRR - vR, vR - vL) : 0.f;
+    const float sLv = actL ? dxrf * lim<LIM>(p.limiter, vR - vL, vL - vLL) : 0.f;
+    const float sRv = actR ? dxrf * lim<LIM>(p.limiter, vRR - vR, vR - vL) : 0.f;
 
     // FP64: w face values, beds and CorrectSlopes (as face_flux)
     const double wEL = wL + sLw * 0.5 * dx;
@@ -2342,6 +2349,7 @@ struct kst_batch {
     Params* d_par = nullptr;
     int *d_on = nullptr, *d_off = nullptr, *d_status = nullptr;
     int visc = -1, mixed = -1;     // fixed by the first scenario attached
+    int anylim = 0;                // a running scenario's limiter is not MinMod2
     int grid = 0;
     hipGraphExec_t gexec = nullptr;
     std::string err;
@@ -3582,22 +3590,27 @@ constexpr int kBatchIters = 8;
 typedef void (*BatchStageFn)(Batch);
 struct BatchSet { BatchStageFn fn[4]; };
 
-template <bool VISC, bool MIX>
+// MinMod2 (the reference's default) at compile time, or any limiter per
+// scenario (LIM_RUNTIME), once a scenario with another one joins
+template <bool VISC, bool MIX, int L>
 BatchSet make_batch_set() {
-    constexpr int L = KST_LIMITER_MINMOD2;
     return BatchSet{{batch_stage_kernel<1, VISC, L, MIX>, batch_stage_kernel<2, VISC, L, MIX>,
                      batch_stage_kernel<3, VISC, L, MIX>, batch_stage_kernel<4, VISC, L, MIX>}};
 }
 
-static const BatchSet& batch_set(bool visc, bool mixed) {
-    static const BatchSet sets[2][2] = {{make_batch_set<false, false>(), make_batch_set<false, true>()},
-                                        {make_batch_set<true, false>(), make_batch_set<true, true>()}};
-    return sets[visc ? 1 : 0][mixed ? 1 : 0];
+static const BatchSet& batch_set(bool visc, bool mixed, bool anylim) {
+    constexpr int M2 = KST_LIMITER_MINMOD2, RT = LIM_RUNTIME;
+    static const BatchSet sets[2][2][2] = {
+        {{make_batch_set<false, false, M2>(), make_batch_set<false, true, M2>()},
+         {make_batch_set<true, false, M2>(), make_batch_set<true, true, M2>()}},
+        {{make_batch_set<false, false, RT>(), make_batch_set<false, true, RT>()},
+         {make_batch_set<true, false, RT>(), make_batch_set<true, true, RT>()}}};
+    return sets[anylim ? 1 : 0][visc ? 1 : 0][mixed ? 1 : 0];
 }
 
 static int batch_launch_iteration(kst_batch* b) {
     const Batch& bt = b->bt;
-    const BatchSet& k = batch_set(b->visc == 1, b->mixed == 1);
+    const BatchSet& k = batch_set(b->visc == 1, b->mixed == 1, b->anylim == 1);
     hipLaunchKernelGGL(batch_activate_kernel, dim3(b->nslot), dim3(256), 0, b->stream, bt);
     hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(BATCH_PLAN_THREADS), 0, b->stream, bt);
     for (int q = 0; q < 4; ++q)
@@ -3671,8 +3684,6 @@ int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot) {
     if (!h->have_state) return batch_err(b, KST_ERR_STATE, "kst_set_state must come before kst_batch_attach");
     if (h->d.morpho) return batch_err(b, KST_ERR_CONFIG, "batches run the hydraulic operator only");
     if (h->p.oned) return batch_err(b, KST_ERR_CONFIG, "batches run 2D grids only");
-    if (h->p.limiter != KST_LIMITER_MINMOD2)
-        return batch_err(b, KST_ERR_CONFIG, "batches use the MinMod2 limiter");
     const int visc = h->p.nu != 0.0 ? 1 : 0, mixed = h->d.mixed ? 1 : 0;
     bool any = false;
     for (int q = 0; q < b->nslot; ++q) any = any || b->on[q];
@@ -3682,11 +3693,13 @@ int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot) {
     for (int q = 0; q < b->nslot && s < 0; ++q)
         if (!b->slots[q]) s = q;
     if (s < 0) return batch_err(b, KST_ERR_CONFIG, "no free slot");
-    if (b->visc != visc || b->mixed != mixed) {
+    const int anylim = ((any && b->anylim) || h->p.limiter != KST_LIMITER_MINMOD2) ? 1 : 0;
+    if (b->visc != visc || b->mixed != mixed || b->anylim != anylim) {
         // the graph's kernels depend on the setting
         if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
         b->visc = visc;
         b->mixed = mixed;
+        b->anylim = anylim;
     }
     BCHK(b, hipStreamSynchronize(h->stream));
     BCHK(b, hipStreamSynchronize(b->stream));

@@ -169,3 +169,32 @@ def test_full_ensemble_batch_equals_alone(gpu):
     b.close()
     for g in gs:
         g.close()
+
+
+def test_batch_mixes_limiters(gpu):
+    """Scenarios with each of the reference's limiters in one batch (the
+    kernels take the limiter per scenario then): every one the same bits as
+    alone, in FP64 and in mixed precision."""
+    for prec in (C.PREC_FP64, C.PREC_MIXED):
+        wls = members(prec, 5)
+        for w, lim in zip(wls, ("minmod1", "minmod2", "vanalbada", "weno", "none")):
+            w.cfg.limiter = C.LIMITERS[lim]
+        alone = []
+        for w in wls:
+            g = stepper_for(w)
+            st = g.step_n(40)
+            alone.append((st, g.state()))
+            g.close()
+        b = Batch(8)
+        gs = [stepper_for(w) for w in wls]
+        slots = [b.attach(g) for g in gs]
+        for s in slots:
+            b.start(s, nsteps=40)
+        status = b.run()
+        for k, (s, g) in enumerate(zip(slots, gs)):
+            assert status[s] == SLOT_DONE
+            st = b.stats(s)
+            for key in STATS:
+                assert st[key] == alone[k][0][key], f"{wls[k].name} ({prec}): {key}"
+            same(g.state(), alone[k][1], f"{wls[k].name} ({prec})")
+        b.close()
