@@ -320,9 +320,10 @@ int kst_profile_read(kst_handle* h, double ms[KST_PROFILE_KERNELS],
  * slot keeps its own tiles, dt, retries and time; each iteration walks one
  * work list of the active sub-blocks of all running slots.  Handles join
  * (attach) and leave (detach) between runs without rebuilding the batch's
- * hipGraph.  A batch holds 2D hydraulic scenarios, all of one precision and
- * one eddy-viscosity setting, each with its own limiter (while every running
- * scenario uses MinMod2 the kernels have it at compile time).  While attached, a handle
+ * hipGraph.  A batch holds 2D scenarios, hydraulic or morphodynamic, all of
+ * one precision and one eddy-viscosity setting, each with its own limiter
+ * (while every running scenario uses MinMod2 the kernels have it at compile
+ * time); a morphodynamic scenario takes its Strang step as alone.  While attached, a handle
  * is advanced only through its batch; kst_get_state / kst_get_maxima read it
  * after kst_batch_run returns.  Results are the same bits as running the
  * handle alone. */

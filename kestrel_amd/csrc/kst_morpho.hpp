@@ -36,14 +36,16 @@ struct MWork {
     double dt;
 };
 
+// witem >= 0: that item of the scenario's list (a batch's work list), else
+// the block's pass-th item of a one-scenario launch.
 template <int BY, class DevT>
 __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_no_refine, int kw,
-                            int pass) {
+                            int pass, int witem = -1) {
     const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
     // the small shape's item and tile ahead of the control block (cap_item)
     constexpr bool SPEC = BY < B;
     int wspec = 0, tspec = 0;
-    if (SPEC) {
+    if (SPEC && witem < 0) {
         wspec = cap_item();
         tspec = d.active[min(wspec / spt, p.ntiles)];   // the list holds ntiles + 1 entries
     }
@@ -57,9 +59,9 @@ __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_
     }
     if (!skip) count_work(d.ctrl, kw);
     const int nwork = c->n_active * spt;
-    const bool spec = SPEC && nwork <= (int)gridDim.x;
+    const bool spec = SPEC && witem < 0 && nwork <= (int)gridDim.x;
     bool more = false;
-    const int w = spec ? (pass == 0 ? wspec : nwork) : xcd_item(nwork, pass, &more);
+    const int w = witem >= 0 ? witem : (spec ? (pass == 0 ? wspec : nwork) : xcd_item(nwork, pass, &more));
     if (w >= nwork) skip = 1;
     m.more = more;
     if (!skip) {
@@ -109,7 +111,7 @@ __device__ __forceinline__ double excess_deposition(const DevT& d, const Params&
 // ONED: a 1D grid (one row of cells; its vertex rates and cell topography
 // take the reference's 1D forms, its upper vertex row keeps a zero rate).
 template <int S, int BY, bool ONED, class DevT>
-__device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int pass) {
+__device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int pass, int witem = -1) {
     constexpr int NT = morpho_nt<BY>();
     constexpr int NH = RW * (BY + 4), NE = MR1 * (BY + 2), NVB = MV * (BY + 1);
     __shared__ double hn[NH];              // Hn of intermed_{S-1}, 2-cell halo
@@ -122,7 +124,7 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
     unsigned tidu = threadIdx.x;
     asm("" : "+v"(tidu) : "s"(pass));
     const int tid = (int)(tidu & (NT - 1));
-    if (tid == 0) morpho_work<BY>(d, p, m, false, KW_MORPHO + S - 1, pass);
+    if (tid == 0) morpho_work<BY>(d, p, m, false, KW_MORPHO + S - 1, pass, witem);
     __syncthreads();
     if (__builtin_amdgcn_readfirstlane(m.skip)) return false;   // wave-uniform (a uniform loop)
     const bool more = __builtin_amdgcn_readfirstlane(m.more) != 0;
@@ -788,9 +790,8 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
 // in place in global memory) and walked in consecutive chunks of the sorted
 // list, each chunk as above on the state the chunks before it wrote, until a
 // chunk stops: the same walk, in the same order, as the reference's.
-__global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
-    const auto& d = const_arg(dp);
-    const Params& p = *pp;
+template <class DevT>
+__device__ __forceinline__ void redistribute_body(const DevT& d, const Params& p) {
     __shared__ int sh_n, sh_skip;
     __shared__ unsigned long long sh_ndirty;
     __shared__ double lx[RD_LDS];
@@ -872,9 +873,32 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
     if (tid == 0) c->ndirty = (long long)sh_ndirty;
 }
 
+__global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
+    redistribute_body(const_arg(dp), *pp);
+}
+
 // ComputeInterfacialTopographicData on the new bed (src/TimeStepper.f90:774-779),
 // plus its CFL numerators and prefactors, in (tile, part) items.
-constexpr int TILE_PARTS = 8;
+
+// Whether the face topography of the new bed is due in this attempt (the
+// morphodynamic operator ran and its checks passed).
+template <class DevT>
+__device__ __forceinline__ bool face_topo_due(const DevT& d, const Params& p) {
+    const auto* c = ctrl_view(d);
+    if (c->status != ST_NEW && c->status != ST_RETRY) return false;
+    const double dt = step_dt(d, p, c);
+    return first_operator_ok(c, p, dt) && !c->mref;
+}
+
+// (tile, part) item `it` of the face topography of the new bed.
+template <class DevT>
+__device__ __forceinline__ void face_topo_item(const DevT& d, const Params& p, int it) {
+    const int t = d.active[it / TILE_PARTS], part = it % TILE_PARTS;
+    // a face's static terms read only that face's topography, computed in
+    // the same pass, and cell gamma, which the morphodynamic stage wrote
+    tile_face_topo<true>(d, p, d.TB, t, part, TILE_PARTS);
+    tile_prefactors(d, p, d.TB, t, part, TILE_PARTS);
+}
 
 __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ dp, const Params* __restrict__ pp) {
     const auto& d = const_arg(dp);
@@ -882,11 +906,7 @@ __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ 
     __shared__ int sh_skip, sh_n;
     const auto* c = ctrl_view(d);
     if (threadIdx.x == 0) {
-        int skip = (c->status != ST_NEW && c->status != ST_RETRY);
-        if (!skip) {
-            const double dt = step_dt(d, p, c);
-            if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
-        }
+        const int skip = face_topo_due(d, p) ? 0 : 1;
         if (!skip) count_work(d.ctrl, KW_FACETOPO);
         sh_skip = skip;
         sh_n = c->n_active * TILE_PARTS;
@@ -894,13 +914,7 @@ __global__ __launch_bounds__(256) void face_topo_kernel(const Dev* __restrict__ 
     __syncthreads();
     if (sh_skip) return;
     // (tile, part) items, the grid sized for the active set the host last saw
-    for (int it = blockIdx.x; it < sh_n; it += gridDim.x) {
-        const int t = d.active[it / TILE_PARTS], part = it % TILE_PARTS;
-        // a face's static terms read only that face's topography, computed in
-        // the same pass, and cell gamma, which the morphodynamic stage wrote
-        tile_face_topo<true>(d, p, d.TB, t, part, TILE_PARTS);
-        tile_prefactors(d, p, d.TB, t, part, TILE_PARTS);
-    }
+    for (int it = blockIdx.x; it < sh_n; it += gridDim.x) face_topo_item(d, p, it);
 }
 
 // One of the TILE_PARTS shares of tile t's bed copy.
@@ -939,6 +953,71 @@ __global__ __launch_bounds__(256) void commit_kernel(const Dev* __restrict__ dp,
     const int n = c->n_active * TILE_PARTS;
     for (int it = blockIdx.x; it < n; it += gridDim.x)
         commit_part(d, p, d.active[it / TILE_PARTS], it % TILE_PARTS);
+}
+
+// ---------------------------------------------------------------------------
+// batches with morphodynamic scenarios: the morphodynamic operator and the
+// second hydraulic operator walk the work lists of the morphodynamic slots
+// (Batch::off2: their 16 x 16 sub-blocks, Batch::off3: their (tile, part)
+// items), the redistribution takes one workgroup per slot
+// ---------------------------------------------------------------------------
+
+// The slot of work item w of a list whose slot s starts at off[s].
+__device__ __forceinline__ int batch_slot(const int* off, int nslot, int w) {
+    int lo = 0, hi = nslot - 1;   // last slot with off <= w
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= w) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int S>
+__global__ __launch_bounds__(morpho_nt<B>()) void batch_morpho_kernel(Batch bt) {
+    __shared__ int sh_slot;
+    const int total = bt.off2[bt.nslot];
+    const int m = gridDim.x / 8, x = blockIdx.x % 8;
+    const int per = (total + 7) / 8;
+    const int end = min((x + 1) * per, total);
+    for (int w = x * per + (int)(blockIdx.x / 8); w < end; w += m) {
+        if (threadIdx.x == 0) sh_slot = batch_slot(bt.off2, bt.nslot, w);
+        __syncthreads();
+        const int sl = __builtin_amdgcn_readfirstlane(sh_slot);
+        morpho_pass<S, B, false>(const_arg(bt.dev + sl), generic(const_arg(bt.par + sl)), 0, w - bt.off2[sl]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(RD_THREADS) void batch_redistribute_kernel(Batch bt) {
+    const int s = blockIdx.x;
+    if (s >= bt.nslot || !bt.on[s] || !bt.dev[s].morpho) return;
+    redistribute_body(const_arg(bt.dev + s), generic(const_arg(bt.par + s)));
+}
+
+// kind 0: the face topography of the new bed; 1: the accepted bed's commit
+template <int KIND>
+__global__ __launch_bounds__(256) void batch_tile_part_kernel(Batch bt) {
+    __shared__ int sh_slot, sh_go;
+    const int total = bt.off3[bt.nslot];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        if (threadIdx.x == 0) {
+            const int sl = batch_slot(bt.off3, bt.nslot, w);
+            const auto& d = const_arg(bt.dev + sl);
+            sh_slot = sl;
+            sh_go = KIND == 0 ? (face_topo_due(d, generic(const_arg(bt.par + sl))) ? 1 : 0)
+                              : (ctrl_view(d)->commit ? 1 : 0);
+        }
+        __syncthreads();
+        const int sl = __builtin_amdgcn_readfirstlane(sh_slot);
+        if (__builtin_amdgcn_readfirstlane(sh_go)) {
+            const auto& d = const_arg(bt.dev + sl);
+            const Params& p = generic(const_arg(bt.par + sl));
+            const int it = w - bt.off3[sl];
+            if (KIND == 0) face_topo_item(d, p, it);
+            else commit_part(d, p, d.active[it / TILE_PARTS], it % TILE_PARTS);
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace kst

@@ -2176,13 +2176,20 @@ __global__ void finalize_kernel(const Dev* __restrict__ d, const Params* __restr
 // A slot table of resident scenarios (each with its own state, tiles, dt and
 // control block) and the work list of one iteration: off[s] is the first
 // work item (sub-block of an active tile) of slot s, off[nslot] the total.
+// Tiles' per-tile passes over the bed (face topography of a new bed, the
+// accepted bed's commit) are split into this many (tile, part) items.
+constexpr int TILE_PARTS = 8;
+
 struct Batch {
     const Dev* dev;
     const Params* par;
     const int* on;        // slot holds a scenario
-    int* off;             // [nslot + 1]
+    int* off;             // [nslot + 1] first work item of each slot (16 x 16 sub-blocks)
     int* status;          // [nslot] the scenario's status after each iteration
     int nslot;
+    const Dev* dev2;      // the second hydraulic operator of each slot (morphodynamic slots)
+    int* off2;            // [nslot + 1] as off, morphodynamic slots only
+    int* off3;            // [nslot + 1] (tile, part) items of the morphodynamic slots
 };
 
 constexpr int BATCH_PLAN_THREADS = 1024;
@@ -2195,42 +2202,55 @@ __global__ __launch_bounds__(256) void batch_activate_kernel(Batch bt) {
 }
 
 // Work list of the iteration: the running slots' active sub-blocks.
-__global__ __launch_bounds__(BATCH_PLAN_THREADS) void batch_plan_kernel(Batch bt) {
+// The iteration's work lists (exclusive prefix sums over the slots): off,
+// the running slots' sub-blocks; with morphodynamic slots (lists > 1) also
+// off2, theirs alone, and off3, their (tile, part) items.
+__global__ __launch_bounds__(BATCH_PLAN_THREADS) void batch_plan_kernel(Batch bt, int lists) {
     __shared__ int sc[BATCH_PLAN_THREADS];
-    int carry = 0;
-    for (int base = 0; base < bt.nslot; base += BATCH_PLAN_THREADS) {
-        const int s = base + (int)threadIdx.x;
-        int n = 0;
-        if (s < bt.nslot && bt.on[s]) {
-            const Dev& d = bt.dev[s];
-            const Params& p = bt.par[s];
-            const Ctrl* c = d.ctrl;
-            if (c->status == ST_NEW || c->status == ST_RETRY)
-                n = c->n_active * (((p.nx + B - 1) / B) * ((p.ny + B - 1) / B));
-        }
-        sc[threadIdx.x] = n;
-        __syncthreads();
-        for (int o = 1; o < BATCH_PLAN_THREADS; o <<= 1) {
-            const int v = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
+    for (int kind = 0; kind < lists; ++kind) {
+        int* off = kind == 0 ? bt.off : (kind == 1 ? bt.off2 : bt.off3);
+        int carry = 0;
+        for (int base = 0; base < bt.nslot; base += BATCH_PLAN_THREADS) {
+            const int s = base + (int)threadIdx.x;
+            int n = 0;
+            if (s < bt.nslot && bt.on[s]) {
+                const Dev& d = bt.dev[s];
+                const Params& p = bt.par[s];
+                const Ctrl* c = d.ctrl;
+                const bool run = c->status == ST_NEW || c->status == ST_RETRY;
+                const int spt = ((p.nx + B - 1) / B) * ((p.ny + B - 1) / B);
+                if (kind == 0 && run) n = c->n_active * spt;
+                if (kind == 1 && run && d.morpho) n = c->n_active * spt;
+                if (kind == 2 && d.morpho) n = c->n_active * TILE_PARTS;
+            }
+            sc[threadIdx.x] = n;
             __syncthreads();
-            sc[threadIdx.x] += v;
+            for (int o = 1; o < BATCH_PLAN_THREADS; o <<= 1) {
+                const int v = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
+                __syncthreads();
+                sc[threadIdx.x] += v;
+                __syncthreads();
+            }
+            if (s < bt.nslot) off[s] = carry + sc[threadIdx.x] - n;
+            carry += sc[BATCH_PLAN_THREADS - 1];
             __syncthreads();
         }
-        if (s < bt.nslot) bt.off[s] = carry + sc[threadIdx.x] - n;
-        carry += sc[BATCH_PLAN_THREADS - 1];
+        if (threadIdx.x == 0) off[bt.nslot] = carry;
         __syncthreads();
     }
-    if (threadIdx.x == 0) bt.off[bt.nslot] = carry;
 }
 
 // One RK stage over the work list.  The grid is a multiple of 8 blocks that
 // stay resident and walk the list; XCD x (blocks x, x+8, ...) takes the x-th
 // contiguous eighth of it, so neighbouring sub-blocks share an L2.
+// second: the second hydraulic operator of the morphodynamic slots (dev2, off2)
 template <int STAGE, bool VISC, int LIM, bool MIX>
-__global__ __launch_bounds__(STAGE_NT) void batch_stage_kernel(Batch bt) {
+__global__ __launch_bounds__(STAGE_NT) void batch_stage_kernel(Batch bt, int second) {
     __shared__ SmemT<VISC ? 7 : 5, MIX> s;
     __shared__ int sh_slot;
-    const int total = bt.off[bt.nslot];
+    const int* off = second ? bt.off2 : bt.off;
+    const Dev* dev = second ? bt.dev2 : bt.dev;
+    const int total = off[bt.nslot];
     const int m = gridDim.x / 8, x = blockIdx.x % 8;
     const int per = (total + 7) / 8;
     const int end = min((x + 1) * per, total);
@@ -2239,14 +2259,14 @@ __global__ __launch_bounds__(STAGE_NT) void batch_stage_kernel(Batch bt) {
             int lo = 0, hi = bt.nslot - 1;       // last slot with off <= w
             while (lo < hi) {
                 const int mid = (lo + hi + 1) / 2;
-                if (bt.off[mid] <= w) lo = mid; else hi = mid - 1;
+                if (off[mid] <= w) lo = mid; else hi = mid - 1;
             }
             sh_slot = lo;
         }
         __syncthreads();
         const int sl = __builtin_amdgcn_readfirstlane(sh_slot);
-        stage_body<STAGE, VISC, LIM, MIX>(const_arg(bt.dev + sl), generic(const_arg(bt.par + sl)), s,
-                                          w - bt.off[sl]);
+        stage_body<STAGE, VISC, LIM, MIX>(const_arg(dev + sl), generic(const_arg(bt.par + sl)), s,
+                                          w - off[sl]);
         __syncthreads();
     }
 }
@@ -2346,11 +2366,15 @@ struct kst_batch {
     std::vector<int> on;
     Batch bt{};
     Dev* d_dev = nullptr;
+    Dev* d_dev2 = nullptr;         // each slot's second hydraulic operator
     Params* d_par = nullptr;
     int *d_on = nullptr, *d_off = nullptr, *d_status = nullptr;
+    int *d_off2 = nullptr, *d_off3 = nullptr;
     int visc = -1, mixed = -1;     // fixed by the first scenario attached
     int anylim = 0;                // a running scenario's limiter is not MinMod2
-    int grid = 0;
+    int morpho = 0;                // a running scenario has morphodynamics
+    int grid = 0;                  // resident grids of the stage, morphodynamic and (tile, part) kernels
+    int mgrid = 0, tgrid = 0;
     hipGraphExec_t gexec = nullptr;
     std::string err;
 };
@@ -3587,7 +3611,7 @@ static int batch_err(kst_batch* b, int code, const std::string& msg) {
 
 constexpr int kBatchIters = 8;
 
-typedef void (*BatchStageFn)(Batch);
+typedef void (*BatchStageFn)(Batch, int);
 struct BatchSet { BatchStageFn fn[4]; };
 
 // MinMod2 (the reference's default) at compile time, or any limiter per
@@ -3608,14 +3632,30 @@ static const BatchSet& batch_set(bool visc, bool mixed, bool anylim) {
     return sets[anylim ? 1 : 0][visc ? 1 : 0][mixed ? 1 : 0];
 }
 
+// One iteration: activation, the work lists, the first hydraulic operator;
+// with morphodynamic slots their M(2dt) (three stages, redistribution, face
+// topography of the new bed) and second hydraulic operator; step control;
+// the accepted beds' commit (the launch sequence of one scenario).
 static int batch_launch_iteration(kst_batch* b) {
     const Batch& bt = b->bt;
     const BatchSet& k = batch_set(b->visc == 1, b->mixed == 1, b->anylim == 1);
+    const bool mo = b->morpho != 0;
     hipLaunchKernelGGL(batch_activate_kernel, dim3(b->nslot), dim3(256), 0, b->stream, bt);
-    hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(BATCH_PLAN_THREADS), 0, b->stream, bt);
+    hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(BATCH_PLAN_THREADS), 0, b->stream, bt, mo ? 3 : 1);
     for (int q = 0; q < 4; ++q)
-        hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(STAGE_NT), 0, b->stream, bt);
+        hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(STAGE_NT), 0, b->stream, bt, 0);
+    if (mo) {
+        constexpr int NT = morpho_nt<B>();
+        hipLaunchKernelGGL((batch_morpho_kernel<1>), dim3(b->mgrid), dim3(NT), 0, b->stream, bt);
+        hipLaunchKernelGGL((batch_morpho_kernel<2>), dim3(b->mgrid), dim3(NT), 0, b->stream, bt);
+        hipLaunchKernelGGL((batch_morpho_kernel<3>), dim3(b->mgrid), dim3(NT), 0, b->stream, bt);
+        hipLaunchKernelGGL(batch_redistribute_kernel, dim3(b->nslot), dim3(RD_THREADS), 0, b->stream, bt);
+        hipLaunchKernelGGL((batch_tile_part_kernel<0>), dim3(b->tgrid), dim3(256), 0, b->stream, bt);
+        for (int q = 0; q < 4; ++q)
+            hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(STAGE_NT), 0, b->stream, bt, 1);
+    }
     hipLaunchKernelGGL(batch_finalize_kernel, dim3((b->nslot + 63) / 64), dim3(64), 0, b->stream, bt);
+    if (mo) hipLaunchKernelGGL((batch_tile_part_kernel<1>), dim3(b->tgrid), dim3(256), 0, b->stream, bt);
     BCHK(b, hipGetLastError());
     return 0;
 }
@@ -3658,23 +3698,34 @@ int kst_batch_create(int max_slots, kst_batch** out) {
     if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess)
         return fail(KST_ERR_HIP);
     if (hipMalloc(&b->d_dev, sizeof(Dev) * max_slots) != hipSuccess ||
+        hipMalloc(&b->d_dev2, sizeof(Dev) * max_slots) != hipSuccess ||
         hipMalloc(&b->d_par, sizeof(Params) * max_slots) != hipSuccess ||
         hipMalloc(&b->d_on, sizeof(int) * max_slots) != hipSuccess ||
         hipMalloc(&b->d_off, sizeof(int) * (max_slots + 1)) != hipSuccess ||
+        hipMalloc(&b->d_off2, sizeof(int) * (max_slots + 1)) != hipSuccess ||
+        hipMalloc(&b->d_off3, sizeof(int) * (max_slots + 1)) != hipSuccess ||
         hipMalloc(&b->d_status, sizeof(int) * max_slots) != hipSuccess)
         return fail(KST_ERR_HIP);
     if (hipMemset(b->d_on, 0, sizeof(int) * max_slots) != hipSuccess ||
         hipMemset(b->d_off, 0, sizeof(int) * (max_slots + 1)) != hipSuccess ||
+        hipMemset(b->d_off2, 0, sizeof(int) * (max_slots + 1)) != hipSuccess ||
+        hipMemset(b->d_off3, 0, sizeof(int) * (max_slots + 1)) != hipSuccess ||
         hipMemset(b->d_status, 0, sizeof(int) * max_slots) != hipSuccess)
         return fail(KST_ERR_HIP);
-    b->bt = Batch{b->d_dev, b->d_par, b->d_on, b->d_off, b->d_status, max_slots};
-    // a resident grid: the stage kernel's occupancy on every CU, a multiple of 8
-    int cus = 0, per_cu = 0;
+    b->bt = Batch{b->d_dev, b->d_par, b->d_on, b->d_off, b->d_status, max_slots,
+                  b->d_dev2, b->d_off2, b->d_off3};
+    // resident grids: each kernel's occupancy on every CU, a multiple of 8
+    int cus = 0, per_cu = 0, per_cu_m = 0, per_cu_t = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, batch_stage_kernel<2, true, KST_LIMITER_MINMOD2, false>, STAGE_NT, 0) != hipSuccess)
+            &per_cu, batch_stage_kernel<2, true, KST_LIMITER_MINMOD2, false>, STAGE_NT, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_m, batch_morpho_kernel<2>, morpho_nt<B>(), 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, batch_tile_part_kernel<0>, 256, 0) != hipSuccess)
         return fail(KST_ERR_HIP);
     b->grid = std::max(8, (cus * std::max(per_cu, 1)) / 8 * 8);
+    b->mgrid = std::max(8, (cus * std::max(per_cu_m, 1)) / 8 * 8);
+    b->tgrid = std::max(8, (cus * std::max(per_cu_t, 1)) / 8 * 8);
     *out = b;
     return KST_OK;
 }
@@ -3682,7 +3733,6 @@ int kst_batch_create(int max_slots, kst_batch** out) {
 int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot) {
     if (!b || !h || !slot) return KST_ERR_CONFIG;
     if (!h->have_state) return batch_err(b, KST_ERR_STATE, "kst_set_state must come before kst_batch_attach");
-    if (h->d.morpho) return batch_err(b, KST_ERR_CONFIG, "batches run the hydraulic operator only");
     if (h->p.oned) return batch_err(b, KST_ERR_CONFIG, "batches run 2D grids only");
     const int visc = h->p.nu != 0.0 ? 1 : 0, mixed = h->d.mixed ? 1 : 0;
     bool any = false;
@@ -3694,17 +3744,20 @@ int kst_batch_attach(kst_batch* b, kst_handle* h, int* slot) {
         if (!b->slots[q]) s = q;
     if (s < 0) return batch_err(b, KST_ERR_CONFIG, "no free slot");
     const int anylim = ((any && b->anylim) || h->p.limiter != KST_LIMITER_MINMOD2) ? 1 : 0;
-    if (b->visc != visc || b->mixed != mixed || b->anylim != anylim) {
+    const int morpho = ((any && b->morpho) || h->d.morpho) ? 1 : 0;
+    if (b->visc != visc || b->mixed != mixed || b->anylim != anylim || b->morpho != morpho) {
         // the graph's kernels depend on the setting
         if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
         b->visc = visc;
         b->mixed = mixed;
         b->anylim = anylim;
+        b->morpho = morpho;
     }
     BCHK(b, hipStreamSynchronize(h->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     const int one = 1, st = ST_DONE;
     BCHK(b, hipMemcpy(b->d_dev + s, &h->d, sizeof(Dev), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->d_dev2 + s, &h->d2, sizeof(Dev), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->d_par + s, &h->p, sizeof(Params), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->d_status + s, &st, sizeof(int), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->d_on + s, &one, sizeof(int), hipMemcpyHostToDevice));
@@ -3805,8 +3858,8 @@ void kst_batch_destroy(kst_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    (void)hipFree(b->d_dev); (void)hipFree(b->d_par); (void)hipFree(b->d_on);
-    (void)hipFree(b->d_off); (void)hipFree(b->d_status);
+    (void)hipFree(b->d_dev); (void)hipFree(b->d_dev2); (void)hipFree(b->d_par); (void)hipFree(b->d_on);
+    (void)hipFree(b->d_off); (void)hipFree(b->d_off2); (void)hipFree(b->d_off3); (void)hipFree(b->d_status);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -198,3 +198,39 @@ def test_batch_mixes_limiters(gpu):
                 assert st[key] == alone[k][0][key], f"{wls[k].name} ({prec}): {key}"
             same(g.state(), alone[k][1], f"{wls[k].name} ({prec})")
         b.close()
+
+
+def test_batch_with_morphodynamics_equals_alone(gpu):
+    """Morphodynamic scenarios (the Strang step: three morphodynamic stages,
+    the redistribution walk, the face topography of the new bed, the second
+    hydraulic operator, the bed commit) of different sizes and closures in
+    one batch with hydraulic ones: every one the same bits as alone,
+    counters, fields and the bed."""
+    wls = [workloads.morpho_cap(64, 0, tile=16, radius=10.0, erosion="simple", deposition="simple",
+                                transition="step", damping="rat3"),
+           workloads.morpho_cap(96, 0, tile=16, radius=14.0, psi=0.2),
+           workloads.morpho_cap(64, 0, tile=16, radius=8.0, drag="pouliquen", erosion="fluid",
+                                deposition="simple", transition="none", damping="none")]
+    wls += members(C.PREC_FP64, 2)
+    steps = 40
+    alone = []
+    for w in wls:
+        g = stepper_for(w)
+        st = g.step_n(steps)
+        alone.append((st, g.state(), g.tile_bt()))
+        g.close()
+    assert any(a[0]["redist_cells"] > 0 for a in alone[:3]), "no redistribution exercised"
+    b = Batch(8)
+    gs = [stepper_for(w) for w in wls]
+    slots = [b.attach(g) for g in gs]
+    for s in slots:
+        b.start(s, nsteps=steps)
+    status = b.run()
+    for k, (s, g) in enumerate(zip(slots, gs)):
+        assert status[s] == SLOT_DONE, wls[k].name
+        st = b.stats(s)
+        for key in STATS:
+            assert st[key] == alone[k][0][key], f"{wls[k].name}: {key} {st[key]} vs {alone[k][0][key]}"
+        same(g.state(), alone[k][1], wls[k].name)
+        assert np.array_equal(g.tile_bt(), alone[k][2]), f"{wls[k].name}: bed"
+    b.close()
