@@ -1,13 +1,16 @@
 """Bicubic resampling of elevation data (Bicubic, src/Interp2d.f90:214-282).
 
-GetHeights (src/dem.f90:360-393) maps each vertex into the elevation grid's
-1-based data coordinates and evaluates there a bicubic patch: on the data
-cell holding the point, f(t, u) = sum a_pq t^p u^q is fixed by f, f_x, f_y
-and f_xy at the cell's four corners, the derivatives taken by central
-differences over the neighbouring samples; the 16 coefficients are M times
-the 16 corner values.  Reading the DEM file itself (GDAL, src/dem.f90 and
-src/GeoTiffRead.f90) is not part of this package: the caller passes the
-elevation samples as an array.
+TileHeightData (src/dem.f90:320-340), which GetHeights (src/dem.f90:360-393)
+calls for a DEM topography, gives each vertex the mean of four bicubic
+evaluations, at the vertex offset by +-dx/2 and +-dy/2, each point mapped
+into the elevation grid's 1-based data coordinates (tile_heights).  bicubic()
+is the kernel of that resampling: on the data cell holding a point,
+f(t, u) = sum a_pq t^p u^q is fixed by f, f_x, f_y and f_xy at the cell's four
+corners, the derivatives taken by central differences over the neighbouring
+samples; the 16 coefficients are M times the 16 corner values.  Reading the
+DEM file itself (GDAL, src/dem.f90 and src/GeoTiffRead.f90) is not part of
+this package: the caller passes the elevation samples, the raster's origin
+and pixel size as the file gives them.
 """
 from __future__ import annotations
 
@@ -87,3 +90,26 @@ def bicubic(z: np.ndarray, x, y) -> np.ndarray:
     for p in range(3, -1, -1):
         ans = t * ans + ((a[4 * p + 3] * u + a[4 * p + 2]) * u + a[4 * p + 1]) * u + a[4 * p]
     return ans
+
+
+def tile_heights(z: np.ndarray, xv, yv, dx: float, dy: float, centre=(0.0, 0.0),
+                 origin=(0.0, 0.0), pixel=(1.0, 1.0)) -> np.ndarray:
+    """TileHeightData (src/dem.f90:320-340) at vertices (xv[i], yv[j]) of a
+    domain centred at `centre` (its UTM easting and northing): the four points
+    centre + (xv +- dx/2, yv +- dy/2), taken in the order (-,-), (-,+), (+,-),
+    (+,+), each mapped to data coordinates (x - origin) / pixel + 1 (pixel
+    widths as the raster gives them, negative for a north-up raster's y) and
+    evaluated by bicubic(); the vertex takes 0.25 times their sum, added in
+    that order.  Returns b0[len(xv), len(yv)] (index i along x)."""
+    xv = np.asarray(xv, dtype=np.float64)
+    yv = np.asarray(yv, dtype=np.float64)
+    X, Y = np.meshgrid(xv, yv, indexing="ij")
+    h = []
+    for a in (0, 1):
+        for b in (0, 1):
+            px = centre[0] + X + ((a + 0.0) - 0.5) * dx
+            py = centre[1] + Y + ((b + 0.0) - 0.5) * dy
+            ix = (px - origin[0]) / pixel[0] + 1
+            iy = (py - origin[1]) / pixel[1] + 1
+            h.append(bicubic(z, ix.ravel(), iy.ravel()).reshape(X.shape))
+    return 0.25 * (((h[0] + h[1]) + h[2]) + h[3])

@@ -76,3 +76,38 @@ def test_bicubic_rejects_points_without_margin():
     z = np.zeros((10, 10))
     with pytest.raises(ValueError):
         dem.bicubic(z, np.array([1.5]), np.array([5.0]))
+
+
+def test_tile_heights_four_point_mean():
+    """TileHeightData's vertex value (src/dem.f90:320-340): the mean of the
+    bicubic at the four points vertex +- dx/2, +- dy/2 in data coordinates.
+    A surface the bicubic reproduces (bilinear plus quadratic) gives the
+    four-point mean of its exact values; on a random raster each of the four
+    evaluations is the reference's compiled Bicubic when that is built."""
+    i = np.arange(1, 41, dtype=float)[:, None]
+    j = np.arange(1, 31, dtype=float)[None, :]
+    z = 3.0 + 2.0 * i - j + 0.5 * i * j + 0.25 * i * i - 0.125 * j * j
+    xv = np.array([-4.0, -1.5, 0.0, 2.5, 6.0])
+    yv = np.array([-3.0, 0.0, 1.25, 4.0])
+    centre, origin, pixel, dx, dy = (500.0, 900.0), (480.0, 880.0), (1.0, 1.0), 2.0, 1.0
+    got = dem.tile_heights(z, xv, yv, dx, dy, centre, origin, pixel)
+    X, Y = np.meshgrid(xv, yv, indexing="ij")
+
+    def f(px, py):
+        x = (px - origin[0]) / pixel[0] + 1
+        y = (py - origin[1]) / pixel[1] + 1
+        return 3.0 + 2.0 * x - y + 0.5 * x * y + 0.25 * x * x - 0.125 * y * y
+
+    want = 0.25 * sum(f(centre[0] + X + sx * dx / 2, centre[1] + Y + sy * dy / 2)
+                      for sx in (-1, 1) for sy in (-1, 1))
+    assert np.allclose(got, want, rtol=0, atol=1e-11)
+    # a planted mistake (the vertex itself, no offsets) differs on a curved surface
+    assert not np.allclose(dem.tile_heights(z, xv, yv, 0.0, 0.0, centre, origin, pixel), want, atol=1e-6)
+    if ref_available():
+        rng = np.random.default_rng(11)
+        zr = rng.uniform(-50.0, 500.0, (40, 30))
+        got = dem.tile_heights(zr, xv, yv, dx, dy, centre, origin, pixel)
+        h = [ref_bicubic(zr, ((centre[0] + X + (a - 0.5) * dx) - origin[0]).ravel() / pixel[0] + 1,
+                         ((centre[1] + Y + (b - 0.5) * dy) - origin[1]).ravel() / pixel[1] + 1).reshape(X.shape)
+             for a in (0, 1) for b in (0, 1)]
+        assert np.array_equal(got, 0.25 * (((h[0] + h[1]) + h[2]) + h[3]))
