@@ -252,7 +252,8 @@ void kst_destroy(kst_handle* h);
 
 /* Launch grids (no counterpart in the reference, which loops over its active
  * tiles on the CPU).  By default every launch covers a power-of-two cap of
- * twice the active set the library last read back, and workgroups take
+ * twice the active set the library last read back (1.25 times from 256
+ * tiles), and workgroups take
  * further work items in turn when the set outgrows it between read-backs.
  * kst_set_grid_cap fixes the cap at `tiles` active tiles (clamped to the
  * grid's tile count) so that grids, and the captured graphs, never change;
