@@ -747,79 +747,6 @@ __device__ __forceinline__ double face_bed(const DevT& d, const Params& p, int t
     }
 }
 
-// What phase A of the FP64 stage body reads for one region cell (relative
-// cell (i, j) of tile t): the stage's input state, with stage 2 the RHS of
-// u^n it is formed from, the cell's b0, bt and gamma, whether its tile is
-// active, and u^n's w (the value a ghost tile's faces take; with UN false it
-// is read later, and only for a ghost cell).
-struct CellIn {
-    double w, ru, rv, h, ew, eu, ev, eh, im, b0c, btc, gam, un;
-    int gi, tc;
-    bool act;
-};
-
-template <int STAGE, bool UN, class DevT>
-__device__ __forceinline__ CellIn load_cell(const DevT& d, const Params& p, int t, int i, int j) {
-    int tc, ci, cj;
-    map_cell(p, t, i, j, tc, ci, cj);
-    CellIn c;
-    const int gi = tc * p.NC + cj * p.nx + ci;
-    const int FS = (int)d.FS;
-    c.gi = gi;
-    c.tc = tc;
-    if (STAGE == 1 || STAGE == 2) {
-        c.w = d.Uw[gi]; c.ru = d.Uu[gi]; c.rv = d.Uv[gi]; c.h = d.Uh[gi];
-    } else {
-        const double* S = (STAGE == 3) ? d.S2 : d.S3;
-        c.w = S[gi]; c.ru = S[FS + gi]; c.rv = S[2 * FS + gi]; c.h = S[3 * FS + gi];
-    }
-    if (STAGE == 2) {
-        c.ew = d.R[gi]; c.eu = d.R[FS + gi]; c.ev = d.R[2 * FS + gi];
-        c.eh = d.R[3 * FS + gi]; c.im = d.R[4 * FS + gi];
-    } else {
-        c.ew = c.eu = c.ev = c.eh = c.im = 0.0;
-    }
-    c.b0c = d.b0c[gi]; c.btc = d.T.btc[gi]; c.gam = d.T.cgam[gi];
-    c.act = d.tstate[tc] == KST_TILE_ACTIVE;
-    c.un = UN ? d.S0[gi] : 0.0;
-    return c;
-}
-
-// A tile all eight of whose neighbours exist: t itself when it is one, else
-// an interior tile (the grid has at least three tiles along each dimension
-// unless its boundaries are periodic).  Loads issued for a tile read before
-// the control block say whether it is the block's are made through it, so
-// that a stale entry of the active list addresses nothing out of bounds.
-__device__ __forceinline__ int interior_tile(const Params& p, int t) {
-    const int fallback = p.periodic ? 0 : (p.oned ? 1 : p.ntx + 1);
-    if (t < 0 || t >= p.ntiles) return fallback;
-    if (p.periodic) return t;
-    const int i = t % p.ntx, j = t / p.ntx;
-    if (i == 0 || i == p.ntx - 1) return fallback;
-    if (!p.oned && (j == 0 || j == p.nty - 1)) return fallback;
-    return t;
-}
-
-// The cells of phase A: thread tid's region slot (rx, ry) of a bxa x bya
-// sub-block in the given round; false when it has none.
-template <int BY, bool ONED>
-__device__ __forceinline__ bool region_slot(int tid, int round, int bxa, int bya, int& rx, int& ry) {
-    constexpr int OWN = B * BY;
-    constexpr bool one_round = STAGE_NT >= OWN + 128;
-    const int hq = one_round ? tid - OWN : tid - 128;
-    if (round == 0 && tid < OWN) {
-        rx = tid % B; ry = tid / B;
-        return rx < bxa && ry < bya;
-    }
-    if (hq < 0 || hq >= 128) return false;
-    const int strip = hq >> 5, k = hq & 31, dd = k >> 4, pos = k & 15;
-    if (strip == 0) { rx = -2 + dd; ry = pos; return pos < bya; }
-    if (strip == 1) { rx = bxa + dd; ry = pos; return pos < bya; }
-    if (strip == 2) { rx = pos; ry = -2 + dd; return pos < bxa && !ONED; }
-    rx = pos; ry = bya + dd;
-    return pos < bxa && !ONED;   // no rows beyond a 1D grid's
-}
-
 // STAGE 1: RHS(u^n) -> R.            (IntegrateTo, src/TimeStepper.f90:155,220)
 // STAGE 2: u1 = f(u^n, R); RHS(u1); u2 -> S2.   (src/TimeStepper.f90:371-444)
 // STAGE 3: RHS(u2); u3 -> S3.                   (src/TimeStepper.f90:447-498)
@@ -906,93 +833,9 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     unsigned tidu = threadIdx.x;
     asm("" : "+v"(tidu) : "s"(pass));
     const int tid = (int)(tidu & (STAGE_NT - 1));
-    constexpr bool SPEC = BY < B;
-    constexpr int OWN = B * BY;
-    constexpr bool one_round = STAGE_NT >= OWN + 128;
-    // A sub-block with at most one face per thread (the small shape) issues
-    // its other global loads ahead of the state's so that they overlap: the
-    // CorrectSlopes beds of its faces here (kept in a register until the
-    // state is staged), below what phases B and C read.
-    constexpr bool PRE = !MIX && Smem::NFX + Smem::NFY <= STAGE_NT;
-    constexpr int NXB = BY * (B + 3), NYB = B * (BY + 3);   // x-face and y-face beds
-    static_assert(!PRE || (NXB <= 128 && NYB <= 128 && Smem::NFX <= 128 && Smem::NFY <= 128),
-                  "split_item: one bed and one face per thread, each kind within two waves");
-    const bool trk = STAGE == 4 && p.track_maxima && !d.second;
-    // EARLY: the small shape in FP64 takes its item from cap_item, which needs
-    // no control-block read, so on its first pass every global load of phase
-    // A and the topography phases B and C read are issued before the control
-    // block is read, and the two overlap.  The loads address the guessed
-    // item's tile through interior_tile; a block whose item, once the control
-    // block is read, differs from the guess (an active set that outgrew the
-    // grid, a list entry past the active set) issues them again for its own.
-    constexpr bool EARLY = SPEC && PRE;
-    int e_t = -1, e_bx0 = 0, e_by0 = 0;
-    CellIn ecin{};
-    double ebedv = 0.0;
-    FaceTopo epft{};
-    double ec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // cell bx, by, pxu, pxv, pyv; stage 3: u^n
-    double emxv[6] = {0, 0, 0, 0, 0, 0}, emun[4] = {0, 0, 0, 0};
-    double eside[4] = {0, 0, 0, 0};               // the neighbours' edge beds of row/column tid
-    bool ehs = false;
-    if constexpr (EARLY) {
-        if (witem < 0 && pass == 0) {
-            const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
-            const int ws = cap_item();
-            const int tt = interior_tile(p, d.active[min(ws / spt, p.ntiles)]);   // ntiles + 1 entries
-            const int sub = ws % spt;
-            const int bx0 = (sub % sbx) * B, by0 = (sub / sbx) * BY;
-            const int ea = min(B, p.nx - bx0), eb = min(BY, p.ny - by0);
-            e_t = tt; e_bx0 = bx0; e_by0 = by0;
-            int rx, ry;
-            if (region_slot<BY, Smem::oned>(tid, 0, ea, eb, rx, ry))
-                ecin = load_cell<STAGE, true>(d, p, tt, bx0 + rx, by0 + ry);
-            const int tx = tid % B, ty = tid / B;
-            const bool own_ = tx < ea && ty < eb;
-            const int oi = tt * p.NC + (by0 + ty) * p.nx + (bx0 + tx);
-            if (ea == B && eb == BY) {
-                bool yb;
-                const int q = split_item(tid, NXB, NYB, &yb);
-                if (q >= 0) {
-                    if (!yb) ebedv = face_bed<false>(d, p, tt, bx0 + q % (B + 3) - 1, by0 + q / (B + 3));
-                    else ebedv = face_bed<true>(d, p, tt, by0 + q % (BY + 3) - 1, bx0 + q / (BY + 3));
-                }
-                bool yf;
-                const int f = split_item(tid, Smem::NFX, Smem::NFY, &yf);
-                if (f >= 0) {
-                    if (!yf) epft = load_face_topo_at<false, STAGE != 4>(d, p, tt, bx0, by0, f % (B + 1), f / (B + 1));
-                    else epft = load_face_topo_at<true, STAGE != 4>(d, p, tt, bx0, by0, f / B, f % B);
-                }
-                if (own_ && STAGE == 3) { ec[5] = d.Uw[oi]; ec[6] = d.Uu[oi]; ec[7] = d.Uv[oi]; ec[8] = d.Uh[oi]; }
-            }
-            if (own_) {
-                ec[0] = d.T.cbx[oi]; ec[1] = d.T.cby[oi];
-                ec[2] = d.T.pxu[oi]; ec[3] = d.T.pxv[oi]; ec[4] = d.T.pyv[oi];
-            }
-            if (trk && own_) {
-                const long long FS_ = d.FS;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) emxv[k] = d.maxima[2 * k * FS_ + oi];   // fields 0, 2, ..., 10
-#pragma unroll
-                for (int k = 0; k < 4; ++k) emun[k] = d.S0[k * FS_ + oi];
-            }
-            if (tid < B) {
-                const int r = tid;
-                if (r < eb) {
-                    const int row = (by0 + r) * (p.nx + 1);
-                    if (bx0 == 0) eside[0] = d.T.xib[nb_tile(p, tt, -1, 0) * p.NFX + row + p.nx];
-                    if (bx0 + ea == p.nx) eside[1] = d.T.xib[nb_tile(p, tt, 1, 0) * p.NFX + row];
-                }
-                if (!Smem::oned && r < ea) {
-                    const int col = bx0 + r;
-                    if (by0 == 0) eside[2] = d.T.yib[nb_tile(p, tt, 0, -1) * p.NFY + p.ny * p.nx + col];
-                    if (by0 + eb == p.ny) eside[3] = d.T.yib[nb_tile(p, tt, 0, 1) * p.NFY + col];
-                }
-            }
-            ehs = p.n_sources > 0 && d.has_src[tt];
-        }
-    }
     if (tid == 0) {
         const int sbx = (p.nx + B - 1) / B, sby = (p.ny + BY - 1) / BY, spt = sbx * sby;
+        constexpr bool SPEC = BY < B;
         int wspec = 0, tspec = 0;
         if (SPEC && witem < 0) {
             wspec = cap_item();
@@ -1046,8 +889,6 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     const double dt = s.dt;
     const int FS = (int)d.FS;
     const int bxa = s.bxa, bya = s.bya;
-    // the loads issued ahead are this item's
-    const bool hit = EARLY && witem < 0 && pass == 0 && t == e_t && s.bx0 == e_bx0 && s.by0 == e_by0;
 
     // ---- phase A: stage state + derived variables on the region ----------
     const int tx = tid % B, ty = tid / B;
@@ -1061,11 +902,19 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     // 2 x 16 slots) by the next 128 threads in the same round when there are
     // enough, else by threads 128..255 in a second round (waves 0 and 1 take
     // the extra faces of phase B instead)
+    constexpr int OWN = B * BY;
+    constexpr bool one_round = STAGE_NT >= OWN + 128;
+    // A sub-block with at most one face per thread (the small shape) issues
+    // its other global loads ahead of the state's so that they overlap: the
+    // CorrectSlopes beds of its faces here (kept in a register until the
+    // state is staged), below what phases B and C read.
+    constexpr bool PRE = !MIX && Smem::NFX + Smem::NFY <= STAGE_NT;
     const bool pre = PRE && bxa == B && bya == BY;
+    constexpr int NXB = BY * (B + 3), NYB = B * (BY + 3);   // x-face and y-face beds
+    static_assert(!PRE || (NXB <= 128 && NYB <= 128 && Smem::NFX <= 128 && Smem::NFY <= 128),
+                  "split_item: one bed and one face per thread, each kind within two waves");
     double bedv = 0.0;
-    if (hit) {
-        bedv = ebedv;
-    } else if (pre) {
+    if (pre) {
         bool yb;
         const int q = split_item(tid, NXB, NYB, &yb);
         if (q >= 0) {
@@ -1088,11 +937,9 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             else { rx = pos; ry = bya + dd; valid = pos < bxa && !Smem::oned; }   // no rows beyond a 1D grid's
         }
         if (!valid) continue;
-        const bool ecl = hit && round == 0;   // this cell's loads were issued ahead
-        int tc = 0, ci = 0, cj = 0;
-        if (ecl) tc = ecin.tc;
-        else map_cell(p, t, s.bx0 + rx, s.by0 + ry, tc, ci, cj);
-        const int gi = ecl ? ecin.gi : tc * p.NC + cj * p.nx + ci;
+        int tc, ci, cj;
+        map_cell(p, t, s.bx0 + rx, s.by0 + ry, tc, ci, cj);
+        const int gi = tc * p.NC + cj * p.nx + ci;
         double w, ru, rv, h;
         if constexpr (MIX) {
             w = Un[gi]; ru = Un[FS + gi]; rv = Un[2 * FS + gi]; h = Un[3 * FS + gi];
@@ -1113,27 +960,22 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
                 h = h + (double)Sf[3 * FS + gi];
             }
         } else if (STAGE == 1 || STAGE == 2) {
-            if (ecl) { w = ecin.w; ru = ecin.ru; rv = ecin.rv; h = ecin.h; }
-            else { w = d.Uw[gi]; ru = d.Uu[gi]; rv = d.Uv[gi]; h = d.Uh[gi]; }
+            w = d.Uw[gi]; ru = d.Uu[gi]; rv = d.Uv[gi]; h = d.Uh[gi];
             if (STAGE == 2) {
                 if (round == 0) { pw = w; pru = ru; prv = rv; ph = h; }
-                const double ew = ecl ? ecin.ew : d.R[gi], eu = ecl ? ecin.eu : d.R[FS + gi];
-                const double ev = ecl ? ecin.ev : d.R[2 * FS + gi];
-                const double eh = ecl ? ecin.eh : d.R[3 * FS + gi], im = ecl ? ecin.im : d.R[4 * FS + gi];
+                const double ew = d.R[gi], eu = d.R[FS + gi], ev = d.R[2 * FS + gi];
+                const double eh = d.R[3 * FS + gi], im = d.R[4 * FS + gi];
                 ru = (ru + dt * eu) / (1.0 - dt * im);
                 rv = (rv + dt * ev) / (1.0 - dt * im);
                 w = w + dt * ew;
                 h = h + dt * eh;
             }
-        } else if (ecl) {
-            w = ecin.w; ru = ecin.ru; rv = ecin.rv; h = ecin.h;
         } else {
             const double* S = (STAGE == 3) ? d.S2 : d.S3;
             w = S[gi]; ru = S[FS + gi]; rv = S[2 * FS + gi]; h = S[3 * FS + gi];
         }
-        const double b0c = ecl ? ecin.b0c : d.b0c[gi], btc = ecl ? ecin.btc : d.T.btc[gi];
-        const double gam = ecl ? ecin.gam : d.T.cgam[gi];
-        const bool act = ecl ? ecin.act : d.tstate[tc] == KST_TILE_ACTIVE;
+        const double b0c = d.b0c[gi], btc = d.T.btc[gi], gam = d.T.cgam[gi];
+        const bool act = d.tstate[tc] == KST_TILE_ACTIVE;
         // ComputeDesingularisedVariables (src/HydraulicRHS.f90:762-878) runs
         // on active tiles only; ghost tiles keep SetDefaultTileData's zeros
         double hn = 0.0, psi = 0.0, rho = p.rhow, uu = 0.0, vv = 0.0;
@@ -1155,7 +997,7 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         const int r = rg(rx, ry);
         if constexpr (Smem::oned) s.rv[r] = rv;
         s.w[r] = w; s.h[r] = h; s.u[r] = uu; s.v[r] = vv; s.rho[r] = rho;
-        s.wst[r] = act ? w : (ecl ? ecin.un : Un[gi]);
+        s.wst[r] = act ? w : Un[gi];
         s.gidx[r] = gi;
         s.act[r] = act ? 1 : 0;
         if (round == 0) {
@@ -1170,11 +1012,7 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     FaceTopo pft{};
     double pbx = 0, pby = 0, pgxu = 0, pgxv = 0, pgyv = 0, pw0 = 0, pu0 = 0, pv0 = 0, ph0 = 0;
     const bool prec = pre || STAGE == 1 || STAGE == 4;
-    if (hit) {
-        pft = epft;
-        pbx = ec[0]; pby = ec[1]; pgxu = ec[2]; pgxv = ec[3]; pgyv = ec[4];
-        pw0 = ec[5]; pu0 = ec[6]; pv0 = ec[7]; ph0 = ec[8];
-    } else if (pre) {
+    if (pre) {
         bool yf;
         const int f = split_item(tid, Smem::NFX, Smem::NFY, &yf);
         if (f >= 0) {
@@ -1183,19 +1021,15 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         }
         if (own && STAGE == 3) { pw0 = d.Uw[oidx]; pu0 = d.Uu[oidx]; pv0 = d.Uv[oidx]; ph0 = d.Uh[oidx]; }
     }
-    if (prec && own && !hit) {
+    if (prec && own) {
         pbx = d.T.cbx[oidx]; pby = d.T.cby[oidx];
         pgxu = d.T.pxu[oidx]; pgxv = d.T.pxv[oidx]; pgyv = d.T.pyv[oidx];
     }
     // the last stage's maxima update reads u^n and six maxima fields of the
     // owned cell (each written only by its owner): loaded here too
+    const bool trk = STAGE == 4 && p.track_maxima && !d.second;
     double mxv[6] = {0, 0, 0, 0, 0, 0}, mun[4] = {0, 0, 0, 0};
-    if (hit) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) mxv[k] = emxv[k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) mun[k] = emun[k];
-    } else if (trk && own) {
+    if (trk && own) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) mxv[k] = d.maxima[2 * k * FS + oidx];   // fields 0, 2, ..., 10
 #pragma unroll
@@ -1229,13 +1063,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
                 const int tn = nb_tile(p, t, -1, 0);
                 s.xside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed<Smem::oned>(d, p, tn, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
-                                    : (hit ? eside[0] : d.T.xib[tn * p.NFX + row + p.nx]);
+                                    : d.T.xib[tn * p.NFX + row + p.nx];
             }
             if (s.bx0 + bxa == p.nx) {
                 const int tn = nb_tile(p, t, 1, 0);
                 s.xside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed<Smem::oned>(d, p, tn, 0, s.by0 + r, 0, s.by0 + r + 1)
-                                    : (hit ? eside[1] : d.T.xib[tn * p.NFX + row]);
+                                    : d.T.xib[tn * p.NFX + row];
             }
         }
         if (!Smem::oned && r < bxa) {
@@ -1244,13 +1078,13 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
                 const int tn = nb_tile(p, t, 0, -1);
                 s.yside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed(d, p, tn, col, p.ny, col + 1, p.ny)
-                                    : (hit ? eside[2] : d.T.yib[tn * p.NFY + p.ny * p.nx + col]);
+                                    : d.T.yib[tn * p.NFY + p.ny * p.nx + col];
             }
             if (s.by0 + bya == p.ny) {
                 const int tn = nb_tile(p, t, 0, 1);
                 s.yside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
                                     ? ghost_bed(d, p, tn, col, 0, col + 1, 0)
-                                    : (hit ? eside[3] : d.T.yib[tn * p.NFY + col]);
+                                    : d.T.yib[tn * p.NFY + col];
             }
         }
     }
@@ -1356,7 +1190,7 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     // ExplicitSourceTerms (src/Equations.f90:456-620); with no source the
     // flux terms are 0/(gam*gam) = 0 exactly
     double Ew, Eh;
-    if (p.n_sources > 0 && (hit ? ehs : d.has_src[t] != 0)) {
+    if (p.n_sources > 0 && d.has_src[t]) {
         double Qt = 0.0, psiQt = 0.0;
         const int ti = t % p.ntx, tj = t / p.ntx;
         const int i = s.bx0 + tx, j = s.by0 + ty;
