@@ -1743,6 +1743,20 @@ __device__ __forceinline__ int side_tile(const Params& p, int k, int r) {
     return nb_tile(p, k, di, dj);
 }
 
+// side_tile(p, k, r) for r = 0..3 (N, S, E, W), with one division at most.
+__device__ __forceinline__ void pass_sides(const Params& p, int k, int (&n)[4]) {
+    if (!p.periodic) {
+        n[0] = k + p.ntx; n[1] = k - p.ntx; n[2] = k + 1; n[3] = k - 1;
+        return;
+    }
+    const int tj = k / p.ntx, ti = k - tj * p.ntx;
+    const int row = tj * p.ntx;
+    n[0] = ti + (tj + 1 >= p.nty ? tj + 1 - p.nty : tj + 1) * p.ntx;
+    n[1] = ti + (tj - 1 < 0 ? tj - 1 + p.nty : tj - 1) * p.ntx;
+    n[2] = row + (ti + 1 >= p.ntx ? ti + 1 - p.ntx : ti + 1);
+    n[3] = row + (ti - 1 < 0 ? ti - 1 + p.ntx : ti - 1);
+}
+
 // What AddTile (src/UpdateTiles.f90:56-78) does with neighbour n of an
 // active tile whose edge the flow nears: nothing if n is no tile (id -1 in
 // the reference's 0-based terms) or already active; outside the domain or on
@@ -1751,6 +1765,15 @@ __device__ __forceinline__ int side_tile(const Params& p, int k, int r) {
 enum { ADD_NONE = 0, ADD_TILE = 1, ADD_HALT = 2 };
 __device__ __forceinline__ int add_action(const Params& p, const uint8_t* ts, int n) {
     if (n <= -1 || (n < p.ntiles && ts[n] == KST_TILE_ACTIVE)) return ADD_NONE;
+    if (n >= p.ntiles || (!p.periodic && edge_tile(p, n)))
+        return p.bc == KST_BC_HALT ? ADD_HALT : ADD_NONE;
+    return ADD_TILE;
+}
+
+// The action for neighbour n given its state byte (read by the caller when
+// 0 <= n < ntiles).
+__device__ __forceinline__ int add_action_of(const Params& p, int n, uint8_t tsn) {
+    if (n <= -1 || (n < p.ntiles && tsn == KST_TILE_ACTIVE)) return ADD_NONE;
     if (n >= p.ntiles || (!p.periodic && edge_tile(p, n)))
         return p.bc == KST_BC_HALT ? ADD_HALT : ADD_NONE;
     return ADD_TILE;
@@ -1765,76 +1788,251 @@ __device__ __forceinline__ int ghost_candidate(const Params& p, int k, int r) {
     return nb_tile(p, k, DI[r], DJ[r]);
 }
 
+static constexpr int ACT_NT = 256;      // activation workgroup: 4 waves
+static constexpr int ACT_R = 8;         // active-list entries per thread and round
+static constexpr int MAXCAND = 2048;    // candidate tiles held in LDS
+
+// Ordered append over the workgroup: each thread brings `cnt` items; the
+// return value is the first slot of this thread's items, slots following
+// thread order after the `base` slots already taken; `base` (uniform)
+// grows by the round's total.
+__device__ __forceinline__ int block_append(int cnt, int& base, int* wsum) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int off = base, tot = 0;
+    for (int w = 0; w < ACT_NT / 64; ++w) {
+        if (w < wv) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    base += tot;
+    return off + x - cnt;
+}
+
+// The N, S, E, W passes of CheckIfNearBoundaries over the active list in
+// place (src/TimeStepper.f90:935-1150): a new tile is inserted in order and
+// the later entries of the pass shift, so entries pushed past the pass's
+// initial length are not visited.  One thread walks the list; taken when the
+// candidates do not fit in LDS.  eflags hold each tile's candidate sides.
 template <class DevT>
-__device__ void activate_body(const DevT& d, const Params& p) {
-    __shared__ int newt[MAXNEW];
-    __shared__ int nnew, bad, need;
+__device__ void activate_walk(const DevT& d, const Params& p, int* newt, int& nnew, int& bad) {
+    Ctrl* c = d.ctrl;
+    for (int q = p.oned ? 2 : 0; q < 4 && !bad; ++q) {
+        const int size = c->n_active;
+        for (int tt = 0; tt < size && !bad; ++tt) {
+            const int k = d.active[tt];
+            const int n = side_tile(p, k, q);
+            if (!(d.eflags[k] & (1u << q))) continue;
+            const int act = add_action(p, d.tstate, n);
+            if (act == ADD_HALT) { bad = 1; break; }
+            if (act == ADD_TILE) {
+                int pos = c->n_active;
+                while (pos > 0 && d.active[pos - 1] > n) { d.active[pos] = d.active[pos - 1]; --pos; }
+                d.active[pos] = n;
+                c->n_active++;
+                if (d.tstate[n] == KST_TILE_GHOST) {
+                    for (int gq = 0; gq < c->n_ghost; ++gq)
+                        if (d.ghosts[gq] == n) {
+                            for (int r = gq; r < c->n_ghost - 1; ++r) d.ghosts[r] = d.ghosts[r + 1];
+                            c->n_ghost--;
+                            break;
+                        }
+                }
+                d.tstate[n] = KST_TILE_ACTIVE;
+                d.eflags[n] = 0;
+                if (nnew < MAXNEW) newt[nnew++] = n;
+                else bad = 2;
+            }
+        }
+    }
+}
+
+// Activation after an accepted step (CheckIfNearBoundaries,
+// src/TimeStepper.f90:935-1150; AddTile, src/UpdateTiles.f90:56-78).
+// A flagged side of an active tile is a candidate when its neighbour would
+// be activated or halt the run; sides that are not candidates at the start
+// never act later (tiles only become active), so the walk over the list
+// reduces to a walk over the candidates in list order: a candidate at
+// position i0 of the initial list sits at i0 + (new tiles below it) when
+// the pass reaches it, and is visited if that is below the pass's initial
+// length.  The new tiles then merge into the sorted list, and leave the
+// ghost list in order, in parallel.
+template <class DevT>
+__device__ __forceinline__ void activate_body(const DevT& d, const Params& p) {
+    __shared__ int newt[MAXNEW], srt[MAXNEW], ipos[MAXNEW];
+    __shared__ int cand_k[MAXCAND], cand_e[MAXCAND];
+    __shared__ int wsum[ACT_NT / 64], tl[ACT_NT];
+    __shared__ int nnew, bad, first_move, ng_start;
     Ctrl* c = d.ctrl;
     if (c->status != ST_NEW) return;
     if (threadIdx.x == 0) count_work(c, KW_ACTIVATE);
     if (d.morpho) refresh_ghost_overlay(d, p);
-    // fast path: does any flagged edge face a tile that is not active?
-    if (threadIdx.x == 0) need = 0;
-    __syncthreads();
-    {
-        int mine = 0;
-        for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) {
-            const int k = d.active[q];
-            const unsigned f = d.eflags[k];
-            if (!f) continue;
-            for (int r = p.oned ? 2 : 0; r < 4; ++r)   // E and W only on a 1D grid
-                if ((f & (1u << r)) && add_action(p, d.tstate, side_tile(p, k, r)) != ADD_NONE) mine = 1;
+    const int n0 = c->n_active;
+    const int s0 = p.oned ? 2 : 0;      // E and W only on a 1D grid
+    // candidates in list order; each thread takes ACT_R consecutive entries
+    // with every load of the round in flight at once
+    int ncand = 0;
+    for (int base = 0; base < n0; base += ACT_NT * ACT_R) {
+        const int q0 = base + threadIdx.x * ACT_R;
+        int k[ACT_R];
+        unsigned f[ACT_R];
+        // the loads take clamped (in-range) indices unconditionally and the
+        // results are selected afterwards, so that a round's loads issue
+        // back to back rather than each behind a branch and a wait
+#pragma unroll
+        for (int r = 0; r < ACT_R; ++r) {
+            const int v = d.active[min(q0 + r, n0 - 1)];
+            k[r] = q0 + r < n0 ? v : -1;
         }
-        if (mine) atomicOr(&need, 1);
+#pragma unroll
+        for (int r = 0; r < ACT_R; ++r) {
+            const unsigned v = d.eflags[max(k[r], 0)];
+            f[r] = k[r] >= 0 ? v : 0u;
+        }
+        int nb[ACT_R][4], ts[ACT_R][4];
+#pragma unroll
+        for (int r = 0; r < ACT_R; ++r) {
+            pass_sides(p, max(k[r], 0), nb[r]);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ts[r][s] = d.tstate[min(max(nb[r][s], 0), p.ntiles - 1)];
+        }
+        unsigned b[ACT_R];
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < ACT_R; ++r) {
+            b[r] = 0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (s >= s0 && ((f[r] >> s) & 1u) &&
+                    add_action_of(p, nb[r][s], (uint8_t)ts[r][s]) != ADD_NONE)
+                    b[r] |= 1u << s;
+            if (f[r]) d.eflags[k[r]] = b[r];
+            cnt += b[r] != 0;
+        }
+        int slot = block_append(cnt, ncand, wsum);
+#pragma unroll
+        for (int r = 0; r < ACT_R; ++r)
+            if (b[r]) {
+                if (slot < MAXCAND) { cand_k[slot] = k[r]; cand_e[slot] = ((q0 + r) << 4) | (int)b[r]; }
+                ++slot;
+            }
     }
+    if (ncand == 0) return;   // every flag cleared, nothing to add
     __syncthreads();
-    if (!need) {
-        for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) d.eflags[d.active[q]] = 0;
-        return;
-    }
-    if (threadIdx.x == 0) {
-        nnew = 0;
-        bad = 0;
-        // N, S, E, W passes in order; E and W only on a 1D grid
-        // (CheckIfNearBoundaries, src/TimeStepper.f90:935-940)
-        for (int q = p.oned ? 2 : 0; q < 4 && !bad; ++q) {
-            const int size = c->n_active;
-            for (int tt = 0; tt < size && !bad; ++tt) {
-                const int k = d.active[tt];
-                const int n = side_tile(p, k, q);
-                if (!(d.eflags[k] & (1u << q))) continue;
-                const int act = add_action(p, d.tstate, n);
-                if (act == ADD_HALT) { bad = 1; break; }
-                if (act == ADD_TILE) {
-                    // sorted insertion; later indices of this pass shift as in the reference
-                    int pos = c->n_active;
-                    while (pos > 0 && d.active[pos - 1] > n) { d.active[pos] = d.active[pos - 1]; --pos; }
-                    d.active[pos] = n;
-                    c->n_active++;
-                    if (d.tstate[n] == KST_TILE_GHOST) {
-                        for (int gq = 0; gq < c->n_ghost; ++gq)
-                            if (d.ghosts[gq] == n) {
-                                for (int r = gq; r < c->n_ghost - 1; ++r) d.ghosts[r] = d.ghosts[r + 1];
-                                c->n_ghost--;
-                                break;
-                            }
-                    }
-                    d.tstate[n] = KST_TILE_ACTIVE;
-                    d.eflags[n] = 0;
-                    if (nnew < MAXNEW) newt[nnew++] = n;
-                    else bad = 2;
-                }
+    if (ncand > MAXCAND) {
+        if (threadIdx.x == 0) {
+            nnew = 0;
+            bad = 0;
+            activate_walk(d, p, newt, nnew, bad);
+            if (bad) {
+                c->status = ST_ERROR;
+                c->err = (bad == 1) ? KST_ERR_HALT : KST_ERR_STATE;
             }
         }
-        if (bad) {
-            c->status = ST_ERROR;
-            c->err = (bad == 1) ? KST_ERR_HALT : KST_ERR_STATE;
+        __syncthreads();
+        if (bad) return;
+        for (int q = threadIdx.x; q < c->n_active; q += ACT_NT) d.eflags[d.active[q]] = 0;
+    } else {
+        if (threadIdx.x == 0) {
+            nnew = 0;
+            bad = 0;
+            int size = n0;
+            for (int s = s0; s < 4 && !bad; ++s) {
+                for (int j = 0; j < ncand; ++j) {
+                    const int e = cand_e[j];
+                    if (!(e & (1 << s))) continue;
+                    const int v = cand_k[j];
+                    int below = 0;
+                    for (int m = 0; m < nnew; ++m) below += newt[m] < v;
+                    if ((e >> 4) + below >= size) break;
+                    const int n = side_tile(p, v, s);
+                    const int act = add_action(p, d.tstate, n);
+                    if (act == ADD_HALT) { bad = 1; break; }
+                    if (act == ADD_TILE) {
+                        d.tstate[n] = KST_TILE_ACTIVE;
+                        d.eflags[n] = 0;
+                        if (nnew == MAXNEW) { bad = 2; break; }
+                        newt[nnew++] = n;
+                    }
+                }
+                size = n0 + nnew;
+            }
+            if (bad) {
+                c->status = ST_ERROR;
+                c->err = (bad == 1) ? KST_ERR_HALT : KST_ERR_STATE;
+            }
         }
+        __syncthreads();
+        if (bad) return;
+        for (int j = threadIdx.x; j < ncand; j += ACT_NT) d.eflags[cand_k[j]] = 0;
+        const int nn = nnew;
+        if (nn == 0) return;
+        // the new tiles in order (ranks by counting: the values are distinct)
+        // and each one's place in the initial list
+        for (int m = threadIdx.x; m < nn; m += ACT_NT) {
+            const int v = newt[m];
+            int rk = 0;
+            for (int i = 0; i < nn; ++i) rk += newt[i] < v;
+            srt[rk] = v;
+            int lo = 0, hi = n0;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (d.active[mid] < v) lo = mid + 1;
+                else hi = mid;
+            }
+            ipos[rk] = lo;
+            if (rk == 0) first_move = lo;
+        }
+        __syncthreads();
+        // move the initial entries up by the new tiles below them, the top
+        // round first (every entry is read before any is written)
+        for (int top = ((n0 - 1) / (ACT_NT * ACT_R)) * (ACT_NT * ACT_R); top + ACT_NT * ACT_R > first_move;
+             top -= ACT_NT * ACT_R) {
+            int v[ACT_R];
+#pragma unroll
+            for (int r = 0; r < ACT_R; ++r) {
+                const int q = top + r * ACT_NT + (int)threadIdx.x;
+                v[r] = q < n0 && q >= first_move ? d.active[q] : -1;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < ACT_R; ++r)
+                if (v[r] >= 0) {
+                    int lo = 0, hi = nn;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (srt[mid] < v[r]) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    d.active[top + r * ACT_NT + threadIdx.x + lo] = v[r];
+                }
+            __syncthreads();
+            if (top == 0) break;
+        }
+        for (int m = threadIdx.x; m < nn; m += ACT_NT) d.active[ipos[m] + m] = srt[m];
+        // tiles that were ghosts leave the ghost list, which keeps its order
+        const int ng0 = c->n_ghost;
+        int kept = 0;
+        for (int base = 0; base < ng0; base += ACT_NT) {
+            const int q = base + threadIdx.x;
+            const int g = q < ng0 ? d.ghosts[q] : -1;
+            const bool keep = g >= 0 && d.tstate[g] != KST_TILE_ACTIVE;
+            const int slot = block_append(keep ? 1 : 0, kept, wsum);
+            if (keep) d.ghosts[slot] = g;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            c->n_active = n0 + nn;
+            c->n_ghost = kept;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (bad) return;
-    // clear the flags consumed by this pass
-    for (int q = threadIdx.x; q < c->n_active; q += blockDim.x) d.eflags[d.active[q]] = 0;
     if (nnew == 0) return;
     __syncthreads();
     // ActivateTile (src/UpdateTiles.f90:342-384): cell bt = 0, face
@@ -1849,7 +2047,6 @@ __device__ void activate_body(const DevT& d, const Params& p) {
     }
     for (int q = 0; q < nnew; ++q) dry_tile(d, p, newt[q], true);
     // new ghosts around each new tile (AddGhostTiles, src/UpdateTiles.f90:389-481)
-    __shared__ int ng_start;
     if (threadIdx.x == 0) {
         ng_start = c->n_ghost;
         for (int q = 0; q < nnew; ++q) {
@@ -1866,19 +2063,27 @@ __device__ void activate_body(const DevT& d, const Params& p) {
     __syncthreads();
     for (int q = ng_start; q < c->n_ghost; ++q) dry_tile(d, p, d.ghosts[q], false);
     __syncthreads();
-    // ghosts whose neighbourhood changed rebuild their edge w slopes
-    for (int q = 0; q < c->n_ghost; ++q) {
-        const int g = d.ghosts[q];
-        bool touched = q >= ng_start;
-        for (int r = 0; r < nnew && !touched; ++r)
+    // ghosts whose neighbourhood changed rebuild their edge w slopes, taken
+    // in list order a round of ACT_NT at a time
+    const int ngh = c->n_ghost;
+    for (int base = 0; base < ngh; base += ACT_NT) {
+        const int q = base + threadIdx.x;
+        const int g = q < ngh ? d.ghosts[q] : -1;
+        bool touched = q < ngh && q >= ng_start;
+        for (int r = 0; r < nnew && g >= 0 && !touched; ++r)
             for (int u = 0; u < ghost_count(p) && !touched; ++u)
                 if (ghost_candidate(p, newt[r], u) == g) touched = true;
-        if (touched) ghost_slopes(d, p, g);
+        int cnt = 0;
+        const int slot = block_append(touched ? 1 : 0, cnt, wsum);
+        if (touched) tl[slot] = g;
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) ghost_slopes(d, p, tl[j]);
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(256) void activate_kernel(const Dev* __restrict__ d,
-                                                       const Params* __restrict__ p) {
+__global__ __launch_bounds__(ACT_NT) void activate_kernel(const Dev* __restrict__ d,
+                                                          const Params* __restrict__ p) {
     activate_body(const_arg(d), *p);
 }
 
@@ -2029,7 +2234,7 @@ struct Batch {
 constexpr int BATCH_PLAN_THREADS = 1024;
 
 // Activation of every slot (one block per slot).
-__global__ __launch_bounds__(256) void batch_activate_kernel(Batch bt) {
+__global__ __launch_bounds__(ACT_NT) void batch_activate_kernel(Batch bt) {
     const int s = blockIdx.x;
     if (!bt.on[s]) return;
     activate_body(const_arg(bt.dev + s), generic(const_arg(bt.par + s)));
@@ -2416,7 +2621,7 @@ int launch_iteration(kst_handle* h, bool profiled, bool small, int cap, int grid
     const int nblk_stage = sm ? 8 * ((cap * spt_s + 7) / 8) : (grid_limit > 0 ? std::min(nblk, grid_limit) : nblk);
     const Dev* const A = h->d_args;
     const Params* const P = h->d_par;
-    LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(256), A, P);
+    LAUNCH(KW_ACTIVATE, activate_kernel, dim3(1), dim3(ACT_NT), A, P);
     const StageSet& k = stage_set(visc, p.limiter, d.mixed != 0, sm, p.oned != 0);
     for (int q = 0; q < 4; ++q)
         LAUNCH(KW_STAGE + q, k.fn[q], dim3(nblk_stage), dim3(STAGE_NT), A, P);
@@ -3481,7 +3686,7 @@ static int batch_launch_iteration(kst_batch* b) {
     const Batch& bt = b->bt;
     const BatchSet& k = batch_set(b->visc == 1, b->mixed == 1, b->anylim == 1);
     const bool mo = b->morpho != 0;
-    hipLaunchKernelGGL(batch_activate_kernel, dim3(b->nslot), dim3(256), 0, b->stream, bt);
+    hipLaunchKernelGGL(batch_activate_kernel, dim3(b->nslot), dim3(ACT_NT), 0, b->stream, bt);
     hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(BATCH_PLAN_THREADS), 0, b->stream, bt, mo ? 3 : 1);
     for (int q = 0; q < 4; ++q)
         hipLaunchKernelGGL(k.fn[q], dim3(b->grid), dim3(STAGE_NT), 0, b->stream, bt, 0);
