@@ -1912,16 +1912,19 @@ __device__ __forceinline__ void activate_body(const DevT& d, const Params& p) {
                 if (s >= s0 && ((f[r] >> s) & 1u) &&
                     add_action_of(p, nb[r][s], (uint8_t)ts[r][s]) != ADD_NONE)
                     b[r] |= 1u << s;
-            if (f[r]) d.eflags[k[r]] = b[r];
             cnt += b[r] != 0;
         }
         int slot = block_append(cnt, ncand, wsum);
+        // the flags become the candidate sides after the barriers of the
+        // append, which would otherwise wait for these stores
 #pragma unroll
-        for (int r = 0; r < ACT_R; ++r)
+        for (int r = 0; r < ACT_R; ++r) {
+            if (f[r]) d.eflags[k[r]] = b[r];
             if (b[r]) {
                 if (slot < MAXCAND) { cand_k[slot] = k[r]; cand_e[slot] = ((q0 + r) << 4) | (int)b[r]; }
                 ++slot;
             }
+        }
     }
     if (ncand == 0) return;   // every flag cleared, nothing to add
     __syncthreads();
