@@ -49,7 +49,8 @@ __device__ void morpho_work(const DevT& d, const Params& p, MWork& m, bool need_
         wspec = cap_item();
         tspec = d.active[min(wspec / spt, p.ntiles)];   // the list holds ntiles + 1 entries
     }
-    const auto* c = ctrl_view(d);
+    const CtrlSnap cs = snap_ctrl(ctrl_view(d));
+    const CtrlSnap* c = &cs;
     int skip = (c->status != ST_NEW && c->status != ST_RETRY);
     double dt = 0.0;
     if (!skip) {
@@ -801,16 +802,18 @@ __device__ __forceinline__ void redistribute_body(const DevT& d, const Params& p
     Ctrl* c = d.ctrl;
     const int tid = threadIdx.x;
     if (tid == 0) {
-        int skip = (c->status != ST_NEW && c->status != ST_RETRY);
+        const CtrlSnap cs = snap_ctrl(ctrl_view(d));
+        const long long ndirty = ctrl_view(d)->ndirty;
+        int skip = (cs.status != ST_NEW && cs.status != ST_RETRY);
         if (!skip) {
-            const double dt = step_dt(d, p, c);
-            if (!first_operator_ok(c, p, dt) || c->mref) skip = 1;
+            const double dt = step_dt(d, p, &cs);
+            if (!first_operator_ok(&cs, p, dt) || cs.mref) skip = 1;
         }
-        if (!skip && c->ncand <= 0) skip = 1;
+        if (!skip && cs.ncand <= 0) skip = 1;
         if (!skip) count_work(c, KW_REDIST);
         sh_skip = skip;
-        sh_n = c->ncand;
-        sh_ndirty = c->ndirty;
+        sh_n = cs.ncand;
+        sh_ndirty = ndirty;
     }
     __syncthreads();
     if (sh_skip) return;
@@ -884,7 +887,8 @@ __global__ __launch_bounds__(RD_THREADS) void redistribute_kernel(const Dev* __r
 // morphodynamic operator ran and its checks passed).
 template <class DevT>
 __device__ __forceinline__ bool face_topo_due(const DevT& d, const Params& p) {
-    const auto* c = ctrl_view(d);
+    const CtrlSnap cs = snap_ctrl(ctrl_view(d));
+    const CtrlSnap* c = &cs;
     if (c->status != ST_NEW && c->status != ST_RETRY) return false;
     const double dt = step_dt(d, p, c);
     return first_operator_ok(c, p, dt) && !c->mref;

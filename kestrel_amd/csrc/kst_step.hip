@@ -220,6 +220,29 @@ __device__ __forceinline__ ConstArg<Ctrl>* ctrl_view(const DevT& d) {
     return (ConstArg<Ctrl>*)d.ctrl;
 }
 
+// The fields of the control block that a kernel's start-of-launch checks
+// read, loaded together: the CFL slots sit on cache lines of their own, and
+// read behind each check's branch they would be fetched one after another.
+// Has the member names of Ctrl, so that step_dt and first_operator_ok take
+// it as they take the control block.
+struct CtrlSnap {
+    int status, n_active, mref, cur, ncand;
+    double t, tend, next_t, dt_retry;
+    struct {
+        unsigned long long bits;
+    } slot[6];
+};
+
+template <class CtrlP>
+__device__ __forceinline__ CtrlSnap snap_ctrl(CtrlP c) {
+    CtrlSnap s;
+    s.status = c->status; s.n_active = c->n_active; s.mref = c->mref; s.cur = c->cur; s.ncand = c->ncand;
+    s.t = c->t; s.tend = c->tend; s.next_t = c->next_t; s.dt_retry = c->dt_retry;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s.slot[k].bits = c->slot[k].bits;
+    return s;
+}
+
 // The step's (hydraulic) dt: substep-1 advice clamped to the next
 // output/flux-series time, or to half of it with morphodynamics
 // (src/TimeStepper.f90:155-168), or the refined dt of a retried step.
@@ -841,18 +864,21 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
             wspec = cap_item();
             tspec = d.active[min(wspec / spt, p.ntiles)];   // the list holds ntiles + 1 entries
         }
-        const auto* c = ctrl_view(d);
+        const CtrlSnap cs = snap_ctrl(ctrl_view(d));
+        const CtrlSnap* c = &cs;
         int skip = 0;
         if (c->status == ST_DONE || c->status == ST_ERROR) skip = 1;
         if (STAGE == 1 && !d.second && c->status != ST_NEW) skip = 1;
-        const int sb = d.second ? 3 : 0;
         double dt = 0.0;
         if (!skip && (STAGE >= 2 || d.second)) {
             dt = step_dt(d, p, c);
             if (d.second && (!first_operator_ok(c, p, dt) || c->mref)) skip = 1;
             if (d.second && STAGE >= 2 && 0.9 * advised_from_slot(p, c->slot[3].bits, c->n_active) < dt) skip = 1;
-            if (STAGE >= 3 && advised_from_slot(p, c->slot[sb + 1].bits, c->n_active) < dt) skip = 1;
-            if (STAGE >= 4 && advised_from_slot(p, c->slot[sb + 2].bits, c->n_active) < dt) skip = 1;
+            // constant slot indices keep the snapshot in registers
+            const unsigned long long s1 = d.second ? c->slot[4].bits : c->slot[1].bits;
+            const unsigned long long s2 = d.second ? c->slot[5].bits : c->slot[2].bits;
+            if (STAGE >= 3 && advised_from_slot(p, s1, c->n_active) < dt) skip = 1;
+            if (STAGE >= 4 && advised_from_slot(p, s2, c->n_active) < dt) skip = 1;
         }
         s.dt = dt;
         s.cur = c->cur;
