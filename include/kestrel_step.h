@@ -291,6 +291,16 @@ int kst_autotune_choices(const kst_handle* h, int max, int32_t* bucket, int32_t*
  * small candidate sets.  0 restores a default. */
 int kst_set_redistribution_chunk(kst_handle* h, int chunk, int lds_sort);
 
+/* Activation candidates (no counterpart in the reference, whose
+ * CheckIfNearBoundaries walks the active list itself,
+ * src/TimeStepper.f90:935-1150).  After an accepted step the device collects
+ * the flagged sides whose neighbour would be activated or halt the run, and
+ * walks up to `cap` of them from LDS (default and maximum 2048); a step with
+ * more walks the active list in place, as the reference does.  Results do not
+ * depend on it; the tests lower it to run that walk.  0 restores the
+ * default. */
+int kst_set_activation_candidates(kst_handle* h, int cap);
+
 /* Diagnostics (no counterpart in the reference, which has no timers;
  * src/Messages.f90:275-283 prints only t and dt).  While profiling is on, the
  * library launches one iteration at a time and brackets every kernel launch

@@ -149,6 +149,7 @@ struct Dev {
     int cand_cap;
     int rd_chunk;                     // candidates walked at once (<= RD_THREADS)
     int rd_lds_sort;                  // largest padded candidate count sorted in LDS (<= RD_LDS)
+    int act_cand;                     // activation candidates walked from LDS (<= MAXCAND)
     int morpho;
     // Redistribution may write ghost tiles' copies of shared vertices and
     // ghost cells' w, Hnpsi into intermed3 (src/Redistribute.f90:362-470); the
@@ -1954,7 +1955,7 @@ __device__ __forceinline__ void activate_body(const DevT& d, const Params& p) {
     }
     if (ncand == 0) return;   // every flag cleared, nothing to add
     __syncthreads();
-    if (ncand > MAXCAND) {
+    if (ncand > d.act_cand) {
         if (threadIdx.x == 0) {
             nnew = 0;
             bad = 0;
@@ -3026,6 +3027,7 @@ int kst_create(const kst_config* cfg, kst_handle** out) {
         d.dirty_cap = 1 << 16;
         rc |= dalloc(h, &d.dirty, (size_t)d.dirty_cap);
     }
+    d.act_cand = MAXCAND;
     rc |= dalloc(h, &d.maxima, 11 * FS);
     rc |= dalloc(h, &d.tstate, (size_t)h->ntiles); rc |= dalloc(h, &d.has_src, (size_t)h->ntiles);
     rc |= dalloc(h, &d.active, (size_t)h->ntiles + 1); rc |= dalloc(h, &d.ghosts, (size_t)h->ntiles + 1);
@@ -3642,6 +3644,18 @@ int kst_set_redistribution_chunk(kst_handle* h, int chunk, int lds_sort) {
     const int c = chunk ? chunk : RD_THREADS, l = lds_sort ? lds_sort : RD_LDS;
     h->d.rd_chunk = h->d2.rd_chunk = c;
     h->d.rd_lds_sort = h->d2.rd_lds_sort = l;
+    const Dev both[2] = {h->d, h->d2};
+    HIPCHK(h, hipMemcpyAsync(h->d_args, both, sizeof both, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return KST_OK;
+}
+
+int kst_set_activation_candidates(kst_handle* h, int cap) {
+    if (!h) return KST_ERR_CONFIG;
+    if (cap < 0 || cap > MAXCAND)
+        return set_err(h, KST_ERR_CONFIG, "kst_set_activation_candidates: cap must lie in 1.." +
+                                              std::to_string(MAXCAND) + " (0: default)");
+    h->d.act_cand = h->d2.act_cand = cap ? cap : MAXCAND;
     const Dev both[2] = {h->d, h->d2};
     HIPCHK(h, hipMemcpyAsync(h->d_args, both, sizeof both, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -36,7 +36,7 @@ EXPORTS = ["kst_config_defaults", "kst_config_derive", "kst_create", "kst_set_to
            "kst_batch_run", "kst_batch_stats", "kst_batch_last_error", "kst_batch_destroy",
            "kst_get_cell_topography", "kst_set_maxima", "kst_get_output", "kst_volume", "kst_set_grid_cap",
            "kst_set_redistribution_chunk", "kst_autotune", "kst_autotune_choices",
-           "kst_volume_noalloc", "kst_set_cell_topography"]
+           "kst_volume_noalloc", "kst_set_cell_topography", "kst_set_activation_candidates"]
 
 
 class KstStats(ctypes.Structure):
@@ -82,6 +82,8 @@ def lib():
         L.kst_set_grid_cap.restype = ctypes.c_int
         L.kst_set_redistribution_chunk.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.kst_set_redistribution_chunk.restype = ctypes.c_int
+        L.kst_set_activation_candidates.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.kst_set_activation_candidates.restype = ctypes.c_int
         L.kst_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.kst_autotune.restype = ctypes.c_int
         i32p = ctypes.POINTER(ctypes.c_int32)
@@ -252,6 +254,11 @@ class Stepper:
         """Walk at most `chunk` redistribution candidates at once and sort at
         most `lds_sort` in LDS (0: defaults); results do not depend on them."""
         self._check(self.L.kst_set_redistribution_chunk(self.h, int(chunk), int(lds_sort)))
+
+    def set_activation_candidates(self, cap: int):
+        """Walk at most `cap` activation candidates from LDS, more through the
+        active list itself (0: default); results do not depend on it."""
+        self._check(self.L.kst_set_activation_candidates(self.h, int(cap)))
 
     def autotune(self, enable: bool = True):
         """Choose the launch shape per active-set bucket from timed trials."""
