@@ -149,10 +149,13 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
         map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
         const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
         // ghost tiles keep the derived fields they were given: zero
-        // (SetDefaultTileData), or the Dirichlet boundary depth
-        double h = 0.0;
-        if (d.tstate[tc] == KST_TILE_ACTIVE) h = compute_hn(Win[gi], d.b0c[gi], Tin.btc[gi], Tin.cgam[gi]);
-        else if (p.bc == KST_BC_DIRICHLET && dirichlet_ghost(p, tc)) h = p.bc_hn;
+        // (SetDefaultTileData), or the Dirichlet boundary depth.  The inputs
+        // are loaded whatever the tile's state (every tile holds the cell)
+        // and the value selected, so that the loads do not wait behind the
+        // state's branch.
+        const bool act = d.tstate[tc] == KST_TILE_ACTIVE;
+        double h = compute_hn(Win[gi], d.b0c[gi], Tin.btc[gi], Tin.cgam[gi]);
+        if (!act) h = (p.bc == KST_BC_DIRICHLET && dirichlet_ghost(p, tc)) ? p.bc_hn : 0.0;
         hn[q] = h < 0.0 ? 0.0 : h;
     }
     __syncthreads();
@@ -165,15 +168,23 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
         int tc, ci, cj;
         map_cell(p, t, bx0 + rx, by0 + ry, tc, ci, cj);
         const size_t gi = (size_t)tc * p.NC + cj * p.nx + ci;
+        // the cell's inputs, loaded with its state (every tile holds the
+        // cell) and held by the empty asm, so that the loads issue together
+        // rather than one after another behind the state's branch
         const double bxc = Tin.cbx[gi], byc = Tin.cby[gi];
+        const double hin = Hin[gi], btc = Tin.btc[gi];
+        const double ua = (S == 1) ? d.S3[FS + gi] : d.UV[gi];
+        const double va = (S == 1) ? (ONED ? 0.0 : d.S3[2 * FS + gi]) : d.UV[FS + gi];
+        const int st = d.tstate[tc];
+        asm volatile("" ::"v"(st), "v"(bxc), "v"(byc), "v"(hin), "v"(btc), "v"(ua), "v"(va));
         sbx[q] = bxc;
         sby[q] = byc;
         double e = 0.0;
-        const bool on = d.tstate[tc] == KST_TILE_ACTIVE;
+        const bool on = st == KST_TILE_ACTIVE;
         const bool own = rx >= 0 && ry >= 0 && rx < bxa && ry < bya;
         if (on) {
             const double h = hn[rg(rx, ry)];
-            double hp = Hin[gi];
+            double hp = hin;
             if (hp < 0.0) hp = 0.0;
             const double den = h * h + dmax(h * h, eps * eps);
             const double psi = dmin(2.0 * h * hp / den, p.max_pack);
@@ -182,17 +193,17 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
             double u, v;
             if (S == 1) {
                 const double rho = density(p, psi);
-                u = 2.0 * h * d.S3[FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
-                v = ONED ? 0.0 : 2.0 * h * d.S3[2 * FS + gi] / (h * h + dmax(h * h, eps * eps)) / rho;
+                u = 2.0 * h * ua / (h * h + dmax(h * h, eps * eps)) / rho;
+                v = ONED ? 0.0 : 2.0 * h * va / (h * h + dmax(h * h, eps * eps)) / rho;
                 if (own) { d.UV[gi] = u; d.UV[FS + gi] = v; }
             } else {
-                u = d.UV[gi];
-                v = d.UV[FS + gi];
+                u = ua;
+                v = va;
             }
             // the S and N neighbours only on a 2D grid (src/MorphodynamicRHS.f90:113-135)
             if (!(h < eps || hn[rg(rx - 1, ry)] < eps || hn[rg(rx + 1, ry)] < eps ||
                   (!ONED && (hn[rg(rx, ry - 1)] < eps || hn[rg(rx, ry + 1)] < eps)))) {
-                CellState cs{h, u, v, psi, Tin.btc[gi], bxc, byc};
+                CellState cs{h, u, v, psi, btc, bxc, byc};
                 e = erosion_minus_deposition(p, cs);
             }
         }
@@ -337,15 +348,19 @@ __device__ __forceinline__ bool morpho_pass(const DevT& d, const Params& p, int 
 // The grid is sized for the active set the host last saw; a block whose
 // share of the list holds more items takes them in turn.
 template <int S, int BY = B, bool ONED = false>
-__global__ __launch_bounds__(morpho_nt<BY>()) void morpho_kernel(const Dev* __restrict__ dp,
+// (the 16 x 16 shape keeps 4 waves per SIMD, two of its 512-thread blocks
+// per CU: at most 128 VGPRs)
+__global__ __launch_bounds__(morpho_nt<BY>(), BY == B ? 4 : 1) void morpho_kernel(const Dev* __restrict__ dp,
                                                                const Params* __restrict__ pp) {
     for (int pass = 0;; ++pass) {
         // fresh argument pointers per pass: loads through them are not
         // hoisted out of the loop and held in registers across passes
+        // (Params through the constant address space: through a plain
+        // pointer from the asm its fields would be flat vector loads)
         const Dev* d = dp;
-        const Params* p = pp;
+        ConstArg<Params>* p = (ConstArg<Params>*)pp;
         asm("" : "+s"(d), "+s"(p) : "s"(pass));
-        if (!morpho_pass<S, BY, ONED>(const_arg(d), *p, pass)) break;
+        if (!morpho_pass<S, BY, ONED>(const_arg(d), generic(*p), pass)) break;
         __syncthreads();
     }
 }
