@@ -86,8 +86,10 @@ enum {
     KW_N = 16
 };
 
+// An atomic whose result is unused: the thread does not wait for it (a
+// read-modify-write would hold block 0 for a round trip before its barrier).
 __device__ __forceinline__ void count_work(const Ctrl* c, int id) {
-    if (blockIdx.x == 0) const_cast<Ctrl*>(c)->work[id] += 1;
+    if (blockIdx.x == 0) atomicAdd((unsigned long long*)&const_cast<Ctrl*>(c)->work[id], 1ULL);
 }
 
 // Bed-dependent topography of one generation, per tile.  Each tile holds its
@@ -894,7 +896,8 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
         bool more = false;
         if (spec) w = pass == 0 ? wspec : nwork;
         else if (witem < 0) w = xcd_item(nwork, pass, &more);
-        if (!skip && w == 0) d.ctrl->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)] += 1;
+        if (!skip && w == 0)
+            atomicAdd((unsigned long long*)&d.ctrl->work[KW_STAGE + STAGE - 1 + (d.second ? 4 : 0)], 1ULL);
         if (w >= nwork) skip = 1;
         if (!skip) {
             s.tile = spec ? tspec : d.active[w / spt];
@@ -1082,38 +1085,34 @@ __device__ __forceinline__ bool stage_body(const DevT& d, const Params& p, Smem&
     // stage a ghost neighbour's copy of the shared vertices is the one the
     // redistribution may have written (intermed3, src/Redistribute.f90:362-410)
     if (tid < B) {
+        // the sides that apply, whether the neighbour is a ghost (in the last
+        // stage of a morphodynamic step), then the loads issued together and
+        // stored after: one round trip rather than one per side
         const int r = tid;
         const bool ovl = STAGE == 4 && d.morpho;
-        if (r < bya) {
-            const int row = (s.by0 + r) * (p.nx + 1);
-            if (s.bx0 == 0) {
-                const int tn = nb_tile(p, t, -1, 0);
-                s.xside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
-                                    ? ghost_bed<Smem::oned>(d, p, tn, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
-                                    : d.T.xib[tn * p.NFX + row + p.nx];
-            }
-            if (s.bx0 + bxa == p.nx) {
-                const int tn = nb_tile(p, t, 1, 0);
-                s.xside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
-                                    ? ghost_bed<Smem::oned>(d, p, tn, 0, s.by0 + r, 0, s.by0 + r + 1)
-                                    : d.T.xib[tn * p.NFX + row];
-            }
-        }
-        if (!Smem::oned && r < bxa) {
-            const int col = s.bx0 + r;
-            if (s.by0 == 0) {
-                const int tn = nb_tile(p, t, 0, -1);
-                s.yside[0][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
-                                    ? ghost_bed(d, p, tn, col, p.ny, col + 1, p.ny)
-                                    : d.T.yib[tn * p.NFY + p.ny * p.nx + col];
-            }
-            if (s.by0 + bya == p.ny) {
-                const int tn = nb_tile(p, t, 0, 1);
-                s.yside[1][r] = (ovl && d.tstate[tn] != KST_TILE_ACTIVE)
-                                    ? ghost_bed(d, p, tn, col, 0, col + 1, 0)
-                                    : d.T.yib[tn * p.NFY + col];
-            }
-        }
+        const bool sx0 = r < bya && s.bx0 == 0, sx1 = r < bya && s.bx0 + bxa == p.nx;
+        const bool sy0 = !Smem::oned && r < bxa && s.by0 == 0;
+        const bool sy1 = !Smem::oned && r < bxa && s.by0 + bya == p.ny;
+        const int row = (s.by0 + r) * (p.nx + 1), col = s.bx0 + r;
+        const int tW = nb_tile(p, t, -1, 0), tE = nb_tile(p, t, 1, 0);
+        const int tS = Smem::oned ? t : nb_tile(p, t, 0, -1), tN = Smem::oned ? t : nb_tile(p, t, 0, 1);
+        const bool gW = ovl && sx0 && d.tstate[tW] != KST_TILE_ACTIVE;
+        const bool gE = ovl && sx1 && d.tstate[tE] != KST_TILE_ACTIVE;
+        const bool gS = ovl && sy0 && d.tstate[tS] != KST_TILE_ACTIVE;
+        const bool gN = ovl && sy1 && d.tstate[tN] != KST_TILE_ACTIVE;
+        double vW = 0.0, vE = 0.0, vS = 0.0, vN = 0.0;
+        if (sx0)
+            vW = gW ? ghost_bed<Smem::oned>(d, p, tW, p.nx, s.by0 + r, p.nx, s.by0 + r + 1)
+                    : d.T.xib[tW * p.NFX + row + p.nx];
+        if (sx1)
+            vE = gE ? ghost_bed<Smem::oned>(d, p, tE, 0, s.by0 + r, 0, s.by0 + r + 1) : d.T.xib[tE * p.NFX + row];
+        if (sy0)
+            vS = gS ? ghost_bed(d, p, tS, col, p.ny, col + 1, p.ny) : d.T.yib[tS * p.NFY + p.ny * p.nx + col];
+        if (sy1) vN = gN ? ghost_bed(d, p, tN, col, 0, col + 1, 0) : d.T.yib[tN * p.NFY + col];
+        if (sx0) s.xside[0][r] = vW;
+        if (sx1) s.xside[1][r] = vE;
+        if (sy0) s.yside[0][r] = vS;
+        if (sy1) s.yside[1][r] = vN;
     }
     __syncthreads();
 
