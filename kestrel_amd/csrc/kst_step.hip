@@ -1484,17 +1484,18 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
             gl = (i == 0) ? T.cgam[tW * p.NC + j * nx + nx - 1] : T.cgam[t * p.NC + j * nx + i - 1];
             gr_ = (i == nx) ? T.cgam[tE * p.NC + j * nx] : T.cgam[t * p.NC + j * nx + i];
         }
+        // the vertex columns either side: this tile's, or across the tile
+        // edge the neighbour's (selected per lane, so that a wave's edge and
+        // interior faces load together rather than one branch after another)
+        const double* bR = (i == nx) ? bE : b;
+        const double* btR = (i == nx) ? btE : bt;
+        const int iR = (i == nx) ? 1 : i + 1;
+        const double* bL = (i == 0) ? bW : b;
+        const double* btL = (i == 0) ? btW : bt;
+        const int iL = (i == 0) ? nx - 1 : i - 1;
         Kahan kx;
-        if (i >= 1 && i <= nx - 1) {
-            kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(V(b, i + 1, j + 1)); kx.add(V(bt, i + 1, j + 1));
-            kx.add(-V(b, i - 1, j)); kx.add(-V(bt, i - 1, j)); kx.add(-V(b, i - 1, j + 1)); kx.add(-V(bt, i - 1, j + 1));
-        } else if (i == 0) {
-            kx.add(V(b, 1, j)); kx.add(V(bt, 1, j)); kx.add(V(b, 1, j + 1)); kx.add(V(bt, 1, j + 1));
-            kx.add(-V(bW, nx - 1, j)); kx.add(-V(btW, nx - 1, j)); kx.add(-V(bW, nx - 1, j + 1)); kx.add(-V(btW, nx - 1, j + 1));
-        } else {
-            kx.add(V(bE, 1, j)); kx.add(V(btE, 1, j)); kx.add(V(bE, 1, j + 1)); kx.add(V(btE, 1, j + 1));
-            kx.add(-V(b, nx - 1, j)); kx.add(-V(bt, nx - 1, j)); kx.add(-V(b, nx - 1, j + 1)); kx.add(-V(bt, nx - 1, j + 1));
-        }
+        kx.add(V(bR, iR, j)); kx.add(V(btR, iR, j)); kx.add(V(bR, iR, j + 1)); kx.add(V(btR, iR, j + 1));
+        kx.add(-V(bL, iL, j)); kx.add(-V(btL, iL, j)); kx.add(-V(bL, iL, j + 1)); kx.add(-V(btL, iL, j + 1));
         const double dbdx = 0.25 * p.dxr * kx.s;
         const double btf = 0.5 * (V(bt, i, j) + V(bt, i, j + 1));
         Kahan ky;
@@ -1529,17 +1530,15 @@ __device__ void tile_face_topo(const DevT& d, const Params& p, const TopoT& T, i
         kx.add(V(b, i + 1, j)); kx.add(V(bt, i + 1, j)); kx.add(-V(b, i, j)); kx.add(-V(bt, i, j));
         const double dbdx = p.dxr * kx.s;
         const double btf = 0.5 * (V(bt, i, j) + V(bt, i + 1, j));
+        const double* bU = (j == ny) ? bN : b;
+        const double* btU = (j == ny) ? btN : bt;
+        const int jU = (j == ny) ? 1 : j + 1;
+        const double* bD = (j == 0) ? bS : b;
+        const double* btD = (j == 0) ? btS : bt;
+        const int jD = (j == 0) ? ny - 1 : j - 1;
         Kahan ky;
-        if (j >= 1 && j <= ny - 1) {
-            ky.add(V(b, i + 1, j + 1)); ky.add(V(bt, i + 1, j + 1)); ky.add(V(b, i, j + 1)); ky.add(V(bt, i, j + 1));
-            ky.add(-V(b, i + 1, j - 1)); ky.add(-V(bt, i + 1, j - 1)); ky.add(-V(b, i, j - 1)); ky.add(-V(bt, i, j - 1));
-        } else if (j == 0) {
-            ky.add(V(b, i + 1, 1)); ky.add(V(bt, i + 1, 1)); ky.add(V(b, i, 1)); ky.add(V(bt, i, 1));
-            ky.add(-V(bS, i + 1, ny - 1)); ky.add(-V(btS, i + 1, ny - 1)); ky.add(-V(bS, i, ny - 1)); ky.add(-V(btS, i, ny - 1));
-        } else {
-            ky.add(V(bN, i + 1, 1)); ky.add(V(btN, i + 1, 1)); ky.add(V(bN, i, 1)); ky.add(V(btN, i, 1));
-            ky.add(-V(b, i + 1, ny - 1)); ky.add(-V(bt, i + 1, ny - 1)); ky.add(-V(b, i, ny - 1)); ky.add(-V(bt, i, ny - 1));
-        }
+        ky.add(V(bU, i + 1, jU)); ky.add(V(btU, i + 1, jU)); ky.add(V(bU, i, jU)); ky.add(V(btU, i, jU));
+        ky.add(-V(bD, i + 1, jD)); ky.add(-V(btD, i + 1, jD)); ky.add(-V(bD, i, jD)); ky.add(-V(btD, i, jD));
         const double dbdy = 0.25 * p.dyr * ky.s;
         double ib = V(bt, i, j) + V(bt, i + 1, j);
         ib = ib + V(b, i, j);
