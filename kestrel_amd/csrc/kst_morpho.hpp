@@ -941,22 +941,41 @@ template <class DevT>
 __device__ void commit_part(const DevT& d, const Params& p, int t, int part) {
     const int lane0 = threadIdx.x + part * blockDim.x, stride = blockDim.x * TILE_PARTS;
     const size_t NV = (size_t)(p.nx + 1) * (p.ny + 1);
-    for (size_t v = lane0; v < NV; v += stride) d.TA.vbt[t * NV + v] = d.TB.vbt[t * NV + v];
-    for (int k = lane0; k < p.NC; k += stride) {
-        const size_t g = (size_t)t * p.NC + k;
-        d.TA.btc[g] = d.TB.btc[g]; d.TA.cbx[g] = d.TB.cbx[g]; d.TA.cby[g] = d.TB.cby[g];
-        d.TA.cgam[g] = d.TB.cgam[g]; d.TA.pxu[g] = d.TB.pxu[g]; d.TA.pxv[g] = d.TB.pxv[g];
-        d.TA.pyv[g] = d.TB.pyv[g];
-    }
-    for (int k = lane0; k < p.NFX; k += stride) {
-        const size_t f = (size_t)t * p.NFX + k;
-        d.TA.xbt[f] = d.TB.xbt[f]; d.TA.xby[f] = d.TB.xby[f]; d.TA.xgam[f] = d.TB.xgam[f];
-        d.TA.xib[f] = d.TB.xib[f]; d.TA.xnp[f] = d.TB.xnp[f]; d.TA.xnm[f] = d.TB.xnm[f];
-    }
-    for (int k = lane0; k < p.NFY; k += stride) {
-        const size_t f = (size_t)t * p.NFY + k;
-        d.TA.ybt[f] = d.TB.ybt[f]; d.TA.ybx[f] = d.TB.ybx[f]; d.TA.ygam[f] = d.TB.ygam[f];
-        d.TA.yib[f] = d.TB.yib[f]; d.TA.ynp[f] = d.TB.ynp[f]; d.TA.ynm[f] = d.TB.ynm[f];
+    const int nv = (int)NV, nmax = max(max(nv, p.NC), max(p.NFX, p.NFY));
+    // every array's element k read first, then written: written group by
+    // group, each group's loads would wait for the stores before them (the
+    // two generations are not known to be distinct)
+    for (int k = lane0; k < nmax; k += stride) {
+        const bool hv = k < nv, hc = k < p.NC, hx = k < p.NFX, hy = k < p.NFY;
+        const size_t v = (size_t)t * NV + k, g = (size_t)t * p.NC + k;
+        const size_t fx = (size_t)t * p.NFX + k, fy = (size_t)t * p.NFY + k;
+        double a = 0.0, c[7] = {}, x[6] = {}, y[6] = {};
+        if (hv) a = d.TB.vbt[v];
+        if (hc) {
+            c[0] = d.TB.btc[g]; c[1] = d.TB.cbx[g]; c[2] = d.TB.cby[g]; c[3] = d.TB.cgam[g];
+            c[4] = d.TB.pxu[g]; c[5] = d.TB.pxv[g]; c[6] = d.TB.pyv[g];
+        }
+        if (hx) {
+            x[0] = d.TB.xbt[fx]; x[1] = d.TB.xby[fx]; x[2] = d.TB.xgam[fx];
+            x[3] = d.TB.xib[fx]; x[4] = d.TB.xnp[fx]; x[5] = d.TB.xnm[fx];
+        }
+        if (hy) {
+            y[0] = d.TB.ybt[fy]; y[1] = d.TB.ybx[fy]; y[2] = d.TB.ygam[fy];
+            y[3] = d.TB.yib[fy]; y[4] = d.TB.ynp[fy]; y[5] = d.TB.ynm[fy];
+        }
+        if (hv) d.TA.vbt[v] = a;
+        if (hc) {
+            d.TA.btc[g] = c[0]; d.TA.cbx[g] = c[1]; d.TA.cby[g] = c[2]; d.TA.cgam[g] = c[3];
+            d.TA.pxu[g] = c[4]; d.TA.pxv[g] = c[5]; d.TA.pyv[g] = c[6];
+        }
+        if (hx) {
+            d.TA.xbt[fx] = x[0]; d.TA.xby[fx] = x[1]; d.TA.xgam[fx] = x[2];
+            d.TA.xib[fx] = x[3]; d.TA.xnp[fx] = x[4]; d.TA.xnm[fx] = x[5];
+        }
+        if (hy) {
+            d.TA.ybt[fy] = y[0]; d.TA.ybx[fy] = y[1]; d.TA.ygam[fy] = y[2];
+            d.TA.yib[fy] = y[3]; d.TA.ynp[fy] = y[4]; d.TA.ynm[fy] = y[5];
+        }
     }
 }
 
