@@ -582,7 +582,15 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
     const int NX = p.ntx * nx, NY = p.nty * ny;
     const size_t NV = (size_t)(nx + 1) * (ny + 1);
     if (tid == 0) { s_stop = n; s_done = 0; s_ghost = 0; }
-    if (tid < n) { lk[tid] = d.cand_key[base + tid]; lx[tid] = d.cand_ex[base + tid]; lev[tid] = 0; }
+    // each candidate's tile is read from the active list here and used after
+    // the ranking, which that load overlaps
+    int ctile = 0, ccell = 0;
+    if (tid < n) {
+        const long long kk = d.cand_key[base + tid];
+        lk[tid] = kk; lx[tid] = d.cand_ex[base + tid]; lev[tid] = 0;
+        ctile = d.active[(int)(kk / p.NC)];
+        ccell = (int)(kk % p.NC);
+    }
     __syncthreads();
     // ---- rank of each candidate in (excess, scan order): P threads per
     // candidate count over interleaved parts of the list
@@ -603,10 +611,7 @@ __device__ bool redistribute_small(const DevT& d, const Params& p, int base, int
             atomicAdd(&lev[q], r);
         }
         __syncthreads();
-        if (tid < n) {
-            const long long kk = lk[tid];
-            cellid[lev[tid]] = d.active[(int)(kk / p.NC)] * p.NC + (int)(kk % p.NC);
-        }
+        if (tid < n) cellid[lev[tid]] = ctile * p.NC + ccell;
     }
     __syncthreads();
     int k = 0, i = 0, j = 0, I = 0, J = 0;
