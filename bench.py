@@ -65,12 +65,12 @@ STEP_WORDS = {"hydraulic": 42, "morpho": 134}
 # profiles/stage_kernel_plain_cap_2048.md, profiles/kernels_morpho_hills_1024.md.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_TRAFFIC = {
-    ("plain_cap_2048", "stage_kernel<2>"): (217657.9, 51263.5, 1634048.0),
-    ("plain_cap_2048", "stage_kernel<3>"): (204239.9, 51263.5, 1634048.0),
-    ("plain_cap_2048", "stage_kernel<4>"): (217718.4, 113545.8, 1634048.0),
-    ("morpho_hills_1024", "redistribute_kernel"): (92.4, 35.6, 5795.84),
-    ("morpho_hills_1024", "stage_kernel<2>"): (1092.4, 181.9, 5795.84),
-    ("morpho_hills_1024", "morpho_kernel<3>"): (632.8, 317.8, 5795.84),
+    ("plain_cap_2048", "stage_kernel<2>"): (212956.2, 51263.5, 1634048.0),
+    ("plain_cap_2048", "stage_kernel<3>"): (202253.5, 51263.5, 1634048.0),
+    ("plain_cap_2048", "stage_kernel<4>"): (214820.1, 113545.8, 1634048.0),
+    ("morpho_hills_1024", "redistribute_kernel"): (92.3, 35.6, 5795.84),
+    ("morpho_hills_1024", "stage_kernel<2>"): (1111.6, 181.9, 5795.84),
+    ("morpho_hills_1024", "morpho_kernel<3>"): (645.8, 317.7, 5795.84),
 }
 # VALU wave-instructions per productive dispatch (SQ_INSTS_VALU, active cells
 # per launch of that run): profiles/stage_kernel_plain_cap_2048_counters.md.
